@@ -1,0 +1,268 @@
+#!/usr/bin/env python3
+"""Benchmark of the structure-factor hot path on one MI355X.
+
+Headline (`value`): G pixel-atom terms/s of one large pattern, the
+"large single pattern" configuration: a 100,000-atom C/N/O/S molecule in a
+12 nm-radius sphere with per-atom B factors U[10, 40] A^2, on a 1024 x 1024
+detector (75 um pixels at 0.1 m, lambda 0.15 nm). A step is one
+spsim_hip_sf_pattern call (q list, packed atoms and output already in HBM).
+
+Also reported, from the "batched uneven instances" configuration (512
+molecules with log-uniform atom counts in [2e3, 2e5] at random orientations on
+a 512 x 512 detector): patterns/s of the persistent work-list kernel replayed
+from a hipGraph versus the plain one-launch-per-pattern path.
+
+The CPU baseline times the FP64 oracle (oracle/oracle.c, all host threads) on
+a bounded pixel sample of the headline workload; it is a reference point, not
+the target.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+# Flop-equivalents per pixel-atom term: 10 FP32 flops + sin + cos
+# at 8 flop-equivalents each (transcendentals run at 32/clk/CU on gfx950 against
+# 256 FP32 flops/clk/CU), plus exp2 and a multiply for the per-atom Debye-Waller factor.
+W_EQ_PLAIN = 26.0
+W_EQ_DW = 36.0
+PEAK_FP32_TFLOPS = 157.3
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--no-batched", action="store_true", help="skip the batched vs plain measurement")
+    p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    p.add_argument("--batched-patterns", type=int, default=512)
+    p.add_argument("--profile", action="store_true", help="headline workload only (for rocprofv3)")
+    return p.parse_args()
+
+
+def rel_l2(a, b):
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def headline(args, torch, dv, wl, _lib):
+    dev = torch.device("cuda", 0)
+    det, Z, pos, B = wl.config2()
+    q = dv.hkl_detector(det.n, det.n, det.width, det.width, det.distance, det.wavelength, device=dev)
+    atoms = dv.DeviceAtoms(dv.pack_atoms(Z, pos, atom_b=B), dev)
+    ff = dv.form_factor_table(device=dev)
+    npix = det.npix
+    I = torch.empty(npix, dtype=torch.float32, device=dev)
+    F = torch.empty((npix, 2), dtype=torch.float32, device=dev)
+    terms = float(npix) * atoms.natoms
+
+    # Tile choice: a short sweep of (pixels per lane, LDS chunk) with one atom
+    # range per tile (a single kernel per step).
+    candidates = [dv.make_tile(k, c, 1) for k in (2, 4, 8) for c in (128, 256)]
+    best, best_ms = None, 1e30
+    for t in candidates:
+        dv.structure_factor(q, atoms, ff, F=F, I=I, tile=t)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        dv.structure_factor(q, atoms, ff, F=F, I=I, tile=t)
+        e1.record()
+        e1.synchronize()
+        ms = e0.elapsed_time(e1)
+        if ms < best_ms:
+            best, best_ms = t, ms
+    tile = best
+
+    for _ in range(args.warmup):
+        dv.structure_factor(q, atoms, ff, F=F, I=I, tile=tile)
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for e0, e1 in ev:
+        e0.record()
+        dv.structure_factor(q, atoms, ff, F=F, I=I, tile=tile)
+        e1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kernel_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+    ms_per_step = wall * 1e3 / args.steps
+
+    # accuracy of the timed output on a pixel sample against the FP64 oracle
+    err = None
+    try:
+        import oracle
+
+        rng = np.random.default_rng(0)
+        sub = np.sort(rng.choice(npix, 256, replace=False))
+        hkl = q.cpu().numpy()
+        _, I_ref = oracle.sf_f64(hkl[sub], Z, pos, atom_b=B, threads=min(16, os.cpu_count() or 1))
+        err = rel_l2(I.cpu().numpy()[sub], I_ref)
+    except Exception as e:  # the oracle is a checker only
+        print(f"accuracy check skipped: {e}", file=sys.stderr)
+
+    achieved = terms * W_EQ_DW / (kernel_ms * 1e-3) / 1e12
+    res = {
+        "terms": terms,
+        "kernel_ms": kernel_ms,
+        "ms_per_step": ms_per_step,
+        "value": terms / (ms_per_step * 1e-3) / 1e9,
+        "tile": {"pixels_per_lane": tile.pixels_per_lane, "waves": tile.waves, "chunk_atoms": tile.chunk_atoms,
+                 "atom_splits": tile.atom_splits},
+        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                     "kernel": "sf_pattern_kernel", "w_eq_flops_per_term": W_EQ_DW,
+                     "terms_per_launch": terms, "kernel_ms": round(kernel_ms, 4)},
+        "rel_err_vs_fp64": err,
+        "natoms": atoms.natoms, "npix": npix, "Z": Z, "pos": pos, "B": B, "hkl": q,
+    }
+    return res
+
+
+def batched(args, torch, dv, wl, _lib):
+    dev = torch.device("cuda", 0)
+    det, mols, rots = wl.config3(args.batched_patterns)
+    hkl = wl  # placeholder to keep names short below
+    import oracle  # noqa: F401  (not used for timing)
+
+    q = dv.hkl_detector(det.n, det.n, det.width, det.width, det.distance, det.wavelength, device=dev).cpu().numpy()
+    offs = np.zeros(len(mols) + 1, np.int32)
+    offs[1:] = np.cumsum([len(m[0]) for m in mols])
+    Zc = np.ascontiguousarray(np.concatenate([m[0] for m in mols]), np.int32)
+    Pc = np.ascontiguousarray(np.concatenate([m[1] for m in mols]), np.float32)
+    pm = np.arange(len(mols), dtype=np.int32)
+    R = np.ascontiguousarray(rots.reshape(-1, 9), np.float64)
+    L = _lib.load()
+    plan = C.c_void_p()
+    vp = lambda a: a.ctypes.data_as(C.c_void_p)
+    _lib.check(L.spsim_hip_batch_create(vp(q), det.npix, len(mols), vp(offs), vp(Zc), vp(Pc), None, None, 0,
+                                        len(mols), vp(pm), vp(R), None, C.c_float(0.0), None, C.byref(plan)),
+               "spsim_hip_batch_create")
+    stream = torch.cuda.Stream()
+    sp = C.c_void_p(stream.cuda_stream)
+    try:
+        best = _lib.Tile()
+        rate = C.c_double(0)
+        _lib.check(L.spsim_hip_batch_autotune(plan, sp, C.byref(best), C.byref(rate)), "spsim_hip_batch_autotune")
+        terms = C.c_double(0)
+        L.spsim_hip_batch_terms(plan, C.byref(terms))
+        I = torch.empty((len(mols), det.npix), dtype=torch.float32, device=dev)
+        with torch.cuda.stream(stream):
+            _lib.check(L.spsim_hip_batch_run(plan, None, C.c_void_p(I.data_ptr()), 1, sp), "batch_run")  # capture + warm
+            stream.synchronize()
+            reps = 2
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(reps):
+                _lib.check(L.spsim_hip_batch_run(plan, None, C.c_void_p(I.data_ptr()), 1, sp), "batch_run")
+            e1.record(stream)
+            e1.synchronize()
+            ms_batched = e0.elapsed_time(e1) / reps
+            I_b = I.clone()
+            t0 = time.perf_counter()
+            _lib.check(L.spsim_hip_batch_run_plain(plan, None, C.c_void_p(I.data_ptr()), sp), "batch_run_plain")
+            stream.synchronize()
+            ms_plain = (time.perf_counter() - t0) * 1e3
+        agree = rel_l2(I.cpu().numpy(), I_b.cpu().numpy())
+        n = len(mols)
+        return {
+            "workload": f"{n} molecules, N log-uniform in [2e3, 2e5] (mean {offs[-1] / n:.0f}), radius 5 nm (N/5000)^(1/3), "
+                        f"uniform random orientations, 512x512 detector 75 um at 0.1 m, lambda 0.2 nm",
+            "terms": terms.value,
+            "batched_ms": round(ms_batched, 2), "plain_ms": round(ms_plain, 2),
+            "patterns_per_s_batched": round(n / (ms_batched * 1e-3), 2),
+            "patterns_per_s_plain": round(n / (ms_plain * 1e-3), 2),
+            "speedup_batched_vs_plain": round(ms_plain / ms_batched, 3),
+            "gterms_per_s_batched": round(terms.value / (ms_batched * 1e-3) / 1e9, 1),
+            "roofline_frac_batched": round(terms.value * W_EQ_PLAIN / (ms_batched * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, 4),
+            "tile": {"pixels_per_lane": best.pixels_per_lane, "chunk_atoms": best.chunk_atoms},
+            "batched_vs_plain_rel_l2": agree,
+            "graph": True,
+        }
+    finally:
+        L.spsim_hip_batch_destroy(plan)
+
+
+def cpu_baseline(h):
+    import oracle
+
+    threads = os.cpu_count() or 1
+    Z, pos, B = h["Z"], h["pos"], h["B"]
+    hkl = h["hkl"].cpu().numpy()
+    rng = np.random.default_rng(1)
+    n = 128
+    spent = 0.0
+    done_terms = 0.0
+    while True:
+        sub = rng.choice(hkl.shape[0], n, replace=False)
+        t0 = time.perf_counter()
+        oracle.sf_f64(hkl[sub], Z, pos, atom_b=B, threads=threads)
+        dt = time.perf_counter() - t0
+        spent += dt
+        done_terms += n * len(Z)
+        if spent >= 8.0 or n >= 65536:
+            break
+        n = int(min(65536, max(n * 2, n * 8.0 / max(dt, 1e-3) / 2)))
+    rate = done_terms / spent
+    return {"value": round(rate / 1e9, 4), "unit": "G pixel-atom terms/s", "cores": threads, "kind": "port",
+            "sample": f"{int(done_terms / len(Z))} random pixels of the 1024x1024 headline pattern x {len(Z)} atoms "
+                      f"with per-atom B, FP64 oracle (oracle/oracle.c, OpenMP over pixels), {spent:.1f} s"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is "
+                                   "described in DESIGN.md but not built)"}))
+        sys.exit(2)
+    import torch
+
+    from spsim_amd import _lib
+    from spsim_amd import device as dv
+    from spsim_amd import workloads as wl
+
+    _lib.load()
+    h = headline(args, torch, dv, wl, _lib)
+    out = {
+        "metric": "G pixel-atom terms/s + patterns/s, batched vs plain, at <=1e-5 rel err vs FP64",
+        "value": round(h["value"], 2),
+        "unit": "G pixel-atom terms/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(h["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "large single pattern: 100000-atom C/N/O/S (62:17:20:1) molecule uniform in a 12 nm-radius "
+                        "sphere, per-atom B U[10,40] A^2, 1024x1024 detector, 75 um pixels at 0.1 m, lambda 0.15 nm; "
+                        "FP64 phase, FP32 sincos and sums",
+            "natoms": h["natoms"], "npix": h["npix"], "terms_per_step": h["terms"], "tile": h["tile"],
+            "parallelism": "1 GPU",
+        },
+        "roofline": h["roofline"],
+        "rel_err_vs_fp64": h["rel_err_vs_fp64"],
+    }
+    if not args.profile and not args.no_batched:
+        out["batched"] = batched(args, torch, dv, wl, _lib)
+    if not args.profile and not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(h)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,205 @@
+/*
+ * spsim-mi355x C-ABI: device-resident entry points of the structure-factor hot
+ * path on AMD MI355X (gfx950).
+ *
+ * These have no counterpart in the reference's public headers; each one
+ * replaces the inner part of a reference function so that callers that keep
+ * data in HBM (PyTorch, the batched pipeline, the benchmark) can skip the
+ * host round trip the reference API implies:
+ *
+ *   spsim_hip_hkl_detector      <- get_HKL_list_for_detector   src/diffraction.c:237-297
+ *   spsim_hip_rotate_hkl        <- apply_orientation_to_HKL_list src/diffraction.c:300-323
+ *   spsim_hip_sf_pattern        <- compute_pattern_on_list      src/diffraction.c:770-853
+ *                                  (and cuda_compute_pattern_on_list2, src/diffraction.cu:235-339)
+ *   spsim_hip_batch_*           <- the per-shot loop of main/simulate_shot, src/spsim.c:392-396,
+ *                                  run as one persistent launch over many molecules/orientations
+ *   spsim_hip_detector_chain    <- calculate_thomson_correction .. calculate_noiseless_detector_output
+ *                                  src/diffraction.c:1109-1204, src/noise.c:32-51,
+ *                                  src/amplification.c:54-153 (fused)
+ *   spsim_hip_sf_adjoint        <- (no reference counterpart) gradients of a Poisson
+ *                                  negative log-likelihood w.r.t. atom positions and B factors
+ *
+ * Conventions: all pointers named d_* are device pointers (hipMalloc or
+ * torch.cuda memory) on the current HIP device; `stream` is a hipStream_t
+ * passed as void* (NULL = the default stream). Every function returns
+ * SPSIM_HIP_OK or a negative error code and never exits the process;
+ * spsim_hip_last_error() describes the last failure on the calling thread.
+ * Calls are stream-ordered and asynchronous unless stated otherwise.
+ */
+#ifndef SPSIM_HIP_H
+#define SPSIM_HIP_H 1
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPSIM_HIP_OK 0
+#define SPSIM_HIP_ERR_INVALID (-1)  /* bad argument (null pointer, negative size, bad shape) */
+#define SPSIM_HIP_ERR_HIP (-2)      /* a HIP runtime call failed */
+#define SPSIM_HIP_ERR_NOMEM (-3)    /* host or device allocation failed */
+#define SPSIM_HIP_ERR_NODEVICE (-4) /* no usable gfx950 device */
+
+/* Species index of a packed atom: 0..99 = Cromer-Mann row (atomic number Z),
+   100 = unit point scatterer (f = 1, no B factor), used for delta_atoms. */
+#define SPSIM_HIP_NSPECIES 101
+#define SPSIM_HIP_UNIT_SPECIES 100
+#define SPSIM_HIP_FF_STRIDE 9 /* a1..a4, b1..b4, c  (b in A^2) */
+
+/* How raw (Z, pos) atoms are turned into packed records. */
+#define SPSIM_HIP_PACK_PLAIN 0  /* compute_pattern_on_list: skip Z==0, delta -> unit species */
+#define SPSIM_HIP_PACK_VECTOR 1 /* vector_compute_pattern_on_list quirks (see diffraction.h) */
+
+/* Packed atom record in HBM (16 bytes): position in metres and the real
+   amplitude weight (sqrt of the beam illumination at the atom, 1 for a flat beam). */
+typedef struct {
+  float x, y, z, w;
+} spsim_hip_atom;
+
+/* A run of packed atoms of one species: atoms [start, start+count). */
+typedef struct {
+  int start, count, species, reserved;
+} spsim_hip_segment;
+
+/* Kernel tiling: pixels per lane, waves per workgroup, atoms staged per LDS chunk. */
+typedef struct {
+  int pixels_per_lane; /* 1, 2, 4 or 8 */
+  int waves;           /* 4 */
+  int chunk_atoms;     /* 64, 128 or 256 */
+  int atom_splits;     /* >= 1: atom ranges per pixel tile (single pattern); 0 = automatic */
+} spsim_hip_tile;
+
+/* Library / device */
+const char *spsim_hip_last_error(void);
+int spsim_hip_device_count(int *count);
+/* Number of compute units of the current device. */
+int spsim_hip_cu_count(int *cus);
+int spsim_hip_synchronize(void *stream);
+
+/* Default Cromer-Mann table [101][9] (row 100 = unit scatterer) into host memory.
+   The values equal the reference's include/atomsf.cdata rows; this function does
+   not apply the ./atomsf.lib override (diffraction.c does, like the reference). */
+int spsim_hip_default_form_factors(float *out_101x9);
+
+/* Host-side packing (no device work): sorts atoms by species, drops/relabels
+   per `mode`, and writes packed records, per-atom B (if b != NULL), the
+   permutation perm[packed] = original index and the species segments.
+   Capacities: out_atoms/out_b/out_perm >= natoms, out_segs >= SPSIM_HIP_NSPECIES.
+   amp may be NULL (all 1). */
+int spsim_hip_pack_atoms(const int *Z, const float *pos, const float *amp, const float *b, int natoms,
+                         int delta_atoms, int mode, spsim_hip_atom *out_atoms, float *out_b, int *out_perm,
+                         spsim_hip_segment *out_segs, int *out_natoms, int *out_nsegs);
+
+/* Scattering vectors of a flat (spherical = 0) or spherical-grid detector, exactly
+   as get_HKL_list_for_detector: FP64 geometry, FP32 [ny][nx][3] store (m^-1). */
+int spsim_hip_hkl_detector(int nx, int ny, float width, float height, float distance, float center_x,
+                           float center_y, int spherical, float wavelength, float *d_hkl, void *stream);
+
+/* d_out[i] = R * d_in[i] for a row-major 3x3 R, formed in FP64 and rounded to FP32. */
+int spsim_hip_rotate_hkl(const float *d_in, int npix, const double *rot9, float *d_out, void *stream);
+
+/* One pattern: F[i] = sum_j f_s(|q_i|) exp(-B_eff s^2) w_j e^{-B_j s^2} exp(-2 pi i q_i.r_j),
+   I[i] = |F[i]|^2, with s = |q| / 2 in A^-1, B_eff = b_global (0 for the unit species).
+   d_atom_b may be NULL (no per-atom Debye-Waller factor). d_F ([npix][2]) or d_I
+   may be NULL. The nsegs segments must tile the packed atoms [0, natoms) in order
+   (as spsim_hip_pack_atoms writes them). tile may be NULL (default tiling). Uses a library-owned device
+   workspace when atom_splits > 1 (grown on demand, kept until spsim_hip_release). */
+int spsim_hip_sf_pattern(const float *d_hkl, int npix, const spsim_hip_atom *d_atoms, const float *d_atom_b,
+                         const spsim_hip_segment *d_segs, int nsegs, int natoms, const float *d_ff,
+                         float b_global, float *d_F, float *d_I, const spsim_hip_tile *tile, void *stream);
+
+/* Free the library's cached device workspace and graphs on the current device. */
+int spsim_hip_release(void);
+
+/* ---- Batched persistent launch over many molecules and orientations ---- */
+
+typedef struct spsim_hip_batch spsim_hip_batch;
+
+/* Molecules are given as host arrays, concatenated: molecule m owns raw atoms
+   [mol_offsets[m], mol_offsets[m+1]). atom_b may be NULL. Pattern p shows
+   molecule pattern_mol[p] rotated by rot9[p] (row-major 3x3, applied to q).
+   hkl is the shared [npix][3] host q list. The plan copies everything to the
+   device, builds the cost-sorted (pattern, pixel-tile) work list and owns its
+   device memory until spsim_hip_batch_destroy. tile may be NULL (autotuned). */
+int spsim_hip_batch_create(const float *hkl, int npix, int nmol, const int *mol_offsets, const int *Z,
+                           const float *pos, const float *amp, const float *atom_b, int delta_atoms,
+                           int npatterns, const int *pattern_mol, const double *rot9, const float *ff,
+                           float b_global, const spsim_hip_tile *tile, spsim_hip_batch **out);
+/* Runs the persistent kernel: d_I [npatterns][npix] (required), d_F [npatterns][npix][2]
+   (may be NULL). use_graph != 0 replays a captured hipGraph (captured on first use
+   for this stream and these output pointers). */
+int spsim_hip_batch_run(spsim_hip_batch *b, float *d_F, float *d_I, int use_graph, void *stream);
+/* Plain path for comparison: one single-pattern launch per pattern with a host
+   synchronisation after each, no work list and no graph. */
+int spsim_hip_batch_run_plain(spsim_hip_batch *b, float *d_F, float *d_I, void *stream);
+/* Tile of the persistent kernel; setting it rebuilds the work list. */
+int spsim_hip_batch_set_tile(spsim_hip_batch *b, const spsim_hip_tile *tile);
+int spsim_hip_batch_get_tile(const spsim_hip_batch *b, spsim_hip_tile *tile, int *blocks);
+/* Autotuner: times every supported tile (pixels per lane x LDS chunk) on a
+   slice of the plan's own work (patterns spread over its atom-count range),
+   keeps the fastest in pixel-atom terms/s and returns it (best_rate may be
+   NULL). Synchronises the stream. */
+int spsim_hip_batch_autotune(spsim_hip_batch *b, void *stream, spsim_hip_tile *best, double *best_rate);
+/* Total pixel-atom terms of one run (sum over patterns of natoms * npix). */
+int spsim_hip_batch_terms(const spsim_hip_batch *b, double *terms);
+void spsim_hip_batch_destroy(spsim_hip_batch *b);
+
+/* ---- Detector chain (fused epilogue) ---- */
+
+typedef struct {
+  int nx, ny;             /* raw pixels */
+  int bin_x, bin_y;       /* binning; binned image is (ny/bin_y) x (nx/bin_x) */
+  float width, height;    /* m */
+  float distance;         /* m */
+  float pixel_width, pixel_height;
+  float center_x, center_y;
+  int spherical;
+  int polarization;       /* POLARIZATION_* */
+  float beam_intensity;   /* photons / m^2 */
+  float quantum_efficiency;
+  float wavelength;       /* m */
+  float electron_hole_production_energy; /* J; 0 = one electron per photon (noisy branch) */
+  float dark_current;     /* e- / (pixel s) */
+  float exposure_time;    /* s */
+  float readout_noise;    /* e- rms per binned pixel */
+  float linear_full_well; /* e- */
+  float maximum_value;    /* ADC counts */
+} spsim_hip_detector;
+
+/* For each of npatterns intensity images d_I[p*npix .. ] (npix = nx*ny) runs
+   Thomson x solid angle x beam -> photons, Poisson(photons*QE) -> count,
+   electrons, binned readout with dark current + Gaussian readout noise clamped
+   at 0 and the ADC factor, and the noiseless binned output. Randomness comes
+   from Philox-4x32-10 keyed by seed, with counters (pattern_base + p, pixel),
+   so results do not depend on launch shape. Any output pointer may be NULL.
+   Raw outputs are [npatterns][npix], binned ones [npatterns][nbinned]. */
+int spsim_hip_detector_chain(const spsim_hip_detector *det, const float *d_I, int npatterns, uint64_t seed,
+                             uint64_t pattern_base, float *d_thomson, float *d_solid_angle, float *d_photons,
+                             float *d_count, float *d_electrons, float *d_real_output,
+                             float *d_noiseless_output, void *stream);
+
+/* ---- Adjoint ---- */
+
+/* Poisson NLL of expected counts lambda_i = K_i * I_i against counts y_i:
+   loss = sum_i (lambda_i - y_i log lambda_i); d_gI[i] = dloss/dI_i. d_K may be
+   NULL (K = 1). d_loss (1 double) may be NULL. */
+int spsim_hip_poisson_nll(const float *d_I, const float *d_y, const float *d_K, int npix, double *d_loss,
+                          float *d_gI, void *stream);
+
+/* Given the forward F of the same inputs (d_F [npix][2]) and g_i = dL/dI_i (d_gI),
+   writes d_grad_pos [natoms][3] = dL/dr_j (per packed atom, in units of L per
+   metre) and d_grad_b [natoms] = dL/dB_j (units of L per A^2), recomputing the
+   phases per pixel chunk instead of storing atom x pixel terms. h_segs is the
+   HOST copy of the segments (as spsim_hip_pack_atoms returns them); d_grad_b
+   may be NULL. Uses the library workspace and synchronises the stream. */
+int spsim_hip_sf_adjoint(const float *d_hkl, int npix, const spsim_hip_atom *d_atoms, const float *d_atom_b,
+                         const spsim_hip_segment *h_segs, int nsegs, int natoms, const float *d_ff,
+                         float b_global, const float *d_F, const float *d_gI, float *d_grad_pos,
+                         float *d_grad_b, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPSIM_HIP_H */

@@ -1,0 +1,211 @@
+"""ORACLE — test infrastructure only.
+
+NumPy/ctypes front end of oracle/oracle.c, the CPU restatement of spsim's
+structure-factor hot path (see the header of oracle.c for what each function
+restates and how it is pinned). Only tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg use it, always as the checker, never as the thing
+measured or shipped.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+_LIB = None
+
+_f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+def lib() -> C.CDLL:
+    global _LIB
+    if _LIB is None:
+        path = HERE / "liboracle.so"
+        if not path.exists():
+            from . import build as _b
+
+            _b.build()
+        L = C.CDLL(str(path))
+        L.oracle_form_factor.restype = C.c_double
+        L.oracle_form_factor.argtypes = [C.c_double, C.c_int, C.c_double, _f32p]
+        L.oracle_hkl_detector.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float,
+                                          C.c_int, C.c_float, _f32p]
+        L.oracle_rotate_hkl.argtypes = [_f32p, C.c_int, _f64p, _f32p]
+        L.oracle_sf_f64.argtypes = [_f32p, C.c_int, _i32p, _f32p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int,
+                                    _f32p, C.c_float, C.c_void_p, _f64p, C.c_int]
+        L.oracle_sf_refsem.argtypes = [_f32p, C.c_int, _i32p, _f32p, C.c_void_p, C.c_int, C.c_int, _f32p, C.c_float,
+                                       C.c_void_p, _f32p, C.c_int]
+        L.oracle_sf_grad_f64.argtypes = [_f32p, C.c_int, _i32p, _f32p, C.c_void_p, C.c_void_p, C.c_int, _f32p,
+                                         C.c_float, _f64p, _f64p, C.c_void_p, C.c_int]
+        L.oracle_thomson.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int,
+                                     _f32p]
+        L.oracle_solid_angle.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float,
+                                         C.c_float, C.c_float, C.c_int, _f32p]
+        L.oracle_photons.argtypes = [_f32p, _f32p, _f32p, C.c_float, C.c_size_t, _f32p]
+        L.oracle_electrons_per_photon.restype = C.c_float
+        L.oracle_electrons_per_photon.argtypes = [C.c_float, C.c_float, C.c_int]
+        L.oracle_noiseless_output.argtypes = [_f32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float,
+                                              C.c_float, C.c_float, _f32p]
+        L.oracle_real_output.argtypes = [_f32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, _f32p,
+                                         C.c_float, C.c_float, _f32p]
+        L.oracle_philox4x32_10.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+        L.oracle_std_normal.restype = C.c_double
+        L.oracle_std_normal.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32]
+        L.oracle_poisson.restype = C.c_double
+        L.oracle_poisson.argtypes = [C.c_double, C.c_uint64, C.c_uint32, C.c_uint32]
+        _LIB = L
+    return _LIB
+
+
+def _opt(a, dtype):
+    if a is None:
+        return None, None
+    arr = np.ascontiguousarray(a, dtype=dtype)
+    return arr, arr.ctypes.data_as(C.c_void_p)
+
+
+# --------------------------------------------------------------- form factors
+
+def default_form_factors() -> np.ndarray:
+    """[101][9] table: rows 0..99 from the Cromer-Mann table the product ships
+    (spsim_amd/csrc/host/cromer_mann.c, parsed as text), row 100 the unit scatterer."""
+    import re
+
+    text = (HERE.parent / "spsim_amd" / "csrc" / "host" / "cromer_mann.c").read_text()
+    body = text[text.index("= {") + 3:]
+    rows = re.findall(r"\{([^{}]*)\}", body)
+    vals = np.array([[float(v.strip().rstrip("f")) for v in r.split(",")] for r in rows], dtype=np.float32)
+    assert vals.shape == (100, 9), vals.shape
+    unit = np.zeros((1, 9), np.float32)
+    unit[0, 8] = 1.0
+    return np.ascontiguousarray(np.vstack([vals, unit]))
+
+
+def form_factor(d: float, Z: int, B: float = 0.0, ff: np.ndarray | None = None) -> float:
+    ff = default_form_factors() if ff is None else np.ascontiguousarray(ff, np.float32)
+    return lib().oracle_form_factor(float(d), int(Z), float(B), ff.ravel())
+
+
+# --------------------------------------------------------------- geometry
+
+def hkl_detector(nx, ny, width, height, distance, center_x, center_y, spherical, wavelength) -> np.ndarray:
+    out = np.zeros(nx * ny * 3, np.float32)
+    lib().oracle_hkl_detector(nx, ny, width, height, distance, center_x, center_y, spherical, wavelength, out)
+    return out.reshape(ny * nx, 3)
+
+
+def rotate_hkl(hkl: np.ndarray, R: np.ndarray) -> np.ndarray:
+    hkl = np.ascontiguousarray(hkl, np.float32).reshape(-1, 3)
+    out = np.empty_like(hkl)
+    lib().oracle_rotate_hkl(hkl.ravel(), hkl.shape[0], np.ascontiguousarray(R, np.float64).ravel(), out.ravel())
+    return out
+
+
+# --------------------------------------------------------------- structure factor
+
+def sf_f64(hkl, Z, pos, amp=None, atom_b=None, delta=False, vector_mode=False, B=0.0, ff=None, threads=0):
+    """FP64 oracle. Returns (F complex128 [npix], I float64 [npix])."""
+    hkl = np.ascontiguousarray(hkl, np.float32).reshape(-1, 3)
+    Z = np.ascontiguousarray(Z, np.int32)
+    pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+    ff = default_form_factors() if ff is None else np.ascontiguousarray(ff, np.float32)
+    amp_a, amp_p = _opt(amp, np.float32)
+    b_a, b_p = _opt(atom_b, np.float32)
+    n = hkl.shape[0]
+    F = np.zeros(2 * n, np.float64)
+    I = np.zeros(n, np.float64)
+    lib().oracle_sf_f64(hkl.ravel(), n, Z, pos.ravel(), amp_p, b_p, Z.shape[0], int(delta), int(vector_mode),
+                        ff.ravel(), float(B), F.ctypes.data_as(C.c_void_p), I, int(threads))
+    return F[0::2] + 1j * F[1::2], I
+
+
+def sf_refsem(hkl, Z, pos, illum=None, delta=False, B=0.0, ff=None, threads=0):
+    """compute_pattern_on_list in the reference's own precision. Returns (F complex64, I float32)."""
+    hkl = np.ascontiguousarray(hkl, np.float32).reshape(-1, 3)
+    Z = np.ascontiguousarray(Z, np.int32)
+    pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+    ff = default_form_factors() if ff is None else np.ascontiguousarray(ff, np.float32)
+    il_a, il_p = _opt(illum, np.float32)
+    n = hkl.shape[0]
+    F = np.zeros(2 * n, np.float32)
+    I = np.zeros(n, np.float32)
+    lib().oracle_sf_refsem(hkl.ravel(), n, Z, pos.ravel(), il_p, Z.shape[0], int(delta), ff.ravel(), float(B),
+                           F.ctypes.data_as(C.c_void_p), I, int(threads))
+    return (F[0::2] + 1j * F[1::2]).astype(np.complex64), I
+
+
+def sf_grad_f64(hkl, Z, pos, g, amp=None, atom_b=None, B=0.0, ff=None, threads=0):
+    """Gradient of sum_p g_p I_p w.r.t. positions [N,3] and B factors [N] (FP64)."""
+    hkl = np.ascontiguousarray(hkl, np.float32).reshape(-1, 3)
+    Z = np.ascontiguousarray(Z, np.int32)
+    pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+    ff = default_form_factors() if ff is None else np.ascontiguousarray(ff, np.float32)
+    amp_a, amp_p = _opt(amp, np.float32)
+    b_a, b_p = _opt(atom_b, np.float32)
+    g = np.ascontiguousarray(g, np.float64)
+    N = Z.shape[0]
+    gp = np.zeros(3 * N, np.float64)
+    gb = np.zeros(N, np.float64)
+    lib().oracle_sf_grad_f64(hkl.ravel(), hkl.shape[0], Z, pos.ravel(), amp_p, b_p, N, ff.ravel(), float(B), g, gp,
+                             gb.ctypes.data_as(C.c_void_p), int(threads))
+    return gp.reshape(N, 3), gb
+
+
+# --------------------------------------------------------------- detector chain
+
+def thomson(nx, ny, width, height, distance, cx=0.0, cy=0.0, polarization=0):
+    out = np.zeros(nx * ny, np.float32)
+    lib().oracle_thomson(nx, ny, width, height, distance, cx, cy, polarization, out)
+    return out
+
+
+def solid_angle(nx, ny, width, height, distance, pixel_width, pixel_height, cx=0.0, cy=0.0, spherical=0):
+    out = np.zeros(nx * ny, np.float32)
+    lib().oracle_solid_angle(nx, ny, width, height, distance, pixel_width, pixel_height, cx, cy, spherical, out)
+    return out
+
+
+def photons(th, om, I, beam):
+    th, om, I = (np.ascontiguousarray(a, np.float32) for a in (th, om, I))
+    out = np.zeros_like(I)
+    lib().oracle_photons(th, om, I, float(beam), I.size, out)
+    return out
+
+
+def electrons_per_photon(wavelength, e_h, noiseless=False):
+    return lib().oracle_electrons_per_photon(wavelength, e_h, int(noiseless))
+
+
+def noiseless_output(ph, nx, ny, bx, by, qe, epp, full_well, max_value):
+    ph = np.ascontiguousarray(ph, np.float32)
+    out = np.zeros((nx // bx) * (ny // by), np.float32)
+    lib().oracle_noiseless_output(ph, nx, ny, bx, by, qe, epp, full_well, max_value, out)
+    return out
+
+
+def real_output(el, nx, ny, bx, by, dark, exposure, noise, full_well, max_value):
+    el = np.ascontiguousarray(el, np.float32)
+    noise = np.ascontiguousarray(noise, np.float32)
+    out = np.zeros((nx // bx) * (ny // by), np.float32)
+    lib().oracle_real_output(el, nx, ny, bx, by, dark, exposure, noise, full_well, max_value, out)
+    return out
+
+
+def philox4x32_10(ctr, key):
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    lib().oracle_philox4x32_10(c, k, o)
+    return list(o)
+
+
+def std_normal(seed, idx, pattern, purpose):
+    return lib().oracle_std_normal(seed, idx, pattern, purpose)
+
+
+def poisson(mean, seed, idx, pattern):
+    return lib().oracle_poisson(float(mean), int(seed), int(idx), int(pattern))

@@ -1,0 +1,95 @@
+"""ctypes bindings of libspsim_amd.so (the C-ABI in include/spsim_hip.h and the
+drop-in API in include/*.h).
+
+The library is loaded from the package directory; there is no fallback: if it
+is missing or no gfx950 device is present, calls fail with RuntimeError.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libspsim_amd.so"
+
+SPSIM_HIP_OK = 0
+NSPECIES = 101
+UNIT_SPECIES = 100
+FF_STRIDE = 9
+PACK_PLAIN = 0
+PACK_VECTOR = 1
+
+
+class Atom(C.Structure):
+    _fields_ = [("x", C.c_float), ("y", C.c_float), ("z", C.c_float), ("w", C.c_float)]
+
+
+class Segment(C.Structure):
+    _fields_ = [("start", C.c_int), ("count", C.c_int), ("species", C.c_int), ("reserved", C.c_int)]
+
+
+class Tile(C.Structure):
+    _fields_ = [("pixels_per_lane", C.c_int), ("waves", C.c_int), ("chunk_atoms", C.c_int),
+                ("atom_splits", C.c_int)]
+
+
+class DetectorParams(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int), ("bin_x", C.c_int), ("bin_y", C.c_int),
+        ("width", C.c_float), ("height", C.c_float), ("distance", C.c_float),
+        ("pixel_width", C.c_float), ("pixel_height", C.c_float),
+        ("center_x", C.c_float), ("center_y", C.c_float),
+        ("spherical", C.c_int), ("polarization", C.c_int),
+        ("beam_intensity", C.c_float), ("quantum_efficiency", C.c_float), ("wavelength", C.c_float),
+        ("electron_hole_production_energy", C.c_float), ("dark_current", C.c_float),
+        ("exposure_time", C.c_float), ("readout_noise", C.c_float), ("linear_full_well", C.c_float),
+        ("maximum_value", C.c_float),
+    ]
+
+
+_vp = C.c_void_p
+_i = C.c_int
+_f = C.c_float
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "spsim_hip_last_error": (C.c_char_p, []),
+    "spsim_hip_device_count": (_i, [C.POINTER(_i)]),
+    "spsim_hip_cu_count": (_i, [C.POINTER(_i)]),
+    "spsim_hip_synchronize": (_i, [_vp]),
+    "spsim_hip_default_form_factors": (_i, [_vp]),
+    "spsim_hip_pack_atoms": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _vp, _vp, _vp, _vp, C.POINTER(_i), C.POINTER(_i)]),
+    "spsim_hip_hkl_detector": (_i, [_i, _i, _f, _f, _f, _f, _f, _i, _f, _vp, _vp]),
+    "spsim_hip_rotate_hkl": (_i, [_vp, _i, _vp, _vp, _vp]),
+    "spsim_hip_sf_pattern": (_i, [_vp, _i, _vp, _vp, _vp, _i, _i, _vp, _f, _vp, _vp, C.POINTER(Tile), _vp]),
+    "spsim_hip_release": (_i, []),
+    "spsim_hip_detector_chain": (_i, [C.POINTER(DetectorParams), _vp, _i, C.c_uint64, C.c_uint64,
+                                      _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+}
+
+_LIB: C.CDLL | None = None
+
+
+def load() -> C.CDLL:
+    """Loads libspsim_amd.so and declares the C-ABI signatures."""
+    global _LIB
+    if _LIB is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is not built; run `python -m spsim_amd.build`")
+        L = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name, None)
+            if fn is None:
+                continue
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def last_error() -> str:
+    return (load().spsim_hip_last_error() or b"").decode()
+
+
+def check(rc: int, what: str) -> None:
+    if rc != SPSIM_HIP_OK:
+        raise RuntimeError(f"{what} failed ({rc}): {last_error()}")

@@ -1,0 +1,375 @@
+// Batched structure factors: many molecules and orientations in one persistent
+// launch over a cost-sorted (pattern, pixel-tile) work list, optionally
+// replayed from a captured hipGraph, plus the plain one-launch-per-pattern path
+// and the tile autotuner.
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "common.hpp"
+#include "sf_kernels.hpp"
+
+using spsim::MolInfo;
+
+struct spsim_hip_batch {
+  int npix = 0, npatterns = 0, nmol = 0;
+  spsim_hip_tile tile{};
+  int blocks = 0;
+  int cus = 0;
+  bool has_b = false;
+  float b_global = 0.0f;
+  int nitems = 0;
+  double terms = 0.0;
+
+  // device memory
+  float *hkl = nullptr;
+  spsim_hip_atom *atoms = nullptr;
+  float *atom_b = nullptr;
+  spsim_hip_segment *segs = nullptr;  // per molecule, starts relative to the molecule's first atom
+  MolInfo *mols = nullptr;
+  int *pattern_mol = nullptr;
+  double *rot = nullptr;
+  int2 *items = nullptr;
+  unsigned *queue = nullptr;
+  float *ff = nullptr;
+  float *hkl_rot = nullptr;  // plain path scratch [npix][3]
+
+  // host copies
+  std::vector<MolInfo> h_mols;
+  std::vector<int> h_pattern_mol;
+  std::vector<double> h_rot;
+
+  // captured graph
+  hipGraphExec_t exec = nullptr;
+  hipStream_t g_stream = nullptr;
+  float *g_F = nullptr, *g_I = nullptr;
+};
+
+namespace spsim {
+namespace {
+
+template <class T>
+int dmalloc(T **p, size_t n) {
+  *p = nullptr;
+  hipError_t e = hipMalloc((void **)p, std::max<size_t>(n, 1) * sizeof(T));
+  if (e != hipSuccess) return fail(SPSIM_HIP_ERR_NOMEM, "hipMalloc(%zu) failed: %s", n * sizeof(T), hipGetErrorString(e));
+  return SPSIM_HIP_OK;
+}
+
+template <class T>
+int dupload(T **p, const T *h, size_t n) {
+  SPSIM_TRY(dmalloc(p, n));
+  if (n) SPSIM_HIP_TRY(hipMemcpy(*p, h, n * sizeof(T), hipMemcpyHostToDevice));
+  return SPSIM_HIP_OK;
+}
+
+void free_batch(spsim_hip_batch *b) {
+  if (!b) return;
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  void *ptrs[] = {b->hkl, b->atoms, b->atom_b, b->segs, b->mols, b->pattern_mol, b->rot, b->items, b->queue, b->ff,
+                  b->hkl_rot};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  delete b;
+}
+
+// Work items (pattern, tile) for tile size `tp`, heaviest pattern first so the
+// persistent workgroups finish together (longest-processing-time order).
+std::vector<int2> build_items(const spsim_hip_batch &b, int tp, const std::vector<int> *only_patterns = nullptr,
+                              int max_tiles = -1) {
+  std::vector<int> pats;
+  if (only_patterns) {
+    pats = *only_patterns;
+  } else {
+    pats.resize(b.npatterns);
+    std::iota(pats.begin(), pats.end(), 0);
+  }
+  auto natoms = [&](int p) { const MolInfo &m = b.h_mols[b.h_pattern_mol[p]]; return m.atom_end - m.atom_begin; };
+  std::stable_sort(pats.begin(), pats.end(), [&](int a, int c) { return natoms(a) > natoms(c); });
+  int tiles = (b.npix + tp - 1) / tp;
+  if (max_tiles > 0) tiles = std::min(tiles, max_tiles);
+  std::vector<int2> items;
+  items.reserve((size_t)pats.size() * tiles);
+  for (int p : pats)
+    for (int t = 0; t < tiles; ++t) items.push_back(make_int2(p, t));
+  return items;
+}
+
+BatchLaunch launch_args(const spsim_hip_batch *b, const int2 *items, int nitems, unsigned *queue, float *F, float *I) {
+  BatchLaunch a;
+  a.hkl = b->hkl;
+  a.npix = b->npix;
+  a.atoms = b->atoms;
+  a.atom_b = b->has_b ? b->atom_b : nullptr;
+  a.segs = b->segs;
+  a.mols = b->mols;
+  a.pattern_mol = b->pattern_mol;
+  a.rot = b->rot;
+  a.items = items;
+  a.nitems = nitems;
+  a.queue = queue;
+  a.ff = b->ff;
+  a.b_global = b->b_global;
+  a.F = (float2 *)F;
+  a.I = I;
+  return a;
+}
+
+int set_tile(spsim_hip_batch *b, const spsim_hip_tile &t) {
+  if (!tile_supported(t)) return fail(SPSIM_HIP_ERR_INVALID, "unsupported tile");
+  int per_cu = 1;
+  SPSIM_TRY(batch_blocks_per_cu(t, b->has_b, &per_cu));
+  std::vector<int2> items = build_items(*b, tile_pixels(t));
+  if (b->items) SPSIM_HIP_TRY(hipFree(b->items));
+  b->items = nullptr;
+  SPSIM_TRY(dupload(&b->items, items.data(), items.size()));
+  b->nitems = (int)items.size();
+  b->tile = t;
+  b->blocks = b->cus * per_cu;
+  if (b->exec) {
+    (void)hipGraphExecDestroy(b->exec);
+    b->exec = nullptr;
+  }
+  return SPSIM_HIP_OK;
+}
+
+}  // namespace
+}  // namespace spsim
+
+extern "C" {
+
+int spsim_hip_batch_create(const float *hkl, int npix, int nmol, const int *mol_offsets, const int *Z,
+                           const float *pos, const float *amp, const float *atom_b, int delta_atoms, int npatterns,
+                           const int *pattern_mol, const double *rot9, const float *ff, float b_global,
+                           const spsim_hip_tile *tile, spsim_hip_batch **out) {
+  using namespace spsim;
+  if (!out) return fail(SPSIM_HIP_ERR_INVALID, "out is NULL");
+  *out = nullptr;
+  if (npix <= 0 || nmol <= 0 || npatterns <= 0) return fail(SPSIM_HIP_ERR_INVALID, "npix, nmol and npatterns must be > 0");
+  if (npix > (1 << 28)) return fail(SPSIM_HIP_ERR_INVALID, "npix too large");
+  if (!hkl || !mol_offsets || !pattern_mol || !rot9) return fail(SPSIM_HIP_ERR_INVALID, "NULL input array");
+  if (mol_offsets[0] != 0) return fail(SPSIM_HIP_ERR_INVALID, "mol_offsets[0] must be 0");
+  for (int m = 0; m < nmol; ++m)
+    if (mol_offsets[m + 1] < mol_offsets[m]) return fail(SPSIM_HIP_ERR_INVALID, "mol_offsets must be non-decreasing");
+  const int total = mol_offsets[nmol];
+  if (total > 0 && (!Z || !pos)) return fail(SPSIM_HIP_ERR_INVALID, "Z/pos is NULL");
+  for (int p = 0; p < npatterns; ++p)
+    if (pattern_mol[p] < 0 || pattern_mol[p] >= nmol) return fail(SPSIM_HIP_ERR_INVALID, "pattern_mol[%d] out of range", p);
+  int cus = 0;
+  SPSIM_TRY(ensure_device(&cus));
+
+  spsim_hip_batch *b = new spsim_hip_batch();
+  b->npix = npix;
+  b->npatterns = npatterns;
+  b->nmol = nmol;
+  b->cus = cus;
+  b->has_b = atom_b != nullptr;
+  b->b_global = b_global;
+
+  // Pack every molecule: species-sorted atoms with segments relative to the molecule.
+  std::vector<spsim_hip_atom> h_atoms(std::max(total, 1));
+  std::vector<float> h_b(std::max(total, 1), 0.0f);
+  std::vector<spsim_hip_segment> h_segs;
+  std::vector<spsim_hip_segment> tmp(SPSIM_HIP_NSPECIES);
+  int packed_total = 0;
+  b->h_mols.resize(nmol);
+  for (int m = 0; m < nmol; ++m) {
+    const int o = mol_offsets[m], n = mol_offsets[m + 1] - o;
+    int nat = 0, nseg = 0;
+    int rc = spsim_hip_pack_atoms(Z + o, pos + 3 * (size_t)o, amp ? amp + o : nullptr, atom_b ? atom_b + o : nullptr, n,
+                                  delta_atoms, SPSIM_HIP_PACK_PLAIN, h_atoms.data() + packed_total,
+                                  h_b.data() + packed_total, nullptr, tmp.data(), &nat, &nseg);
+    if (rc != SPSIM_HIP_OK) {
+      free_batch(b);
+      return fail(rc, "molecule %d: atomic numbers must lie in [0, 99]", m);
+    }
+    b->h_mols[m] = MolInfo{packed_total, packed_total + nat, (int)h_segs.size(), nseg};
+    h_segs.insert(h_segs.end(), tmp.begin(), tmp.begin() + nseg);
+    packed_total += nat;
+  }
+  b->h_pattern_mol.assign(pattern_mol, pattern_mol + npatterns);
+  b->h_rot.assign(rot9, rot9 + 9 * (size_t)npatterns);
+  for (int p = 0; p < npatterns; ++p) {
+    const MolInfo &m = b->h_mols[pattern_mol[p]];
+    b->terms += (double)(m.atom_end - m.atom_begin) * npix;
+  }
+
+  float ffh[SPSIM_HIP_NSPECIES * SPSIM_HIP_FF_STRIDE];
+  if (ff)
+    std::memcpy(ffh, ff, sizeof ffh);
+  else
+    spsim_hip_default_form_factors(ffh);
+
+  int rc = SPSIM_HIP_OK;
+  if ((rc = dupload(&b->hkl, hkl, 3 * (size_t)npix)) ||
+      (rc = dupload(&b->atoms, h_atoms.data(), (size_t)std::max(packed_total, 1))) ||
+      (b->has_b && (rc = dupload(&b->atom_b, h_b.data(), (size_t)std::max(packed_total, 1)))) ||
+      (rc = dupload(&b->segs, h_segs.data(), h_segs.size())) || (rc = dupload(&b->mols, b->h_mols.data(), (size_t)nmol)) ||
+      (rc = dupload(&b->pattern_mol, pattern_mol, (size_t)npatterns)) ||
+      (rc = dupload(&b->rot, rot9, 9 * (size_t)npatterns)) || (rc = dmalloc(&b->queue, 1)) ||
+      (rc = dupload(&b->ff, ffh, (size_t)SPSIM_HIP_NSPECIES * SPSIM_HIP_FF_STRIDE)) ||
+      (rc = dmalloc(&b->hkl_rot, 3 * (size_t)npix))) {
+    free_batch(b);
+    return rc;
+  }
+  const int avg_atoms = packed_total / std::max(nmol, 1);
+  spsim_hip_tile t = tile ? *tile : default_tile(avg_atoms, npix, b->has_b);
+  if ((rc = set_tile(b, t)) != SPSIM_HIP_OK) {
+    free_batch(b);
+    return rc;
+  }
+  *out = b;
+  return SPSIM_HIP_OK;
+}
+
+int spsim_hip_batch_set_tile(spsim_hip_batch *b, const spsim_hip_tile *tile) {
+  using namespace spsim;
+  if (!b || !tile) return fail(SPSIM_HIP_ERR_INVALID, "NULL argument");
+  return set_tile(b, *tile);
+}
+
+int spsim_hip_batch_get_tile(const spsim_hip_batch *b, spsim_hip_tile *tile, int *blocks) {
+  using namespace spsim;
+  if (!b || !tile) return fail(SPSIM_HIP_ERR_INVALID, "NULL argument");
+  *tile = b->tile;
+  if (blocks) *blocks = b->blocks;
+  return SPSIM_HIP_OK;
+}
+
+int spsim_hip_batch_run(spsim_hip_batch *b, float *d_F, float *d_I, int use_graph, void *stream) {
+  using namespace spsim;
+  if (!b || !d_I) return fail(SPSIM_HIP_ERR_INVALID, "NULL argument");
+  hipStream_t s = (hipStream_t)stream;
+  BatchLaunch a = launch_args(b, b->items, b->nitems, b->queue, d_F, d_I);
+  if (!use_graph) return launch_batch(a, b->tile, b->blocks, s);
+  if (!s) return fail(SPSIM_HIP_ERR_INVALID, "graph replay needs a non-default stream");
+  if (!b->exec || b->g_stream != s || b->g_F != d_F || b->g_I != d_I) {
+    if (b->exec) {
+      SPSIM_HIP_TRY(hipGraphExecDestroy(b->exec));
+      b->exec = nullptr;
+    }
+    hipGraph_t g = nullptr;
+    SPSIM_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    int rc = launch_batch(a, b->tile, b->blocks, s);
+    hipError_t e = hipStreamEndCapture(s, &g);
+    if (rc != SPSIM_HIP_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
+    b->g_stream = s;
+    b->g_F = d_F;
+    b->g_I = d_I;
+  }
+  SPSIM_HIP_TRY(hipGraphLaunch(b->exec, s));
+  return SPSIM_HIP_OK;
+}
+
+int spsim_hip_batch_run_plain(spsim_hip_batch *b, float *d_F, float *d_I, void *stream) {
+  using namespace spsim;
+  if (!b || !d_I) return fail(SPSIM_HIP_ERR_INVALID, "NULL argument");
+  hipStream_t s = (hipStream_t)stream;
+  for (int p = 0; p < b->npatterns; ++p) {
+    const MolInfo &m = b->h_mols[b->h_pattern_mol[p]];
+    SPSIM_TRY(spsim_hip_rotate_hkl(b->hkl, b->npix, &b->h_rot[9 * (size_t)p], b->hkl_rot, s));
+    SPSIM_TRY(spsim_hip_sf_pattern(b->hkl_rot, b->npix, b->atoms + m.atom_begin, b->has_b ? b->atom_b + m.atom_begin : nullptr,
+                                   b->segs + m.seg_begin, m.nseg, m.atom_end - m.atom_begin, b->ff, b->b_global,
+                                   d_F ? d_F + 2 * (size_t)p * b->npix : nullptr, d_I + (size_t)p * b->npix, nullptr, s));
+    SPSIM_HIP_TRY(hipStreamSynchronize(s));
+  }
+  return SPSIM_HIP_OK;
+}
+
+int spsim_hip_batch_terms(const spsim_hip_batch *b, double *terms) {
+  using namespace spsim;
+  if (!b || !terms) return fail(SPSIM_HIP_ERR_INVALID, "NULL argument");
+  *terms = b->terms;
+  return SPSIM_HIP_OK;
+}
+
+// Times each supported tile on a representative slice of the work (patterns
+// spread over the cost range, the first tiles of the detector) and keeps the
+// fastest in pixel-atom terms per second. Leaves the plan on the best tile.
+int spsim_hip_batch_autotune(spsim_hip_batch *b, void *stream, spsim_hip_tile *best_out, double *best_rate) {
+  using namespace spsim;
+  if (!b) return fail(SPSIM_HIP_ERR_INVALID, "NULL plan");
+  hipStream_t s = (hipStream_t)stream;
+  // Sample patterns at cost quantiles.
+  std::vector<int> order(b->npatterns);
+  std::iota(order.begin(), order.end(), 0);
+  auto natoms = [&](int p) { const MolInfo &m = b->h_mols[b->h_pattern_mol[p]]; return m.atom_end - m.atom_begin; };
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return natoms(x) > natoms(y); });
+  std::vector<int> sample;
+  const int want = std::min(b->npatterns, 6);
+  for (int i = 0; i < want; ++i) sample.push_back(order[(size_t)i * b->npatterns / want]);
+  const int ks[4] = {1, 2, 4, 8};
+  const int cs[3] = {64, 128, 256};
+  float *dI = nullptr;
+  unsigned *q = nullptr;
+  SPSIM_TRY(dmalloc(&dI, (size_t)b->npatterns * b->npix));
+  if (dmalloc(&q, 1) != SPSIM_HIP_OK) {
+    (void)hipFree(dI);
+    return SPSIM_HIP_ERR_NOMEM;
+  }
+  hipEvent_t e0, e1;
+  SPSIM_HIP_TRY(hipEventCreate(&e0));
+  SPSIM_HIP_TRY(hipEventCreate(&e1));
+  spsim_hip_tile best = b->tile;
+  double best_r = -1.0;
+  int rc = SPSIM_HIP_OK;
+  for (int ki = 0; ki < 4 && rc == SPSIM_HIP_OK; ++ki) {
+    for (int ci = 0; ci < 3 && rc == SPSIM_HIP_OK; ++ci) {
+      spsim_hip_tile t{ks[ki], 4, cs[ci], 0};
+      const int tp = tile_pixels(t);
+      int per_cu = 1;
+      if ((rc = batch_blocks_per_cu(t, b->has_b, &per_cu)) != SPSIM_HIP_OK) break;
+      const int blocks = b->cus * per_cu;
+      // enough tiles per sampled pattern for ~4 items per workgroup
+      const int tiles = std::max(1, std::min((b->npix + tp - 1) / tp, (4 * blocks + want - 1) / std::max(want, 1)));
+      std::vector<int2> items = build_items(*b, tp, &sample, tiles);
+      double terms = 0.0;
+      for (const int2 &it : items) {
+        const int px = std::min(tp, b->npix - it.y * tp);
+        terms += (double)natoms(it.x) * px;
+      }
+      int2 *ditems = nullptr;
+      if ((rc = dupload(&ditems, items.data(), items.size())) != SPSIM_HIP_OK) break;
+      BatchLaunch a = launch_args(b, ditems, (int)items.size(), q, nullptr, dI);
+      float ms = 0.0f;
+      rc = launch_batch(a, t, blocks, s);  // warm-up
+      if (rc == SPSIM_HIP_OK) {
+        hipEventRecord(e0, s);
+        rc = launch_batch(a, t, blocks, s);
+        hipEventRecord(e1, s);
+        hipError_t e = hipEventSynchronize(e1);
+        if (e != hipSuccess) rc = hip_fail(e, "hipEventSynchronize");
+        else hipEventElapsedTime(&ms, e0, e1);
+      }
+      (void)hipFree(ditems);
+      if (rc == SPSIM_HIP_OK && ms > 0.0f) {
+        const double r = terms / (ms * 1e-3);
+        if (r > best_r) {
+          best_r = r;
+          best = t;
+        }
+      }
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(dI);
+  (void)hipFree(q);
+  if (rc != SPSIM_HIP_OK) return rc;
+  SPSIM_TRY(set_tile(b, best));
+  if (best_out) *best_out = best;
+  if (best_rate) *best_rate = best_r;
+  return SPSIM_HIP_OK;
+}
+
+void spsim_hip_batch_destroy(spsim_hip_batch *b) { spsim::free_batch(b); }
+
+}  // extern "C"

@@ -1,0 +1,39 @@
+// Internal helpers shared by the HIP translation units of libspsim_amd.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+
+#include "spsim_hip.h"
+
+namespace spsim {
+
+// Records a formatted message as the calling thread's last error and returns code.
+int fail(int code, const char *fmt, ...);
+// Records a HIP failure (with the runtime's message) and returns SPSIM_HIP_ERR_HIP.
+int hip_fail(hipError_t e, const char *what);
+
+// Checks that the current device is gfx950 and caches its CU count.
+int ensure_device(int *cus);
+
+// Per-device grow-only scratch allocation owned by the library.
+int workspace(size_t bytes, void **ptr);
+void workspace_release();
+
+constexpr double kLog2e = 1.4426950408889634073599246810019;
+
+}  // namespace spsim
+
+#define SPSIM_HIP_TRY(call)                                 \
+  do {                                                      \
+    hipError_t spsim_e_ = (call);                           \
+    if (spsim_e_ != hipSuccess) return spsim::hip_fail(spsim_e_, #call); \
+  } while (0)
+
+#define SPSIM_TRY(call)          \
+  do {                           \
+    int spsim_rc_ = (call);      \
+    if (spsim_rc_ != SPSIM_HIP_OK) return spsim_rc_; \
+  } while (0)

@@ -1,0 +1,161 @@
+"""Device-resident Python front end of the C-ABI (include/spsim_hip.h).
+
+PyTorch provides device memory and the stream; every computation is a call
+into libspsim_amd.so with raw device pointers. Inputs are NumPy arrays or
+CUDA (HIP) tensors; outputs are torch tensors on the current device.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import Atom, DetectorParams, Segment, Tile, check
+
+
+def _ptr(t: torch.Tensor | None) -> C.c_void_p | None:
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise ValueError("expected a device tensor")
+    if not t.is_contiguous():
+        raise ValueError("expected a contiguous tensor")
+    return C.c_void_p(t.data_ptr())
+
+
+def _stream(stream: torch.cuda.Stream | None = None) -> C.c_void_p:
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return C.c_void_p(s.cuda_stream)
+
+
+def default_form_factors() -> np.ndarray:
+    out = np.zeros((_lib.NSPECIES, _lib.FF_STRIDE), np.float32)
+    check(_lib.load().spsim_hip_default_form_factors(out.ctypes.data_as(C.c_void_p)), "spsim_hip_default_form_factors")
+    return out
+
+
+@dataclass
+class PackedAtoms:
+    """Species-sorted atoms in host memory (see spsim_hip_pack_atoms)."""
+
+    atoms: np.ndarray  # [n, 4] float32: x, y, z, amplitude
+    b: np.ndarray | None  # [n] float32 per-atom B (A^2) or None
+    perm: np.ndarray  # [n] int32: packed index -> original index
+    segs: np.ndarray  # [nsegs, 4] int32: start, count, species, 0
+
+    @property
+    def natoms(self) -> int:
+        return int(self.atoms.shape[0])
+
+
+def pack_atoms(Z, pos, amp=None, atom_b=None, delta_atoms=False, mode=_lib.PACK_PLAIN) -> PackedAtoms:
+    Z = np.ascontiguousarray(Z, np.int32)
+    pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+    n = Z.shape[0]
+    if pos.shape[0] != n:
+        raise ValueError("Z and pos have different atom counts")
+    amp_a = None if amp is None else np.ascontiguousarray(amp, np.float32)
+    b_a = None if atom_b is None else np.ascontiguousarray(atom_b, np.float32)
+    atoms = np.zeros((max(n, 1), 4), np.float32)
+    outb = np.zeros(max(n, 1), np.float32)
+    perm = np.zeros(max(n, 1), np.int32)
+    segs = np.zeros((_lib.NSPECIES, 4), np.int32)
+    nat, nseg = C.c_int(0), C.c_int(0)
+    vp = lambda a: None if a is None else a.ctypes.data_as(C.c_void_p)
+    check(_lib.load().spsim_hip_pack_atoms(vp(Z), vp(pos), vp(amp_a), vp(b_a), n, int(bool(delta_atoms)), int(mode),
+                                           vp(atoms), vp(outb), vp(perm), vp(segs), C.byref(nat), C.byref(nseg)),
+          "spsim_hip_pack_atoms")
+    k = nat.value
+    return PackedAtoms(atoms[:k].copy(), outb[:k].copy() if b_a is not None else None, perm[:k].copy(),
+                       segs[:nseg.value].copy())
+
+
+class DeviceAtoms:
+    """PackedAtoms uploaded to HBM."""
+
+    def __init__(self, packed: PackedAtoms, device=None):
+        dev = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+        self.packed = packed
+        self.natoms = packed.natoms
+        self.nsegs = int(packed.segs.shape[0])
+        self.atoms = torch.from_numpy(packed.atoms.reshape(-1, 4) if packed.natoms else np.zeros((1, 4), np.float32)).to(dev)
+        self.b = None if packed.b is None else torch.from_numpy(packed.b if packed.natoms else np.zeros(1, np.float32)).to(dev)
+        self.segs = torch.from_numpy(packed.segs if self.nsegs else np.zeros((1, 4), np.int32)).to(dev)
+        self.perm = torch.from_numpy(packed.perm.astype(np.int64)).to(dev)
+
+
+def form_factor_table(ff: np.ndarray | None = None, device=None) -> torch.Tensor:
+    ff = default_form_factors() if ff is None else np.ascontiguousarray(ff, np.float32).reshape(_lib.NSPECIES, 9)
+    dev = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+    return torch.from_numpy(ff).to(dev)
+
+
+def make_tile(pixels_per_lane=4, chunk_atoms=256, atom_splits=0, waves=4) -> Tile:
+    return Tile(pixels_per_lane, waves, chunk_atoms, atom_splits)
+
+
+def structure_factor(hkl: torch.Tensor, atoms: DeviceAtoms, ff: torch.Tensor, B: float = 0.0,
+                     F: torch.Tensor | None = None, I: torch.Tensor | None = None, tile: Tile | None = None,
+                     stream=None, want_F: bool = True):
+    """One pattern on the q list hkl ([npix, 3] float32 device tensor).
+    Returns (F [npix, 2] float32 or None, I [npix] float32)."""
+    npix = int(hkl.shape[0])
+    if want_F and F is None:
+        F = torch.empty((npix, 2), dtype=torch.float32, device=hkl.device)
+    if I is None:
+        I = torch.empty((npix,), dtype=torch.float32, device=hkl.device)
+    rc = _lib.load().spsim_hip_sf_pattern(_ptr(hkl), npix, _ptr(atoms.atoms), _ptr(atoms.b), _ptr(atoms.segs),
+                                          atoms.nsegs, atoms.natoms, _ptr(ff), float(B), _ptr(F if want_F else None),
+                                          _ptr(I), C.byref(tile) if tile is not None else None, _stream(stream))
+    check(rc, "spsim_hip_sf_pattern")
+    return (F if want_F else None), I
+
+
+def hkl_detector(nx, ny, width, height, distance, wavelength, center_x=0.0, center_y=0.0, spherical=0, device=None,
+                 stream=None) -> torch.Tensor:
+    dev = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+    out = torch.empty((nx * ny, 3), dtype=torch.float32, device=dev)
+    check(_lib.load().spsim_hip_hkl_detector(nx, ny, width, height, distance, center_x, center_y, int(spherical),
+                                             wavelength, _ptr(out), _stream(stream)), "spsim_hip_hkl_detector")
+    return out
+
+
+def rotate_hkl(hkl: torch.Tensor, R, out: torch.Tensor | None = None, stream=None) -> torch.Tensor:
+    R = np.ascontiguousarray(R, np.float64).reshape(9)
+    if out is None:
+        out = torch.empty_like(hkl)
+    check(_lib.load().spsim_hip_rotate_hkl(_ptr(hkl), int(hkl.shape[0]), R.ctypes.data_as(C.c_void_p), _ptr(out),
+                                           _stream(stream)), "spsim_hip_rotate_hkl")
+    return out
+
+
+def detector_params(**kw) -> DetectorParams:
+    p = DetectorParams()
+    p.bin_x = p.bin_y = 1
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def detector_chain(det: DetectorParams, I: torch.Tensor, npatterns: int = 1, seed: int = 0, pattern_base: int = 0,
+                   outputs=("real_output", "noiseless_output"), stream=None) -> dict:
+    """Runs the fused detector chain; returns the requested outputs as tensors."""
+    npix = det.nx * det.ny
+    nb = (det.nx // det.bin_x) * (det.ny // det.bin_y)
+    names = ["thomson", "solid_angle", "photons", "count", "electrons", "real_output", "noiseless_output"]
+    sizes = [npix] * 5 + [nb] * 2
+    res = {}
+    ptrs = []
+    for n, s in zip(names, sizes):
+        if n in outputs:
+            t = torch.empty((npatterns, s), dtype=torch.float32, device=I.device)
+            res[n] = t
+            ptrs.append(_ptr(t))
+        else:
+            ptrs.append(None)
+    check(_lib.load().spsim_hip_detector_chain(C.byref(det), _ptr(I), npatterns, seed, pattern_base, *ptrs,
+                                               _stream(stream)), "spsim_hip_detector_chain")
+    return res

@@ -1,0 +1,121 @@
+"""Pins the oracle's detector chain (src/diffraction.c:1109-1204,
+src/noise.c:32-51, src/amplification.c:54-153) with closed forms, the Philox
+known-answer vectors and distribution tests."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+
+R0 = 2.81794e-15
+HC = 1.98644521e-25
+
+
+def test_philox_known_answers():
+    # Random123 kat_vectors, philox4x32_10
+    assert oracle.philox4x32_10([0, 0, 0, 0], [0, 0]) == [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]
+    assert oracle.philox4x32_10([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2) == [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]
+    assert oracle.philox4x32_10([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344], [0xA4093822, 0x299F31D0]) == \
+        [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]
+
+
+def geometry(nx=9, ny=7, pw=1e-3, dist=0.05, cx=2e-4, cy=-3e-4):
+    w, h = np.float32(nx * pw).item(), np.float32(ny * pw).item()
+    x, y = np.meshgrid(np.arange(nx), np.arange(ny))
+    f = lambda v: float(np.float32(v))
+    px = ((x - (nx - 1) / 2) / nx) * f(w) - f(cx)
+    py = ((y - (ny - 1) / 2) / ny) * f(h) - f(cy)
+    return nx, ny, w, h, dist, cx, cy, px.ravel(), py.ravel()
+
+
+def test_thomson_polarisation_closed_forms():
+    nx, ny, w, h, dist, cx, cy, px, py = geometry()
+    d = float(np.float32(dist))
+    r2 = px ** 2 + py ** 2 + d * d
+    np.testing.assert_allclose(oracle.thomson(nx, ny, w, h, dist, cx, cy, 0), R0 ** 2, rtol=1e-7)
+    # sin(chi) = py / r  ->  cos^2 chi = 1 - py^2 / r^2
+    np.testing.assert_allclose(oracle.thomson(nx, ny, w, h, dist, cx, cy, 1), R0 ** 2 * (1 - py ** 2 / r2), rtol=1e-6)
+    np.testing.assert_allclose(oracle.thomson(nx, ny, w, h, dist, cx, cy, 2), R0 ** 2 * (1 - px ** 2 / r2), rtol=1e-6)
+    np.testing.assert_allclose(oracle.thomson(nx, ny, w, h, dist, cx, cy, 3), R0 ** 2 * (1 + d * d / r2) / 2,
+                               rtol=1e-6)
+
+
+def test_solid_angle_closed_forms():
+    nx, ny, w, h, dist, cx, cy, px, py = geometry()
+    pw = ph = 1e-3
+    A = float(np.float32(pw)) * float(np.float32(ph))
+    d = float(np.float32(dist))
+    r = np.sqrt(px ** 2 + py ** 2 + d * d)
+    # A/d^2 cos^3(atan(rho/d)) = A d / r^3
+    np.testing.assert_allclose(oracle.solid_angle(nx, ny, w, h, dist, pw, ph, cx, cy, 0), A * d / r ** 3, rtol=1e-6)
+    np.testing.assert_allclose(oracle.solid_angle(nx, ny, w, h, dist, pw, ph, cx, cy, 1), A / d ** 2, rtol=1e-6)
+
+
+def test_photons_and_electrons():
+    rng = np.random.default_rng(0)
+    th, om, I = (rng.uniform(0.5, 2, 50).astype(np.float32) for _ in range(3))
+    np.testing.assert_allclose(oracle.photons(th, om, I, 3.0), th.astype(np.float64) * om * I * 3.0, rtol=1e-6)
+    lam, eh = 1.3e-9, 5.8e-19
+    assert oracle.electrons_per_photon(lam, eh) == pytest.approx(HC / lam / eh, rel=1e-6)
+    assert oracle.electrons_per_photon(lam, 0.0) == 1.0
+    assert math.isinf(oracle.electrons_per_photon(lam, 0.0, noiseless=True))
+
+
+def test_binned_outputs_by_hand():
+    nx, ny, bx, by = 4, 4, 2, 2
+    ph = np.arange(16, dtype=np.float32)
+    qe, epp, fw, mx = 0.5, 2.0, 100.0, 50.0
+    out = oracle.noiseless_output(ph, nx, ny, bx, by, qe, epp, fw, mx)
+    # bins: {0,1,4,5}, {2,3,6,7}, {8,9,12,13}, {10,11,14,15}; each x qe*epp*adc = 0.5
+    np.testing.assert_allclose(out, np.array([10, 18, 42, 50]) * 0.5)
+    el = np.full(16, 3.0, np.float32)
+    noise = np.array([1.0, -100.0, 0.0, 2.5], np.float32)
+    out = oracle.real_output(el, nx, ny, bx, by, 0.25, 2.0, noise, fw, mx)
+    # each bin: 4 * (3 + 0.5) + noise, clamped at 0, times adc = 0.5
+    np.testing.assert_allclose(out, np.array([15.0, 0.0, 14.0, 16.5]) * 0.5)
+    # no ADC scaling when full well or maximum value is zero
+    out = oracle.real_output(el, nx, ny, bx, by, 0.0, 0.0, noise * 0, 0.0, mx)
+    np.testing.assert_allclose(out, 12.0)
+
+
+@pytest.mark.parametrize("mean", [0.0, 0.3, 4.0, 25.0, 300.0, 999.0, 5000.0, 2.5e5])
+def test_poisson_moments(mean):
+    n = 40000
+    x = np.array([oracle.poisson(mean, 7, i, 3) for i in range(n)])
+    assert np.all(x == np.floor(x)) and np.all(x >= 0)
+    if mean == 0:
+        assert np.all(x == 0)
+        return
+    if mean < 1000:
+        m, v = mean, mean
+    else:
+        # (int)(mean + sqrt(mean) N(0,1)) truncates: mean - 1/2, variance mean + 1/12
+        m, v = mean - 0.5, mean + 1.0 / 12
+    se_m = math.sqrt(v / n)
+    se_v = v * math.sqrt(2.0 / n) * (1 + 1.0 / max(mean, 1))
+    assert abs(x.mean() - m) < 5 * se_m + 1e-12
+    assert abs(x.var() - v) < 5 * se_v
+
+
+def test_poisson_small_mean_pmf():
+    mean, n = 1.7, 60000
+    x = np.array([oracle.poisson(mean, 11, i, 0) for i in range(n)]).astype(int)
+    for k in range(6):
+        p = math.exp(-mean) * mean ** k / math.factorial(k)
+        assert abs(np.mean(x == k) - p) < 5 * math.sqrt(p * (1 - p) / n)
+
+
+def test_poisson_streams_are_independent():
+    a = np.array([oracle.poisson(50.0, 1, i, 0) for i in range(5000)])
+    b = np.array([oracle.poisson(50.0, 1, i, 1) for i in range(5000)])
+    c = np.array([oracle.poisson(50.0, 2, i, 0) for i in range(5000)])
+    assert abs(np.corrcoef(a, b)[0, 1]) < 0.06 and abs(np.corrcoef(a, c)[0, 1]) < 0.06
+
+
+def test_std_normal_moments():
+    n = 40000
+    x = np.array([oracle.std_normal(5, i, 0, 2) for i in range(n)])
+    assert abs(x.mean()) < 5 / math.sqrt(n)
+    assert abs(x.var() - 1) < 5 * math.sqrt(2 / n)
+    assert abs(np.mean(x < -1.0) - 0.158655) < 0.01
