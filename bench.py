@@ -31,7 +31,7 @@ import numpy as np
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
-# Flop-equivalents per pixel-atom term (see DESIGN.md): 10 FP32 flops + sin + cos
+# Flop-equivalents per pixel-atom term: 10 FP32 flops + sin + cos
 # at 8 flop-equivalents each (transcendentals run at 32/clk/CU on gfx950 against
 # 256 FP32 flops/clk/CU), plus exp2 and a multiply for the per-atom Debye-Waller factor.
 W_EQ_PLAIN = 26.0
@@ -133,9 +133,6 @@ def headline(args, torch, dv, wl, _lib):
 def batched(args, torch, dv, wl, _lib):
     dev = torch.device("cuda", 0)
     det, mols, rots = wl.config3(args.batched_patterns)
-    hkl = wl  # placeholder to keep names short below
-    import oracle  # noqa: F401  (not used for timing)
-
     q = dv.hkl_detector(det.n, det.n, det.width, det.width, det.distance, det.wavelength, device=dev).cpu().numpy()
     offs = np.zeros(len(mols) + 1, np.int32)
     offs[1:] = np.cumsum([len(m[0]) for m in mols])
@@ -223,8 +220,8 @@ def cpu_baseline(h):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is "
-                                   "described in DESIGN.md but not built)"}))
+        print(json.dumps({"error": "this benchmark runs on one GPU; --gpus must be 1 (sharding across GPUs "
+                                   "is designed but not built)"}))
         sys.exit(2)
     import torch
 

@@ -64,6 +64,17 @@ SIGNATURES = {
     "spsim_hip_release": (_i, []),
     "spsim_hip_detector_chain": (_i, [C.POINTER(DetectorParams), _vp, _i, C.c_uint64, C.c_uint64,
                                       _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "spsim_hip_sf_adjoint": (_i, [_vp, _i, _vp, _vp, _vp, _i, _i, _vp, _f, _vp, _vp, _vp, _vp, _vp]),
+    "spsim_hip_poisson_nll": (_i, [_vp, _vp, _vp, _i, _vp, _vp, _vp]),
+    "spsim_hip_batch_create": (_i, [_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp, _vp, _vp, _f,
+                                    C.POINTER(Tile), C.POINTER(_vp)]),
+    "spsim_hip_batch_run": (_i, [_vp, _vp, _vp, _i, _vp]),
+    "spsim_hip_batch_run_plain": (_i, [_vp, _vp, _vp, _vp]),
+    "spsim_hip_batch_terms": (_i, [_vp, C.POINTER(C.c_double)]),
+    "spsim_hip_batch_set_tile": (_i, [_vp, C.POINTER(Tile)]),
+    "spsim_hip_batch_get_tile": (_i, [_vp, C.POINTER(Tile), C.POINTER(_i)]),
+    "spsim_hip_batch_autotune": (_i, [_vp, _vp, C.POINTER(Tile), C.POINTER(C.c_double)]),
+    "spsim_hip_batch_destroy": (None, [_vp]),
 }
 
 _LIB: C.CDLL | None = None

@@ -1,0 +1,115 @@
+"""Adjoint kernel (spsim_hip_sf_adjoint) and the torch.autograd front end:
+gradients of a Poisson NLL w.r.t. positions and per-atom B factors against the
+FP64 oracle gradient (itself pinned by finite differences) and against a dense
+FP64 PyTorch autograd reference."""
+import numpy as np
+import pytest
+
+import oracle
+from spsim_amd import device as dv
+from spsim_amd import torch_ops as to
+from spsim_amd import workloads as wl
+from tests.conftest import oracle_threads, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def small_problem(natoms=120, n=24, seed=0, with_b=True):
+    rng = np.random.default_rng(seed)
+    Z, pos, B = wl.random_molecule(natoms, 2.5e-9, rng, atom_b_range=(10.0, 40.0) if with_b else None)
+    det = wl.Detector(n, 1e-3, 0.1, 0.15e-9)
+    q = oracle.hkl_detector(det.n, det.n, det.width, det.width, det.distance, 0, 0, 0, det.wavelength)
+    return Z, pos, B, q
+
+
+def test_adjoint_matches_oracle_gradient(gpu):
+    import torch
+
+    Z, pos, B, q = small_problem()
+    Z[5] = 0  # skipped atom: zero gradient
+    rng = np.random.default_rng(1)
+    g = rng.normal(size=q.shape[0])
+    gp_ref, gb_ref = oracle.sf_grad_f64(q, Z, pos, g, atom_b=B, B=2.0, threads=oracle_threads())
+    layout = to.SpeciesLayout(Z, gpu)
+    hkl = torch.from_numpy(q).to(gpu)
+    ff = dv.form_factor_table(device=gpu)
+    P = torch.from_numpy(pos).to(gpu).requires_grad_(True)
+    Bt = torch.from_numpy(B).to(gpu).requires_grad_(True)
+    I = to.intensity(P, hkl, layout, ff, atom_b=Bt, b_global=2.0)
+    (I * torch.from_numpy(g.astype(np.float32)).to(gpu)).sum().backward()
+    gp = P.grad.cpu().numpy()
+    gb = Bt.grad.cpu().numpy()
+    assert rel_l2(gp, gp_ref) < 2e-5
+    assert rel_l2(gb, gb_ref) < 2e-5
+    assert np.all(gp[5] == 0) and gb[5] == 0
+
+
+def dense_torch_intensity(q, Z, pos, B, ff, torch):
+    """FP64 dense reference on the CPU: I_p = |sum_j f_Zj(q_p) e^{-B_j s_p^2} e^{-2 pi i q_p.r_j}|^2."""
+    qd = torch.from_numpy(q.astype(np.float64))
+    d = torch.linalg.norm(qd, dim=1)
+    s2 = (d * 1e-10) ** 2 / 4
+    f = torch.zeros((q.shape[0], len(Z)), dtype=torch.float64)
+    for z in np.unique(Z):
+        c = torch.from_numpy(ff[z].astype(np.float64))
+        fz = sum(c[i] * torch.exp(-c[4 + i] * s2) for i in range(4)) + c[8]
+        f[:, torch.from_numpy(np.nonzero(Z == z)[0])] = fz[:, None]
+    a = f * torch.exp(-B[None, :] * s2[:, None])
+    ph = 2 * np.pi * (qd @ pos.T)
+    re = (a * torch.cos(ph)).sum(1)
+    im = -(a * torch.sin(ph)).sum(1)
+    return re * re + im * im
+
+
+def test_config4_gradient_vs_dense_fp64_torch(gpu):
+    """Differentiation configuration at N = 500: Poisson NLL of a pattern
+    simulated at perturbed coordinates against counts drawn at the true ones."""
+    import torch
+
+    rng = np.random.default_rng(44)
+    Z, pos, _ = wl.random_molecule(500, 7e-9 * (500 / 20000) ** (1 / 3), rng)
+    B = np.full(500, 20.0, np.float32)
+    det = wl.Detector(48, 1.6e-3, 0.1, 0.2e-9)
+    q = oracle.hkl_detector(det.n, det.n, det.width, det.width, det.distance, 0, 0, 0, det.wavelength)
+    ff = oracle.default_form_factors()
+    _, I_true = oracle.sf_f64(q, Z, pos, atom_b=B)
+    K = (1e3 / I_true.mean()) * np.ones_like(I_true)
+    y = rng.poisson(K * I_true).astype(np.float32)
+    pert = (pos + rng.normal(0, 0.05e-9, pos.shape)).astype(np.float32)
+
+    # GPU: hand-written adjoint behind torch.autograd
+    layout = to.SpeciesLayout(Z, gpu)
+    hkl = torch.from_numpy(q).to(gpu)
+    fft = dv.form_factor_table(device=gpu)
+    P = torch.from_numpy(pert).to(gpu).requires_grad_(True)
+    Bt = torch.from_numpy(B).to(gpu).requires_grad_(True)
+    I = to.intensity(P, hkl, layout, fft, atom_b=Bt)
+    loss = to.poisson_nll(I, torch.from_numpy(y).to(gpu), torch.from_numpy(K.astype(np.float32)).to(gpu))
+    loss.backward()
+
+    # CPU: dense FP64 autograd
+    Pd = torch.from_numpy(pert.astype(np.float64)).requires_grad_(True)
+    Bd = torch.from_numpy(B.astype(np.float64)).requires_grad_(True)
+    Id = dense_torch_intensity(q, Z, Pd, Bd, ff, torch)
+    lam = torch.from_numpy(K) * Id
+    Ld = (lam - torch.from_numpy(y.astype(np.float64)) * torch.log(lam)).sum()
+    Ld.backward()
+    assert float(loss) == pytest.approx(float(Ld), rel=1e-5)
+    assert rel_l2(P.grad.cpu().numpy(), Pd.grad.numpy()) < 1e-4
+    assert rel_l2(Bt.grad.cpu().numpy(), Bd.grad.numpy()) < 1e-4
+
+
+def test_poisson_nll_kernel(gpu):
+    import torch
+
+    rng = np.random.default_rng(9)
+    I = rng.uniform(0.1, 5, 5000).astype(np.float32)
+    y = rng.poisson(3.0, 5000).astype(np.float32)
+    K = rng.uniform(0.5, 2, 5000).astype(np.float32)
+    It = torch.from_numpy(I).to(gpu).requires_grad_(True)
+    loss = to.poisson_nll(It, torch.from_numpy(y).to(gpu), torch.from_numpy(K).to(gpu))
+    loss.backward()
+    lam = K.astype(np.float64) * I
+    ref = (lam - y * np.log(lam)).sum()
+    assert float(loss) == pytest.approx(ref, rel=1e-12)
+    np.testing.assert_allclose(It.grad.cpu().numpy(), K * (1 - y / lam), rtol=1e-6)

@@ -1,0 +1,176 @@
+"""Batched persistent launch (spsim_hip_batch_*) against the oracle, against
+the plain per-pattern path, and under repeated hipGraph replays."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle
+from spsim_amd import _lib
+from spsim_amd import workloads as wl
+from tests.conftest import oracle_threads, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def vp(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def make_plan(mols, rots, pattern_mol, q, atom_b=None, tile=None, b_global=0.0):
+    offs = np.zeros(len(mols) + 1, np.int32)
+    offs[1:] = np.cumsum([len(m[0]) for m in mols])
+    Z = np.ascontiguousarray(np.concatenate([m[0] for m in mols]), np.int32)
+    P = np.ascontiguousarray(np.concatenate([m[1] for m in mols]), np.float32).reshape(-1, 3)
+    R = np.ascontiguousarray(np.asarray(rots).reshape(-1, 9), np.float64)
+    pm = np.ascontiguousarray(pattern_mol, np.int32)
+    plan = C.c_void_p()
+    B = None if atom_b is None else np.ascontiguousarray(atom_b, np.float32)
+    _lib.check(_lib.load().spsim_hip_batch_create(vp(q), q.shape[0], len(mols), vp(offs), vp(Z), vp(P), None,
+                                                  None if B is None else vp(B), 0, len(pm), vp(pm), vp(R), None,
+                                                  float(b_global), C.byref(tile) if tile else None, C.byref(plan)),
+               "batch_create")
+    return plan, offs, Z, P
+
+
+@pytest.fixture(scope="module")
+def setup():
+    rng = np.random.default_rng(21)
+    sizes = [1, 0, 333, 2000, 5000]  # includes a molecule without atoms
+    mols = [wl.random_molecule(n, 2e-9 + 1e-9 * (n / 5000.0) ** (1 / 3), rng)[:2] if n else
+            (np.zeros(0, np.int32), np.zeros((0, 3), np.float32)) for n in sizes]
+    npat = 9
+    pattern_mol = np.array([4, 3, 2, 1, 0, 4, 2, 3, 4], np.int32)
+    rots = wl.uniform_rotations(npat, rng)
+    det = wl.Detector(40, 5e-4, 0.1, 0.2e-9)  # 1600 pixels: ragged tiles
+    q = oracle.hkl_detector(det.n, det.n, det.width, det.width, det.distance, 0, 0, 0, det.wavelength)
+    return mols, rots, pattern_mol, q
+
+
+def reference(setup):
+    mols, rots, pattern_mol, q = setup
+    out = []
+    for p in range(len(pattern_mol)):
+        Z, pos = mols[pattern_mol[p]]
+        qr = oracle.rotate_hkl(q, rots[p])
+        out.append(oracle.sf_f64(qr, Z, pos, threads=oracle_threads())[1] if len(Z) else np.zeros(q.shape[0]))
+    return np.stack(out)
+
+
+def test_batched_paths_match_oracle_and_each_other(gpu, setup):
+    import torch
+
+    mols, rots, pattern_mol, q = setup
+    ref = reference(setup)
+    L = _lib.load()
+    plan, *_ = make_plan(mols, rots, pattern_mol, q)
+    try:
+        npat, npix = len(pattern_mol), q.shape[0]
+        stream = torch.cuda.Stream()
+        sp = C.c_void_p(stream.cuda_stream)
+        I = torch.zeros((npat, npix), dtype=torch.float32, device=gpu)
+        F = torch.zeros((npat, npix, 2), dtype=torch.float32, device=gpu)
+        _lib.check(L.spsim_hip_batch_run(plan, C.c_void_p(F.data_ptr()), C.c_void_p(I.data_ptr()), 0, sp), "run")
+        stream.synchronize()
+        Ib = I.cpu().numpy()
+        for p in range(npat):
+            if ref[p].max() == 0:
+                assert np.all(Ib[p] == 0)
+            else:
+                assert rel_l2(Ib[p], ref[p]) < 1e-5, p
+        Fb = F.cpu().numpy()
+        np.testing.assert_allclose(Fb[..., 0] ** 2 + Fb[..., 1] ** 2, Ib, rtol=1e-5, atol=1e-6 * Ib.max())
+        # plain path: one single-pattern launch per pattern
+        I2 = torch.zeros_like(I)
+        _lib.check(L.spsim_hip_batch_run_plain(plan, None, C.c_void_p(I2.data_ptr()), sp), "plain")
+        stream.synchronize()
+        assert rel_l2(I2.cpu().numpy(), Ib) < 1e-6
+        # graph replays, with eager work and synchronisation in between
+        for rep in range(4):
+            I.zero_()
+            torch.cuda.synchronize()
+            with torch.cuda.stream(stream):
+                _lib.check(L.spsim_hip_batch_run(plan, C.c_void_p(F.data_ptr()), C.c_void_p(I.data_ptr()), 1, sp),
+                           "graph")
+            stream.synchronize()
+            assert np.array_equal(I.cpu().numpy(), Ib), rep
+        terms = C.c_double()
+        L.spsim_hip_batch_terms(plan, C.byref(terms))
+        assert terms.value == sum(len(mols[m][0]) for m in pattern_mol) * npix
+    finally:
+        L.spsim_hip_batch_destroy(plan)
+
+
+def test_autotune_and_every_tile(gpu, setup):
+    import torch
+
+    mols, rots, pattern_mol, q = setup
+    ref = reference(setup)
+    L = _lib.load()
+    plan, *_ = make_plan(mols, rots, pattern_mol, q)
+    try:
+        stream = torch.cuda.Stream()
+        sp = C.c_void_p(stream.cuda_stream)
+        best = _lib.Tile()
+        rate = C.c_double()
+        _lib.check(L.spsim_hip_batch_autotune(plan, sp, C.byref(best), C.byref(rate)), "autotune")
+        assert best.pixels_per_lane in (1, 2, 4, 8) and best.chunk_atoms in (64, 128, 256) and rate.value > 0
+        I = torch.zeros((len(pattern_mol), q.shape[0]), dtype=torch.float32, device=gpu)
+        for k in (1, 2, 4, 8):
+            for c in (64, 128, 256):
+                t = _lib.Tile(k, 4, c, 0)
+                _lib.check(L.spsim_hip_batch_set_tile(plan, C.byref(t)), "set_tile")
+                _lib.check(L.spsim_hip_batch_run(plan, None, C.c_void_p(I.data_ptr()), 0, sp), "run")
+                stream.synchronize()
+                Ib = I.cpu().numpy()
+                for p in range(len(pattern_mol)):
+                    if ref[p].max() > 0:
+                        assert rel_l2(Ib[p], ref[p]) < 1e-5, (k, c, p)
+    finally:
+        L.spsim_hip_batch_destroy(plan)
+
+
+def test_batched_per_atom_b(gpu):
+    import torch
+
+    rng = np.random.default_rng(4)
+    mols, Bs = [], []
+    for n in (700, 1500):
+        Z, pos, B = wl.random_molecule(n, 3e-9, rng, atom_b_range=(10.0, 40.0))
+        mols.append((Z, pos))
+        Bs.append(B)
+    rots = wl.uniform_rotations(3, rng)
+    pm = np.array([1, 0, 1], np.int32)
+    det = wl.Detector(32, 1e-3, 0.1, 0.15e-9)
+    q = oracle.hkl_detector(det.n, det.n, det.width, det.width, det.distance, 0, 0, 0, det.wavelength)
+    L = _lib.load()
+    plan, *_ = make_plan(mols, rots, pm, q, atom_b=np.concatenate(Bs), b_global=4.0)
+    try:
+        I = torch.zeros((3, q.shape[0]), dtype=torch.float32, device=gpu)
+        _lib.check(L.spsim_hip_batch_run(plan, None, C.c_void_p(I.data_ptr()), 0, None), "run")
+        torch.cuda.synchronize()
+        for p in range(3):
+            Z, pos = mols[pm[p]]
+            _, ref = oracle.sf_f64(oracle.rotate_hkl(q, rots[p]), Z, pos, atom_b=Bs[pm[p]], B=4.0)
+            assert rel_l2(I.cpu().numpy()[p], ref) < 1e-5
+    finally:
+        L.spsim_hip_batch_destroy(plan)
+
+
+def test_invalid_arguments_are_rejected(gpu):
+    L = _lib.load()
+    plan = C.c_void_p()
+    q = np.zeros((4, 3), np.float32)
+    offs = np.array([0, 1], np.int32)
+    Z = np.array([6], np.int32)
+    P = np.zeros(3, np.float32)
+    R = np.eye(3).ravel()
+    bad_pm = np.array([3], np.int32)
+    rc = L.spsim_hip_batch_create(vp(q), 4, 1, vp(offs), vp(Z), vp(P), None, None, 0, 1, vp(bad_pm), vp(R), None,
+                                  0.0, None, C.byref(plan))
+    assert rc == -1 and "out of range" in _lib.last_error()
+    badZ = np.array([150], np.int32)
+    pm = np.array([0], np.int32)
+    rc = L.spsim_hip_batch_create(vp(q), 4, 1, vp(offs), vp(badZ), vp(P), None, None, 0, 1, vp(pm), vp(R), None,
+                                  0.0, None, C.byref(plan))
+    assert rc == -1

@@ -1,0 +1,139 @@
+"""Fused detector chain (spsim_hip_detector_chain) against the oracle: exact
+deterministic stages, identical random draws for the same Philox counters,
+and distribution tests of the noise."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from spsim_amd import device as dv
+
+pytestmark = pytest.mark.gpu
+
+
+def params(nx=64, ny=48, bx=4, by=4, pol=0, **kw):
+    pw = 2e-5
+    p = dict(nx=nx, ny=ny, bin_x=bx, bin_y=by, width=float(np.float32(nx * pw)), height=float(np.float32(ny * pw)),
+             distance=0.05, pixel_width=pw, pixel_height=pw, center_x=3e-5, center_y=-1e-5, spherical=0,
+             polarization=pol, beam_intensity=1e25, quantum_efficiency=0.15, wavelength=1.3e-9,
+             electron_hole_production_energy=5.8e-19, dark_current=0.1, exposure_time=1e-13, readout_noise=10.0,
+             linear_full_well=2e5, maximum_value=65535.0)
+    p.update(kw)
+    return p
+
+
+def run_chain(gpu, p, I, seed=3, base=0, npatterns=1):
+    import torch
+
+    det = dv.detector_params(**p)
+    It = torch.from_numpy(np.ascontiguousarray(I, np.float32)).to(gpu)
+    out = dv.detector_chain(det, It, npatterns, seed, base,
+                            outputs=("thomson", "solid_angle", "photons", "count", "electrons", "real_output",
+                                     "noiseless_output"))
+    torch.cuda.synchronize()
+    return {k: v.cpu().numpy().reshape(npatterns, -1) for k, v in out.items()}
+
+
+@pytest.mark.parametrize("pol", [0, 1, 2, 3])
+def test_geometry_stages(gpu, pol):
+    p = params(pol=pol)
+    I = np.ones(p["nx"] * p["ny"], np.float32)
+    o = run_chain(gpu, p, I)
+    th = oracle.thomson(p["nx"], p["ny"], p["width"], p["height"], p["distance"], p["center_x"], p["center_y"], pol)
+    om = oracle.solid_angle(p["nx"], p["ny"], p["width"], p["height"], p["distance"], p["pixel_width"],
+                            p["pixel_height"], p["center_x"], p["center_y"], 0)
+    # FP64 libm (asin, cos, atan) on host and device may differ in the last
+    # FP64 bit, which can move the FP32 result by one ulp.
+    np.testing.assert_array_max_ulp(o["thomson"][0], th, maxulp=1)
+    np.testing.assert_array_max_ulp(o["solid_angle"][0], om, maxulp=1)
+    assert np.mean(o["solid_angle"][0] == om) > 0.95
+
+
+def test_deterministic_stages_exact(gpu):
+    p = params()
+    rng = np.random.default_rng(0)
+    I = rng.gamma(0.5, 2e6, p["nx"] * p["ny"]).astype(np.float32)
+    o = run_chain(gpu, p, I)
+    ph = oracle.photons(o["thomson"][0], o["solid_angle"][0], I, p["beam_intensity"])
+    assert np.array_equal(o["photons"][0], ph)
+    epp = oracle.electrons_per_photon(p["wavelength"], p["electron_hole_production_energy"])
+    np.testing.assert_array_equal(o["electrons"][0], o["count"][0] * np.float32(epp))
+    epp_nl = oracle.electrons_per_photon(p["wavelength"], p["electron_hole_production_energy"], noiseless=True)
+    nl = oracle.noiseless_output(ph, p["nx"], p["ny"], 4, 4, p["quantum_efficiency"], epp_nl, p["linear_full_well"],
+                                 p["maximum_value"])
+    assert np.array_equal(o["noiseless_output"][0], nl)
+
+
+def test_random_draws_match_oracle_counters(gpu):
+    p = params(nx=96, ny=80, beam_intensity=1e32)
+    rng = np.random.default_rng(1)
+    # intensities spanning the three Poisson branches (mean < 10, < 1000, >= 1000):
+    # mean = QE r0^2 Omega beam I ~ 1.9e-5 I here
+    I = (10.0 ** rng.uniform(4.2, 8.6, p["nx"] * p["ny"])).astype(np.float32)
+    seed, base = 99, 7
+    o = run_chain(gpu, p, I, seed=seed, base=base)
+    lam = (o["photons"][0] * np.float32(p["quantum_efficiency"])).astype(np.float64)
+    assert lam.min() < 5 and lam.max() > 2000
+    ref = np.array([oracle.poisson(l, seed, i, base) for i, l in enumerate(lam)])
+    same = np.mean(o["count"][0] == ref)
+    assert same > 0.999, same
+    # readout noise per binned pixel from the same counters
+    nb = (p["nx"] // 4) * (p["ny"] // 4)
+    noise = np.array([p["readout_noise"] * oracle.std_normal(seed, b, base, 2) for b in range(nb)], np.float32)
+    ro = oracle.real_output(o["electrons"][0], p["nx"], p["ny"], 4, 4, p["dark_current"], p["exposure_time"], noise,
+                            p["linear_full_well"], p["maximum_value"])
+    np.testing.assert_allclose(o["real_output"][0], ro, rtol=1e-6, atol=1e-3)
+
+
+@pytest.mark.parametrize("mean", [0.4, 6.0, 40.0, 700.0, 4000.0])
+def test_poisson_moments_on_gpu(gpu, mean):
+    p = params(nx=256, ny=256, bx=1, by=1)
+    o0 = run_chain(gpu, p, np.ones(256 * 256, np.float32))
+    ph1 = o0["photons"][0].astype(np.float64)
+    # choose I so that photons * QE is close to `mean` in every pixel
+    I = (mean / (ph1 * p["quantum_efficiency"])).astype(np.float32)
+    o = run_chain(gpu, p, I, seed=5)
+    lam = (o["photons"][0] * np.float32(p["quantum_efficiency"])).astype(np.float64)
+    x = o["count"][0].astype(np.float64)
+    n = x.size
+    if mean < 1000:
+        m, v = lam, lam
+    else:
+        m, v = lam - 0.5, lam + 1.0 / 12
+    z = (x - m).sum() / math.sqrt(v.sum())
+    assert abs(z) < 5, z
+    var_ratio = ((x - m) ** 2).sum() / v.sum()
+    assert abs(var_ratio - 1) < 5 * math.sqrt(2.0 / n) * 1.5, var_ratio
+
+
+def test_readout_noise_and_clamp(gpu):
+    p = params(nx=512, ny=512, bx=1, by=1, beam_intensity=0.0, dark_current=0.0, linear_full_well=0.0)
+    o = run_chain(gpu, p, np.zeros(512 * 512, np.float32), seed=8)
+    r = o["real_output"][0].astype(np.float64)
+    s = p["readout_noise"]
+    # max(N(0, s), 0): mean s/sqrt(2 pi), P(0) = 1/2, second moment s^2/2
+    n = r.size
+    assert abs(np.mean(r == 0) - 0.5) < 5 * math.sqrt(0.25 / n)
+    assert abs(r.mean() - s / math.sqrt(2 * math.pi)) < 5 * s * math.sqrt(0.5 / n)
+    assert abs((r ** 2).mean() - s * s / 2) < 5 * s * s * math.sqrt(2.0 / n)
+
+
+def test_patterns_and_seeds_are_independent(gpu):
+    p = params(nx=128, ny=128, bx=1, by=1, beam_intensity=1e32)
+    I = np.full(128 * 128, 3e7, np.float32)  # mean ~ 570 photons
+    o = run_chain(gpu, p, np.concatenate([I, I]), seed=1, base=0, npatterns=2)
+    a, b = o["count"][0], o["count"][1]
+    assert not np.array_equal(a, b)
+    o2 = run_chain(gpu, p, I, seed=1, base=1)
+    assert np.array_equal(o2["count"][0], b)  # pattern index = pattern_base + p
+    assert abs(np.corrcoef(a, b)[0, 1]) < 0.05
+
+
+def test_ragged_binning_raw_outputs(gpu):
+    p = params(nx=30, ny=21, bx=4, by=4)
+    I = np.full(30 * 21, 2e6, np.float32)
+    o = run_chain(gpu, p, I)
+    th = oracle.thomson(30, 21, p["width"], p["height"], p["distance"], p["center_x"], p["center_y"], 0)
+    assert np.array_equal(o["thomson"][0], th)  # every raw pixel, including those outside full bins
+    assert o["real_output"].shape[1] == (30 // 4) * (21 // 4)
