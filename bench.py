@@ -49,6 +49,8 @@ def parse():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--batched-patterns", type=int, default=512)
     p.add_argument("--profile", action="store_true", help="headline workload only (for rocprofv3)")
+    p.add_argument("--kernel", choices=("auto", "pattern", "grid"), default="auto",
+                   help="restrict the headline sweep to one structure-factor kernel")
     return p.parse_args()
 
 
@@ -68,31 +70,42 @@ def headline(args, torch, dv, wl, _lib):
     F = torch.empty((npix, 2), dtype=torch.float32, device=dev)
     terms = float(npix) * atoms.natoms
 
-    # Tile choice: a short sweep of (pixels per lane, LDS chunk) with one atom
-    # range per tile (a single kernel per step).
-    candidates = [dv.make_tile(k, c, 1) for k in (2, 4, 8) for c in (128, 256)]
-    best, best_ms = None, 1e30
-    for t in candidates:
-        dv.structure_factor(q, atoms, ff, F=F, I=I, tile=t)
+    # Kernel and tile choice: a short sweep over the per-pixel kernel
+    # (pixels per lane x LDS chunk) and the mirror-symmetric detector kernel
+    # (quads per lane x LDS chunk), one atom range per tile (one kernel per step).
+    def run(kind, t):
+        if kind == "grid":
+            dv.structure_factor_grid(q, det.n, det.n, atoms, ff, F=F, I=I, tile=t)
+        else:
+            dv.structure_factor(q, atoms, ff, F=F, I=I, tile=t)
+
+    candidates = [("pattern", dv.make_tile(k, c, 1)) for k in (2, 4) for c in (128, 256)]
+    candidates += [("grid", dv.make_tile(k, c, 1)) for k in (1, 2) for c in (128, 256)]
+    if args.kernel != "auto":
+        candidates = [c for c in candidates if c[0] == args.kernel]
+    best, best_ms, sweep = None, 1e30, []
+    for kind, t in candidates:
+        run(kind, t)
         torch.cuda.synchronize()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record()
-        dv.structure_factor(q, atoms, ff, F=F, I=I, tile=t)
+        run(kind, t)
         e1.record()
         e1.synchronize()
         ms = e0.elapsed_time(e1)
+        sweep.append({"kernel": kind, "lane": t.pixels_per_lane, "chunk": t.chunk_atoms, "ms": round(ms, 3)})
         if ms < best_ms:
-            best, best_ms = t, ms
-    tile = best
+            best, best_ms = (kind, t), ms
+    kind, tile = best
 
     for _ in range(args.warmup):
-        dv.structure_factor(q, atoms, ff, F=F, I=I, tile=tile)
+        run(kind, tile)
     torch.cuda.synchronize()
     ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
     t0 = time.perf_counter()
     for e0, e1 in ev:
         e0.record()
-        dv.structure_factor(q, atoms, ff, F=F, I=I, tile=tile)
+        run(kind, tile)
         e1.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -118,11 +131,12 @@ def headline(args, torch, dv, wl, _lib):
         "kernel_ms": kernel_ms,
         "ms_per_step": ms_per_step,
         "value": terms / (ms_per_step * 1e-3) / 1e9,
-        "tile": {"pixels_per_lane": tile.pixels_per_lane, "waves": tile.waves, "chunk_atoms": tile.chunk_atoms,
-                 "atom_splits": tile.atom_splits},
+        "tile": {"kernel": kind, "per_lane": tile.pixels_per_lane, "waves": tile.waves,
+                 "chunk_atoms": tile.chunk_atoms, "atom_splits": tile.atom_splits, "sweep": sweep},
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
-                     "kernel": "sf_pattern_kernel", "w_eq_flops_per_term": W_EQ_DW,
+                     "kernel": "sf_grid_kernel" if kind == "grid" else "sf_pattern_kernel",
+                     "w_eq_flops_per_term": W_EQ_DW,
                      "terms_per_launch": terms, "kernel_ms": round(kernel_ms, 4)},
         "rel_err_vs_fp64": err,
         "natoms": atoms.natoms, "npix": npix, "Z": Z, "pos": pos, "B": B, "hkl": q,
@@ -143,7 +157,7 @@ def batched(args, torch, dv, wl, _lib):
     L = _lib.load()
     plan = C.c_void_p()
     vp = lambda a: a.ctypes.data_as(C.c_void_p)
-    _lib.check(L.spsim_hip_batch_create(vp(q), det.npix, len(mols), vp(offs), vp(Zc), vp(Pc), None, None, 0,
+    _lib.check(L.spsim_hip_batch_create(vp(q), det.n, det.n, len(mols), vp(offs), vp(Zc), vp(Pc), None, None, 0,
                                         len(mols), vp(pm), vp(R), None, C.c_float(0.0), None, C.byref(plan)),
                "spsim_hip_batch_create")
     stream = torch.cuda.Stream()
@@ -183,7 +197,8 @@ def batched(args, torch, dv, wl, _lib):
             "speedup_batched_vs_plain": round(ms_plain / ms_batched, 3),
             "gterms_per_s_batched": round(terms.value / (ms_batched * 1e-3) / 1e9, 1),
             "roofline_frac_batched": round(terms.value * W_EQ_PLAIN / (ms_batched * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, 4),
-            "tile": {"pixels_per_lane": best.pixels_per_lane, "chunk_atoms": best.chunk_atoms},
+            "tile": {"kernel": "sf_batch_grid_kernel" if L.spsim_hip_batch_uses_grid(plan) else "sf_batch_kernel",
+                     "per_lane": best.pixels_per_lane, "chunk_atoms": best.chunk_atoms},
             "batched_vs_plain_rel_l2": agree,
             "graph": True,
         }

@@ -99,15 +99,31 @@ int spsim_hip_hkl_detector(int nx, int ny, float width, float height, float dist
 /* d_out[i] = R * d_in[i] for a row-major 3x3 R, formed in FP64 and rounded to FP32. */
 int spsim_hip_rotate_hkl(const float *d_in, int npix, const double *rot9, float *d_out, void *stream);
 
-/* One pattern: F[i] = sum_j f_s(|q_i|) exp(-B_eff s^2) w_j e^{-B_j s^2} exp(-2 pi i q_i.r_j),
-   I[i] = |F[i]|^2, with s = |q| / 2 in A^-1, B_eff = b_global (0 for the unit species).
-   d_atom_b may be NULL (no per-atom Debye-Waller factor). d_F ([npix][2]) or d_I
-   may be NULL. The nsegs segments must tile the packed atoms [0, natoms) in order
-   (as spsim_hip_pack_atoms writes them). tile may be NULL (default tiling). Uses a library-owned device
-   workspace when atom_splits > 1 (grown on demand, kept until spsim_hip_release). */
+/* One pattern: F[i] = sum_j f_s(|q'_i|) exp(-B_eff s^2) w_j e^{-B_j s^2} exp(-2 pi i q'_i.r_j),
+   I[i] = |F[i]|^2, with q'_i = R q_i, s = |q| / 2 in A^-1, B_eff = b_global (0 for the
+   unit species). rot9 (host, row-major 3x3) may be NULL (R = identity); the
+   kernels apply it exactly, in FP64, as R^T to the atoms. d_atom_b may be NULL
+   (no per-atom Debye-Waller factor). d_F ([npix][2]) or d_I may be NULL. The
+   nsegs segments must tile the packed atoms [0, natoms) in order (as
+   spsim_hip_pack_atoms writes them). tile may be NULL (default tiling). Uses a
+   library-owned device workspace when atom_splits > 1 (grown on demand, kept
+   until spsim_hip_release). */
 int spsim_hip_sf_pattern(const float *d_hkl, int npix, const spsim_hip_atom *d_atoms, const float *d_atom_b,
                          const spsim_hip_segment *d_segs, int nsegs, int natoms, const float *d_ff,
-                         float b_global, float *d_F, float *d_I, const spsim_hip_tile *tile, void *stream);
+                         float b_global, const double *rot9, float *d_F, float *d_I, const spsim_hip_tile *tile,
+                         void *stream);
+
+/* One pattern on a full nx x ny detector whose q list is [ny][nx][3] (as
+   spsim_hip_hkl_detector writes it); same result as spsim_hip_sf_pattern.
+   When the list is mirror-symmetric (a detector centred on the beam: the pixel
+   mirrored in x has q = (-qx, qy, qz), in y (qx, -qy, qz), bit for bit), each
+   lane evaluates the four mirror pixels together and shares their sines,
+   cosines, form factor and Debye-Waller factor; workgroups whose pixels are not
+   symmetric fall back to the per-pixel sum. tile->pixels_per_lane counts quads
+   per lane here (1 or 2); tile may be NULL. */
+int spsim_hip_sf_grid(const float *d_hkl, int nx, int ny, const spsim_hip_atom *d_atoms, const float *d_atom_b,
+                      const spsim_hip_segment *d_segs, int nsegs, int natoms, const float *d_ff, float b_global,
+                      const double *rot9, float *d_F, float *d_I, const spsim_hip_tile *tile, void *stream);
 
 /* Free the library's cached device workspace and graphs on the current device. */
 int spsim_hip_release(void);
@@ -118,21 +134,26 @@ typedef struct spsim_hip_batch spsim_hip_batch;
 
 /* Molecules are given as host arrays, concatenated: molecule m owns raw atoms
    [mol_offsets[m], mol_offsets[m+1]). atom_b may be NULL. Pattern p shows
-   molecule pattern_mol[p] rotated by rot9[p] (row-major 3x3, applied to q).
-   hkl is the shared [npix][3] host q list. The plan copies everything to the
-   device, builds the cost-sorted (pattern, pixel-tile) work list and owns its
-   device memory until spsim_hip_batch_destroy. tile may be NULL (autotuned). */
-int spsim_hip_batch_create(const float *hkl, int npix, int nmol, const int *mol_offsets, const int *Z,
+   molecule pattern_mol[p] with q rotated by rot9[p] (row-major 3x3, q' = R q,
+   applied exactly in FP64, as in spsim_hip_sf_pattern). hkl is the shared
+   [ny][nx][3] host q list. When that list is mirror-symmetric the plan runs the
+   quad kernel (tile->pixels_per_lane = quads per lane), otherwise the per-pixel
+   kernel. The plan copies everything to the device, builds the cost-sorted
+   (pattern, tile) work list and owns its device memory until
+   spsim_hip_batch_destroy. tile may be NULL (default tiling). */
+int spsim_hip_batch_create(const float *hkl, int nx, int ny, int nmol, const int *mol_offsets, const int *Z,
                            const float *pos, const float *amp, const float *atom_b, int delta_atoms,
                            int npatterns, const int *pattern_mol, const double *rot9, const float *ff,
                            float b_global, const spsim_hip_tile *tile, spsim_hip_batch **out);
 /* Runs the persistent kernel: d_I [npatterns][npix] (required), d_F [npatterns][npix][2]
    (may be NULL). use_graph != 0 replays a captured hipGraph (captured on first use
-   for this stream and these output pointers). */
+   for this stream and these output pointers; the stream must not be NULL). */
 int spsim_hip_batch_run(spsim_hip_batch *b, float *d_F, float *d_I, int use_graph, void *stream);
 /* Plain path for comparison: one single-pattern launch per pattern with a host
    synchronisation after each, no work list and no graph. */
 int spsim_hip_batch_run_plain(spsim_hip_batch *b, float *d_F, float *d_I, void *stream);
+/* 1 when the plan runs the mirror-symmetric quad kernel, 0 for the per-pixel kernel. */
+int spsim_hip_batch_uses_grid(const spsim_hip_batch *b);
 /* Tile of the persistent kernel; setting it rebuilds the work list. */
 int spsim_hip_batch_set_tile(spsim_hip_batch *b, const spsim_hip_tile *tile);
 int spsim_hip_batch_get_tile(const spsim_hip_batch *b, spsim_hip_tile *tile, int *blocks);

@@ -36,7 +36,7 @@ def lib() -> C.CDLL:
                                           C.c_int, C.c_float, _f32p]
         L.oracle_rotate_hkl.argtypes = [_f32p, C.c_int, _f64p, _f32p]
         L.oracle_sf_f64.argtypes = [_f32p, C.c_int, _i32p, _f32p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int,
-                                    _f32p, C.c_float, C.c_void_p, _f64p, C.c_int]
+                                    _f32p, C.c_float, C.c_void_p, _f64p, C.c_int, C.c_void_p]
         L.oracle_sf_refsem.argtypes = [_f32p, C.c_int, _i32p, _f32p, C.c_void_p, C.c_int, C.c_int, _f32p, C.c_float,
                                        C.c_void_p, _f32p, C.c_int]
         L.oracle_sf_grad_f64.argtypes = [_f32p, C.c_int, _i32p, _f32p, C.c_void_p, C.c_void_p, C.c_int, _f32p,
@@ -107,8 +107,9 @@ def rotate_hkl(hkl: np.ndarray, R: np.ndarray) -> np.ndarray:
 
 # --------------------------------------------------------------- structure factor
 
-def sf_f64(hkl, Z, pos, amp=None, atom_b=None, delta=False, vector_mode=False, B=0.0, ff=None, threads=0):
-    """FP64 oracle. Returns (F complex128 [npix], I float64 [npix])."""
+def sf_f64(hkl, Z, pos, amp=None, atom_b=None, delta=False, vector_mode=False, B=0.0, ff=None, threads=0, R=None):
+    """FP64 oracle. R (3x3) rotates q exactly (q' = R q) first.
+    Returns (F complex128 [npix], I float64 [npix])."""
     hkl = np.ascontiguousarray(hkl, np.float32).reshape(-1, 3)
     Z = np.ascontiguousarray(Z, np.int32)
     pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
@@ -118,8 +119,9 @@ def sf_f64(hkl, Z, pos, amp=None, atom_b=None, delta=False, vector_mode=False, B
     n = hkl.shape[0]
     F = np.zeros(2 * n, np.float64)
     I = np.zeros(n, np.float64)
+    R_a, R_p = _opt(None if R is None else np.asarray(R, np.float64).reshape(9), np.float64)
     lib().oracle_sf_f64(hkl.ravel(), n, Z, pos.ravel(), amp_p, b_p, Z.shape[0], int(delta), int(vector_mode),
-                        ff.ravel(), float(B), F.ctypes.data_as(C.c_void_p), I, int(threads))
+                        ff.ravel(), float(B), F.ctypes.data_as(C.c_void_p), I, int(threads), R_p)
     return F[0::2] + 1j * F[1::2], I
 
 

@@ -126,17 +126,24 @@ static void cis_turns(double t, double *c, double *s) {
 /*
  * FP64 evaluation on FP32 inputs. amp may be NULL (1), atom_b may be NULL.
  * vector_mode selects the vector path's semantics (src/diffraction.c:856-968).
+ * R (row-major 3x3, may be NULL) rotates every q exactly in FP64 first.
  * F_out is [npix][2] (may be NULL), I_out [npix].
  */
 int oracle_sf_f64(const float *hkl, int npix, const int *Z, const float *pos, const float *amp, const float *atom_b,
                   int natoms, int delta, int vector_mode, const float *ff, float B, double *F_out, double *I_out,
-                  int nthreads) {
+                  int nthreads, const double *R) {
 #ifdef _OPENMP
   if (nthreads > 0) omp_set_num_threads(nthreads);
 #pragma omp parallel for schedule(dynamic, 16)
 #endif
   for (int i = 0; i < npix; ++i) {
-    const double qx = hkl[3 * i], qy = hkl[3 * i + 1], qz = hkl[3 * i + 2];
+    double qx = hkl[3 * i], qy = hkl[3 * i + 1], qz = hkl[3 * i + 2];
+    if (R) { /* q' = R q, unrounded */
+      const double x = qx, y = qy, z = qz;
+      qx = R[0] * x + R[1] * y + R[2] * z;
+      qy = R[3] * x + R[4] * y + R[5] * z;
+      qz = R[6] * x + R[7] * y + R[8] * z;
+    }
     const double d = sqrt(qx * qx + qy * qy + qz * qz);
     const double s2 = (d * 1e-10) * (d * 1e-10) * 0.25;
     double fcache[NSPEC];
@@ -242,7 +249,7 @@ int oracle_sf_grad_f64(const float *hkl, int npix, const int *Z, const float *po
     free(I);
     return -1;
   }
-  oracle_sf_f64(hkl, npix, Z, pos, amp, atom_b, natoms, 0, 0, ff, B, F, I, nthreads);
+  oracle_sf_f64(hkl, npix, Z, pos, amp, atom_b, natoms, 0, 0, ff, B, F, I, nthreads, NULL);
 #ifdef _OPENMP
   if (nthreads > 0) omp_set_num_threads(nthreads);
 #pragma omp parallel for schedule(dynamic, 4)

@@ -60,14 +60,16 @@ SIGNATURES = {
     "spsim_hip_pack_atoms": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _vp, _vp, _vp, _vp, C.POINTER(_i), C.POINTER(_i)]),
     "spsim_hip_hkl_detector": (_i, [_i, _i, _f, _f, _f, _f, _f, _i, _f, _vp, _vp]),
     "spsim_hip_rotate_hkl": (_i, [_vp, _i, _vp, _vp, _vp]),
-    "spsim_hip_sf_pattern": (_i, [_vp, _i, _vp, _vp, _vp, _i, _i, _vp, _f, _vp, _vp, C.POINTER(Tile), _vp]),
+    "spsim_hip_sf_pattern": (_i, [_vp, _i, _vp, _vp, _vp, _i, _i, _vp, _f, _vp, _vp, _vp, C.POINTER(Tile), _vp]),
+    "spsim_hip_sf_grid": (_i, [_vp, _i, _i, _vp, _vp, _vp, _i, _i, _vp, _f, _vp, _vp, _vp, C.POINTER(Tile), _vp]),
     "spsim_hip_release": (_i, []),
     "spsim_hip_detector_chain": (_i, [C.POINTER(DetectorParams), _vp, _i, C.c_uint64, C.c_uint64,
                                       _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "spsim_hip_sf_adjoint": (_i, [_vp, _i, _vp, _vp, _vp, _i, _i, _vp, _f, _vp, _vp, _vp, _vp, _vp]),
     "spsim_hip_poisson_nll": (_i, [_vp, _vp, _vp, _i, _vp, _vp, _vp]),
-    "spsim_hip_batch_create": (_i, [_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp, _vp, _vp, _f,
+    "spsim_hip_batch_create": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp, _vp, _vp, _f,
                                     C.POINTER(Tile), C.POINTER(_vp)]),
+    "spsim_hip_batch_uses_grid": (_i, [_vp]),
     "spsim_hip_batch_run": (_i, [_vp, _vp, _vp, _i, _vp]),
     "spsim_hip_batch_run_plain": (_i, [_vp, _vp, _vp, _vp]),
     "spsim_hip_batch_terms": (_i, [_vp, C.POINTER(C.c_double)]),

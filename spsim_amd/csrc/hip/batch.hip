@@ -13,7 +13,8 @@
 using spsim::MolInfo;
 
 struct spsim_hip_batch {
-  int npix = 0, npatterns = 0, nmol = 0;
+  int npix = 0, nx = 0, ny = 0, npatterns = 0, nmol = 0;
+  bool grid = false;  // mirror-symmetric detector: quad work items
   spsim_hip_tile tile{};
   int blocks = 0;
   int cus = 0;
@@ -33,7 +34,6 @@ struct spsim_hip_batch {
   int2 *items = nullptr;
   unsigned *queue = nullptr;
   float *ff = nullptr;
-  float *hkl_rot = nullptr;  // plain path scratch [npix][3]
 
   // host copies
   std::vector<MolInfo> h_mols;
@@ -67,8 +67,7 @@ int dupload(T **p, const T *h, size_t n) {
 void free_batch(spsim_hip_batch *b) {
   if (!b) return;
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
-  void *ptrs[] = {b->hkl, b->atoms, b->atom_b, b->segs, b->mols, b->pattern_mol, b->rot, b->items, b->queue, b->ff,
-                  b->hkl_rot};
+  void *ptrs[] = {b->hkl, b->atoms, b->atom_b, b->segs, b->mols, b->pattern_mol, b->rot, b->items, b->queue, b->ff};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   delete b;
@@ -76,7 +75,7 @@ void free_batch(spsim_hip_batch *b) {
 
 // Work items (pattern, tile) for tile size `tp`, heaviest pattern first so the
 // persistent workgroups finish together (longest-processing-time order).
-std::vector<int2> build_items(const spsim_hip_batch &b, int tp, const std::vector<int> *only_patterns = nullptr,
+std::vector<int2> build_items(const spsim_hip_batch &b, int tile_units_, const std::vector<int> *only_patterns = nullptr,
                               int max_tiles = -1) {
   std::vector<int> pats;
   if (only_patterns) {
@@ -87,7 +86,8 @@ std::vector<int2> build_items(const spsim_hip_batch &b, int tp, const std::vecto
   }
   auto natoms = [&](int p) { const MolInfo &m = b.h_mols[b.h_pattern_mol[p]]; return m.atom_end - m.atom_begin; };
   std::stable_sort(pats.begin(), pats.end(), [&](int a, int c) { return natoms(a) > natoms(c); });
-  int tiles = (b.npix + tp - 1) / tp;
+  const int units = b.grid ? grid_quads(b.nx, b.ny) : b.npix;
+  int tiles = (units + tile_units_ - 1) / tile_units_;
   if (max_tiles > 0) tiles = std::min(tiles, max_tiles);
   std::vector<int2> items;
   items.reserve((size_t)pats.size() * tiles);
@@ -100,6 +100,8 @@ BatchLaunch launch_args(const spsim_hip_batch *b, const int2 *items, int nitems,
   BatchLaunch a;
   a.hkl = b->hkl;
   a.npix = b->npix;
+  a.nx = b->nx;
+  a.ny = b->ny;
   a.atoms = b->atoms;
   a.atom_b = b->has_b ? b->atom_b : nullptr;
   a.segs = b->segs;
@@ -117,10 +119,10 @@ BatchLaunch launch_args(const spsim_hip_batch *b, const int2 *items, int nitems,
 }
 
 int set_tile(spsim_hip_batch *b, const spsim_hip_tile &t) {
-  if (!tile_supported(t)) return fail(SPSIM_HIP_ERR_INVALID, "unsupported tile");
+  if (!tile_supported(t, b->grid)) return fail(SPSIM_HIP_ERR_INVALID, "unsupported tile");
   int per_cu = 1;
-  SPSIM_TRY(batch_blocks_per_cu(t, b->has_b, &per_cu));
-  std::vector<int2> items = build_items(*b, tile_pixels(t));
+  SPSIM_TRY(batch_blocks_per_cu(t, b->has_b, b->grid, &per_cu));
+  std::vector<int2> items = build_items(*b, tile_units(t));
   if (b->items) SPSIM_HIP_TRY(hipFree(b->items));
   b->items = nullptr;
   SPSIM_TRY(dupload(&b->items, items.data(), items.size()));
@@ -139,15 +141,17 @@ int set_tile(spsim_hip_batch *b, const spsim_hip_tile &t) {
 
 extern "C" {
 
-int spsim_hip_batch_create(const float *hkl, int npix, int nmol, const int *mol_offsets, const int *Z,
+int spsim_hip_batch_create(const float *hkl, int nx, int ny, int nmol, const int *mol_offsets, const int *Z,
                            const float *pos, const float *amp, const float *atom_b, int delta_atoms, int npatterns,
                            const int *pattern_mol, const double *rot9, const float *ff, float b_global,
                            const spsim_hip_tile *tile, spsim_hip_batch **out) {
   using namespace spsim;
   if (!out) return fail(SPSIM_HIP_ERR_INVALID, "out is NULL");
   *out = nullptr;
-  if (npix <= 0 || nmol <= 0 || npatterns <= 0) return fail(SPSIM_HIP_ERR_INVALID, "npix, nmol and npatterns must be > 0");
-  if (npix > (1 << 28)) return fail(SPSIM_HIP_ERR_INVALID, "npix too large");
+  if (nx <= 0 || ny <= 0 || nmol <= 0 || npatterns <= 0)
+    return fail(SPSIM_HIP_ERR_INVALID, "nx, ny, nmol and npatterns must be > 0");
+  if ((long long)nx * ny > (1 << 28)) return fail(SPSIM_HIP_ERR_INVALID, "detector too large");
+  const int npix = nx * ny;
   if (!hkl || !mol_offsets || !pattern_mol || !rot9) return fail(SPSIM_HIP_ERR_INVALID, "NULL input array");
   if (mol_offsets[0] != 0) return fail(SPSIM_HIP_ERR_INVALID, "mol_offsets[0] must be 0");
   for (int m = 0; m < nmol; ++m)
@@ -161,6 +165,9 @@ int spsim_hip_batch_create(const float *hkl, int npix, int nmol, const int *mol_
 
   spsim_hip_batch *b = new spsim_hip_batch();
   b->npix = npix;
+  b->nx = nx;
+  b->ny = ny;
+  b->grid = host_mirror_symmetric(hkl, nx, ny);
   b->npatterns = npatterns;
   b->nmol = nmol;
   b->cus = cus;
@@ -208,13 +215,12 @@ int spsim_hip_batch_create(const float *hkl, int npix, int nmol, const int *mol_
       (rc = dupload(&b->segs, h_segs.data(), h_segs.size())) || (rc = dupload(&b->mols, b->h_mols.data(), (size_t)nmol)) ||
       (rc = dupload(&b->pattern_mol, pattern_mol, (size_t)npatterns)) ||
       (rc = dupload(&b->rot, rot9, 9 * (size_t)npatterns)) || (rc = dmalloc(&b->queue, 1)) ||
-      (rc = dupload(&b->ff, ffh, (size_t)SPSIM_HIP_NSPECIES * SPSIM_HIP_FF_STRIDE)) ||
-      (rc = dmalloc(&b->hkl_rot, 3 * (size_t)npix))) {
+      (rc = dupload(&b->ff, ffh, (size_t)SPSIM_HIP_NSPECIES * SPSIM_HIP_FF_STRIDE))) {
     free_batch(b);
     return rc;
   }
   const int avg_atoms = packed_total / std::max(nmol, 1);
-  spsim_hip_tile t = tile ? *tile : default_tile(avg_atoms, npix, b->has_b);
+  spsim_hip_tile t = tile ? *tile : default_tile(avg_atoms, npix, b->grid);
   if ((rc = set_tile(b, t)) != SPSIM_HIP_OK) {
     free_batch(b);
     return rc;
@@ -242,7 +248,7 @@ int spsim_hip_batch_run(spsim_hip_batch *b, float *d_F, float *d_I, int use_grap
   if (!b || !d_I) return fail(SPSIM_HIP_ERR_INVALID, "NULL argument");
   hipStream_t s = (hipStream_t)stream;
   BatchLaunch a = launch_args(b, b->items, b->nitems, b->queue, d_F, d_I);
-  if (!use_graph) return launch_batch(a, b->tile, b->blocks, s);
+  if (!use_graph) return launch_batch(a, b->tile, b->grid, b->blocks, s);
   if (!s) return fail(SPSIM_HIP_ERR_INVALID, "graph replay needs a non-default stream");
   if (!b->exec || b->g_stream != s || b->g_F != d_F || b->g_I != d_I) {
     if (b->exec) {
@@ -251,7 +257,7 @@ int spsim_hip_batch_run(spsim_hip_batch *b, float *d_F, float *d_I, int use_grap
     }
     hipGraph_t g = nullptr;
     SPSIM_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    int rc = launch_batch(a, b->tile, b->blocks, s);
+    int rc = launch_batch(a, b->tile, b->grid, b->blocks, s);
     hipError_t e = hipStreamEndCapture(s, &g);
     if (rc != SPSIM_HIP_OK) {
       if (g) (void)hipGraphDestroy(g);
@@ -275,14 +281,24 @@ int spsim_hip_batch_run_plain(spsim_hip_batch *b, float *d_F, float *d_I, void *
   hipStream_t s = (hipStream_t)stream;
   for (int p = 0; p < b->npatterns; ++p) {
     const MolInfo &m = b->h_mols[b->h_pattern_mol[p]];
-    SPSIM_TRY(spsim_hip_rotate_hkl(b->hkl, b->npix, &b->h_rot[9 * (size_t)p], b->hkl_rot, s));
-    SPSIM_TRY(spsim_hip_sf_pattern(b->hkl_rot, b->npix, b->atoms + m.atom_begin, b->has_b ? b->atom_b + m.atom_begin : nullptr,
-                                   b->segs + m.seg_begin, m.nseg, m.atom_end - m.atom_begin, b->ff, b->b_global,
-                                   d_F ? d_F + 2 * (size_t)p * b->npix : nullptr, d_I + (size_t)p * b->npix, nullptr, s));
+    const spsim_hip_atom *atoms = b->atoms + m.atom_begin;
+    const float *ab = b->has_b ? b->atom_b + m.atom_begin : nullptr;
+    const double *R = &b->h_rot[9 * (size_t)p];
+    float *F = d_F ? d_F + 2 * (size_t)p * b->npix : nullptr;
+    float *I = d_I + (size_t)p * b->npix;
+    const int nat = m.atom_end - m.atom_begin;
+    if (b->grid)
+      SPSIM_TRY(spsim_hip_sf_grid(b->hkl, b->nx, b->ny, atoms, ab, b->segs + m.seg_begin, m.nseg, nat, b->ff,
+                                  b->b_global, R, F, I, nullptr, s));
+    else
+      SPSIM_TRY(spsim_hip_sf_pattern(b->hkl, b->npix, atoms, ab, b->segs + m.seg_begin, m.nseg, nat, b->ff,
+                                     b->b_global, R, F, I, nullptr, s));
     SPSIM_HIP_TRY(hipStreamSynchronize(s));
   }
   return SPSIM_HIP_OK;
 }
+
+int spsim_hip_batch_uses_grid(const spsim_hip_batch *b) { return b && b->grid ? 1 : 0; }
 
 int spsim_hip_batch_terms(const spsim_hip_batch *b, double *terms) {
   using namespace spsim;
@@ -291,9 +307,10 @@ int spsim_hip_batch_terms(const spsim_hip_batch *b, double *terms) {
   return SPSIM_HIP_OK;
 }
 
-// Times each supported tile on a representative slice of the work (patterns
-// spread over the cost range, the first tiles of the detector) and keeps the
-// fastest in pixel-atom terms per second. Leaves the plan on the best tile.
+// Times each supported tile of the plan's kernel on a representative slice of
+// the work (patterns spread over the cost range, the first tiles of the
+// detector) and keeps the fastest in pixel-atom terms per second. Leaves the
+// plan on the best tile.
 int spsim_hip_batch_autotune(spsim_hip_batch *b, void *stream, spsim_hip_tile *best_out, double *best_rate) {
   using namespace spsim;
   if (!b) return fail(SPSIM_HIP_ERR_INVALID, "NULL plan");
@@ -307,6 +324,7 @@ int spsim_hip_batch_autotune(spsim_hip_batch *b, void *stream, spsim_hip_tile *b
   const int want = std::min(b->npatterns, 6);
   for (int i = 0; i < want; ++i) sample.push_back(order[(size_t)i * b->npatterns / want]);
   const int ks[4] = {1, 2, 4, 8};
+  const int nks = b->grid ? 2 : 4;  // quads per lane {1, 2} or pixels per lane {1, 2, 4, 8}
   const int cs[3] = {64, 128, 256};
   float *dI = nullptr;
   unsigned *q = nullptr;
@@ -321,29 +339,32 @@ int spsim_hip_batch_autotune(spsim_hip_batch *b, void *stream, spsim_hip_tile *b
   spsim_hip_tile best = b->tile;
   double best_r = -1.0;
   int rc = SPSIM_HIP_OK;
-  for (int ki = 0; ki < 4 && rc == SPSIM_HIP_OK; ++ki) {
+  for (int ki = 0; ki < nks && rc == SPSIM_HIP_OK; ++ki) {
     for (int ci = 0; ci < 3 && rc == SPSIM_HIP_OK; ++ci) {
       spsim_hip_tile t{ks[ki], 4, cs[ci], 0};
-      const int tp = tile_pixels(t);
+      const int tu = tile_units(t);
       int per_cu = 1;
-      if ((rc = batch_blocks_per_cu(t, b->has_b, &per_cu)) != SPSIM_HIP_OK) break;
+      if ((rc = batch_blocks_per_cu(t, b->has_b, b->grid, &per_cu)) != SPSIM_HIP_OK) break;
       const int blocks = b->cus * per_cu;
+      const int units = b->grid ? grid_quads(b->nx, b->ny) : b->npix;
       // enough tiles per sampled pattern for ~4 items per workgroup
-      const int tiles = std::max(1, std::min((b->npix + tp - 1) / tp, (4 * blocks + want - 1) / std::max(want, 1)));
-      std::vector<int2> items = build_items(*b, tp, &sample, tiles);
+      const int tiles = std::max(1, std::min((units + tu - 1) / tu, (4 * blocks + want - 1) / std::max(want, 1)));
+      std::vector<int2> items = build_items(*b, tu, &sample, tiles);
       double terms = 0.0;
       for (const int2 &it : items) {
-        const int px = std::min(tp, b->npix - it.y * tp);
-        terms += (double)natoms(it.x) * px;
+        // pixel-atom terms covered by the item (a quad counts 4 pixels; the
+        // duplicated middle row/column of odd sizes is ignored here)
+        const int n_units = std::min(tu, units - it.y * tu);
+        terms += (double)natoms(it.x) * n_units * (b->grid ? 4 : 1);
       }
       int2 *ditems = nullptr;
       if ((rc = dupload(&ditems, items.data(), items.size())) != SPSIM_HIP_OK) break;
       BatchLaunch a = launch_args(b, ditems, (int)items.size(), q, nullptr, dI);
       float ms = 0.0f;
-      rc = launch_batch(a, t, blocks, s);  // warm-up
+      rc = launch_batch(a, t, b->grid, blocks, s);  // warm-up
       if (rc == SPSIM_HIP_OK) {
         hipEventRecord(e0, s);
-        rc = launch_batch(a, t, blocks, s);
+        rc = launch_batch(a, t, b->grid, blocks, s);
         hipEventRecord(e1, s);
         hipError_t e = hipEventSynchronize(e1);
         if (e != hipSuccess) rc = hip_fail(e, "hipEventSynchronize");

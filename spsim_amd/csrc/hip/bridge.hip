@@ -114,9 +114,9 @@ int spsim_bridge_rotate_hkl(float *list, int npix, const double *rot9) {
   return SPSIM_HIP_OK;
 }
 
-int spsim_bridge_structure_factor(const float *hkl, int npix, const spsim_hip_atom *atoms, const float *atom_b,
-                                  const spsim_hip_segment *segs, int nsegs, int natoms, const float *ff, float b_global,
-                                  Complex *F, float *I) {
+static int structure_factor_impl(const float *hkl, int npix, int nx, int ny, const spsim_hip_atom *atoms,
+                                 const float *atom_b, const spsim_hip_segment *segs, int nsegs, int natoms,
+                                 const float *ff, float b_global, Complex *F, float *I) {
   using namespace spsim;
   SPSIM_TRY(ensure_device(nullptr));
   if (npix <= 0) return SPSIM_HIP_OK;
@@ -133,11 +133,27 @@ int spsim_bridge_structure_factor(const float *hkl, int npix, const spsim_hip_at
   SPSIM_TRY(b.upload(&dff, ff, (size_t)SPSIM_HIP_NSPECIES * SPSIM_HIP_FF_STRIDE, st.s));
   if (F) SPSIM_TRY(b.alloc(&dF, (size_t)npix * 2));
   SPSIM_TRY(b.alloc(&dI, (size_t)npix));
-  SPSIM_TRY(spsim_hip_sf_pattern(dq, npix, da, db, ds, nsegs, natoms, dff, b_global, dF, dI, nullptr, st.s));
+  if (nx > 0)
+    SPSIM_TRY(spsim_hip_sf_grid(dq, nx, ny, da, db, ds, nsegs, natoms, dff, b_global, nullptr, dF, dI, nullptr, st.s));
+  else
+    SPSIM_TRY(spsim_hip_sf_pattern(dq, npix, da, db, ds, nsegs, natoms, dff, b_global, nullptr, dF, dI, nullptr, st.s));
   if (F) SPSIM_TRY(download((float *)F, (const float *)dF, (size_t)npix * 2, st.s));
   SPSIM_TRY(download(I, (const float *)dI, (size_t)npix, st.s));
   SPSIM_HIP_TRY(hipStreamSynchronize(st.s));
   return SPSIM_HIP_OK;
+}
+
+int spsim_bridge_structure_factor(const float *hkl, int npix, const spsim_hip_atom *atoms, const float *atom_b,
+                                  const spsim_hip_segment *segs, int nsegs, int natoms, const float *ff, float b_global,
+                                  Complex *F, float *I) {
+  return structure_factor_impl(hkl, npix, 0, 0, atoms, atom_b, segs, nsegs, natoms, ff, b_global, F, I);
+}
+
+int spsim_bridge_structure_factor_grid(const float *hkl, int nx, int ny, const spsim_hip_atom *atoms,
+                                       const float *atom_b, const spsim_hip_segment *segs, int nsegs, int natoms,
+                                       const float *ff, float b_global, Complex *F, float *I) {
+  if (nx < 0 || ny < 0) return spsim::fail(SPSIM_HIP_ERR_INVALID, "negative detector size");
+  return structure_factor_impl(hkl, nx * ny, nx, ny, atoms, atom_b, segs, nsegs, natoms, ff, b_global, F, I);
 }
 
 int spsim_bridge_thomson(const spsim_hip_detector *det, float *out) {

@@ -4,14 +4,15 @@
 // compute_pattern_on_list, src/diffraction.c:770-853).
 //
 // Layout of the work:
-//  * A workgroup (4 waves, 256 lanes) owns a pixel tile; each lane owns K
-//    pixels strided by 256, so the tile is 256*K pixels and global loads and
-//    stores of q, F and I stay coalesced.
+//  * A workgroup (4 waves, 256 lanes) owns a tile of pixels (per-pixel
+//    kernels: K pixels per lane strided by 256) or of mirror quads (grid
+//    kernels, below), so global loads and stores of q, F and I stay coalesced.
 //  * Atoms arrive pre-grouped by species (one segment per element). For each
 //    segment the form factor f_Z(|q|) is evaluated once per pixel, then the
 //    segment's atoms are streamed through LDS in chunks of CHUNK records
-//    (positions widened to FP64 when staged). Every lane reads the same atom
-//    record (an LDS broadcast) and applies it to its K pixels.
+//    (positions widened to FP64 when staged, and rotated by R^T when the
+//    pattern carries an orientation). Every lane reads the same atom record
+//    (an LDS broadcast) and applies it to all of its pixels.
 //  * Phase: q.r is formed in FP64 (q and r are FP32 inputs, so each product is
 //    exact) and reduced to a fraction of a turn with v_fract_f64 before the
 //    conversion to FP32; v_sin_f32/v_cos_f32 take their argument in turns.
@@ -49,42 +50,65 @@ __device__ __forceinline__ float form_factor(const float *__restrict__ c, float 
   return f;
 }
 
-// Loads q for one pixel (zero for pixels past the end). With ROT the vector is
-// rotated by R in FP64 with a fixed fma order and rounded to FP32, which is
-// bit-identical to spsim_hip_rotate_hkl and the host rotation.
-template <bool ROT>
-__device__ __forceinline__ void load_q(const float *__restrict__ hkl, int pix, bool valid, const double *R, double &qx,
-                                       double &qy, double &qz, float &s2) {
+// Loads q for one pixel (zero for pixels past the end) and s^2 = (|q| 1e-10 / 2)^2
+// in A^-2. |q| is rotation invariant, so s^2 is the same with or without R.
+__device__ __forceinline__ void load_q(const float *__restrict__ hkl, int pix, bool valid, double &qx, double &qy,
+                                       double &qz, float &s2) {
   float x = 0.0f, y = 0.0f, z = 0.0f;
   if (valid) {
     x = hkl[3 * pix + 0];
     y = hkl[3 * pix + 1];
     z = hkl[3 * pix + 2];
   }
-  if constexpr (ROT) {
-    const double dx = x, dy = y, dz = z;
-    const float rx = (float)__builtin_fma(R[2], dz, __builtin_fma(R[1], dy, R[0] * dx));
-    const float ry = (float)__builtin_fma(R[5], dz, __builtin_fma(R[4], dy, R[3] * dx));
-    const float rz = (float)__builtin_fma(R[8], dz, __builtin_fma(R[7], dy, R[6] * dx));
-    x = rx;
-    y = ry;
-    z = rz;
-  }
   qx = x;
   qy = y;
   qz = z;
-  // s = |q| * 1e-10 / 2 in 1/A  ->  s^2 = |q|^2 * 2.5e-21
   s2 = (float)((qx * qx + qy * qy + qz * qz) * 2.5e-21);
 }
 
+// Stages atoms [c0, c0+n) into LDS: positions widened to FP64 and, with R,
+// replaced by R^T r; with per-atom B the amplitude is stored as log2 w. Must be
+// called by every thread of the workgroup; the first barrier protects the
+// previous chunk's readers.
+template <bool HAS_B>
+__device__ __forceinline__ void stage_chunk(const spsim_hip_atom *__restrict__ atoms, const float *__restrict__ atom_b,
+                                            int c0, int n, const double *R, AtomLds *__restrict__ sm) {
+  __syncthreads();
+  for (int t = threadIdx.x; t < n; t += kThreads) {
+    const spsim_hip_atom a = atoms[c0 + t];
+    AtomLds r;
+    const double x = a.x, y = a.y, z = a.z;
+    if (R) {
+      r.x = __builtin_fma(R[6], z, __builtin_fma(R[3], y, R[0] * x));
+      r.y = __builtin_fma(R[7], z, __builtin_fma(R[4], y, R[1] * x));
+      r.z = __builtin_fma(R[8], z, __builtin_fma(R[5], y, R[2] * x));
+    } else {
+      r.x = x;
+      r.y = y;
+      r.z = z;
+    }
+    if constexpr (HAS_B) {
+      r.w = log2f(a.w);
+      r.b = atom_b[c0 + t];
+    } else {
+      r.w = a.w;
+      r.b = 0.0f;
+    }
+    sm[t] = r;
+  }
+  __syncthreads();
+}
+
 // Sums the atoms [a_begin, a_end) of the given segments into Fr, Fi for the
-// lane's K pixels. Must be called by every thread of the workgroup.
+// lane's K pixels (Fi accumulates minus the imaginary part). Must be called by
+// every thread of the workgroup.
 template <int K, int CHUNK, bool HAS_B>
 __device__ __forceinline__ void tile_sum(const double (&qx)[K], const double (&qy)[K], const double (&qz)[K],
                                          const float (&s2)[K], const spsim_hip_atom *__restrict__ atoms,
                                          const float *__restrict__ atom_b, const spsim_hip_segment *__restrict__ segs,
                                          int nsegs, int a_begin, int a_end, const float *__restrict__ ff,
-                                         float b_global, AtomLds *__restrict__ sm, double (&Fr)[K], double (&Fi)[K]) {
+                                         float b_global, const double *R, AtomLds *__restrict__ sm, double (&Fr)[K],
+                                         double (&Fi)[K]) {
   float nl2[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
@@ -107,31 +131,13 @@ __device__ __forceinline__ void tile_sum(const double (&qx)[K], const double (&q
 
     for (int c0 = lo; c0 < hi; c0 += CHUNK) {
       const int n = min(CHUNK, hi - c0);
-      __syncthreads();  // previous chunk fully consumed
-      for (int t = threadIdx.x; t < n; t += kThreads) {
-        const spsim_hip_atom a = atoms[c0 + t];
-        AtomLds r;
-        r.x = a.x;
-        r.y = a.y;
-        r.z = a.z;
-        if constexpr (HAS_B) {
-          r.w = log2f(a.w);
-          r.b = atom_b[c0 + t];
-        } else {
-          r.w = a.w;
-          r.b = 0.0f;
-        }
-        sm[t] = r;
-      }
-      __syncthreads();
-
+      stage_chunk<HAS_B>(atoms, atom_b, c0, n, R, sm);
       float sr[K], si[K];
 #pragma unroll
       for (int k = 0; k < K; ++k) {
         sr[k] = 0.0f;
         si[k] = 0.0f;
       }
-#pragma unroll 2
       for (int j = 0; j < n; ++j) {
         const AtomLds a = sm[j];
 #pragma unroll
@@ -159,39 +165,212 @@ __device__ __forceinline__ void tile_sum(const double (&qx)[K], const double (&q
   }
 }
 
-// One pattern: blockIdx.x = pixel tile, blockIdx.y = atom split.
+// ---- Mirror-symmetric detectors ------------------------------------------
+//
+// A detector centred on the beam has an exactly mirror-symmetric q list: the
+// pixel mirrored in x has q = (-qx, qy, qz), the one mirrored in y (qx, -qy, qz)
+// (the geometry negates exactly and rounds identically). One lane then owns the
+// four mirror pixels of a "quad" and, per atom, forms
+//   a = qx rx, b = qy ry, c = qz rz   (FP64, exact products),
+// the phases c, a + b and a - b reduced to turns, and their sines and cosines
+// (6 transcendentals for 4 terms instead of 8). With u = w e^{-2 pi i c}:
+//   pixel (x, y):   sum u e^{-2 pi i (a+b)},   pixel (x', y'): sum u e^{+2 pi i (a+b)}
+//   pixel (x, y'):  sum u e^{-2 pi i (a-b)},   pixel (x', y):  sum u e^{+2 pi i (a-b)}
+// which are assembled from the four real sums P = sum u cos, Q = sum u sin per
+// angle. |q| is equal across the quad, so the form factor and the per-atom
+// Debye-Waller factor are shared too. Each workgroup checks the symmetry of its
+// quads bit for bit and falls back to the per-pixel sum if any quad is not
+// symmetric, so results never depend on the assumption. An orientation R is
+// applied to the atoms (R^T r), which keeps the q list symmetric.
+
+// Quad index -> the four mirror pixels; the representative (x, y) lies in the
+// lower half of each axis. Middle rows/columns of odd sizes repeat a pixel.
+__device__ __forceinline__ void quad_pixels(int qidx, int nx, int ny, int &p0, int &p1, int &p2, int &p3) {
+  const int hx = (nx + 1) / 2;
+  const int x = qidx % hx, y = qidx / hx;
+  const int xm = nx - 1 - x, ym = ny - 1 - y;
+  p0 = y * nx + x;
+  p1 = y * nx + xm;
+  p2 = ym * nx + x;
+  p3 = ym * nx + xm;
+}
+
+__device__ __forceinline__ bool mirrored(const float *__restrict__ hkl, int p0, int p, float sx, float sy) {
+  return hkl[3 * p + 0] == sx * hkl[3 * p0 + 0] && hkl[3 * p + 1] == sy * hkl[3 * p0 + 1] &&
+         hkl[3 * p + 2] == hkl[3 * p0 + 2];
+}
+
+template <int K, int CHUNK, bool HAS_B>
+__device__ __forceinline__ void quad_sum(const double (&qx)[K], const double (&qy)[K], const double (&qz)[K],
+                                         const float (&s2)[K], const spsim_hip_atom *__restrict__ atoms,
+                                         const float *__restrict__ atom_b, const spsim_hip_segment *__restrict__ segs,
+                                         int nsegs, int a_begin, int a_end, const float *__restrict__ ff,
+                                         float b_global, const double *R, AtomLds *__restrict__ sm,
+                                         double (&Fr)[4 * K], double (&Fi)[4 * K]) {
+  float nl2[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) nl2[k] = -s2[k] * (float)kLog2e;
+#pragma unroll
+  for (int i = 0; i < 4 * K; ++i) {
+    Fr[i] = 0.0;
+    Fi[i] = 0.0;
+  }
+  for (int sg = 0; sg < nsegs; ++sg) {
+    const int s_start = segs[sg].start;
+    const int s_end = s_start + segs[sg].count;
+    const int species = segs[sg].species;
+    const int lo = max(s_start, a_begin);
+    const int hi = min(s_end, a_end);
+    if (lo >= hi) continue;
+    const float Bs = (species == SPSIM_HIP_UNIT_SPECIES) ? 0.0f : b_global;
+    const float *c = ff + species * SPSIM_HIP_FF_STRIDE;
+    float f[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) f[k] = form_factor(c, s2[k], Bs);
+    for (int c0 = lo; c0 < hi; c0 += CHUNK) {
+      const int n = min(CHUNK, hi - c0);
+      stage_chunk<HAS_B>(atoms, atom_b, c0, n, R, sm);
+      float pr[K], pi[K], qr[K], qi[K], p2r[K], p2i[K], q2r[K], q2i[K];
+#pragma unroll
+      for (int k = 0; k < K; ++k) pr[k] = pi[k] = qr[k] = qi[k] = p2r[k] = p2i[k] = q2r[k] = q2i[k] = 0.0f;
+      for (int j = 0; j < n; ++j) {
+        const AtomLds at = sm[j];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          const double ta = qx[k] * at.x;
+          const double tb = qy[k] * at.y;
+          const double tc = qz[k] * at.z;
+          const float fc = (float)__builtin_amdgcn_fract(tc);
+          const float fs = (float)__builtin_amdgcn_fract(ta + tb);
+          const float fd = (float)__builtin_amdgcn_fract(ta - tb);
+          float w;
+          if constexpr (HAS_B) {
+            w = __builtin_amdgcn_exp2f(__builtin_fmaf(at.b, nl2[k], at.w));
+          } else {
+            w = at.w;
+          }
+          const float ur = w * __builtin_amdgcn_cosf(fc);
+          const float ui = w * __builtin_amdgcn_sinf(fc);
+          const float cs = __builtin_amdgcn_cosf(fs), ss = __builtin_amdgcn_sinf(fs);
+          const float cd = __builtin_amdgcn_cosf(fd), sd = __builtin_amdgcn_sinf(fd);
+          pr[k] = __builtin_fmaf(ur, cs, pr[k]);
+          pi[k] = __builtin_fmaf(ui, cs, pi[k]);
+          qr[k] = __builtin_fmaf(ur, ss, qr[k]);
+          qi[k] = __builtin_fmaf(ui, ss, qi[k]);
+          p2r[k] = __builtin_fmaf(ur, cd, p2r[k]);
+          p2i[k] = __builtin_fmaf(ui, cd, p2i[k]);
+          q2r[k] = __builtin_fmaf(ur, sd, q2r[k]);
+          q2i[k] = __builtin_fmaf(ui, sd, q2i[k]);
+        }
+      }
+      // Fi accumulates minus the imaginary part, as in tile_sum.
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const double fk = f[k];
+        Fr[4 * k + 0] = __builtin_fma(fk, (double)pr[k] - (double)qi[k], Fr[4 * k + 0]);  // (x, y): c + (a+b)
+        Fi[4 * k + 0] = __builtin_fma(fk, (double)pi[k] + (double)qr[k], Fi[4 * k + 0]);
+        Fr[4 * k + 1] = __builtin_fma(fk, (double)p2r[k] + (double)q2i[k], Fr[4 * k + 1]);  // (x', y): c - (a-b)
+        Fi[4 * k + 1] = __builtin_fma(fk, (double)p2i[k] - (double)q2r[k], Fi[4 * k + 1]);
+        Fr[4 * k + 2] = __builtin_fma(fk, (double)p2r[k] - (double)q2i[k], Fr[4 * k + 2]);  // (x, y'): c + (a-b)
+        Fi[4 * k + 2] = __builtin_fma(fk, (double)p2i[k] + (double)q2r[k], Fi[4 * k + 2]);
+        Fr[4 * k + 3] = __builtin_fma(fk, (double)pr[k] + (double)qi[k], Fr[4 * k + 3]);  // (x', y'): c - (a+b)
+        Fi[4 * k + 3] = __builtin_fma(fk, (double)pi[k] - (double)qr[k], Fi[4 * k + 3]);
+      }
+    }
+  }
+}
+
+// Writes one pixel's result: a split partial (F, exact sign) or F and I.
+__device__ __forceinline__ void store_pixel(int p, double Fr, double Fi_neg, int npix, int split, double2 *partial,
+                                            float2 *F, float *I) {
+  if (partial) {
+    partial[(size_t)split * npix + p] = make_double2(Fr, -Fi_neg);
+  } else {
+    if (F) F[p] = make_float2((float)Fr, (float)(-Fi_neg));
+    if (I) I[p] = (float)(Fr * Fr + Fi_neg * Fi_neg);
+  }
+}
+
+// A tile of K quads per lane of an nx x ny detector list: per-quad symmetry
+// check (workgroup vote), then the quad sum or the per-pixel fallback.
+template <int K, int CHUNK, bool HAS_B>
+__device__ __forceinline__ void grid_tile(const float *__restrict__ hkl, int nx, int ny, int quad_base,
+                                          const spsim_hip_atom *__restrict__ atoms, const float *__restrict__ atom_b,
+                                          const spsim_hip_segment *__restrict__ segs, int nsegs, int a_begin,
+                                          int a_end, const float *__restrict__ ff, float b_global, const double *R,
+                                          AtomLds *__restrict__ sm, int (&pix)[4 * K], double (&Fr)[4 * K],
+                                          double (&Fi)[4 * K]) {
+  const int nquads = grid_quads(nx, ny);
+  bool sym = true;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int qd = quad_base + threadIdx.x + k * kThreads;
+    if (qd < nquads) {
+      quad_pixels(qd, nx, ny, pix[4 * k], pix[4 * k + 1], pix[4 * k + 2], pix[4 * k + 3]);
+      const int p0 = pix[4 * k];
+      sym = sym && mirrored(hkl, p0, pix[4 * k + 1], -1.0f, 1.0f) && mirrored(hkl, p0, pix[4 * k + 2], 1.0f, -1.0f) &&
+            mirrored(hkl, p0, pix[4 * k + 3], -1.0f, -1.0f);
+    } else {
+      pix[4 * k] = pix[4 * k + 1] = pix[4 * k + 2] = pix[4 * k + 3] = -1;
+    }
+  }
+  const bool fallback = __syncthreads_or(!sym);
+  if (!fallback) {
+    double qx[K], qy[K], qz[K];
+    float s2[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) load_q(hkl, pix[4 * k], pix[4 * k] >= 0, qx[k], qy[k], qz[k], s2[k]);
+    quad_sum<K, CHUNK, HAS_B>(qx, qy, qz, s2, atoms, atom_b, segs, nsegs, a_begin, a_end, ff, b_global, R, sm, Fr, Fi);
+  } else {
+    double qx[4 * K], qy[4 * K], qz[4 * K];
+    float s2[4 * K];
+#pragma unroll
+    for (int i = 0; i < 4 * K; ++i) load_q(hkl, pix[i], pix[i] >= 0, qx[i], qy[i], qz[i], s2[i]);
+    tile_sum<4 * K, CHUNK, HAS_B>(qx, qy, qz, s2, atoms, atom_b, segs, nsegs, a_begin, a_end, ff, b_global, R, sm, Fr,
+                                  Fi);
+  }
+}
+
+// ---- single-pattern kernels (blockIdx.x = tile, blockIdx.y = atom split) ----
+
 template <int K, int CHUNK, bool HAS_B>
 __global__ __launch_bounds__(kThreads) void sf_pattern_kernel(PatternLaunch a, double2 *__restrict__ partial) {
   __shared__ AtomLds sm[CHUNK];
-  const int tile = blockIdx.x;
-  const int split = blockIdx.y;
-  const int nsplit = gridDim.y;
+  const int split = blockIdx.y, nsplit = gridDim.y;
   const int a_begin = (int)(((long long)a.natoms * split) / nsplit);
   const int a_end = (int)(((long long)a.natoms * (split + 1)) / nsplit);
-  const int base = tile * (kThreads * K) + threadIdx.x;
-
+  const int base = blockIdx.x * (kThreads * K) + threadIdx.x;
   double qx[K], qy[K], qz[K];
   float s2[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const int pix = base + k * kThreads;
-    load_q<false>(a.hkl, pix, pix < a.npix, nullptr, qx[k], qy[k], qz[k], s2[k]);
+    load_q(a.hkl, pix, pix < a.npix, qx[k], qy[k], qz[k], s2[k]);
   }
   double Fr[K], Fi[K];
   tile_sum<K, CHUNK, HAS_B>(qx, qy, qz, s2, a.atoms, a.atom_b, a.segs, a.nsegs, a_begin, a_end, a.ff, a.b_global,
-                            sm, Fr, Fi);
+                            a.has_rot ? a.rot.r : nullptr, sm, Fr, Fi);
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const int pix = base + k * kThreads;
-    if (pix >= a.npix) continue;
-    // exp(-i phi): imaginary part is minus the sine sum.
-    if (partial) {
-      partial[(size_t)split * a.npix + pix] = make_double2(Fr[k], -Fi[k]);
-    } else {
-      if (a.F) a.F[pix] = make_float2((float)Fr[k], (float)(-Fi[k]));
-      if (a.I) a.I[pix] = (float)(Fr[k] * Fr[k] + Fi[k] * Fi[k]);
-    }
+    if (pix < a.npix) store_pixel(pix, Fr[k], Fi[k], a.npix, split, partial, a.F, a.I);
   }
+}
+
+template <int K, int CHUNK, bool HAS_B>
+__global__ __launch_bounds__(kThreads) void sf_grid_kernel(PatternLaunch a, int nx, int ny,
+                                                           double2 *__restrict__ partial) {
+  __shared__ AtomLds sm[CHUNK];
+  const int split = blockIdx.y, nsplit = gridDim.y;
+  const int a_begin = (int)(((long long)a.natoms * split) / nsplit);
+  const int a_end = (int)(((long long)a.natoms * (split + 1)) / nsplit);
+  int pix[4 * K];
+  double Fr[4 * K], Fi[4 * K];
+  grid_tile<K, CHUNK, HAS_B>(a.hkl, nx, ny, blockIdx.x * (kThreads * K), a.atoms, a.atom_b, a.segs, a.nsegs, a_begin,
+                             a_end, a.ff, a.b_global, a.has_rot ? a.rot.r : nullptr, sm, pix, Fr, Fi);
+#pragma unroll
+  for (int i = 0; i < 4 * K; ++i)
+    if (pix[i] >= 0) store_pixel(pix[i], Fr[i], Fi[i], a.npix, split, partial, a.F, a.I);
 }
 
 // Deterministic reduction of the atom splits of one pattern, in split order.
@@ -211,114 +390,113 @@ __global__ __launch_bounds__(kThreads) void sf_reduce_kernel(const double2 *__re
 
 __global__ void reset_queue_kernel(unsigned *q) { *q = 0u; }
 
-// Persistent kernel over the (pattern, pixel-tile) work list. Each workgroup
-// takes the next item from the queue until the list is exhausted; the item
-// index is uniform across the workgroup, so every wave leaves the loop together.
+// ---- persistent batched kernels ----
+//
+// Workgroups take the next (pattern, tile) item from the queue until the list
+// is exhausted. The item index is uniform across the workgroup, so every wave
+// leaves the loop together. Each pattern's orientation R is applied to the
+// atoms while they are staged.
+
+__device__ __forceinline__ int next_item(unsigned *queue, int *s_item) {
+  __syncthreads();
+  if (threadIdx.x == 0) *s_item = (int)atomicAdd(queue, 1u);
+  __syncthreads();
+  return *s_item;
+}
+
 template <int K, int CHUNK, bool HAS_B>
 __global__ __launch_bounds__(kThreads) void sf_batch_kernel(BatchLaunch a) {
   __shared__ AtomLds sm[CHUNK];
   __shared__ int s_item;
   for (;;) {
-    __syncthreads();
-    if (threadIdx.x == 0) s_item = (int)atomicAdd(a.queue, 1u);
-    __syncthreads();
-    const int item = s_item;
+    const int item = next_item(a.queue, &s_item);
     if (item >= a.nitems) break;
     const int2 it = a.items[item];
     const int p = it.x;
     const MolInfo m = a.mols[a.pattern_mol[p]];
-    const double *R = a.rot + 9 * (size_t)p;
     const int base = it.y * (kThreads * K) + threadIdx.x;
-
     double qx[K], qy[K], qz[K];
     float s2[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const int pix = base + k * kThreads;
-      load_q<true>(a.hkl, pix, pix < a.npix, R, qx[k], qy[k], qz[k], s2[k]);
+      load_q(a.hkl, pix, pix < a.npix, qx[k], qy[k], qz[k], s2[k]);
     }
     double Fr[K], Fi[K];
     // segment starts are relative to the molecule's first packed atom
     tile_sum<K, CHUNK, HAS_B>(qx, qy, qz, s2, a.atoms + m.atom_begin, HAS_B ? a.atom_b + m.atom_begin : nullptr,
-                              a.segs + m.seg_begin, m.nseg, 0, m.atom_end - m.atom_begin, a.ff, a.b_global, sm, Fr,
-                              Fi);
+                              a.segs + m.seg_begin, m.nseg, 0, m.atom_end - m.atom_begin, a.ff, a.b_global,
+                              a.rot + 9 * (size_t)p, sm, Fr, Fi);
     const size_t off = (size_t)p * a.npix;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const int pix = base + k * kThreads;
-      if (pix >= a.npix) continue;
-      if (a.F) a.F[off + pix] = make_float2((float)Fr[k], (float)(-Fi[k]));
-      a.I[off + pix] = (float)(Fr[k] * Fr[k] + Fi[k] * Fi[k]);
+      if (pix < a.npix) store_pixel(pix, Fr[k], Fi[k], a.npix, 0, nullptr, a.F ? a.F + off : nullptr, a.I + off);
     }
   }
 }
 
-// Instantiation table: (K, CHUNK) pairs x HAS_B.
-#define SPSIM_FOR_EACH_TILE(X) \
-  X(1, 64) X(1, 128) X(1, 256) \
-  X(2, 64) X(2, 128) X(2, 256) \
-  X(4, 64) X(4, 128) X(4, 256) \
-  X(8, 64) X(8, 128) X(8, 256)
+template <int K, int CHUNK, bool HAS_B>
+__global__ __launch_bounds__(kThreads) void sf_batch_grid_kernel(BatchLaunch a) {
+  __shared__ AtomLds sm[CHUNK];
+  __shared__ int s_item;
+  for (;;) {
+    const int item = next_item(a.queue, &s_item);
+    if (item >= a.nitems) break;
+    const int2 it = a.items[item];
+    const int p = it.x;
+    const MolInfo m = a.mols[a.pattern_mol[p]];
+    int pix[4 * K];
+    double Fr[4 * K], Fi[4 * K];
+    grid_tile<K, CHUNK, HAS_B>(a.hkl, a.nx, a.ny, it.y * (kThreads * K), a.atoms + m.atom_begin,
+                               HAS_B ? a.atom_b + m.atom_begin : nullptr, a.segs + m.seg_begin, m.nseg, 0,
+                               m.atom_end - m.atom_begin, a.ff, a.b_global, a.rot + 9 * (size_t)p, sm, pix, Fr, Fi);
+    const size_t off = (size_t)p * a.npix;
+#pragma unroll
+    for (int i = 0; i < 4 * K; ++i)
+      if (pix[i] >= 0) store_pixel(pix[i], Fr[i], Fi[i], a.npix, 0, nullptr, a.F ? a.F + off : nullptr, a.I + off);
+  }
+}
+
+// ---- instantiation tables ----
+
+#define SPSIM_PIXEL_TILES(X) \
+  X(1, 64) X(1, 128) X(1, 256) X(2, 64) X(2, 128) X(2, 256) X(4, 64) X(4, 128) X(4, 256) X(8, 64) X(8, 128) X(8, 256)
+#define SPSIM_QUAD_TILES(X) X(1, 64) X(1, 128) X(1, 256) X(2, 64) X(2, 128) X(2, 256)
+
+enum class Kind { kPattern, kGrid, kBatch, kBatchGrid };
 
 template <bool HAS_B>
-const void *pattern_kernel_ptr(int K, int C) {
-#define X(k_, c_) \
-  if (K == k_ && C == c_) return (const void *)&sf_pattern_kernel<k_, c_, HAS_B>;
-  SPSIM_FOR_EACH_TILE(X)
-#undef X
+const void *kernel_ptr(Kind kind, int K, int C) {
+#define P(k_, c_) \
+  if (K == k_ && C == c_) return kind == Kind::kPattern ? (const void *)&sf_pattern_kernel<k_, c_, HAS_B> \
+                                                          : (const void *)&sf_batch_kernel<k_, c_, HAS_B>;
+#define Q(k_, c_) \
+  if (K == k_ && C == c_) return kind == Kind::kGrid ? (const void *)&sf_grid_kernel<k_, c_, HAS_B> \
+                                                       : (const void *)&sf_batch_grid_kernel<k_, c_, HAS_B>;
+  if (kind == Kind::kPattern || kind == Kind::kBatch) {
+    SPSIM_PIXEL_TILES(P)
+  } else {
+    SPSIM_QUAD_TILES(Q)
+  }
+#undef P
+#undef Q
   return nullptr;
 }
 
-template <bool HAS_B>
-const void *batch_kernel_ptr(int K, int C) {
-#define X(k_, c_) \
-  if (K == k_ && C == c_) return (const void *)&sf_batch_kernel<k_, c_, HAS_B>;
-  SPSIM_FOR_EACH_TILE(X)
-#undef X
-  return nullptr;
+const void *kernel_for(Kind kind, const spsim_hip_tile &t, bool has_b) {
+  if (t.waves != 4) return nullptr;
+  return has_b ? kernel_ptr<true>(kind, t.pixels_per_lane, t.chunk_atoms)
+               : kernel_ptr<false>(kind, t.pixels_per_lane, t.chunk_atoms);
 }
 
-const void *pattern_kernel(const spsim_hip_tile &t, bool has_b) {
-  return has_b ? pattern_kernel_ptr<true>(t.pixels_per_lane, t.chunk_atoms)
-               : pattern_kernel_ptr<false>(t.pixels_per_lane, t.chunk_atoms);
-}
-const void *batch_kernel(const spsim_hip_tile &t, bool has_b) {
-  return has_b ? batch_kernel_ptr<true>(t.pixels_per_lane, t.chunk_atoms)
-               : batch_kernel_ptr<false>(t.pixels_per_lane, t.chunk_atoms);
-}
-
-}  // namespace
-
-bool tile_supported(const spsim_hip_tile &t) {
-  return t.waves == 4 && pattern_kernel(t, false) != nullptr && t.atom_splits >= 0;
-}
-
-spsim_hip_tile default_tile(int natoms, int npix, bool has_b) {
-  (void)has_b;
-  spsim_hip_tile t;
-  t.waves = 4;
-  t.pixels_per_lane = npix >= 4 * 256 * 256 ? 4 : (npix >= 2 * 256 * 64 ? 2 : 1);
-  t.chunk_atoms = natoms >= 256 ? 256 : (natoms >= 128 ? 128 : 64);
-  t.atom_splits = 0;
-  return t;
-}
-
-int auto_splits(const spsim_hip_tile &t, int natoms, int npix, int cus) {
-  const int tiles = (npix + tile_pixels(t) - 1) / tile_pixels(t);
-  const long long want_items = 4LL * cus * 2;  // ~4 rounds of 2 workgroups per CU
-  long long s = (want_items + tiles - 1) / std::max(tiles, 1);
-  const long long max_s = std::max(1, natoms / std::max(1, 2 * t.chunk_atoms));
-  s = std::min(s, max_s);
-  s = std::max(1LL, std::min(s, 64LL));
-  return (int)s;
-}
-
-int launch_pattern(const PatternLaunch &a, const spsim_hip_tile &t, int splits, hipStream_t s) {
+int launch_single(Kind kind, const PatternLaunch &a, int nx, int ny, const spsim_hip_tile &t, int splits,
+                  hipStream_t s) {
   if (a.npix <= 0) return SPSIM_HIP_OK;
-  const bool has_b = a.atom_b != nullptr;
-  const void *k = pattern_kernel(t, has_b);
-  if (!k) return fail(SPSIM_HIP_ERR_INVALID, "unsupported tile (K=%d chunk=%d)", t.pixels_per_lane, t.chunk_atoms);
-  const int tiles = (a.npix + tile_pixels(t) - 1) / tile_pixels(t);
+  const void *k = kernel_for(kind, t, a.atom_b != nullptr);
+  if (!k) return fail(SPSIM_HIP_ERR_INVALID, "unsupported tile (per lane %d, chunk %d)", t.pixels_per_lane, t.chunk_atoms);
+  const int units = kind == Kind::kGrid ? grid_quads(nx, ny) : a.npix;
+  const int tiles = (units + tile_units(t) - 1) / tile_units(t);
   if (splits < 1) splits = 1;
   double2 *partial = nullptr;
   if (splits > 1) {
@@ -327,8 +505,13 @@ int launch_pattern(const PatternLaunch &a, const spsim_hip_tile &t, int splits, 
     partial = (double2 *)ws;
   }
   PatternLaunch arg = a;
-  void *params[] = {(void *)&arg, (void *)&partial};
-  SPSIM_HIP_TRY(hipLaunchKernel(k, dim3(tiles, splits), dim3(kThreads), params, 0, s));
+  if (kind == Kind::kGrid) {
+    void *params[] = {(void *)&arg, (void *)&nx, (void *)&ny, (void *)&partial};
+    SPSIM_HIP_TRY(hipLaunchKernel(k, dim3(tiles, splits), dim3(kThreads), params, 0, s));
+  } else {
+    void *params[] = {(void *)&arg, (void *)&partial};
+    SPSIM_HIP_TRY(hipLaunchKernel(k, dim3(tiles, splits), dim3(kThreads), params, 0, s));
+  }
   if (splits > 1) {
     hipLaunchKernelGGL(sf_reduce_kernel, dim3((a.npix + kThreads - 1) / kThreads), dim3(kThreads), 0, s,
                        (const double2 *)partial, splits, a.npix, a.F, a.I);
@@ -337,19 +520,98 @@ int launch_pattern(const PatternLaunch &a, const spsim_hip_tile &t, int splits, 
   return SPSIM_HIP_OK;
 }
 
-int batch_blocks_per_cu(const spsim_hip_tile &t, bool has_b, int *blocks) {
-  const void *k = batch_kernel(t, has_b);
-  if (!k) return fail(SPSIM_HIP_ERR_INVALID, "unsupported tile (K=%d chunk=%d)", t.pixels_per_lane, t.chunk_atoms);
+int check_common(const float *d_hkl, long long npix, const spsim_hip_atom *d_atoms, const spsim_hip_segment *d_segs,
+                 int nsegs, int natoms, const float *d_ff) {
+  if (npix < 0 || nsegs < 0 || natoms < 0)
+    return fail(SPSIM_HIP_ERR_INVALID, "negative size (npix=%lld nsegs=%d natoms=%d)", npix, nsegs, natoms);
+  if (npix > (1LL << 28)) return fail(SPSIM_HIP_ERR_INVALID, "npix=%lld exceeds 2^28", npix);
+  if (npix > 0 && !d_hkl) return fail(SPSIM_HIP_ERR_INVALID, "d_hkl is NULL");
+  if (nsegs > 0 && (!d_atoms || !d_segs)) return fail(SPSIM_HIP_ERR_INVALID, "d_atoms/d_segs is NULL");
+  if (nsegs == 0 && natoms != 0) return fail(SPSIM_HIP_ERR_INVALID, "natoms=%d but no segments", natoms);
+  if (!d_ff) return fail(SPSIM_HIP_ERR_INVALID, "d_ff is NULL");
+  return SPSIM_HIP_OK;
+}
+
+PatternLaunch make_launch(const float *d_hkl, int npix, const spsim_hip_atom *d_atoms, const float *d_atom_b,
+                          const spsim_hip_segment *d_segs, int nsegs, int natoms, const float *d_ff, float b_global,
+                          const double *rot9, float *d_F, float *d_I) {
+  PatternLaunch a;
+  a.hkl = d_hkl;
+  a.npix = npix;
+  a.atoms = d_atoms;
+  a.atom_b = d_atom_b;
+  a.segs = d_segs;
+  a.nsegs = nsegs;
+  a.natoms = natoms;
+  a.ff = d_ff;
+  a.b_global = b_global;
+  a.F = (float2 *)d_F;
+  a.I = d_I;
+  a.has_rot = rot9 != nullptr;
+  for (int i = 0; i < 9; ++i) a.rot.r[i] = rot9 ? rot9[i] : (i % 4 == 0 ? 1.0 : 0.0);
+  return a;
+}
+
+}  // namespace
+
+bool tile_supported(const spsim_hip_tile &t, bool grid) {
+  return kernel_for(grid ? Kind::kGrid : Kind::kPattern, t, false) != nullptr && t.atom_splits >= 0;
+}
+
+spsim_hip_tile default_tile(int natoms, int npix, bool grid) {
+  spsim_hip_tile t;
+  t.waves = 4;
+  if (grid)
+    t.pixels_per_lane = 1;
+  else
+    t.pixels_per_lane = npix >= 4 * 256 * 256 ? 2 : 1;
+  t.chunk_atoms = natoms >= 256 ? 256 : (natoms >= 128 ? 128 : 64);
+  t.atom_splits = 0;
+  return t;
+}
+
+int auto_splits(int tiles, int natoms, int chunk_atoms, int cus) {
+  const long long want_items = 4LL * cus * 2;  // ~4 rounds of 2 workgroups per CU
+  long long s = (want_items + tiles - 1) / std::max(tiles, 1);
+  const long long max_s = std::max(1, natoms / std::max(1, 2 * chunk_atoms));
+  s = std::min(s, max_s);
+  s = std::max(1LL, std::min(s, 64LL));
+  return (int)s;
+}
+
+bool host_mirror_symmetric(const float *hkl, int nx, int ny) {
+  for (int y = 0; y < ny; ++y) {
+    for (int x = 0; x < nx; ++x) {
+      const float *q = hkl + 3 * ((size_t)y * nx + x);
+      const float *mx = hkl + 3 * ((size_t)y * nx + (nx - 1 - x));
+      const float *my = hkl + 3 * ((size_t)(ny - 1 - y) * nx + x);
+      if (!(mx[0] == -q[0] && mx[1] == q[1] && mx[2] == q[2])) return false;
+      if (!(my[0] == q[0] && my[1] == -q[1] && my[2] == q[2])) return false;
+    }
+  }
+  return true;
+}
+
+int launch_pattern(const PatternLaunch &a, const spsim_hip_tile &t, int splits, hipStream_t s) {
+  return launch_single(Kind::kPattern, a, 0, 0, t, splits, s);
+}
+
+int launch_grid(const PatternLaunch &a, int nx, int ny, const spsim_hip_tile &t, int splits, hipStream_t s) {
+  return launch_single(Kind::kGrid, a, nx, ny, t, splits, s);
+}
+
+int batch_blocks_per_cu(const spsim_hip_tile &t, bool has_b, bool grid, int *blocks) {
+  const void *k = kernel_for(grid ? Kind::kBatchGrid : Kind::kBatch, t, has_b);
+  if (!k) return fail(SPSIM_HIP_ERR_INVALID, "unsupported tile (per lane %d, chunk %d)", t.pixels_per_lane, t.chunk_atoms);
   int nb = 0;
   SPSIM_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, kThreads, 0));
   *blocks = std::max(1, nb);
   return SPSIM_HIP_OK;
 }
 
-int launch_batch(const BatchLaunch &a, const spsim_hip_tile &t, int blocks, hipStream_t s) {
-  const bool has_b = a.atom_b != nullptr;
-  const void *k = batch_kernel(t, has_b);
-  if (!k) return fail(SPSIM_HIP_ERR_INVALID, "unsupported tile (K=%d chunk=%d)", t.pixels_per_lane, t.chunk_atoms);
+int launch_batch(const BatchLaunch &a, const spsim_hip_tile &t, bool grid, int blocks, hipStream_t s) {
+  const void *k = kernel_for(grid ? Kind::kBatchGrid : Kind::kBatch, t, a.atom_b != nullptr);
+  if (!k) return fail(SPSIM_HIP_ERR_INVALID, "unsupported tile (per lane %d, chunk %d)", t.pixels_per_lane, t.chunk_atoms);
   hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(1), 0, s, a.queue);
   SPSIM_HIP_TRY(hipGetLastError());
   if (a.nitems <= 0) return SPSIM_HIP_OK;
@@ -363,34 +625,42 @@ int launch_batch(const BatchLaunch &a, const spsim_hip_tile &t, int blocks, hipS
 
 extern "C" int spsim_hip_sf_pattern(const float *d_hkl, int npix, const spsim_hip_atom *d_atoms, const float *d_atom_b,
                                     const spsim_hip_segment *d_segs, int nsegs, int natoms, const float *d_ff,
-                                    float b_global, float *d_F, float *d_I, const spsim_hip_tile *tile, void *stream) {
+                                    float b_global, const double *rot9, float *d_F, float *d_I,
+                                    const spsim_hip_tile *tile, void *stream) {
   using namespace spsim;
-  if (npix < 0 || nsegs < 0 || natoms < 0)
-    return fail(SPSIM_HIP_ERR_INVALID, "negative size (npix=%d nsegs=%d natoms=%d)", npix, nsegs, natoms);
-  if (npix > 0 && !d_hkl) return fail(SPSIM_HIP_ERR_INVALID, "d_hkl is NULL");
-  if (nsegs > 0 && (!d_atoms || !d_segs)) return fail(SPSIM_HIP_ERR_INVALID, "d_atoms/d_segs is NULL");
-  if (nsegs == 0 && natoms != 0) return fail(SPSIM_HIP_ERR_INVALID, "natoms=%d but no segments", natoms);
-  if (!d_ff) return fail(SPSIM_HIP_ERR_INVALID, "d_ff is NULL");
-  if (npix > (1 << 28)) return fail(SPSIM_HIP_ERR_INVALID, "npix=%d exceeds 2^28", npix);
+  SPSIM_TRY(check_common(d_hkl, npix, d_atoms, d_segs, nsegs, natoms, d_ff));
   int cus = 0;
   SPSIM_TRY(ensure_device(&cus));
   if (npix == 0) return SPSIM_HIP_OK;
-  spsim_hip_tile t = tile ? *tile : default_tile(natoms, npix, d_atom_b != nullptr);
-  if (!tile_supported(t))
+  const spsim_hip_tile t = tile ? *tile : default_tile(natoms, npix, false);
+  if (!tile_supported(t, false))
     return fail(SPSIM_HIP_ERR_INVALID, "unsupported tile (pixels_per_lane=%d waves=%d chunk_atoms=%d)",
                 t.pixels_per_lane, t.waves, t.chunk_atoms);
-  const int splits = t.atom_splits > 0 ? t.atom_splits : auto_splits(t, natoms, npix, cus);
-  PatternLaunch a;
-  a.hkl = d_hkl;
-  a.npix = npix;
-  a.atoms = d_atoms;
-  a.atom_b = d_atom_b;
-  a.segs = d_segs;
-  a.nsegs = nsegs;
-  a.natoms = natoms;
-  a.ff = d_ff;
-  a.b_global = b_global;
-  a.F = (float2 *)d_F;
-  a.I = d_I;
+  const int tiles = (npix + tile_units(t) - 1) / tile_units(t);
+  const int splits = t.atom_splits > 0 ? t.atom_splits : auto_splits(tiles, natoms, t.chunk_atoms, cus);
+  const PatternLaunch a =
+      make_launch(d_hkl, npix, d_atoms, d_atom_b, d_segs, nsegs, natoms, d_ff, b_global, rot9, d_F, d_I);
   return launch_pattern(a, t, splits, (hipStream_t)stream);
+}
+
+extern "C" int spsim_hip_sf_grid(const float *d_hkl, int nx, int ny, const spsim_hip_atom *d_atoms,
+                                 const float *d_atom_b, const spsim_hip_segment *d_segs, int nsegs, int natoms,
+                                 const float *d_ff, float b_global, const double *rot9, float *d_F, float *d_I,
+                                 const spsim_hip_tile *tile, void *stream) {
+  using namespace spsim;
+  if (nx < 0 || ny < 0) return fail(SPSIM_HIP_ERR_INVALID, "negative detector size %d x %d", nx, ny);
+  const long long npix = (long long)nx * ny;
+  SPSIM_TRY(check_common(d_hkl, npix, d_atoms, d_segs, nsegs, natoms, d_ff));
+  int cus = 0;
+  SPSIM_TRY(ensure_device(&cus));
+  if (npix == 0) return SPSIM_HIP_OK;
+  const spsim_hip_tile t = tile ? *tile : default_tile(natoms, (int)npix, true);
+  if (!tile_supported(t, true))
+    return fail(SPSIM_HIP_ERR_INVALID, "unsupported grid tile (quads_per_lane=%d waves=%d chunk_atoms=%d)",
+                t.pixels_per_lane, t.waves, t.chunk_atoms);
+  const int tiles = (grid_quads(nx, ny) + tile_units(t) - 1) / tile_units(t);
+  const int splits = t.atom_splits > 0 ? t.atom_splits : auto_splits(tiles, natoms, t.chunk_atoms, cus);
+  const PatternLaunch a =
+      make_launch(d_hkl, (int)npix, d_atoms, d_atom_b, d_segs, nsegs, natoms, d_ff, b_global, rot9, d_F, d_I);
+  return launch_grid(a, nx, ny, t, splits, (hipStream_t)stream);
 }

@@ -12,6 +12,12 @@ struct MolInfo {
   int atom_begin, atom_end, seg_begin, nseg;
 };
 
+// Row-major rotation R applied to q (q' = R q). Kernels apply R^T to the atoms
+// instead, in FP64, which gives the same phase q'.r = q.(R^T r).
+struct Rot {
+  double r[9];
+};
+
 struct PatternLaunch {
   const float *hkl;
   int npix;
@@ -24,11 +30,14 @@ struct PatternLaunch {
   float b_global;
   float2 *F;  // may be null
   float *I;   // may be null
+  int has_rot;
+  Rot rot;
 };
 
 struct BatchLaunch {
   const float *hkl;
   int npix;
+  int nx, ny;  // detector shape (grid work items)
   const spsim_hip_atom *atoms;
   const float *atom_b;  // may be null
   const spsim_hip_segment *segs;
@@ -44,18 +53,27 @@ struct BatchLaunch {
   float *I;   // [npatterns][npix]
 };
 
-// Valid tile parameters: pixels_per_lane in {1,2,4,8}, waves == 4, chunk in {64,128,256}.
-bool tile_supported(const spsim_hip_tile &t);
-inline int tile_pixels(const spsim_hip_tile &t) { return t.pixels_per_lane * t.waves * 64; }
+// Valid tiles: per-pixel kernels take pixels_per_lane in {1,2,4,8}; the
+// mirror-symmetric (grid) kernels take quads per lane in {1,2}; waves == 4;
+// chunk_atoms in {64,128,256}.
+bool tile_supported(const spsim_hip_tile &t, bool grid);
+// Pixels (per-pixel kernels) or quads (grid kernels) covered by one tile.
+inline int tile_units(const spsim_hip_tile &t) { return t.pixels_per_lane * t.waves * 64; }
+__host__ __device__ inline int grid_quads(int nx, int ny) { return ((nx + 1) / 2) * ((ny + 1) / 2); }
 
 // One pattern. splits >= 1; with splits > 1 the library workspace holds the partial sums.
 int launch_pattern(const PatternLaunch &a, const spsim_hip_tile &t, int splits, hipStream_t s);
-// Persistent kernel: grid of `blocks` workgroups pulls work items from the queue.
-int launch_batch(const BatchLaunch &a, const spsim_hip_tile &t, int blocks, hipStream_t s);
+// Same on an nx x ny detector list, exploiting its mirror symmetry where present.
+int launch_grid(const PatternLaunch &a, int nx, int ny, const spsim_hip_tile &t, int splits, hipStream_t s);
+// Persistent kernel: `blocks` workgroups pull work items from the queue. With
+// grid != 0 the items are quad tiles of the a.nx x a.ny detector.
+int launch_batch(const BatchLaunch &a, const spsim_hip_tile &t, bool grid, int blocks, hipStream_t s);
 // Workgroups of the batch kernel that fit on one CU at once for this tile.
-int batch_blocks_per_cu(const spsim_hip_tile &t, bool has_b, int *blocks);
+int batch_blocks_per_cu(const spsim_hip_tile &t, bool has_b, bool grid, int *blocks);
 // Default tile and split count for a single pattern of npix pixels and natoms atoms.
-spsim_hip_tile default_tile(int natoms, int npix, bool has_b);
-int auto_splits(const spsim_hip_tile &t, int natoms, int npix, int cus);
+spsim_hip_tile default_tile(int natoms, int npix, bool grid);
+int auto_splits(int tiles, int natoms, int chunk_atoms, int cus);
+// True when the [ny][nx][3] host list is exactly mirror-symmetric in x and y.
+bool host_mirror_symmetric(const float *hkl, int nx, int ny);
 
 }  // namespace spsim

@@ -25,6 +25,12 @@ int spsim_bridge_structure_factor(const float *hkl, int npix, const spsim_hip_at
                                   const spsim_hip_segment *segs, int nsegs, int natoms, const float *ff, float b_global,
                                   Complex *F, float *I);
 
+/* Same on a full nx x ny detector list ([ny][nx][3]); uses the mirror-symmetric
+   kernel where the list allows it. */
+int spsim_bridge_structure_factor_grid(const float *hkl, int nx, int ny, const spsim_hip_atom *atoms,
+                                       const float *atom_b, const spsim_hip_segment *segs, int nsegs, int natoms,
+                                       const float *ff, float b_global, Complex *F, float *I);
+
 /* Detector stages. Inputs are host arrays of nx*ny raw pixels; binned outputs
    have (nx/bin_x)*(ny/bin_y) entries. */
 int spsim_bridge_thomson(const spsim_hip_detector *det, float *out);

@@ -125,6 +125,23 @@ static double now_seconds(void) {
   return ts.tv_sec + 1e-9 * ts.tv_nsec;
 }
 
+/* Detector dimensions of a q list when they are known (nx*ny == size): the
+   caller's CCD when it matches, otherwise a square list. The grid kernel
+   checks the mirror symmetry it uses, so a wrong guess only costs speed. */
+static void grid_dims(int size, const Options *opts, int *nx, int *ny) {
+  *nx = *ny = 0;
+  if (opts && opts->detector && opts->detector->nx > 0 && opts->detector->ny > 0 &&
+      (long long)opts->detector->nx * opts->detector->ny == size) {
+    *nx = opts->detector->nx;
+    *ny = opts->detector->ny;
+    return;
+  }
+  int n = (int)sqrt((double)size);
+  while ((long long)n * n > size) --n;
+  while ((long long)(n + 1) * (n + 1) <= size) ++n;
+  if ((long long)n * n == size) *nx = *ny = n;
+}
+
 static Diffraction_Pattern *compute_pattern(Molecule *mol, float *HKL_list, int HKL_list_size, float B,
                                             Experiment *exp, Options *opts, int mode) {
   const double t0 = now_seconds();
@@ -157,8 +174,14 @@ static Diffraction_Pattern *compute_pattern(Molecule *mol, float *HKL_list, int 
     fprintf(stderr, "spsim: compute_pattern_on_list: atomic numbers must lie in [0, 99]\n");
     goto fail;
   }
-  rc = spsim_bridge_structure_factor(HKL_list, HKL_list_size, atoms, NULL, segs, nsegs, natoms,
-                                     spsim_form_factor_table(), B, res->F, res->ints);
+  int gx = 0, gy = 0;
+  grid_dims(HKL_list_size, opts, &gx, &gy);
+  if (gx > 0)
+    rc = spsim_bridge_structure_factor_grid(HKL_list, gx, gy, atoms, NULL, segs, nsegs, natoms,
+                                            spsim_form_factor_table(), B, res->F, res->ints);
+  else
+    rc = spsim_bridge_structure_factor(HKL_list, HKL_list_size, atoms, NULL, segs, nsegs, natoms,
+                                       spsim_form_factor_table(), B, res->F, res->ints);
   if (rc != SPSIM_HIP_OK) {
     spsim_report("compute_pattern_on_list", rc);
     goto fail;

@@ -97,20 +97,51 @@ def make_tile(pixels_per_lane=4, chunk_atoms=256, atom_splits=0, waves=4) -> Til
     return Tile(pixels_per_lane, waves, chunk_atoms, atom_splits)
 
 
+def _rot(R):
+    if R is None:
+        return None, None
+    a = np.ascontiguousarray(R, np.float64).reshape(9)
+    return a, a.ctypes.data_as(C.c_void_p)
+
+
 def structure_factor(hkl: torch.Tensor, atoms: DeviceAtoms, ff: torch.Tensor, B: float = 0.0,
                      F: torch.Tensor | None = None, I: torch.Tensor | None = None, tile: Tile | None = None,
-                     stream=None, want_F: bool = True):
-    """One pattern on the q list hkl ([npix, 3] float32 device tensor).
-    Returns (F [npix, 2] float32 or None, I [npix] float32)."""
+                     stream=None, want_F: bool = True, R=None):
+    """One pattern on the q list hkl ([npix, 3] float32 device tensor), with q
+    rotated exactly by R (3x3) when given. Returns (F [npix, 2] float32 or
+    None, I [npix] float32)."""
     npix = int(hkl.shape[0])
     if want_F and F is None:
         F = torch.empty((npix, 2), dtype=torch.float32, device=hkl.device)
     if I is None:
         I = torch.empty((npix,), dtype=torch.float32, device=hkl.device)
+    Ra, Rp = _rot(R)
     rc = _lib.load().spsim_hip_sf_pattern(_ptr(hkl), npix, _ptr(atoms.atoms), _ptr(atoms.b), _ptr(atoms.segs),
-                                          atoms.nsegs, atoms.natoms, _ptr(ff), float(B), _ptr(F if want_F else None),
-                                          _ptr(I), C.byref(tile) if tile is not None else None, _stream(stream))
+                                          atoms.nsegs, atoms.natoms, _ptr(ff), float(B), Rp,
+                                          _ptr(F if want_F else None), _ptr(I),
+                                          C.byref(tile) if tile is not None else None, _stream(stream))
     check(rc, "spsim_hip_sf_pattern")
+    return (F if want_F else None), I
+
+
+def structure_factor_grid(hkl: torch.Tensor, nx: int, ny: int, atoms: DeviceAtoms, ff: torch.Tensor,
+                          B: float = 0.0, F: torch.Tensor | None = None, I: torch.Tensor | None = None,
+                          tile: Tile | None = None, stream=None, want_F: bool = True, R=None):
+    """One pattern on a full nx x ny detector list ([ny*nx, 3]), q rotated
+    exactly by R when given; uses the mirror-symmetric kernel where the list is
+    symmetric (tile.pixels_per_lane = quads per lane, 1 or 2). Returns (F or None, I)."""
+    npix = nx * ny
+    if int(hkl.shape[0]) != npix:
+        raise ValueError("hkl must have nx*ny rows")
+    if want_F and F is None:
+        F = torch.empty((npix, 2), dtype=torch.float32, device=hkl.device)
+    if I is None:
+        I = torch.empty((npix,), dtype=torch.float32, device=hkl.device)
+    Ra, Rp = _rot(R)
+    rc = _lib.load().spsim_hip_sf_grid(_ptr(hkl), nx, ny, _ptr(atoms.atoms), _ptr(atoms.b), _ptr(atoms.segs),
+                                       atoms.nsegs, atoms.natoms, _ptr(ff), float(B), Rp, _ptr(F if want_F else None),
+                                       _ptr(I), C.byref(tile) if tile is not None else None, _stream(stream))
+    check(rc, "spsim_hip_sf_grid")
     return (F if want_F else None), I
 
 

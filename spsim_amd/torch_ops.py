@@ -54,7 +54,7 @@ class _Intensity(torch.autograd.Function):
         F = torch.empty((npix, 2), dtype=torch.float32, device=hkl.device)
         I = torch.empty((npix,), dtype=torch.float32, device=hkl.device)
         check(_lib.load().spsim_hip_sf_pattern(_p(hkl), npix, _p(atoms), _p(pb), _p(layout.segs), layout.nsegs,
-                                               layout.natoms, _p(ff), float(b_global), _p(F), _p(I),
+                                               layout.natoms, _p(ff), float(b_global), None, _p(F), _p(I),
                                                C.byref(tile) if tile is not None else None, _sp()),
               "spsim_hip_sf_pattern")
         ctx.save_for_backward(atoms, pb if pb is not None else torch.empty(0, device=hkl.device), F)

@@ -17,7 +17,7 @@ def vp(a):
     return a.ctypes.data_as(C.c_void_p)
 
 
-def make_plan(mols, rots, pattern_mol, q, atom_b=None, tile=None, b_global=0.0):
+def make_plan(mols, rots, pattern_mol, q, nx, ny, atom_b=None, tile=None, b_global=0.0):
     offs = np.zeros(len(mols) + 1, np.int32)
     offs[1:] = np.cumsum([len(m[0]) for m in mols])
     Z = np.ascontiguousarray(np.concatenate([m[0] for m in mols]), np.int32)
@@ -26,45 +26,49 @@ def make_plan(mols, rots, pattern_mol, q, atom_b=None, tile=None, b_global=0.0):
     pm = np.ascontiguousarray(pattern_mol, np.int32)
     plan = C.c_void_p()
     B = None if atom_b is None else np.ascontiguousarray(atom_b, np.float32)
-    _lib.check(_lib.load().spsim_hip_batch_create(vp(q), q.shape[0], len(mols), vp(offs), vp(Z), vp(P), None,
+    _lib.check(_lib.load().spsim_hip_batch_create(vp(q), nx, ny, len(mols), vp(offs), vp(Z), vp(P), None,
                                                   None if B is None else vp(B), 0, len(pm), vp(pm), vp(R), None,
                                                   float(b_global), C.byref(tile) if tile else None, C.byref(plan)),
                "batch_create")
     return plan, offs, Z, P
 
 
-@pytest.fixture(scope="module")
-def setup():
+@pytest.fixture(scope="module", params=["centred", "off-centre"])
+def setup(request):
+    """Five molecules (one empty) shown in nine orientations on a 40 x 40
+    detector; centred (mirror-symmetric: quad kernel) or off-centre (per-pixel)."""
     rng = np.random.default_rng(21)
-    sizes = [1, 0, 333, 2000, 5000]  # includes a molecule without atoms
+    sizes = [1, 0, 333, 2000, 5000]
     mols = [wl.random_molecule(n, 2e-9 + 1e-9 * (n / 5000.0) ** (1 / 3), rng)[:2] if n else
             (np.zeros(0, np.int32), np.zeros((0, 3), np.float32)) for n in sizes]
     npat = 9
     pattern_mol = np.array([4, 3, 2, 1, 0, 4, 2, 3, 4], np.int32)
     rots = wl.uniform_rotations(npat, rng)
-    det = wl.Detector(40, 5e-4, 0.1, 0.2e-9)  # 1600 pixels: ragged tiles
-    q = oracle.hkl_detector(det.n, det.n, det.width, det.width, det.distance, 0, 0, 0, det.wavelength)
-    return mols, rots, pattern_mol, q
+    det = wl.Detector(40, 5e-4, 0.1, 0.2e-9)
+    cx = 0.0 if request.param == "centred" else 1.5e-4
+    q = oracle.hkl_detector(det.n, det.n, det.width, det.width, det.distance, cx, 0, 0, det.wavelength)
+    return mols, rots, pattern_mol, q, det.n, request.param == "centred"
 
 
 def reference(setup):
-    mols, rots, pattern_mol, q = setup
+    mols, rots, pattern_mol, q, n, _ = setup
     out = []
     for p in range(len(pattern_mol)):
         Z, pos = mols[pattern_mol[p]]
-        qr = oracle.rotate_hkl(q, rots[p])
-        out.append(oracle.sf_f64(qr, Z, pos, threads=oracle_threads())[1] if len(Z) else np.zeros(q.shape[0]))
+        # batched patterns rotate q exactly (q' = R q in FP64)
+        out.append(oracle.sf_f64(q, Z, pos, R=rots[p], threads=oracle_threads())[1] if len(Z) else np.zeros(q.shape[0]))
     return np.stack(out)
 
 
 def test_batched_paths_match_oracle_and_each_other(gpu, setup):
     import torch
 
-    mols, rots, pattern_mol, q = setup
+    mols, rots, pattern_mol, q, n, centred = setup
     ref = reference(setup)
     L = _lib.load()
-    plan, *_ = make_plan(mols, rots, pattern_mol, q)
+    plan, *_ = make_plan(mols, rots, pattern_mol, q, n, n)
     try:
+        assert L.spsim_hip_batch_uses_grid(plan) == int(centred)
         npat, npix = len(pattern_mol), q.shape[0]
         stream = torch.cuda.Stream()
         sp = C.c_void_p(stream.cuda_stream)
@@ -104,19 +108,20 @@ def test_batched_paths_match_oracle_and_each_other(gpu, setup):
 def test_autotune_and_every_tile(gpu, setup):
     import torch
 
-    mols, rots, pattern_mol, q = setup
+    mols, rots, pattern_mol, q, n, centred = setup
     ref = reference(setup)
     L = _lib.load()
-    plan, *_ = make_plan(mols, rots, pattern_mol, q)
+    plan, *_ = make_plan(mols, rots, pattern_mol, q, n, n)
     try:
         stream = torch.cuda.Stream()
         sp = C.c_void_p(stream.cuda_stream)
         best = _lib.Tile()
         rate = C.c_double()
         _lib.check(L.spsim_hip_batch_autotune(plan, sp, C.byref(best), C.byref(rate)), "autotune")
-        assert best.pixels_per_lane in (1, 2, 4, 8) and best.chunk_atoms in (64, 128, 256) and rate.value > 0
+        assert best.pixels_per_lane in ((1, 2) if centred else (1, 2, 4, 8)) and best.chunk_atoms in (64, 128, 256)
+        assert rate.value > 0
         I = torch.zeros((len(pattern_mol), q.shape[0]), dtype=torch.float32, device=gpu)
-        for k in (1, 2, 4, 8):
+        for k in ((1, 2) if centred else (1, 2, 4, 8)):
             for c in (64, 128, 256):
                 t = _lib.Tile(k, 4, c, 0)
                 _lib.check(L.spsim_hip_batch_set_tile(plan, C.byref(t)), "set_tile")
@@ -144,14 +149,14 @@ def test_batched_per_atom_b(gpu):
     det = wl.Detector(32, 1e-3, 0.1, 0.15e-9)
     q = oracle.hkl_detector(det.n, det.n, det.width, det.width, det.distance, 0, 0, 0, det.wavelength)
     L = _lib.load()
-    plan, *_ = make_plan(mols, rots, pm, q, atom_b=np.concatenate(Bs), b_global=4.0)
+    plan, *_ = make_plan(mols, rots, pm, q, det.n, det.n, atom_b=np.concatenate(Bs), b_global=4.0)
     try:
         I = torch.zeros((3, q.shape[0]), dtype=torch.float32, device=gpu)
         _lib.check(L.spsim_hip_batch_run(plan, None, C.c_void_p(I.data_ptr()), 0, None), "run")
         torch.cuda.synchronize()
         for p in range(3):
             Z, pos = mols[pm[p]]
-            _, ref = oracle.sf_f64(oracle.rotate_hkl(q, rots[p]), Z, pos, atom_b=Bs[pm[p]], B=4.0)
+            _, ref = oracle.sf_f64(q, Z, pos, atom_b=Bs[pm[p]], B=4.0, R=rots[p])
             assert rel_l2(I.cpu().numpy()[p], ref) < 1e-5
     finally:
         L.spsim_hip_batch_destroy(plan)
@@ -166,11 +171,11 @@ def test_invalid_arguments_are_rejected(gpu):
     P = np.zeros(3, np.float32)
     R = np.eye(3).ravel()
     bad_pm = np.array([3], np.int32)
-    rc = L.spsim_hip_batch_create(vp(q), 4, 1, vp(offs), vp(Z), vp(P), None, None, 0, 1, vp(bad_pm), vp(R), None,
+    rc = L.spsim_hip_batch_create(vp(q), 2, 2, 1, vp(offs), vp(Z), vp(P), None, None, 0, 1, vp(bad_pm), vp(R), None,
                                   0.0, None, C.byref(plan))
     assert rc == -1 and "out of range" in _lib.last_error()
     badZ = np.array([150], np.int32)
     pm = np.array([0], np.int32)
-    rc = L.spsim_hip_batch_create(vp(q), 4, 1, vp(offs), vp(badZ), vp(P), None, None, 0, 1, vp(pm), vp(R), None,
+    rc = L.spsim_hip_batch_create(vp(q), 2, 2, 1, vp(offs), vp(badZ), vp(P), None, None, 0, 1, vp(pm), vp(R), None,
                                   0.0, None, C.byref(plan))
     assert rc == -1

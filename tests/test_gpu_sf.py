@@ -173,3 +173,88 @@ def test_sf_delta_atoms_and_vector_mode(gpu):
             _, I_ref = oracle.sf_f64(hkl, Z, pos, amp=amp, delta=delta, vector_mode=bool(mode), B=10.0)
             _, I = _gpu_sf(gpu, hkl, Z, pos, amp=amp, delta=delta, mode=mode, B=10.0)
             assert rel_l2(I, I_ref) < TOL_F64, (delta, mode)
+
+
+def _gpu_grid(gpu, hkl, nx, ny, Z, pos, atom_b=None, B=0.0, tile=None):
+    import torch
+
+    da = dv.DeviceAtoms(dv.pack_atoms(Z, pos, atom_b=atom_b), gpu)
+    ff = dv.form_factor_table(device=gpu)
+    q = torch.from_numpy(np.ascontiguousarray(hkl, np.float32)).to(gpu)
+    F, I = dv.structure_factor_grid(q, nx, ny, da, ff, B=B, tile=tile)
+    torch.cuda.synchronize()
+    F = F.cpu().numpy()
+    return F[:, 0] + 1j * F[:, 1], I.cpu().numpy()
+
+
+@pytest.mark.parametrize("nx,ny", [(64, 64), (45, 45), (50, 33), (1, 7), (2, 2)])
+def test_sf_grid_mirror_kernel_vs_f64(gpu, nx, ny):
+    rng = np.random.default_rng(30)
+    Z, pos, B = wl.random_molecule(900, 4e-9, rng, atom_b_range=(10.0, 40.0))
+    hkl = oracle.hkl_detector(nx, ny, np.float32(nx * 3e-4).item(), np.float32(ny * 3e-4).item(), 0.05, 0.0, 0.0, 0,
+                              0.15e-9)
+    F_ref, I_ref = oracle.sf_f64(hkl, Z, pos, atom_b=B, B=3.0, threads=oracle_threads())
+    for has_b in (False, True):
+        if not has_b:
+            F_ref, I_ref = oracle.sf_f64(hkl, Z, pos, B=3.0, threads=oracle_threads())
+        for tile in (None, dv.make_tile(1, 64, 1), dv.make_tile(2, 256, 3)):
+            F, I = _gpu_grid(gpu, hkl, nx, ny, Z, pos, atom_b=B if has_b else None, B=3.0, tile=tile)
+            assert rel_l2(I, I_ref) < TOL_F64, (has_b, tile)
+            assert rel_l2(np.c_[F.real, F.imag], np.c_[F_ref.real, F_ref.imag]) < TOL_F64
+
+
+def test_sf_grid_falls_back_without_symmetry(gpu):
+    rng = np.random.default_rng(31)
+    Z, pos, _ = wl.random_molecule(500, 3e-9, rng)
+    n = 40
+    # off-centre detector: no mirror symmetry anywhere
+    hkl = oracle.hkl_detector(n, n, np.float32(n * 5e-4).item(), np.float32(n * 5e-4).item(), 0.1, 1e-4, -2e-4, 0,
+                              0.2e-9)
+    _, I_ref = oracle.sf_f64(hkl, Z, pos)
+    _, I = _gpu_grid(gpu, hkl, n, n, Z, pos)
+    assert rel_l2(I, I_ref) < TOL_F64
+    # a symmetric detector with one pixel disturbed: only that workgroup falls back
+    hkl2 = _det_hkl(wl.Detector(n, 5e-4, 0.1, 0.2e-9)).copy()
+    hkl2[7, 0] *= np.float32(1.001)
+    _, I_ref2 = oracle.sf_f64(hkl2, Z, pos)
+    _, I2 = _gpu_grid(gpu, hkl2, n, n, Z, pos)
+    assert rel_l2(I2, I_ref2) < TOL_F64
+    # an arbitrary (non-detector) list of n*n vectors
+    hkl3 = rng.normal(0, 2e9, size=(n * n, 3)).astype(np.float32)
+    _, I_ref3 = oracle.sf_f64(hkl3, Z, pos)
+    _, I3 = _gpu_grid(gpu, hkl3, n, n, Z, pos)
+    assert rel_l2(I3, I_ref3) < TOL_F64
+
+
+def test_sf_grid_config2_subset_vs_f64(gpu):
+    det, Z, pos, B = wl.config2()
+    hkl = _det_hkl(det)
+    F, I = _gpu_grid(gpu, hkl, det.n, det.n, Z, pos, atom_b=B)
+    rng = np.random.default_rng(23)
+    sub = np.sort(rng.choice(hkl.shape[0], 4096, replace=False))
+    _, I_ref = oracle.sf_f64(hkl[sub], Z, pos, atom_b=B, threads=oracle_threads())
+    err = rel_l2(I[sub], I_ref)
+    print(f"config2 (mirror kernel) subset rel err {err:.3e}")
+    assert err < TOL_F64
+
+
+@pytest.mark.parametrize("grid", [False, True])
+def test_sf_exact_rotation(gpu, grid):
+    """q' = R q applied exactly (atoms rotated by R^T in FP64) in both kernels."""
+    import torch
+
+    rng = np.random.default_rng(32)
+    Z, pos, B = wl.random_molecule(1200, 5e-9, rng, atom_b_range=(10.0, 40.0))
+    R = wl.uniform_rotations(1, rng)[0]
+    n = 36
+    hkl = _det_hkl(wl.Detector(n, 8e-4, 0.1, 0.15e-9))
+    _, I_ref = oracle.sf_f64(hkl, Z, pos, atom_b=B, R=R, threads=oracle_threads())
+    da = dv.DeviceAtoms(dv.pack_atoms(Z, pos, atom_b=B), gpu)
+    ff = dv.form_factor_table(device=gpu)
+    q = torch.from_numpy(hkl).to(gpu)
+    if grid:
+        _, I = dv.structure_factor_grid(q, n, n, da, ff, R=R)
+    else:
+        _, I = dv.structure_factor(q, da, ff, R=R)
+    torch.cuda.synchronize()
+    assert rel_l2(I.cpu().numpy(), I_ref) < TOL_F64
