@@ -140,6 +140,28 @@ typedef struct {
 /* src/config.c:30-87 */
 Options *set_defaults(void);
 
+/* include/config.h:184, src/config.c:89-426. Reads a libconfig-format file
+ * (e.g. spsim.conf) into opt and fills the derived fields (nx, ny, nz,
+ * wavelength or photon_energy, beam_intensity). Exits with status 1 on a
+ * syntax error and with status 0 for input_type "pdb" without pdb_filename,
+ * as the reference does; spsim_read_options_file reports the same conditions
+ * as a status instead. */
+void read_options_file(char *filename, Options *opt);
+
+/* include/config.h:187, src/config.c:428-642. Writes the options as a
+ * libconfig-format file (the spsim CLI writes spsim.confout). */
+void write_options_file(char *filename, Options *res);
+
+/* No counterpart in the reference: read_options_file for callers that cannot
+ * exit. Returns 0, -1 (unreadable file, syntax error or a path longer than
+ * 1023 characters) or -2 (input_type "pdb" without pdb_filename); err, when
+ * not NULL, receives "<file>:<line>: <message>". */
+int spsim_read_options_file(const char *filename, Options *opt, char *err, int errlen);
+
+/* No counterpart in the reference: write_options_file with a status
+ * (0, or -1 when the file cannot be written). */
+int spsim_write_options_file(const char *filename, const Options *opt);
+
 #ifdef __cplusplus
 }
 #endif

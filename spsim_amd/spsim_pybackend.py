@@ -209,6 +209,10 @@ def wrap(ptr):
 def _arg(x):
     if isinstance(x, Proxy):
         return x._ptr
+    if isinstance(x, str):
+        return x.encode()
+    if hasattr(x, "__fspath__"):
+        return str(x).encode()
     return x
 
 
@@ -217,6 +221,8 @@ _P = C.POINTER
 _SIGS = {
     # config.h
     "set_defaults": (_P(Options), []),
+    "read_options_file": (None, [C.c_char_p, _P(Options)]),
+    "write_options_file": (None, [C.c_char_p, _P(Options)]),
     # molecule.h
     "alloc_mol": (_P(Molecule), []),
     "add_atom_to_mol": (None, [_P(Molecule), C.c_int, C.c_float, C.c_float, C.c_float]),

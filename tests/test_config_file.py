@@ -1,0 +1,341 @@
+"""spsim.conf reader and writer (read_options_file / write_options_file,
+reference src/config.c:89-642), on the CPU through the C library.
+
+What pins it: the libconfig grammar (comments, separators, string
+concatenation and escapes, hex and 64-bit integers, booleans, groups, arrays,
+lists, @include), the reference's key-to-field table and enum strings
+(src/config.c:105-386), the derived fields of src/config.c:386-421 recomputed
+here in float32 arithmetic, and write -> read round trips of every field the
+writer emits. When the reference tree is present its example configuration
+(examples/spsim.conf) is read as well.
+"""
+import ctypes as C
+import math
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from spsim_amd import _lib
+from spsim_amd import spsim_pybackend as sp
+
+f32 = np.float32
+REF_EXAMPLE = Path("/root/reference/examples/spsim.conf")
+
+
+def read(tmp_path, text, opts=None, name="in.conf"):
+    p = tmp_path / name
+    p.write_text(text)
+    opts = opts or sp.set_defaults()
+    err = C.create_string_buffer(640)
+    rc = _lib.load().spsim_read_options_file(str(p).encode(), opts._ptr, err, len(err))
+    return rc, opts, err.value.decode()
+
+
+@pytest.fixture(autouse=True)
+def _sigs():
+    L = _lib.load()
+    L.spsim_read_options_file.restype = C.c_int
+    L.spsim_read_options_file.argtypes = [C.c_char_p, C.c_void_p, C.c_char_p, C.c_int]
+    L.spsim_write_options_file.restype = C.c_int
+    L.spsim_write_options_file.argtypes = [C.c_char_p, C.c_void_p]
+
+
+EXAMPLE = """
+# an example in the style of the spsim CLI input
+number_of_dimensions = 2;
+input_type = "chemical_formula";
+chemical_formula = {
+  atom6 = 3100;   // carbon
+  atom7 = 850;
+  atom8 = 1000;
+  atom16 = 50;
+  atom1 = 0;      /* a zero quantity is skipped */
+};
+detector_distance = 1.000000e-02;
+detector_width = 2.680000e-02;
+detector_height = 2.600000e-02;
+detector_pixel_width = 4.000000e-05;
+detector_pixel_height = 4.000000e-05;
+detector_quantum_efficiency = 1.500000e-01;
+detector_electron_hole_production_energy = 5.800000e-19;
+detector_readout_noise = 1.000000e+01;
+detector_dark_current = 1.000000e-01;
+detector_linear_full_well = 2.000000e+05;
+detector_binning = 4;
+detector_binning_y = 2;
+detector_maximum_value = 65535.0;
+experiment_wavelength = 1.00000e-10;
+experiment_exposure_time = 1.000000e-13;
+experiment_beam_intensity = 2.102500000e+30;
+experiment_polarization = "unpolarized";
+box_type = "parallelepipedic";
+random_orientation = 1;
+number_of_patterns = 7;
+random_seed = 1234;
+b_factor = 12.5;
+"""
+
+
+def test_example_fields(tmp_path):
+    rc, o, err = read(tmp_path, EXAMPLE)
+    assert rc == 0, err
+    d, e = o.detector, o.experiment
+    assert o.input_type == sp.CHEM_FORMULA
+    cf = o.chem_formula
+    assert cf.n_elements == 4
+    assert [cf.atomic_number[i] for i in range(4)] == [6, 7, 8, 16]
+    assert [cf.quantity[i] for i in range(4)] == [3100, 850, 1000, 50]
+    assert d.distance == f32(1e-2) and d.width == f32(2.68e-2) and d.height == f32(2.6e-2)
+    assert d.pixel_width == f32(4e-5) and d.quantum_efficiency == f32(0.15)
+    assert d.electron_hole_production_energy == f32(5.8e-19)
+    assert d.readout_noise == f32(10) and d.dark_current == f32(0.1) and d.linear_full_well == f32(2e5)
+    # detector_binning sets x and y (z = 1) before the per-axis keys apply
+    assert (d.binning_x, d.binning_y, d.binning_z) == (4, 2, 1)
+    assert d.maximum_value == f32(65535)
+    assert e.wavelength == f32(1e-10) and e.exposure_time == f32(1e-13) and e.beam_intensity == f32(2.1025e30)
+    assert e.polarization == sp.POLARIZATION_UNPOLARIZED
+    assert o.box_type == sp.BOX_PARALLEL
+    assert (o.random_orientation, o.n_patterns, o.random_seed) == (1, 7, 1234)
+    assert o.b_factor == f32(12.5)
+    # untouched keys keep set_defaults' values
+    assert o.vectorize == 1 and o.wavelength_samples == 5 and o.box_dimension == f32(1e-9)
+    # derived fields: rint of the float32 quotient; photon energy from the wavelength
+    assert d.nx == int(np.rint(f32(2.68e-2) / f32(4e-5))) == 670
+    assert d.ny == int(np.rint(f32(2.6e-2) / f32(4e-5))) == 650
+    assert d.nz == 1 and d.binning_z == 1
+    assert e.photon_energy == f32(1.240e-6 / float(f32(1e-10)))
+
+
+def test_photon_energy_gives_wavelength_and_beam_energy_gives_intensity(tmp_path):
+    rc, o, err = read(tmp_path, """
+        experiment_photon_energy = 1500.0;
+        experiment_beam_energy = 2e-3;
+        experiment_focal_diameter = 1e-7;
+        detector_width = 1e-2; detector_height = 1e-2;
+        detector_pixel_width = 1e-4; detector_pixel_height = 1e-4;
+    """)
+    assert rc == 0, err
+    e = o.experiment
+    lam = f32(1.240e-6 / float(f32(1500)))
+    assert e.wavelength == lam
+    eV = f32(1.240e-6 / float(lam))
+    nph = f32(float(f32(2e-3)) * 6.24150974e18 / float(eV))
+    area = f32(f32(1e-7) * f32(1e-7))
+    assert e.beam_intensity == f32(nph / area)
+    # closed form: E / (h c / lambda) / d^2, to float rounding
+    assert math.isclose(e.beam_intensity, 2e-3 / (1.602176634e-19 * 1500) / 1e-14, rel_tol=2e-3)
+    # an explicit beam intensity is kept
+    rc, o, _ = read(tmp_path, "experiment_photon_energy = 1500.0; experiment_beam_energy = 2e-3;"
+                              "experiment_focal_diameter = 1e-7; experiment_beam_intensity = 5.0;", name="b.conf")
+    assert rc == 0 and o.experiment.beam_intensity == f32(5)
+
+
+def test_three_d_detector_forces_polarization_ignore(tmp_path):
+    rc, o, err = read(tmp_path, """
+        detector_width = 1e-3; detector_height = 1e-3; detector_depth = 1e-3;
+        detector_pixel_width = 1e-4; detector_pixel_height = 1e-4; detector_pixel_depth = 2.5e-4;
+        experiment_wavelength = 1e-9; experiment_polarization = "vertical";
+    """)
+    assert rc == 0, err
+    assert (o.detector.nx, o.detector.ny, o.detector.nz) == (10, 10, 4)
+    assert o.experiment.polarization == sp.POLARIZATION_IGNORE
+
+
+@pytest.mark.parametrize("name,value", [("ignore", 0), ("vertical", 1), ("horizontal", 2), ("unpolarized", 3)])
+def test_polarization_names(tmp_path, name, value):
+    rc, o, _ = read(tmp_path, f'experiment_wavelength = 1e-9; experiment_polarization = "{name}";')
+    assert rc == 0 and o.experiment.polarization == value
+
+
+def test_unknown_enum_strings_leave_fields(tmp_path):
+    o = sp.set_defaults()
+    o.experiment.polarization = 2
+    rc, o, _ = read(tmp_path, 'experiment_wavelength = 1e-9; experiment_polarization = "circular";'
+                              'box_type = "cubic"; input_type = "none";', opts=o)
+    assert rc == 0
+    assert o.experiment.polarization == 2 and o.box_type == sp.BOX_SPHERICAL and o.input_type == 0
+
+
+def test_grammar(tmp_path):
+    text = r"""
+    /* block
+       comment */
+    experiment_wavelength : 2.5e-10   # colon separator, no semicolon
+    pdb_filename = "dir/" "mol\x41.pdb" ;   // adjacent literals, hex escape
+    input_type = "pdb",
+    precalculated_sf = "a\"b\\c";
+    hkl_grid = "grid.h5";
+    random_seed = 0x10;
+    number_of_patterns = 5L;
+    use_cuda = true;
+    vectorize = FALSE;
+    detector_distance = 1;              // integer for a float key
+    number_of_patterns_unused = 12345678901L;
+    fast_exit = 2.9;                    // float for an integer key
+    unused_array = [1, 2, 3];
+    unused_list = ( "a", 1.5, { x = 1; }, [ ] );
+    unused_group = { nested = { deeper = "x"; }; };
+    b_factor = -.5;
+    """
+    rc, o, err = read(tmp_path, text)
+    assert rc == 0, err
+    assert o.experiment.wavelength == f32(2.5e-10)
+    assert o.input_type == sp.PDB
+    assert bytes(o._ptr.contents.pdb_filename).rstrip(b"\0") == b"dir/molA.pdb"
+    assert bytes(o._ptr.contents.sf_filename).rstrip(b"\0") == b'a"b\\c'
+    assert bytes(o._ptr.contents.hkl_grid_filename).rstrip(b"\0") == b"grid.h5"
+    assert o.random_seed == 16 and o.n_patterns == 5
+    assert o.use_cuda == 1 and o.vectorize == 0
+    assert o.detector.distance == f32(1.0)
+    assert o.fast_exit == 2
+    assert o.b_factor == f32(-0.5)
+
+
+def test_include(tmp_path):
+    (tmp_path / "det.inc").write_text("detector_distance = 0.25;\ndetector_binning = 3;\n")
+    import os
+
+    cwd = os.getcwd()
+    os.chdir(tmp_path)
+    try:
+        rc, o, err = read(tmp_path, 'experiment_wavelength = 1e-9;\n@include "det.inc"\nrandom_seed = 9;')
+    finally:
+        os.chdir(cwd)
+    assert rc == 0, err
+    assert o.detector.distance == f32(0.25) and o.detector.binning_x == 3 and o.random_seed == 9
+
+
+@pytest.mark.parametrize("text,line", [
+    ("a = 1;\nb = ;\n", 2),
+    ("a = 1;\n\n c = \"open\n", 3),
+    ("a = 1;\nb = 2;\na = 3;\n", 3),
+    ("x = { y = 1;\n", 2),
+    ("/* never closed\n\n", 3),
+    ("k = [1, {a=1;}];", 1),
+    ("k = 12abc;", 1),
+    ("= 3;", 1),
+    ("k = \"bad \\q escape\";", 1),
+])
+def test_syntax_errors_report_the_line(tmp_path, text, line):
+    rc, _, err = read(tmp_path, text)
+    assert rc == -1
+    assert f"in.conf:{line}:" in err, err
+
+
+def test_pdb_without_filename(tmp_path):
+    rc, o, err = read(tmp_path, 'input_type = "pdb";')
+    assert rc == -2 and "pdb_filename" in err
+
+
+def test_missing_file(tmp_path):
+    err = C.create_string_buffer(256)
+    rc = _lib.load().spsim_read_options_file(str(tmp_path / "nope.conf").encode(), sp.set_defaults()._ptr, err, 256)
+    assert rc == -1 and b"file I/O error" in err.value
+
+
+def test_round_trip_every_written_field(tmp_path):
+    rng = np.random.default_rng(5)
+    o = sp.set_defaults()
+    d, e = o.detector, o.experiment
+    float_fields = {
+        d: ["distance", "width", "height", "depth", "center_x", "center_y", "center_z", "pixel_width",
+            "pixel_height", "pixel_depth", "quantum_efficiency", "electron_hole_production_energy",
+            "readout_noise", "dark_current", "linear_full_well", "maximum_value", "gaussian_blurring",
+            "real_space_blurring"],
+        e: ["wavelength", "bandwidth", "exposure_time", "beam_intensity", "beam_center_x", "beam_center_y",
+            "beam_fwhm", "beam_energy", "photon_energy", "focal_diameter"],
+        o: ["box_dimension", "b_factor"],
+    }
+    int_fields = {
+        d: ["binning_x", "binning_y", "binning_z", "spherical"],
+        o: ["use_fft_for_sf", "use_nfft_for_sf", "origin_to_com", "random_orientation", "n_patterns", "vectorize",
+            "delta_atoms", "fast_exit", "use_cuda", "wavelength_samples", "random_seed", "output_photons",
+            "output_noiseless_photons", "output_count", "output_noiseless_count", "output_solid_angles",
+            "output_scattering_factors", "output_real_space", "verbosity_level"],
+    }
+    want = {}
+    for obj, names in float_fields.items():
+        for n in names:
+            v = f32(rng.uniform(0.1, 10) * 10.0 ** rng.integers(-20, 20))
+            setattr(obj, n, float(v))
+            want[(id(obj), n)] = v
+    # consistent wavelength / photon energy so the derived-field check stays quiet
+    e.wavelength = float(f32(1e-10))
+    e.photon_energy = float(f32(1.240e-6 / float(f32(1e-10))))
+    want[(id(e), "wavelength")] = f32(e.wavelength)
+    want[(id(e), "photon_energy")] = f32(e.photon_energy)
+    d.pixel_depth = 0.0
+    want[(id(d), "pixel_depth")] = f32(0)
+    for obj, names in int_fields.items():
+        for n in names:
+            v = int(rng.integers(-1000, 1000))
+            setattr(obj, n, v)
+            want[(id(obj), n)] = v
+    d.binning_z = 1
+    want[(id(d), "binning_z")] = 1
+    o.euler_orientation = np.array([0.1, -2.5, 3.0], np.float32)
+    o.crystal_size = np.array([2, 3, 4], np.int32)
+    o.crystal_cell = np.array([1e-9, 2e-9, 3e-9, 80, 90, 100], np.float32)
+    o.input_type = sp.CHEM_FORMULA
+    Zs = (C.c_int * 3)(1, 6, 79)
+    Qs = (C.c_int * 3)(10, 20, 30)
+    o.chem_formula.n_elements = 3
+    o.chem_formula._ptr.contents.atomic_number = C.cast(Zs, C.POINTER(C.c_int))
+    o.chem_formula._ptr.contents.quantity = C.cast(Qs, C.POINTER(C.c_int))
+    e.polarization = sp.POLARIZATION_HORIZONTAL
+    o.box_type = sp.BOX_PARALLEL
+    o.sf_filename = "sf \"file\".h5"
+    path = tmp_path / "out.conf"
+    assert _lib.load().spsim_write_options_file(str(path).encode(), o._ptr) == 0
+    o.chem_formula._ptr.contents.atomic_number = None
+    o.chem_formula._ptr.contents.quantity = None
+    o.chem_formula.n_elements = 0
+
+    rc, r, err = read(tmp_path, path.read_text(), name="back.conf")
+    assert rc == 0, err
+    objs = {id(d): r.detector, id(e): r.experiment, id(o): r}
+    for (oid, n), v in want.items():
+        got = getattr(objs[oid], n)
+        if isinstance(v, np.floating):
+            assert f32(got) == v, (n, got, v)
+        else:
+            assert got == v, (n, got, v)
+    assert np.array_equal(r.euler_orientation, o.euler_orientation)
+    assert np.array_equal(r.crystal_size, o.crystal_size)
+    assert np.array_equal(r.crystal_cell, o.crystal_cell)
+    assert r.input_type == sp.CHEM_FORMULA and r.chem_formula.n_elements == 3
+    assert [r.chem_formula.atomic_number[i] for i in range(3)] == [1, 6, 79]
+    assert [r.chem_formula.quantity[i] for i in range(3)] == [10, 20, 30]
+    assert r.experiment.polarization == sp.POLARIZATION_HORIZONTAL and r.box_type == sp.BOX_PARALLEL
+    assert bytes(r._ptr.contents.sf_filename).rstrip(b"\0") == b'sf "file".h5'
+    text = path.read_text()
+    # the writer's key order follows the reference's writer
+    keys = [l.split("=")[0].strip() for l in text.splitlines() if "=" in l and not l.startswith(" ")]
+    assert keys[:4] == ["input_type", "chemical_formula", "box_type", "detector_distance"]
+    assert keys.index("experiment_polarization") == keys.index("use_cuda") - 1
+    assert keys[-1] == "precalculated_sf"
+
+
+def test_pybackend_names(tmp_path):
+    p = tmp_path / "c.conf"
+    p.write_text('experiment_wavelength = 3e-10; detector_width = 1e-2; detector_pixel_width = 1e-4;'
+                 'detector_height = 1e-2; detector_pixel_height = 1e-4;')
+    o = sp.set_defaults()
+    sp.read_options_file(str(p), o)
+    assert o.detector.nx == 100 and o.experiment.wavelength == f32(3e-10)
+    sp.write_options_file(str(tmp_path / "c.out"), o)
+    assert "experiment_wavelength = %.8e;" % float(f32(3e-10)) in (tmp_path / "c.out").read_text()
+
+
+@pytest.mark.skipif(not REF_EXAMPLE.exists(), reason="reference tree not mounted")
+def test_reference_example_file(tmp_path):
+    o = sp.set_defaults()
+    err = C.create_string_buffer(640)
+    rc = _lib.load().spsim_read_options_file(str(REF_EXAMPLE).encode(), o._ptr, err, 640)
+    assert rc == 0, err.value
+    assert o.input_type == sp.PDB
+    assert bytes(o._ptr.contents.pdb_filename).rstrip(b"\0") == b"1aon.pdb"
+    assert (o.detector.nx, o.detector.ny) == (670, 650)
+    assert o.crystal_size.tolist() == [2, 2, 2]
+    assert o.experiment.beam_intensity == f32(2.1025e30)
