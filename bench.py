@@ -48,6 +48,9 @@ def parse():
     p.add_argument("--no-batched", action="store_true", help="skip the batched vs plain measurement")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--batched-patterns", type=int, default=512)
+    p.add_argument("--pipeline-patterns", type=int, default=256)
+    p.add_argument("--no-pipeline", action="store_true", help="skip the full-pipeline measurement")
+    p.add_argument("--no-adjoint", action="store_true", help="skip the differentiation measurement")
     p.add_argument("--profile", action="store_true", help="headline workload only (for rocprofv3)")
     p.add_argument("--kernel", choices=("auto", "pattern", "grid"), default="auto",
                    help="restrict the headline sweep to one structure-factor kernel")
@@ -206,6 +209,132 @@ def batched(args, torch, dv, wl, _lib):
         L.spsim_hip_batch_destroy(plan)
 
 
+def pipeline(args, torch, dv, wl, _lib):
+    """Full pipeline, batched: 256 orientations of one 50,000-atom molecule on a
+    1024 x 1024 detector (20 um pixels at 0.05 m, lambda 1.3 nm), then the fused
+    Thomson / solid angle / Poisson / readout chain with 4 x 4 binning."""
+    dev = torch.device("cuda", 0)
+    det, Z, pos, rots = wl.config5(args.pipeline_patterns)
+    n, npat = det.n, len(rots)
+    q = dv.hkl_detector(n, n, det.width, det.width, det.distance, det.wavelength, device=dev).cpu().numpy()
+    offs = np.array([0, len(Z)], np.int32)
+    pm = np.zeros(npat, np.int32)
+    R = np.ascontiguousarray(rots.reshape(-1, 9), np.float64)
+    L = _lib.load()
+    plan = C.c_void_p()
+    vp = lambda a: a.ctypes.data_as(C.c_void_p)
+    _lib.check(L.spsim_hip_batch_create(vp(q), n, n, 1, vp(offs), vp(np.ascontiguousarray(Z, np.int32)),
+                                        vp(np.ascontiguousarray(pos, np.float32)), None, None, 0, npat, vp(pm), vp(R),
+                                        None, C.c_float(0.0), None, C.byref(plan)), "spsim_hip_batch_create")
+    # detector of the full-pipeline configuration; e-h energy and exposure from the reference README
+    detp = dv.detector_params(nx=n, ny=n, bin_x=4, bin_y=4, width=det.width, height=det.width, distance=det.distance,
+                              pixel_width=det.pixel, pixel_height=det.pixel, beam_intensity=1e25,
+                              quantum_efficiency=0.15, wavelength=det.wavelength,
+                              electron_hole_production_energy=5.8e-19, dark_current=0.1, exposure_time=1e-13,
+                              readout_noise=10.0, linear_full_well=2e5, maximum_value=65535.0)
+    stream = torch.cuda.Stream()
+    sp = C.c_void_p(stream.cuda_stream)
+    try:
+        best = _lib.Tile()
+        _lib.check(L.spsim_hip_batch_autotune(plan, sp, C.byref(best), None), "autotune")
+        terms = C.c_double(0)
+        L.spsim_hip_batch_terms(plan, C.byref(terms))
+        I = torch.empty((npat, n * n), dtype=torch.float32, device=dev)
+        nb = (n // 4) * (n // 4)
+        real = torch.empty((npat, nb), dtype=torch.float32, device=dev)
+        noiseless = torch.empty((npat, nb), dtype=torch.float32, device=dev)
+
+        def chain():
+            _lib.check(L.spsim_hip_detector_chain(C.byref(detp), C.c_void_p(I.data_ptr()), npat, 5, 0, None, None,
+                                                  None, None, None, C.c_void_p(real.data_ptr()),
+                                                  C.c_void_p(noiseless.data_ptr()), sp), "detector_chain")
+
+        with torch.cuda.stream(stream):
+            _lib.check(L.spsim_hip_batch_run(plan, None, C.c_void_p(I.data_ptr()), 1, sp), "batch_run")
+            chain()
+            stream.synchronize()
+            ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+            ev[0].record(stream)
+            _lib.check(L.spsim_hip_batch_run(plan, None, C.c_void_p(I.data_ptr()), 1, sp), "batch_run")
+            ev[1].record(stream)
+            chain()
+            ev[2].record(stream)
+            ev[2].synchronize()
+            ms_sf, ms_chain = ev[0].elapsed_time(ev[1]), ev[1].elapsed_time(ev[2])
+            t0 = time.perf_counter()
+            _lib.check(L.spsim_hip_batch_run_plain(plan, None, C.c_void_p(I.data_ptr()), sp), "plain")
+            stream.synchronize()
+            ms_plain = (time.perf_counter() - t0) * 1e3
+        chain_bytes = npat * (n * n * 4 + 2 * nb * 4)
+        return {
+            "workload": f"{npat} orientations of one 50000-atom C/N/O/S molecule (radius 10 nm), 1024x1024 detector "
+                        f"20 um at 0.05 m, lambda 1.3 nm, 4x4 binning, QE 0.15, readout 10 e-, dark 0.1, "
+                        f"full well 2e5, max 65535, beam 1e25 photons/m^2 (e-h 5.8e-19 J, exposure 1e-13 s assumed)",
+            "patterns_per_s": round(npat / ((ms_sf + ms_chain) * 1e-3), 2),
+            "structure_factor_ms": round(ms_sf, 2), "detector_chain_ms": round(ms_chain, 3),
+            "gterms_per_s": round(terms.value / (ms_sf * 1e-3) / 1e9, 1),
+            "roofline_frac_structure_factor": round(terms.value * W_EQ_PLAIN / (ms_sf * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, 4),
+            "detector_chain_gbs": round(chain_bytes / (ms_chain * 1e-3) / 1e9, 1),
+            "detector_chain_hbm_frac": round(chain_bytes / (ms_chain * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "plain_structure_factor_ms": round(ms_plain, 2),
+            "speedup_batched_vs_plain": round(ms_plain / ms_sf, 3),
+            "kernel": "sf_batch_grid_kernel" if L.spsim_hip_batch_uses_grid(plan) else "sf_batch_kernel",
+            "tile": {"per_lane": best.pixels_per_lane, "chunk_atoms": best.chunk_atoms},
+        }
+    finally:
+        L.spsim_hip_batch_destroy(plan)
+
+
+def differentiation(args, torch, dv, wl, _lib):
+    """Differentiation configuration: 20,000 atoms (radius 7 nm), per-atom B =
+    20 A^2, 512 x 512 detector; Poisson NLL against counts drawn at 1e3 mean
+    photons/pixel for the true coordinates, evaluated at coordinates perturbed by
+    N(0, 0.05 nm). Times forward (I + loss) and backward (adjoint) passes."""
+    from spsim_amd import torch_ops as to
+
+    dev = torch.device("cuda", 0)
+    det, Z, pos = wl.config4()
+    rng = np.random.default_rng(40)
+    B = np.full(len(Z), 20.0, np.float32)
+    hkl = dv.hkl_detector(det.n, det.n, det.width, det.width, det.distance, det.wavelength, device=dev)
+    layout = to.SpeciesLayout(Z, dev)
+    ff = dv.form_factor_table(device=dev)
+    with torch.no_grad():
+        I_true = to.intensity(torch.from_numpy(pos).to(dev), hkl, layout, ff, atom_b=torch.from_numpy(B).to(dev))
+    K = (1e3 / I_true.mean()).expand_as(I_true).contiguous()
+    y = torch.poisson((K * I_true).double()).float()
+    pert = torch.from_numpy((pos + rng.normal(0, 0.05e-9, pos.shape)).astype(np.float32)).to(dev)
+    Bt = torch.from_numpy(B).to(dev)
+
+    def step():
+        P = pert.clone().requires_grad_(True)
+        Bv = Bt.clone().requires_grad_(True)
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+        e[0].record()
+        loss = to.poisson_nll(to.intensity(P, hkl, layout, ff, atom_b=Bv), y, K)
+        e[1].record()
+        loss.backward()
+        e[2].record()
+        e[2].synchronize()
+        return e[0].elapsed_time(e[1]), e[1].elapsed_time(e[2]), P.grad, Bv.grad
+
+    step()
+    fw, bw = [], []
+    for _ in range(3):
+        f, b, gp, gb = step()
+        fw.append(f)
+        bw.append(b)
+    f, b = float(np.median(fw)), float(np.median(bw))
+    terms = float(det.npix) * len(Z)
+    return {
+        "workload": "20000-atom C/N/O/S molecule (radius 7 nm), per-atom B 20 A^2, 512x512 detector 75 um at 0.1 m, "
+                    "lambda 0.2 nm (pixel size, distance, wavelength assumed); Poisson NLL at 1e3 mean photons/pixel",
+        "forward_ms": round(f, 3), "backward_ms": round(b, 3), "backward_over_forward": round(b / f, 3),
+        "forward_gterms_per_s": round(terms / (f * 1e-3) / 1e9, 1),
+        "grad_finite": bool(torch.isfinite(gp).all().item() and torch.isfinite(gb).all().item()),
+    }
+
+
 def cpu_baseline(h):
     import oracle
 
@@ -271,6 +400,10 @@ def main():
     }
     if not args.profile and not args.no_batched:
         out["batched"] = batched(args, torch, dv, wl, _lib)
+    if not args.profile and not args.no_pipeline:
+        out["pipeline"] = pipeline(args, torch, dv, wl, _lib)
+    if not args.profile and not args.no_adjoint:
+        out["differentiation"] = differentiation(args, torch, dv, wl, _lib)
     if not args.profile and not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(h)
     print(json.dumps(out))

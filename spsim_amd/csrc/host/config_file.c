@@ -11,16 +11,16 @@
  * The Options fields are filled exactly as the reference does it (same keys,
  * same enum strings, `detector_binning` before the per-axis keys, the
  * chemical formula read from `chemical_formula/atomZ` for Z = 0..99 with zero
- * quantities skipped, then the derived fields of src/config.c:386-421), with
- * three deliberate differences, all about values the reference would read as
- * 0 or lose:
- *   - a float key given an integer value (`detector_distance = 1;`) reads
- *     the integer, and an integer key given a float reads it truncated
- *     (libconfig without auto-conversion returns 0 for both);
- *   - booleans read as 1 and 0 (libconfig's integer lookup returns 0);
- *   - the writer spells the real-space blurring key `real_space_blurring`,
- *     the key the reader looks up (the reference writes `real_sapce_blurring`,
- *     src/config.c:545, so the value was lost on a round trip).
+ * quantities skipped, then the derived fields of src/config.c:386-421).
+ * Values keep libconfig's typing without auto-conversion, as the reference's
+ * manual documents (doc/UserManual.tex:91-93): a float key written without a
+ * decimal point or exponent (`detector_readout_noise = 10;`) reads as 0, and
+ * an integer key given a float or a boolean reads as 0. Each such case is
+ * reported on stderr, since the silent 0 is a common mistake. One deliberate
+ * difference: the writer spells the real-space blurring key
+ * `real_space_blurring`, the key the reader looks up (the reference writes
+ * `real_sapce_blurring`, src/config.c:545, so the value was lost on a round
+ * trip).
  * use_cuda is kept as read: this library has no CPU path, so the reference's
  * "no CUDA device" fallback (src/config.c:335-356) does not apply.
  */
@@ -430,24 +430,22 @@ static int read_text(const char *filename, char **text, size_t *len) {
 
 /* ---------------------------------------------------- typed lookups */
 
-static double as_float(const cfg_node *n) {
-  switch (n->type) {
-    case CFG_FLOAT: return n->fval;
-    case CFG_INT:
-    case CFG_INT64:
-    case CFG_BOOL: return (double)n->ival;
-    default: return 0.0;
-  }
+/* libconfig's float lookup: only a float setting has a value. */
+static double as_float(const cfg_node *n, const char *key) {
+  if (n->type == CFG_FLOAT) return n->fval;
+  if (n->type == CFG_INT || n->type == CFG_INT64)
+    fprintf(stderr, "Warning: %s has no decimal point and reads as 0 (write %lld.0)\n", key, n->ival);
+  else
+    fprintf(stderr, "Warning: %s is not a float and reads as 0\n", key);
+  return 0.0;
 }
 
-static int as_int(const cfg_node *n) {
-  switch (n->type) {
-    case CFG_INT:
-    case CFG_BOOL: return (int)n->ival;
-    case CFG_INT64: return (n->ival > INT_MAX || n->ival < INT_MIN) ? 0 : (int)n->ival;
-    case CFG_FLOAT: return (n->fval > (double)INT_MAX || n->fval < (double)INT_MIN) ? 0 : (int)n->fval;
-    default: return 0;
-  }
+/* libconfig's int lookup: int settings, and 64-bit ones that fit in an int. */
+static int as_int(const cfg_node *n, const char *key) {
+  if (n->type == CFG_INT) return (int)n->ival;
+  if (n->type == CFG_INT64 && n->ival <= INT_MAX && n->ival >= INT_MIN) return (int)n->ival;
+  fprintf(stderr, "Warning: %s is not an integer (or does not fit in one) and reads as 0\n", key);
+  return 0;
 }
 
 static const char *as_string(const cfg_node *n) { return n && n->type == CFG_STRING ? n->sval : NULL; }
@@ -572,7 +570,7 @@ static int read_chem_formula(const cfg_node *root, Chem_Formula *cf) {
   for (int i = 0; i < 100; i++) {
     snprintf(path, sizeof path, "chemical_formula/atom%d", i);
     const cfg_node *n = lookup(root, path);
-    int q = n ? as_int(n) : 0;
+    int q = n ? as_int(n, path) : 0;
     if (q) {
       Z[count] = i;
       qty[count] = q;
@@ -678,17 +676,17 @@ static int apply_tree(const cfg_node *root, Options *res, char *err, int errlen)
     }
   }
   if ((n = lookup(root, "detector_binning"))) {
-    res->detector->binning_x = as_int(n);
-    res->detector->binning_y = as_int(n);
+    res->detector->binning_x = as_int(n, "detector_binning");
+    res->detector->binning_y = res->detector->binning_x;
     res->detector->binning_z = 1;
   }
   for (size_t i = 0; i < N_NUMERIC_KEYS; i++) {
     const option_key *k = &numeric_keys[i];
     if (!(n = lookup(root, k->key))) continue;
     if (k->kind == K_FLOAT)
-      *(float *)field_ptr(k, res) = (float)as_float(n);
+      *(float *)field_ptr(k, res) = (float)as_float(n, k->key);
     else
-      *(int *)field_ptr(k, res) = as_int(n);
+      *(int *)field_ptr(k, res) = as_int(n, k->key);
   }
   if ((s = as_string(lookup(root, "experiment_polarization")))) {
     int found = 0;

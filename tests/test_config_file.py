@@ -168,11 +168,7 @@ def test_grammar(tmp_path):
     hkl_grid = "grid.h5";
     random_seed = 0x10;
     number_of_patterns = 5L;
-    use_cuda = true;
-    vectorize = FALSE;
-    detector_distance = 1;              // integer for a float key
     number_of_patterns_unused = 12345678901L;
-    fast_exit = 2.9;                    // float for an integer key
     unused_array = [1, 2, 3];
     unused_list = ( "a", 1.5, { x = 1; }, [ ] );
     unused_group = { nested = { deeper = "x"; }; };
@@ -186,10 +182,39 @@ def test_grammar(tmp_path):
     assert bytes(o._ptr.contents.sf_filename).rstrip(b"\0") == b'a"b\\c'
     assert bytes(o._ptr.contents.hkl_grid_filename).rstrip(b"\0") == b"grid.h5"
     assert o.random_seed == 16 and o.n_patterns == 5
-    assert o.use_cuda == 1 and o.vectorize == 0
-    assert o.detector.distance == f32(1.0)
-    assert o.fast_exit == 2
     assert o.b_factor == f32(-0.5)
+
+
+def test_libconfig_typing(tmp_path, capfd):
+    """Without auto-conversion a float key needs a decimal point or an
+    exponent, and an integer key an integer (doc/UserManual.tex:91-93); the
+    other forms read as 0, with a warning."""
+    o = sp.set_defaults()
+    o.vectorize = 1
+    o.fast_exit = 1
+    o.use_cuda = 1
+    rc, o, err = read(tmp_path, """
+        detector_distance = 1;
+        detector_readout_noise = 10.;
+        detector_dark_current = 1e-1;
+        detector_linear_full_well = 200000L;
+        use_cuda = true;
+        vectorize = FALSE;
+        fast_exit = 2.9;
+        random_seed = 12345678901L;
+        number_of_patterns = 7L;
+        verbosity_level = "2";
+    """, opts=o)
+    assert rc == 0, err
+    assert o.detector.distance == 0 and o.detector.linear_full_well == 0
+    assert o.detector.readout_noise == f32(10) and o.detector.dark_current == f32(0.1)
+    assert (o.use_cuda, o.vectorize, o.fast_exit, o.random_seed, o.verbosity_level) == (0, 0, 0, 0, 0)
+    assert o.n_patterns == 7
+    warn = capfd.readouterr().err
+    for key in ("detector_distance", "detector_linear_full_well", "use_cuda", "vectorize", "fast_exit",
+                "random_seed", "verbosity_level"):
+        assert key in warn, key
+    assert "detector_readout_noise" not in warn and "number_of_patterns" not in warn
 
 
 def test_include(tmp_path):
