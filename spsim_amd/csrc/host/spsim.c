@@ -15,11 +15,21 @@
 #include "spsim.h"
 #include "spsim_internal.h"
 
+/* src/spsim.c:84-97: a random formula molecule (also written to molout.pdb
+ * in the working directory) or a PDB file; exits when input_type is neither. */
 Molecule *get_molecule(Options *opts) {
+  Molecule *mol = NULL;
   if (!opts) return NULL;
-  if (opts->input_type == CHEM_FORMULA) return get_Molecule_from_formula(opts->chem_formula, opts);
-  fprintf(stderr, "spsim: get_molecule: only input_type CHEM_FORMULA is supported\n");
-  return NULL;
+  if (opts->input_type == CHEM_FORMULA) {
+    mol = get_Molecule_from_formula(opts->chem_formula, opts);
+    if (mol) write_pdb_from_mol("molout.pdb", mol);
+  } else if (opts->input_type == PDB) {
+    mol = get_Molecule_from_pdb(opts->pdb_filename);
+  } else {
+    fprintf(stderr, "input_type not specified!\n");
+    exit(0);
+  }
+  return mol;
 }
 
 float *get_HKL_list(Options *opts, int *HKL_list_size) {

@@ -230,6 +230,8 @@ _SIGS = {
     "make_mol": (_P(Molecule), [_P(Image), _P(Image)]),
     "origin_to_center_of_mass": (None, [_P(Molecule)]),
     "get_Molecule_from_formula": (_P(Molecule), [_P(Chem_Formula), _P(Options)]),
+    "get_Molecule_from_pdb": (_P(Molecule), [C.c_char_p]),
+    "write_pdb_from_mol": (None, [C.c_char_p, _P(Molecule)]),
     # diffraction.h
     "get_HKL_list_for_detector": (_P(C.c_float), [_P(CCD), _P(Experiment), _P(C.c_int)]),
     "compute_pattern_on_list": (_P(Diffraction_Pattern), [_P(Molecule), _P(C.c_float), C.c_int, C.c_float,
