@@ -29,6 +29,13 @@ def lib() -> C.CDLL:
             from . import build as _b
 
             _b.build()
+        # PyTorch bundles its own OpenMP runtime under the SONAME libgomp.so.1;
+        # importing it first lets this library share it instead of loading
+        # the system copy beside it.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(str(path))
         L.oracle_form_factor.restype = C.c_double
         L.oracle_form_factor.argtypes = [C.c_double, C.c_int, C.c_double, _f32p]

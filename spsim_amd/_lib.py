@@ -88,6 +88,16 @@ def load() -> C.CDLL:
     if _LIB is None:
         if not LIB_PATH.exists():
             raise RuntimeError(f"{LIB_PATH} is not built; run `python -m spsim_amd.build`")
+        # One HIP runtime per process. The library needs libamdhip64.so.7;
+        # PyTorch ships its own copy and asks for it by another name
+        # (libamdhip64.so), so if this library were loaded first and torch
+        # afterwards, the process would hold two HIP and two HSA runtimes,
+        # and device pointers and streams would cross between them. Loading
+        # torch first makes this library bind to torch's runtime.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(str(LIB_PATH))
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name, None)

@@ -4,6 +4,7 @@ from pathlib import Path
 
 import numpy as np
 import pytest
+import torch  # noqa: F401  (first, so that every native library binds to torch's HIP and OpenMP runtimes)
 
 ROOT = Path(__file__).resolve().parents[1]
 if str(ROOT) not in sys.path:
