@@ -57,6 +57,10 @@ def parse():
     return p.parse_args()
 
 
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def rel_l2(a, b):
     a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
@@ -374,7 +378,9 @@ def main():
     from spsim_amd import workloads as wl
 
     _lib.load()
+    log("headline: large single pattern (tile sweep, then timed steps)")
     h = headline(args, torch, dv, wl, _lib)
+    log(f"headline done: {h['value']:.1f} G terms/s, kernel {h['kernel_ms']:.3f} ms")
     out = {
         "metric": "G pixel-atom terms/s + patterns/s, batched vs plain, at <=1e-5 rel err vs FP64",
         "value": round(h["value"], 2),
@@ -399,13 +405,18 @@ def main():
         "rel_err_vs_fp64": h["rel_err_vs_fp64"],
     }
     if not args.profile and not args.no_batched:
+        log("batched: many molecules, autotune, graph vs plain")
         out["batched"] = batched(args, torch, dv, wl, _lib)
     if not args.profile and not args.no_pipeline:
+        log("pipeline: batched orientations + fused detector chain")
         out["pipeline"] = pipeline(args, torch, dv, wl, _lib)
     if not args.profile and not args.no_adjoint:
+        log("differentiation: forward and adjoint")
         out["differentiation"] = differentiation(args, torch, dv, wl, _lib)
     if not args.profile and not args.no_cpu:
+        log("cpu baseline: FP64 oracle on a pixel sample")
         out["cpu_baseline"] = cpu_baseline(h)
+    log("done")
     print(json.dumps(out))
 
 

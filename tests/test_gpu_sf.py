@@ -193,10 +193,8 @@ def test_sf_grid_mirror_kernel_vs_f64(gpu, nx, ny):
     Z, pos, B = wl.random_molecule(900, 4e-9, rng, atom_b_range=(10.0, 40.0))
     hkl = oracle.hkl_detector(nx, ny, np.float32(nx * 3e-4).item(), np.float32(ny * 3e-4).item(), 0.05, 0.0, 0.0, 0,
                               0.15e-9)
-    F_ref, I_ref = oracle.sf_f64(hkl, Z, pos, atom_b=B, B=3.0, threads=oracle_threads())
     for has_b in (False, True):
-        if not has_b:
-            F_ref, I_ref = oracle.sf_f64(hkl, Z, pos, B=3.0, threads=oracle_threads())
+        F_ref, I_ref = oracle.sf_f64(hkl, Z, pos, atom_b=B if has_b else None, B=3.0, threads=oracle_threads())
         for tile in (None, dv.make_tile(1, 64, 1), dv.make_tile(2, 256, 3)):
             F, I = _gpu_grid(gpu, hkl, nx, ny, Z, pos, atom_b=B if has_b else None, B=3.0, tile=tile)
             assert rel_l2(I, I_ref) < TOL_F64, (has_b, tile)
