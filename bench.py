@@ -38,6 +38,7 @@ W_EQ_PLAIN = 26.0
 W_EQ_DW = 36.0
 PEAK_FP32_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
+COPY_CAL_BYTES = 1 << 29  # 512 MiB
 
 
 def parse():
@@ -378,6 +379,15 @@ def main():
     from spsim_amd import workloads as wl
 
     _lib.load()
+    if args.profile:
+        # A plain copy of known size, for scaling the HBM byte counters
+        # (FETCH_SIZE / WRITE_SIZE) read in the same profiled run.
+        src = torch.ones(COPY_CAL_BYTES // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy calibration: {COPY_CAL_BYTES} bytes read and written")
+        del src, dst
     log("headline: large single pattern (tile sweep, then timed steps)")
     h = headline(args, torch, dv, wl, _lib)
     log(f"headline done: {h['value']:.1f} G terms/s, kernel {h['kernel_ms']:.3f} ms")
