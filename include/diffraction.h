@@ -40,6 +40,18 @@ Diffraction_Pattern *compute_pattern_on_list(Molecule *mol, float *HKL_list, int
 Diffraction_Pattern *vector_compute_pattern_on_list(Molecule *mol, float *HKL_list, int HKL_list_size,
                                                     float B, Experiment *exp, Options *opts);
 
+/* No counterpart in the reference's CPU API: the wavelength-spectrum branch of
+   its GPU path (cuda_compute_pattern_on_list2 with
+   CUDA_spectrum_scattering_from_all_atoms, src/diffraction.cu:300-318, 416-476),
+   which compute_sf takes when use_cuda is set, exp->bandwidth != 0 and
+   opts->wavelength_samples > 1. F_i = sum_j f_Zj(|q_i|) sqrt(illum_j)
+   sum_k W_k exp(-2 pi i (lambda/lambda_k) q_i.r_j) over the samples of
+   spsim_hip_spectrum_samples. As in the reference, an even
+   opts->wavelength_samples is raised by one in place and delta_atoms is not
+   applied. */
+Diffraction_Pattern *spsim_spectrum_pattern_on_list(Molecule *mol, float *HKL_list, int HKL_list_size, float B,
+                                                    Experiment *exp, Options *opts);
+
 /* Reference src/diffraction.c:1146-1187 */
 void calculate_pixel_solid_angle(CCD *det);
 /* Reference src/diffraction.c:1109-1144 */

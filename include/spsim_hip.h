@@ -125,6 +125,33 @@ int spsim_hip_sf_grid(const float *d_hkl, int nx, int ny, const spsim_hip_atom *
                       const spsim_hip_segment *d_segs, int nsegs, int natoms, const float *d_ff, float b_global,
                       const double *rot9, float *d_F, float *d_I, const spsim_hip_tile *tile, void *stream);
 
+/* ---- Wavelength spectrum (replaces the spectrum branch of the reference's GPU
+   path, cuda_compute_pattern_on_list2 src/diffraction.cu:300-318 with the kernel
+   CUDA_spectrum_scattering_from_all_atoms :416-476) ---- */
+
+#define SPSIM_HIP_MAX_SPECTRUM 65 /* wavelength samples per pattern */
+
+/* Samples of a Gaussian spectrum of relative width `bandwidth` (the full
+   range lambda (1 +- bandwidth/2) is +-2 sigma), as the reference draws them
+   (src/diffraction.cu:302-303, 431-460): an even count above 1 is raised by
+   one; lambda_k = lambda (1 - bandwidth/2) + k/(S-1) lambda bandwidth; weights
+   exp(-x^2/2), x = (lambda_k - lambda)/sigma, normalised to sum 1; phase scales
+   s_k = lambda / lambda_k. bandwidth == 0 or nsamples <= 1 gives one sample
+   {1, 1}. Writes at most SPSIM_HIP_MAX_SPECTRUM entries of scales and weights
+   and returns their count, or SPSIM_HIP_ERR_INVALID when more are needed. */
+int spsim_hip_spectrum_samples(double wavelength, double bandwidth, int nsamples, double *scales, double *weights);
+
+/* One pattern under a spectrum: F[i] = sum_j f_s(|q'_i|) w_j e^{-B_j s^2}
+   sum_k weights[k] exp(-2 pi i scales[k] q'_i.r_j), I = |F|^2 (an amplitude
+   average over wavelengths with the form factor at the unscaled |q|, as the
+   reference's spectrum kernel computes it). scales and weights are host arrays
+   of nsamples <= SPSIM_HIP_MAX_SPECTRUM entries. Other arguments as
+   spsim_hip_sf_pattern; tile->pixels_per_lane is 1 or 2. */
+int spsim_hip_sf_spectrum(const float *d_hkl, int npix, const spsim_hip_atom *d_atoms, const float *d_atom_b,
+                          const spsim_hip_segment *d_segs, int nsegs, int natoms, const float *d_ff,
+                          float b_global, const double *rot9, int nsamples, const double *scales,
+                          const double *weights, float *d_F, float *d_I, const spsim_hip_tile *tile, void *stream);
+
 /* Free the library's cached device workspace and graphs on the current device. */
 int spsim_hip_release(void);
 

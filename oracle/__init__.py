@@ -44,6 +44,11 @@ def lib() -> C.CDLL:
         L.oracle_rotate_hkl.argtypes = [_f32p, C.c_int, _f64p, _f32p]
         L.oracle_sf_f64.argtypes = [_f32p, C.c_int, _i32p, _f32p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int,
                                     _f32p, C.c_float, C.c_void_p, _f64p, C.c_int, C.c_void_p]
+        L.oracle_spectrum_samples.restype = C.c_int
+        L.oracle_spectrum_samples.argtypes = [C.c_double, C.c_double, C.c_int, _f64p, _f64p]
+        L.oracle_sf_spectrum_f64.argtypes = [_f32p, C.c_int, _i32p, _f32p, C.c_void_p, C.c_void_p, C.c_int, _f32p,
+                                             C.c_float, C.c_int, _f64p, _f64p, C.c_void_p, _f64p, C.c_int,
+                                             C.c_void_p]
         L.oracle_sf_refsem.argtypes = [_f32p, C.c_int, _i32p, _f32p, C.c_void_p, C.c_int, C.c_int, _f32p, C.c_float,
                                        C.c_void_p, _f32p, C.c_int]
         L.oracle_sf_grad_f64.argtypes = [_f32p, C.c_int, _i32p, _f32p, C.c_void_p, C.c_void_p, C.c_int, _f32p,
@@ -129,6 +134,34 @@ def sf_f64(hkl, Z, pos, amp=None, atom_b=None, delta=False, vector_mode=False, B
     R_a, R_p = _opt(None if R is None else np.asarray(R, np.float64).reshape(9), np.float64)
     lib().oracle_sf_f64(hkl.ravel(), n, Z, pos.ravel(), amp_p, b_p, Z.shape[0], int(delta), int(vector_mode),
                         ff.ravel(), float(B), F.ctypes.data_as(C.c_void_p), I, int(threads), R_p)
+    return F[0::2] + 1j * F[1::2], I
+
+
+def spectrum_samples(wavelength, bandwidth, nsamples):
+    """(scales, weights) of the spectrum path; an even count above 1 is raised by one."""
+    n = max(1, int(nsamples) + 1)
+    sc = np.zeros(n, np.float64)
+    w = np.zeros(n, np.float64)
+    k = lib().oracle_spectrum_samples(float(wavelength), float(bandwidth), int(nsamples), sc, w)
+    return sc[:k].copy(), w[:k].copy()
+
+
+def sf_spectrum_f64(hkl, Z, pos, scales, weights, amp=None, atom_b=None, B=0.0, ff=None, threads=0, R=None):
+    """FP64 spectrum structure factor. Returns (F complex128 [npix], I float64 [npix])."""
+    hkl = np.ascontiguousarray(hkl, np.float32).reshape(-1, 3)
+    Z = np.ascontiguousarray(Z, np.int32)
+    pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+    ff = default_form_factors() if ff is None else np.ascontiguousarray(ff, np.float32)
+    sc = np.ascontiguousarray(scales, np.float64)
+    w = np.ascontiguousarray(weights, np.float64)
+    amp_a, amp_p = _opt(amp, np.float32)
+    b_a, b_p = _opt(atom_b, np.float32)
+    R_a, R_p = _opt(None if R is None else np.asarray(R, np.float64).reshape(9), np.float64)
+    n = hkl.shape[0]
+    F = np.zeros(2 * n, np.float64)
+    I = np.zeros(n, np.float64)
+    lib().oracle_sf_spectrum_f64(hkl.ravel(), n, Z, pos.ravel(), amp_p, b_p, Z.shape[0], ff.ravel(), float(B),
+                                 sc.shape[0], sc, w, F.ctypes.data_as(C.c_void_p), I, int(threads), R_p)
     return F[0::2] + 1j * F[1::2], I
 
 

@@ -18,6 +18,12 @@
  *                             (1 and 2 atoms, atoms at the origin, centrosymmetry, Friedel,
  *                             translation and rotation invariance, delta atoms) and an
  *                             arbitrary-precision (mpmath) sum.
+ *   oracle_spectrum_samples,
+ *   oracle_sf_spectrum_f64    the wavelength-spectrum path of src/diffraction.cu:300-318,
+ *                             416-476 in FP64. Pinned by closed forms (the weights
+ *                             exp(-2 (2k/(S-1) - 1)^2), the scales 1/(1 - bw/2 + k bw/(S-1)),
+ *                             one atom, atoms at the origin, the plain oracle for S = 1) and
+ *                             by linearity over samples for point scatterers.
  *   oracle_sf_refsem          src/diffraction.c:770-853 in the reference's own FP32/FP64 mix.
  *                             Pinned against oracle_sf_f64 within the FP32 error bound.
  *                             The final |F|^2 assumes libspimage's sp_cabs is the FP32
@@ -164,6 +170,85 @@ int oracle_sf_f64(const float *hkl, int npix, const int *Z, const float *pos, co
       cis_turns(t, &c, &s);
       re += fcache[sp] * w * c;
       im -= fcache[sp] * w * s;
+    }
+    if (F_out) {
+      F_out[2 * i] = re;
+      F_out[2 * i + 1] = im;
+    }
+    I_out[i] = re * re + im * im;
+  }
+  return 0;
+}
+
+/*
+ * Wavelength samples of the reference's spectrum path
+ * (src/diffraction.cu:300-303, 431-460), in FP64: an even sample count above 1
+ * is raised by one; lambda_k = lambda (1 - bw/2) + k/(S-1) lambda bw spans
+ * +-2 sigma with sigma = lambda bw / 4; Gaussian weights exp(-x^2/2) with
+ * x = (lambda_k - lambda) / sigma, normalised to sum 1; the phase scale is
+ * s_k = lambda / lambda_k. Returns the sample count used (1 when bw == 0).
+ */
+int oracle_spectrum_samples(double wavelength, double bandwidth, int nsamples, double *scale, double *weight) {
+  if (nsamples > 1 && (nsamples & 1) == 0) nsamples++;
+  if (bandwidth == 0.0 || nsamples <= 1) {
+    scale[0] = 1.0;
+    weight[0] = 1.0;
+    return 1;
+  }
+  const double lo = wavelength * (1.0 - bandwidth / 2.0);
+  const double sigma = (wavelength - lo) / 2.0;
+  double total = 0.0;
+  for (int k = 0; k < nsamples; ++k) {
+    const double lk = lo + ((double)k / (nsamples - 1)) * wavelength * bandwidth;
+    const double x = (lk - wavelength) / sigma;
+    weight[k] = exp(-x * x / 2.0);
+    scale[k] = wavelength / lk;
+    total += weight[k];
+  }
+  for (int k = 0; k < nsamples; ++k) weight[k] /= total;
+  return nsamples;
+}
+
+/*
+ * Spectrum structure factor (src/diffraction.cu:416-476) in FP64 on FP32
+ * inputs: F(q) = sum_j f_Zj(|q|) a_j sum_k w_k e^{-2 pi i s_k q.r_j}. The form
+ * factor is taken at the unscaled |q| and follows the CPU formula (the GPU
+ * path's c-term slip at :450 is not reproduced). Z = 0 atoms are skipped;
+ * amp, atom_b and R as in oracle_sf_f64.
+ */
+int oracle_sf_spectrum_f64(const float *hkl, int npix, const int *Z, const float *pos, const float *amp,
+                           const float *atom_b, int natoms, const float *ff, float B, int nsamples,
+                           const double *scale, const double *weight, double *F_out, double *I_out, int nthreads,
+                           const double *R) {
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 16)
+#endif
+  for (int i = 0; i < npix; ++i) {
+    double qx = hkl[3 * i], qy = hkl[3 * i + 1], qz = hkl[3 * i + 2];
+    if (R) {
+      const double x = qx, y = qy, z = qz;
+      qx = R[0] * x + R[1] * y + R[2] * z;
+      qy = R[3] * x + R[4] * y + R[5] * z;
+      qz = R[6] * x + R[7] * y + R[8] * z;
+    }
+    const double d = sqrt(qx * qx + qy * qy + qz * qz);
+    const double s2 = (d * 1e-10) * (d * 1e-10) * 0.25;
+    double re = 0.0, im = 0.0;
+    for (int j = 0; j < natoms; ++j) {
+      if (Z[j] == 0) continue;
+      double a = oracle_form_factor(d, Z[j], B, ff) * (amp ? (double)amp[j] : 1.0);
+      if (atom_b) a *= exp(-(double)atom_b[j] * s2);
+      const double t = qx * pos[3 * j] + qy * pos[3 * j + 1] + qz * pos[3 * j + 2];
+      double cr = 0.0, ci = 0.0;
+      for (int k = 0; k < nsamples; ++k) {
+        double c, sn;
+        cis_turns(scale[k] * t, &c, &sn);
+        cr += weight[k] * c;
+        ci += weight[k] * sn;
+      }
+      re += a * cr;
+      im -= a * ci;
     }
     if (F_out) {
       F_out[2 * i] = re;

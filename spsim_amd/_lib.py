@@ -12,6 +12,7 @@ from pathlib import Path
 LIB_PATH = Path(__file__).resolve().parent / "libspsim_amd.so"
 
 SPSIM_HIP_OK = 0
+MAX_SPECTRUM = 65  # SPSIM_HIP_MAX_SPECTRUM
 NSPECIES = 101
 UNIT_SPECIES = 100
 FF_STRIDE = 9
@@ -62,6 +63,9 @@ SIGNATURES = {
     "spsim_hip_rotate_hkl": (_i, [_vp, _i, _vp, _vp, _vp]),
     "spsim_hip_sf_pattern": (_i, [_vp, _i, _vp, _vp, _vp, _i, _i, _vp, _f, _vp, _vp, _vp, C.POINTER(Tile), _vp]),
     "spsim_hip_sf_grid": (_i, [_vp, _i, _i, _vp, _vp, _vp, _i, _i, _vp, _f, _vp, _vp, _vp, C.POINTER(Tile), _vp]),
+    "spsim_hip_sf_spectrum": (_i, [_vp, _i, _vp, _vp, _vp, _i, _i, _vp, _f, _vp, _i, _vp, _vp, _vp, _vp,
+                                   C.POINTER(Tile), _vp]),
+    "spsim_hip_spectrum_samples": (_i, [C.c_double, C.c_double, _i, _vp, _vp]),
     "spsim_hip_release": (_i, []),
     "spsim_hip_detector_chain": (_i, [C.POINTER(DetectorParams), _vp, _i, C.c_uint64, C.c_uint64,
                                       _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),

@@ -114,9 +114,12 @@ int spsim_bridge_rotate_hkl(float *list, int npix, const double *rot9) {
   return SPSIM_HIP_OK;
 }
 
+// One structure-factor launch on host arrays: the spectrum kernel when
+// nsamples > 0, else the grid kernel when nx > 0, else the per-pixel kernel.
 static int structure_factor_impl(const float *hkl, int npix, int nx, int ny, const spsim_hip_atom *atoms,
                                  const float *atom_b, const spsim_hip_segment *segs, int nsegs, int natoms,
-                                 const float *ff, float b_global, Complex *F, float *I) {
+                                 const float *ff, float b_global, Complex *F, float *I, int nsamples = 0,
+                                 const double *scales = nullptr, const double *weights = nullptr) {
   using namespace spsim;
   SPSIM_TRY(ensure_device(nullptr));
   if (npix <= 0) return SPSIM_HIP_OK;
@@ -133,7 +136,10 @@ static int structure_factor_impl(const float *hkl, int npix, int nx, int ny, con
   SPSIM_TRY(b.upload(&dff, ff, (size_t)SPSIM_HIP_NSPECIES * SPSIM_HIP_FF_STRIDE, st.s));
   if (F) SPSIM_TRY(b.alloc(&dF, (size_t)npix * 2));
   SPSIM_TRY(b.alloc(&dI, (size_t)npix));
-  if (nx > 0)
+  if (nsamples > 0)
+    SPSIM_TRY(spsim_hip_sf_spectrum(dq, npix, da, db, ds, nsegs, natoms, dff, b_global, nullptr, nsamples, scales,
+                                    weights, dF, dI, nullptr, st.s));
+  else if (nx > 0)
     SPSIM_TRY(spsim_hip_sf_grid(dq, nx, ny, da, db, ds, nsegs, natoms, dff, b_global, nullptr, dF, dI, nullptr, st.s));
   else
     SPSIM_TRY(spsim_hip_sf_pattern(dq, npix, da, db, ds, nsegs, natoms, dff, b_global, nullptr, dF, dI, nullptr, st.s));
@@ -147,6 +153,15 @@ int spsim_bridge_structure_factor(const float *hkl, int npix, const spsim_hip_at
                                   const spsim_hip_segment *segs, int nsegs, int natoms, const float *ff, float b_global,
                                   Complex *F, float *I) {
   return structure_factor_impl(hkl, npix, 0, 0, atoms, atom_b, segs, nsegs, natoms, ff, b_global, F, I);
+}
+
+int spsim_bridge_structure_factor_spectrum(const float *hkl, int npix, const spsim_hip_atom *atoms,
+                                           const float *atom_b, const spsim_hip_segment *segs, int nsegs,
+                                           int natoms, const float *ff, float b_global, int nsamples,
+                                           const double *scales, const double *weights, Complex *F, float *I) {
+  if (nsamples < 1) return spsim::fail(SPSIM_HIP_ERR_INVALID, "nsamples=%d", nsamples);
+  return structure_factor_impl(hkl, npix, 0, 0, atoms, atom_b, segs, nsegs, natoms, ff, b_global, F, I, nsamples,
+                               scales, weights);
 }
 
 int spsim_bridge_structure_factor_grid(const float *hkl, int nx, int ny, const spsim_hip_atom *atoms,

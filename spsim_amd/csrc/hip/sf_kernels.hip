@@ -165,6 +165,82 @@ __device__ __forceinline__ void tile_sum(const double (&qx)[K], const double (&q
   }
 }
 
+// ---- Wavelength spectrum ---------------------------------------------------
+//
+// The reference's spectrum kernel (src/diffraction.cu:416-476) averages the
+// amplitude over wavelength samples: each atom contributes
+// f(|q|) w_j sum_k W_k e^{-2 pi i s_k q.r_j} with phase scales s_k = lambda/lambda_k
+// and normalised weights W_k (the form factor stays at the unscaled |q|). The
+// FP64 phase q.r is formed once per atom and pixel and scaled by each s_k in
+// FP64 before the reduction to turns; the weighted sample sum is then folded
+// into the atom's amplitude like a per-atom weight.
+template <int K, int CHUNK, bool HAS_B>
+__device__ __forceinline__ void spectrum_sum(const double (&qx)[K], const double (&qy)[K], const double (&qz)[K],
+                                             const float (&s2)[K], const spsim_hip_atom *__restrict__ atoms,
+                                             const float *__restrict__ atom_b,
+                                             const spsim_hip_segment *__restrict__ segs, int nsegs, int a_begin,
+                                             int a_end, const float *__restrict__ ff, float b_global, const double *R,
+                                             const Spectrum &sp, AtomLds *__restrict__ sm, double (&Fr)[K],
+                                             double (&Fi)[K]) {
+  float nl2[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    nl2[k] = -s2[k] * (float)kLog2e;
+    Fr[k] = 0.0;
+    Fi[k] = 0.0;
+  }
+  const int ns = sp.n;
+  for (int sg = 0; sg < nsegs; ++sg) {
+    const int s_start = segs[sg].start;
+    const int s_end = s_start + segs[sg].count;
+    const int species = segs[sg].species;
+    const int lo = max(s_start, a_begin);
+    const int hi = min(s_end, a_end);
+    if (lo >= hi) continue;
+    const float Bs = (species == SPSIM_HIP_UNIT_SPECIES) ? 0.0f : b_global;
+    const float *c = ff + species * SPSIM_HIP_FF_STRIDE;
+    float f[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) f[k] = form_factor(c, s2[k], Bs);
+    for (int c0 = lo; c0 < hi; c0 += CHUNK) {
+      const int n = min(CHUNK, hi - c0);
+      stage_chunk<HAS_B>(atoms, atom_b, c0, n, R, sm);
+      float sr[K], si[K];
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        sr[k] = 0.0f;
+        si[k] = 0.0f;
+      }
+      for (int j = 0; j < n; ++j) {
+        const AtomLds a = sm[j];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          const double t = __builtin_fma(qz[k], a.z, __builtin_fma(qy[k], a.y, qx[k] * a.x));
+          float ur = 0.0f, ui = 0.0f;
+          for (int m = 0; m < ns; ++m) {
+            const float ph = (float)__builtin_amdgcn_fract(sp.s[m] * t);
+            ur = __builtin_fmaf(sp.w[m], __builtin_amdgcn_cosf(ph), ur);
+            ui = __builtin_fmaf(sp.w[m], __builtin_amdgcn_sinf(ph), ui);
+          }
+          float wt;
+          if constexpr (HAS_B) {
+            wt = __builtin_amdgcn_exp2f(__builtin_fmaf(a.b, nl2[k], a.w));
+          } else {
+            wt = a.w;
+          }
+          sr[k] = __builtin_fmaf(wt, ur, sr[k]);
+          si[k] = __builtin_fmaf(wt, ui, si[k]);
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        Fr[k] = __builtin_fma((double)f[k], (double)sr[k], Fr[k]);
+        Fi[k] = __builtin_fma((double)f[k], (double)si[k], Fi[k]);
+      }
+    }
+  }
+}
+
 // ---- Mirror-symmetric detectors ------------------------------------------
 //
 // A detector centred on the beam has an exactly mirror-symmetric q list: the
@@ -373,6 +449,31 @@ __global__ __launch_bounds__(kThreads) void sf_grid_kernel(PatternLaunch a, int 
     if (pix[i] >= 0) store_pixel(pix[i], Fr[i], Fi[i], a.npix, split, partial, a.F, a.I);
 }
 
+template <int K, int CHUNK, bool HAS_B>
+__global__ __launch_bounds__(kThreads) void sf_spectrum_kernel(PatternLaunch a, Spectrum sp,
+                                                               double2 *__restrict__ partial) {
+  __shared__ AtomLds sm[CHUNK];
+  const int split = blockIdx.y, nsplit = gridDim.y;
+  const int a_begin = (int)(((long long)a.natoms * split) / nsplit);
+  const int a_end = (int)(((long long)a.natoms * (split + 1)) / nsplit);
+  const int base = blockIdx.x * (kThreads * K) + threadIdx.x;
+  double qx[K], qy[K], qz[K];
+  float s2[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int pix = base + k * kThreads;
+    load_q(a.hkl, pix, pix < a.npix, qx[k], qy[k], qz[k], s2[k]);
+  }
+  double Fr[K], Fi[K];
+  spectrum_sum<K, CHUNK, HAS_B>(qx, qy, qz, s2, a.atoms, a.atom_b, a.segs, a.nsegs, a_begin, a_end, a.ff, a.b_global,
+                                a.has_rot ? a.rot.r : nullptr, sp, sm, Fr, Fi);
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int pix = base + k * kThreads;
+    if (pix < a.npix) store_pixel(pix, Fr[k], Fi[k], a.npix, split, partial, a.F, a.I);
+  }
+}
+
 // Deterministic reduction of the atom splits of one pattern, in split order.
 __global__ __launch_bounds__(kThreads) void sf_reduce_kernel(const double2 *__restrict__ partial, int nsplit, int npix,
                                                              float2 *__restrict__ F, float *__restrict__ I) {
@@ -464,7 +565,7 @@ __global__ __launch_bounds__(kThreads) void sf_batch_grid_kernel(BatchLaunch a) 
   X(1, 64) X(1, 128) X(1, 256) X(2, 64) X(2, 128) X(2, 256) X(4, 64) X(4, 128) X(4, 256) X(8, 64) X(8, 128) X(8, 256)
 #define SPSIM_QUAD_TILES(X) X(1, 64) X(1, 128) X(1, 256) X(2, 64) X(2, 128) X(2, 256)
 
-enum class Kind { kPattern, kGrid, kBatch, kBatchGrid };
+enum class Kind { kPattern, kGrid, kBatch, kBatchGrid, kSpectrum };
 
 template <bool HAS_B>
 const void *kernel_ptr(Kind kind, int K, int C) {
@@ -474,13 +575,18 @@ const void *kernel_ptr(Kind kind, int K, int C) {
 #define Q(k_, c_) \
   if (K == k_ && C == c_) return kind == Kind::kGrid ? (const void *)&sf_grid_kernel<k_, c_, HAS_B> \
                                                        : (const void *)&sf_batch_grid_kernel<k_, c_, HAS_B>;
+#define S(k_, c_) \
+  if (K == k_ && C == c_) return (const void *)&sf_spectrum_kernel<k_, c_, HAS_B>;
   if (kind == Kind::kPattern || kind == Kind::kBatch) {
     SPSIM_PIXEL_TILES(P)
+  } else if (kind == Kind::kSpectrum) {
+    SPSIM_QUAD_TILES(S)
   } else {
     SPSIM_QUAD_TILES(Q)
   }
 #undef P
 #undef Q
+#undef S
   return nullptr;
 }
 
@@ -491,7 +597,7 @@ const void *kernel_for(Kind kind, const spsim_hip_tile &t, bool has_b) {
 }
 
 int launch_single(Kind kind, const PatternLaunch &a, int nx, int ny, const spsim_hip_tile &t, int splits,
-                  hipStream_t s) {
+                  hipStream_t s, const Spectrum *spec = nullptr) {
   if (a.npix <= 0) return SPSIM_HIP_OK;
   const void *k = kernel_for(kind, t, a.atom_b != nullptr);
   if (!k) return fail(SPSIM_HIP_ERR_INVALID, "unsupported tile (per lane %d, chunk %d)", t.pixels_per_lane, t.chunk_atoms);
@@ -507,6 +613,10 @@ int launch_single(Kind kind, const PatternLaunch &a, int nx, int ny, const spsim
   PatternLaunch arg = a;
   if (kind == Kind::kGrid) {
     void *params[] = {(void *)&arg, (void *)&nx, (void *)&ny, (void *)&partial};
+    SPSIM_HIP_TRY(hipLaunchKernel(k, dim3(tiles, splits), dim3(kThreads), params, 0, s));
+  } else if (kind == Kind::kSpectrum) {
+    Spectrum sp = *spec;
+    void *params[] = {(void *)&arg, (void *)&sp, (void *)&partial};
     SPSIM_HIP_TRY(hipLaunchKernel(k, dim3(tiles, splits), dim3(kThreads), params, 0, s));
   } else {
     void *params[] = {(void *)&arg, (void *)&partial};
@@ -600,6 +710,10 @@ int launch_grid(const PatternLaunch &a, int nx, int ny, const spsim_hip_tile &t,
   return launch_single(Kind::kGrid, a, nx, ny, t, splits, s);
 }
 
+int launch_spectrum(const PatternLaunch &a, const Spectrum &sp, const spsim_hip_tile &t, int splits, hipStream_t s) {
+  return launch_single(Kind::kSpectrum, a, 0, 0, t, splits, s, &sp);
+}
+
 int batch_blocks_per_cu(const spsim_hip_tile &t, bool has_b, bool grid, int *blocks) {
   const void *k = kernel_for(grid ? Kind::kBatchGrid : Kind::kBatch, t, has_b);
   if (!k) return fail(SPSIM_HIP_ERR_INVALID, "unsupported tile (per lane %d, chunk %d)", t.pixels_per_lane, t.chunk_atoms);
@@ -663,4 +777,34 @@ extern "C" int spsim_hip_sf_grid(const float *d_hkl, int nx, int ny, const spsim
   const PatternLaunch a =
       make_launch(d_hkl, (int)npix, d_atoms, d_atom_b, d_segs, nsegs, natoms, d_ff, b_global, rot9, d_F, d_I);
   return launch_grid(a, nx, ny, t, splits, (hipStream_t)stream);
+}
+
+extern "C" int spsim_hip_sf_spectrum(const float *d_hkl, int npix, const spsim_hip_atom *d_atoms,
+                                     const float *d_atom_b, const spsim_hip_segment *d_segs, int nsegs, int natoms,
+                                     const float *d_ff, float b_global, const double *rot9, int nsamples,
+                                     const double *scales, const double *weights, float *d_F, float *d_I,
+                                     const spsim_hip_tile *tile, void *stream) {
+  using namespace spsim;
+  SPSIM_TRY(check_common(d_hkl, npix, d_atoms, d_segs, nsegs, natoms, d_ff));
+  if (nsamples < 1 || nsamples > SPSIM_HIP_MAX_SPECTRUM)
+    return fail(SPSIM_HIP_ERR_INVALID, "nsamples=%d outside [1, %d]", nsamples, SPSIM_HIP_MAX_SPECTRUM);
+  if (!scales || !weights) return fail(SPSIM_HIP_ERR_INVALID, "scales/weights is NULL");
+  int cus = 0;
+  SPSIM_TRY(ensure_device(&cus));
+  if (npix == 0) return SPSIM_HIP_OK;
+  spsim_hip_tile t = tile ? *tile : default_tile(natoms, npix, true);
+  if (kernel_for(Kind::kSpectrum, t, false) == nullptr || t.atom_splits < 0)
+    return fail(SPSIM_HIP_ERR_INVALID, "unsupported spectrum tile (pixels_per_lane=%d waves=%d chunk_atoms=%d)",
+                t.pixels_per_lane, t.waves, t.chunk_atoms);
+  Spectrum sp;
+  sp.n = nsamples;
+  for (int k = 0; k < SPSIM_HIP_MAX_SPECTRUM; ++k) {
+    sp.s[k] = k < nsamples ? scales[k] : 0.0;
+    sp.w[k] = k < nsamples ? (float)weights[k] : 0.0f;
+  }
+  const int tiles = (npix + tile_units(t) - 1) / tile_units(t);
+  const int splits = t.atom_splits > 0 ? t.atom_splits : auto_splits(tiles, natoms, t.chunk_atoms, cus);
+  const PatternLaunch a =
+      make_launch(d_hkl, npix, d_atoms, d_atom_b, d_segs, nsegs, natoms, d_ff, b_global, rot9, d_F, d_I);
+  return launch_spectrum(a, sp, t, splits, (hipStream_t)stream);
 }

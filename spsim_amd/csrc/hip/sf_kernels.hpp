@@ -18,6 +18,13 @@ struct Rot {
   double r[9];
 };
 
+// Wavelength samples of a spectrum pattern: phase scales and normalised weights.
+struct Spectrum {
+  int n;
+  float w[SPSIM_HIP_MAX_SPECTRUM];
+  double s[SPSIM_HIP_MAX_SPECTRUM];
+};
+
 struct PatternLaunch {
   const float *hkl;
   int npix;
@@ -65,6 +72,8 @@ __host__ __device__ inline int grid_quads(int nx, int ny) { return ((nx + 1) / 2
 int launch_pattern(const PatternLaunch &a, const spsim_hip_tile &t, int splits, hipStream_t s);
 // Same on an nx x ny detector list, exploiting its mirror symmetry where present.
 int launch_grid(const PatternLaunch &a, int nx, int ny, const spsim_hip_tile &t, int splits, hipStream_t s);
+// Same with a wavelength spectrum (per-pixel kernel; pixels_per_lane 1 or 2).
+int launch_spectrum(const PatternLaunch &a, const Spectrum &sp, const spsim_hip_tile &t, int splits, hipStream_t s);
 // Persistent kernel: `blocks` workgroups pull work items from the queue. With
 // grid != 0 the items are quad tiles of the a.nx x a.ny detector.
 int launch_batch(const BatchLaunch &a, const spsim_hip_tile &t, bool grid, int blocks, hipStream_t s);

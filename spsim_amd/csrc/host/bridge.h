@@ -25,6 +25,13 @@ int spsim_bridge_structure_factor(const float *hkl, int npix, const spsim_hip_at
                                   const spsim_hip_segment *segs, int nsegs, int natoms, const float *ff, float b_global,
                                   Complex *F, float *I);
 
+/* Same under a wavelength spectrum (spsim_hip_sf_spectrum); scales and weights
+   are nsamples host values. */
+int spsim_bridge_structure_factor_spectrum(const float *hkl, int npix, const spsim_hip_atom *atoms,
+                                           const float *atom_b, const spsim_hip_segment *segs, int nsegs,
+                                           int natoms, const float *ff, float b_global, int nsamples,
+                                           const double *scales, const double *weights, Complex *F, float *I);
+
 /* Same on a full nx x ny detector list ([ny][nx][3]); uses the mirror-symmetric
    kernel where the list allows it. */
 int spsim_bridge_structure_factor_grid(const float *hkl, int nx, int ny, const spsim_hip_atom *atoms,

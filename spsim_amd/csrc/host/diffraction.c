@@ -142,12 +142,16 @@ static void grid_dims(int size, const Options *opts, int *nx, int *ny) {
   if ((long long)n * n == size) *nx = *ny = n;
 }
 
+/* nsamples > 0 selects the wavelength-spectrum kernel with the given phase
+   scales and weights (delta_atoms is then ignored, as the reference's spectrum
+   kernel ignores it). */
 static Diffraction_Pattern *compute_pattern(Molecule *mol, float *HKL_list, int HKL_list_size, float B,
-                                            Experiment *exp, Options *opts, int mode) {
+                                            Experiment *exp, Options *opts, int mode, int nsamples,
+                                            const double *scales, const double *weights) {
   const double t0 = now_seconds();
   if (!mol || HKL_list_size < 0 || (HKL_list_size > 0 && !HKL_list) || mol->natoms < 0) return NULL;
   const int N = mol->natoms;
-  const int delta = opts ? opts->delta_atoms : 0;
+  const int delta = (opts && nsamples == 0) ? opts->delta_atoms : 0;
   Diffraction_Pattern *res = (Diffraction_Pattern *)calloc(1, sizeof(Diffraction_Pattern));
   float *amp = (float *)malloc(sizeof(float) * (N > 0 ? N : 1));
   spsim_hip_atom *atoms = (spsim_hip_atom *)malloc(sizeof(spsim_hip_atom) * (N > 0 ? N : 1));
@@ -176,7 +180,11 @@ static Diffraction_Pattern *compute_pattern(Molecule *mol, float *HKL_list, int 
   }
   int gx = 0, gy = 0;
   grid_dims(HKL_list_size, opts, &gx, &gy);
-  if (gx > 0)
+  if (nsamples > 0)
+    rc = spsim_bridge_structure_factor_spectrum(HKL_list, HKL_list_size, atoms, NULL, segs, nsegs, natoms,
+                                                spsim_form_factor_table(), B, nsamples, scales, weights, res->F,
+                                                res->ints);
+  else if (gx > 0)
     rc = spsim_bridge_structure_factor_grid(HKL_list, gx, gy, atoms, NULL, segs, nsegs, natoms,
                                             spsim_form_factor_table(), B, res->F, res->ints);
   else
@@ -207,12 +215,27 @@ fail:
 
 Diffraction_Pattern *compute_pattern_on_list(Molecule *mol, float *HKL_list, int HKL_list_size, float B,
                                              Experiment *exp, Options *opts) {
-  return compute_pattern(mol, HKL_list, HKL_list_size, B, exp, opts, SPSIM_HIP_PACK_PLAIN);
+  return compute_pattern(mol, HKL_list, HKL_list_size, B, exp, opts, SPSIM_HIP_PACK_PLAIN, 0, NULL, NULL);
 }
 
 Diffraction_Pattern *vector_compute_pattern_on_list(Molecule *mol, float *HKL_list, int HKL_list_size, float B,
                                                     Experiment *exp, Options *opts) {
-  return compute_pattern(mol, HKL_list, HKL_list_size, B, exp, opts, SPSIM_HIP_PACK_VECTOR);
+  return compute_pattern(mol, HKL_list, HKL_list_size, B, exp, opts, SPSIM_HIP_PACK_VECTOR, 0, NULL, NULL);
+}
+
+Diffraction_Pattern *spsim_spectrum_pattern_on_list(Molecule *mol, float *HKL_list, int HKL_list_size, float B,
+                                                      Experiment *exp, Options *opts) {
+  double scales[SPSIM_HIP_MAX_SPECTRUM], weights[SPSIM_HIP_MAX_SPECTRUM];
+  if (!exp || !opts) return NULL;
+  /* the reference raises an even sample count in the caller's options (src/diffraction.cu:302-304) */
+  if (opts->wavelength_samples > 1 && (opts->wavelength_samples & 1) == 0) opts->wavelength_samples++;
+  const int n = spsim_hip_spectrum_samples(exp->wavelength, exp->bandwidth, opts->wavelength_samples, scales, weights);
+  if (n < 1) {
+    fprintf(stderr, "spsim: spsim_spectrum_pattern_on_list: wavelength_samples must lie in [1, %d] and the "
+                    "wavelength must be positive\n", SPSIM_HIP_MAX_SPECTRUM);
+    return NULL;
+  }
+  return compute_pattern(mol, HKL_list, HKL_list_size, B, exp, opts, SPSIM_HIP_PACK_PLAIN, n, scales, weights);
 }
 
 void spsim_detector_params(const CCD *det, const Experiment *exp, spsim_hip_detector *p) {

@@ -49,8 +49,12 @@ Diffraction_Pattern *compute_sf(Molecule *mol, float *HKL_list, int HKL_list_siz
     return NULL;
   }
   /* use_cuda / vectorize select the same GPU kernel; vectorize keeps the
-     vector path's semantics (src/spsim.c:127-140). */
-  if (opts->vectorize && !opts->use_cuda)
+     vector path's semantics (src/spsim.c:127-140). With use_cuda, a non-zero
+     bandwidth and more than one wavelength sample, the reference's GPU path
+     averages over the spectrum (src/diffraction.cu:313-318). */
+  if (opts->use_cuda && opts->experiment && opts->experiment->bandwidth != 0 && opts->wavelength_samples > 1)
+    pattern = spsim_spectrum_pattern_on_list(mol, HKL_list, HKL_list_size, opts->b_factor, opts->experiment, opts);
+  else if (opts->vectorize && !opts->use_cuda)
     pattern = vector_compute_pattern_on_list(mol, HKL_list, HKL_list_size, opts->b_factor, opts->experiment, opts);
   else
     pattern = compute_pattern_on_list(mol, HKL_list, HKL_list_size, opts->b_factor, opts->experiment, opts);

@@ -145,6 +145,40 @@ def structure_factor_grid(hkl: torch.Tensor, nx: int, ny: int, atoms: DeviceAtom
     return (F if want_F else None), I
 
 
+def spectrum_samples(wavelength: float, bandwidth: float, nsamples: int):
+    """(scales, weights) of a Gaussian spectrum, as spsim_hip_spectrum_samples draws them."""
+    sc = np.zeros(_lib.MAX_SPECTRUM, np.float64)
+    w = np.zeros(_lib.MAX_SPECTRUM, np.float64)
+    n = _lib.load().spsim_hip_spectrum_samples(float(wavelength), float(bandwidth), int(nsamples),
+                                               sc.ctypes.data_as(C.c_void_p), w.ctypes.data_as(C.c_void_p))
+    if n < 1:
+        raise ValueError(f"spectrum of {nsamples} samples is not supported (at most {_lib.MAX_SPECTRUM})")
+    return sc[:n].copy(), w[:n].copy()
+
+
+def structure_factor_spectrum(hkl: torch.Tensor, atoms: DeviceAtoms, ff: torch.Tensor, scales, weights,
+                              B: float = 0.0, F: torch.Tensor | None = None, I: torch.Tensor | None = None,
+                              tile: Tile | None = None, stream=None, want_F: bool = True, R=None):
+    """One pattern averaged over wavelength samples (phase scales, weights); returns (F or None, I)."""
+    npix = int(hkl.shape[0])
+    sc = np.ascontiguousarray(scales, np.float64)
+    w = np.ascontiguousarray(weights, np.float64)
+    if sc.shape != w.shape:
+        raise ValueError("scales and weights differ in length")
+    if want_F and F is None:
+        F = torch.empty((npix, 2), dtype=torch.float32, device=hkl.device)
+    if I is None:
+        I = torch.empty((npix,), dtype=torch.float32, device=hkl.device)
+    Ra, Rp = _rot(R)
+    rc = _lib.load().spsim_hip_sf_spectrum(_ptr(hkl), npix, _ptr(atoms.atoms), _ptr(atoms.b), _ptr(atoms.segs),
+                                           atoms.nsegs, atoms.natoms, _ptr(ff), float(B), Rp, int(sc.shape[0]),
+                                           sc.ctypes.data_as(C.c_void_p), w.ctypes.data_as(C.c_void_p),
+                                           _ptr(F if want_F else None), _ptr(I),
+                                           C.byref(tile) if tile is not None else None, _stream(stream))
+    check(rc, "spsim_hip_sf_spectrum")
+    return (F if want_F else None), I
+
+
 def hkl_detector(nx, ny, width, height, distance, wavelength, center_x=0.0, center_y=0.0, spherical=0, device=None,
                  stream=None) -> torch.Tensor:
     dev = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
