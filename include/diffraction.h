@@ -29,6 +29,11 @@ typedef struct {
    scattering vectors (m^-1) on the Ewald sphere; *HKL_list_size = nx*ny. */
 float *get_HKL_list_for_detector(CCD *det, Experiment *exp, int *HKL_list_size);
 
+/* Reference src/diffraction.c:325-369 (include/diffraction.h:73). Returns a
+   malloc'd [nz][ny][nx][3] uniform grid q = p / (distance * wavelength) of a
+   3-D detector volume; *HKL_list_size = nx*ny*nz. */
+float *get_HKL_list_for_3d_detector(CCD *det, Experiment *exp, int *HKL_list_size);
+
 /* Reference src/diffraction.c:770-853. F_i = sum_j f_Zj(|q_i|) sqrt(illum_j) exp(-2 pi i q_i.r_j),
    ints_i = |F_i|^2. Atoms with Z == 0 are skipped; delta_atoms forces f = 1. */
 Diffraction_Pattern *compute_pattern_on_list(Molecule *mol, float *HKL_list, int HKL_list_size,

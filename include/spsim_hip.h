@@ -99,6 +99,14 @@ int spsim_hip_hkl_detector(int nx, int ny, float width, float height, float dist
 /* d_out[i] = R * d_in[i] for a row-major 3x3 R, formed in FP64 and rounded to FP32. */
 int spsim_hip_rotate_hkl(const float *d_in, int npix, const double *rot9, float *d_out, void *stream);
 
+/* <- get_HKL_list_for_3d_detector, src/diffraction.c:325-369: the uniform
+   reciprocal-space grid of an nx x ny x nz detector volume, [nz][ny][nx][3],
+   q = p / (distance * wavelength) with p = ((i - (n-1)/2)/n) * extent - center
+   per axis (FP64, stored FP32, bit-identical to the host formula). */
+int spsim_hip_hkl_detector_3d(int nx, int ny, int nz, float width, float height, float depth, float center_x,
+                              float center_y, float center_z, float distance, float wavelength, float *d_hkl,
+                              void *stream);
+
 /* One pattern: F[i] = sum_j f_s(|q'_i|) exp(-B_eff s^2) w_j e^{-B_j s^2} exp(-2 pi i q'_i.r_j),
    I[i] = |F[i]|^2, with q'_i = R q_i, s = |q| / 2 in A^-1, B_eff = b_global (0 for the
    unit species). rot9 (host, row-major 3x3) may be NULL (R = identity); the

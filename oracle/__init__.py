@@ -42,6 +42,8 @@ def lib() -> C.CDLL:
         L.oracle_hkl_detector.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float,
                                           C.c_int, C.c_float, _f32p]
         L.oracle_rotate_hkl.argtypes = [_f32p, C.c_int, _f64p, _f32p]
+        L.oracle_hkl_3d.argtypes = [C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float,
+                                    C.c_float, C.c_float, C.c_float, C.c_float, _f32p]
         L.oracle_sf_f64.argtypes = [_f32p, C.c_int, _i32p, _f32p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int,
                                     _f32p, C.c_float, C.c_void_p, _f64p, C.c_int, C.c_void_p]
         L.oracle_spectrum_samples.restype = C.c_int
@@ -108,6 +110,12 @@ def hkl_detector(nx, ny, width, height, distance, center_x, center_y, spherical,
     out = np.zeros(nx * ny * 3, np.float32)
     lib().oracle_hkl_detector(nx, ny, width, height, distance, center_x, center_y, spherical, wavelength, out)
     return out.reshape(ny * nx, 3)
+
+
+def hkl_3d(nx, ny, nz, width, height, depth, distance, wavelength, center_x=0.0, center_y=0.0, center_z=0.0):
+    out = np.zeros(nx * ny * nz * 3, np.float32)
+    lib().oracle_hkl_3d(nx, ny, nz, width, height, depth, center_x, center_y, center_z, distance, wavelength, out)
+    return out.reshape(-1, 3)
 
 
 def rotate_hkl(hkl: np.ndarray, R: np.ndarray) -> np.ndarray:

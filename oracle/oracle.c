@@ -18,6 +18,10 @@
  *                             (1 and 2 atoms, atoms at the origin, centrosymmetry, Friedel,
  *                             translation and rotation invariance, delta atoms) and an
  *                             arbitrary-precision (mpmath) sum.
+ *   oracle_hkl_3d             src/diffraction.c:325-369. Pinned by its closed form (a
+ *                             uniform grid of spacing extent/(n d lambda), zero at the
+ *                             centre voxel, exactly antisymmetric about it) and 40-digit
+ *                             mpmath values.
  *   oracle_spectrum_samples,
  *   oracle_sf_spectrum_f64    the wavelength-spectrum path of src/diffraction.cu:300-318,
  *                             416-476 in FP64. Pinned by closed forms (the weights
@@ -98,6 +102,25 @@ int oracle_hkl_detector(int nx, int ny, float width, float height, float distanc
       }
     }
   }
+  return 0;
+}
+
+/* src/diffraction.c:325-369: q = p / (distance * wavelength), the product in float. */
+int oracle_hkl_3d(int nx, int ny, int nz, float width, float height, float depth, float center_x, float center_y,
+                  float center_z, float distance, float wavelength, float *out) {
+  const float dl = distance * wavelength;
+  const double r2r = 1.0 / dl;
+  size_t idx = 0;
+  for (int z = 0; z < nz; ++z)
+    for (int y = 0; y < ny; ++y)
+      for (int x = 0; x < nx; ++x) {
+        const double px = ((x - (nx - 1.0) / 2.0) / nx) * width - center_x;
+        const double py = ((y - (ny - 1.0) / 2.0) / ny) * height - center_y;
+        const double pz = ((z - (nz - 1.0) / 2.0) / nz) * depth - center_z;
+        out[idx++] = (float)(px * r2r);
+        out[idx++] = (float)(py * r2r);
+        out[idx++] = (float)(pz * r2r);
+      }
   return 0;
 }
 

@@ -61,6 +61,7 @@ SIGNATURES = {
     "spsim_hip_pack_atoms": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _vp, _vp, _vp, _vp, C.POINTER(_i), C.POINTER(_i)]),
     "spsim_hip_hkl_detector": (_i, [_i, _i, _f, _f, _f, _f, _f, _i, _f, _vp, _vp]),
     "spsim_hip_rotate_hkl": (_i, [_vp, _i, _vp, _vp, _vp]),
+    "spsim_hip_hkl_detector_3d": (_i, [_i, _i, _i, _f, _f, _f, _f, _f, _f, _f, _f, _vp, _vp]),
     "spsim_hip_sf_pattern": (_i, [_vp, _i, _vp, _vp, _vp, _i, _i, _vp, _f, _vp, _vp, _vp, C.POINTER(Tile), _vp]),
     "spsim_hip_sf_grid": (_i, [_vp, _i, _i, _vp, _vp, _vp, _i, _i, _vp, _f, _vp, _vp, _vp, C.POINTER(Tile), _vp]),
     "spsim_hip_sf_spectrum": (_i, [_vp, _i, _vp, _vp, _vp, _i, _i, _vp, _f, _vp, _i, _vp, _vp, _vp, _vp,

@@ -99,6 +99,23 @@ int spsim_bridge_hkl_detector(int nx, int ny, float width, float height, float d
   return SPSIM_HIP_OK;
 }
 
+int spsim_bridge_hkl_detector_3d(int nx, int ny, int nz, float width, float height, float depth, float center_x,
+                                 float center_y, float center_z, float distance, float wavelength, float *out) {
+  using namespace spsim;
+  SPSIM_TRY(ensure_device(nullptr));
+  const size_t n = (size_t)(nx > 0 ? nx : 0) * (ny > 0 ? ny : 0) * (nz > 0 ? nz : 0) * 3;
+  Stream st;
+  SPSIM_TRY(st.create());
+  DevBufs b;
+  float *d = nullptr;
+  SPSIM_TRY(b.alloc(&d, n));
+  SPSIM_TRY(spsim_hip_hkl_detector_3d(nx, ny, nz, width, height, depth, center_x, center_y, center_z, distance,
+                                      wavelength, d, st.s));
+  SPSIM_TRY(download(out, (const float *)d, n, st.s));
+  SPSIM_HIP_TRY(hipStreamSynchronize(st.s));
+  return SPSIM_HIP_OK;
+}
+
 int spsim_bridge_rotate_hkl(float *list, int npix, const double *rot9) {
   using namespace spsim;
   SPSIM_TRY(ensure_device(nullptr));

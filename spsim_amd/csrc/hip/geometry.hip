@@ -1,9 +1,10 @@
 // Detector geometry kernels: scattering vectors per pixel and their rotation.
 //
 // spsim_hip_hkl_detector reproduces get_HKL_list_for_detector
-// (src/diffraction.c:237-297) operation for operation in FP64 with FMA
-// contraction disabled, so the FP32 list it stores is bit-identical to the
-// host computation of the same expressions.
+// (src/diffraction.c:237-297) and spsim_hip_hkl_detector_3d
+// get_HKL_list_for_3d_detector (:325-369) operation for operation in FP64 with
+// FMA contraction disabled, so the FP32 lists they store are bit-identical to
+// the host computation of the same expressions.
 #include "common.hpp"
 
 namespace spsim {
@@ -42,6 +43,33 @@ __global__ __launch_bounds__(256) void hkl_detector_kernel(HklGeom g, float *__r
     o[1] = (float)(g.fourier_py * py / g.py_size);
     o[2] = 0.0f;
   }
+}
+
+// Uniform reciprocal-space grid of a 3-D detector (get_HKL_list_for_3d_detector,
+// src/diffraction.c:325-369): q = p / (distance * wavelength), with the product
+// distance * wavelength rounded to float as the reference forms it.
+struct Hkl3Geom {
+  int nx, ny, nz;
+  double width, height, depth;
+  double center_x, center_y, center_z;
+  double real_to_reciprocal;
+};
+
+__global__ __launch_bounds__(256) void hkl_3d_kernel(Hkl3Geom g, float *__restrict__ out) {
+#pragma clang fp contract(off)
+  const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
+  const long long plane = (long long)g.nx * g.ny;
+  if (idx >= plane * g.nz) return;
+  const int x = (int)(idx % g.nx);
+  const int y = (int)((idx / g.nx) % g.ny);
+  const int z = (int)(idx / plane);
+  const double px = ((x - (g.nx - 1.0) / 2.0) / g.nx) * g.width - g.center_x;
+  const double py = ((y - (g.ny - 1.0) / 2.0) / g.ny) * g.height - g.center_y;
+  const double pz = ((z - (g.nz - 1.0) / 2.0) / g.nz) * g.depth - g.center_z;
+  float *o = out + 3 * idx;
+  o[0] = (float)(px * g.real_to_reciprocal);
+  o[1] = (float)(py * g.real_to_reciprocal);
+  o[2] = (float)(pz * g.real_to_reciprocal);
 }
 
 struct Rot9 {
@@ -105,6 +133,33 @@ extern "C" int spsim_hip_rotate_hkl(const float *d_in, int npix, const double *r
   for (int i = 0; i < 9; ++i) R.r[i] = rot9[i];
   hipLaunchKernelGGL(rotate_hkl_kernel, dim3((npix + 255) / 256), dim3(256), 0, (hipStream_t)stream, d_in, npix, R,
                      d_out);
+  SPSIM_HIP_TRY(hipGetLastError());
+  return SPSIM_HIP_OK;
+}
+
+extern "C" int spsim_hip_hkl_detector_3d(int nx, int ny, int nz, float width, float height, float depth,
+                                         float center_x, float center_y, float center_z, float distance,
+                                         float wavelength, float *d_hkl, void *stream) {
+  using namespace spsim;
+  if (nx < 0 || ny < 0 || nz < 0) return fail(SPSIM_HIP_ERR_INVALID, "negative detector size %d x %d x %d", nx, ny, nz);
+  const long long n = (long long)nx * ny * nz;
+  if (n > (1LL << 28)) return fail(SPSIM_HIP_ERR_INVALID, "detector %d x %d x %d too large", nx, ny, nz);
+  if (n > 0 && !d_hkl) return fail(SPSIM_HIP_ERR_INVALID, "d_hkl is NULL");
+  SPSIM_TRY(ensure_device(nullptr));
+  if (n == 0) return SPSIM_HIP_OK;
+  Hkl3Geom g;
+  g.nx = nx;
+  g.ny = ny;
+  g.nz = nz;
+  g.width = width;
+  g.height = height;
+  g.depth = depth;
+  g.center_x = center_x;
+  g.center_y = center_y;
+  g.center_z = center_z;
+  const float dl = distance * wavelength;  // a float product in the reference
+  g.real_to_reciprocal = 1.0 / (double)dl;
+  hipLaunchKernelGGL(hkl_3d_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, g, d_hkl);
   SPSIM_HIP_TRY(hipGetLastError());
   return SPSIM_HIP_OK;
 }

@@ -18,6 +18,9 @@ extern "C" {
 int spsim_bridge_hkl_detector(int nx, int ny, float width, float height, float distance, float center_x,
                               float center_y, int spherical, float wavelength, float *out);
 
+int spsim_bridge_hkl_detector_3d(int nx, int ny, int nz, float width, float height, float depth, float center_x,
+                                 float center_y, float center_z, float distance, float wavelength, float *out);
+
 int spsim_bridge_rotate_hkl(float *list, int npix, const double *rot9);
 
 /* F (may be NULL) and I receive the pattern of the packed atoms on the q list. */

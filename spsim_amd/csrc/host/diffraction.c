@@ -102,6 +102,23 @@ float *get_HKL_list_for_detector(CCD *det, Experiment *exp, int *HKL_list_size) 
   return list;
 }
 
+float *get_HKL_list_for_3d_detector(CCD *det, Experiment *exp, int *HKL_list_size) {
+  if (!det || !exp || !HKL_list_size) return NULL;
+  const int nx = det->nx, ny = det->ny, nz = det->nz;
+  const size_t n = (size_t)(nx > 0 ? nx : 0) * (ny > 0 ? ny : 0) * (nz > 0 ? nz : 0);
+  float *list = (float *)malloc(sizeof(float) * 3 * n + 1);
+  if (!list) return NULL;
+  int rc = spsim_bridge_hkl_detector_3d(nx, ny, nz, det->width, det->height, det->depth, det->center_x,
+                                        det->center_y, det->center_z, det->distance, exp->wavelength, list);
+  if (rc != SPSIM_HIP_OK) {
+    spsim_report("get_HKL_list_for_3d_detector", rc);
+    free(list);
+    return NULL;
+  }
+  *HKL_list_size = (int)n;
+  return list;
+}
+
 SpRotation *apply_orientation_to_HKL_list(float **HKL_list, int *HKL_list_size, Options *opts) {
   SpRotation *rot;
   double R[9];

@@ -34,10 +34,7 @@ Molecule *get_molecule(Options *opts) {
 
 float *get_HKL_list(Options *opts, int *HKL_list_size) {
   if (!opts || !opts->detector) return NULL;
-  if (opts->detector->nz > 1) {
-    fprintf(stderr, "spsim: get_HKL_list: 3-D detectors (nz > 1) are not supported\n");
-    return NULL;
-  }
+  if (opts->detector->nz > 1) return get_HKL_list_for_3d_detector(opts->detector, opts->experiment, HKL_list_size);
   return get_HKL_list_for_detector(opts->detector, opts->experiment, HKL_list_size);
 }
 

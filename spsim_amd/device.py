@@ -188,6 +188,17 @@ def hkl_detector(nx, ny, width, height, distance, wavelength, center_x=0.0, cent
     return out
 
 
+def hkl_detector_3d(nx, ny, nz, width, height, depth, distance, wavelength, center=(0.0, 0.0, 0.0), device=None,
+                    stream=None) -> torch.Tensor:
+    """Uniform q grid of a 3-D detector volume, [nz*ny*nx, 3]."""
+    dev = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+    out = torch.empty((nx * ny * nz, 3), dtype=torch.float32, device=dev)
+    check(_lib.load().spsim_hip_hkl_detector_3d(nx, ny, nz, width, height, depth, center[0], center[1], center[2],
+                                                distance, wavelength, _ptr(out), _stream(stream)),
+          "spsim_hip_hkl_detector_3d")
+    return out
+
+
 def rotate_hkl(hkl: torch.Tensor, R, out: torch.Tensor | None = None, stream=None) -> torch.Tensor:
     R = np.ascontiguousarray(R, np.float64).reshape(9)
     if out is None:

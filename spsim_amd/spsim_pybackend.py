@@ -234,6 +234,7 @@ _SIGS = {
     "write_pdb_from_mol": (None, [C.c_char_p, _P(Molecule)]),
     # diffraction.h
     "get_HKL_list_for_detector": (_P(C.c_float), [_P(CCD), _P(Experiment), _P(C.c_int)]),
+    "get_HKL_list_for_3d_detector": (_P(C.c_float), [_P(CCD), _P(Experiment), _P(C.c_int)]),
     "compute_pattern_on_list": (_P(Diffraction_Pattern), [_P(Molecule), _P(C.c_float), C.c_int, C.c_float,
                                                            _P(Experiment), _P(Options)]),
     "vector_compute_pattern_on_list": (_P(Diffraction_Pattern), [_P(Molecule), _P(C.c_float), C.c_int, C.c_float,

@@ -89,3 +89,36 @@ def test_rotation_quarter_turns_are_exact():
         out = oracle.rotate_hkl(q, Ri)
         np.testing.assert_allclose(np.linalg.norm(out, axis=1), np.linalg.norm(q, axis=1), rtol=1e-6)
         np.testing.assert_allclose(out, (Ri @ q.astype(np.float64).T).T, rtol=1e-6, atol=1e-3)
+
+
+# ---------------------------------------------------------------- 3-D detector grid
+
+def test_hkl_3d_closed_form():
+    """q = p / (d lambda) on a uniform grid (src/diffraction.c:325-369), with
+    d lambda rounded to float as the reference forms it."""
+    import mpmath as mp
+
+    mp.mp.dps = 40
+    nx, ny, nz = 7, 6, 5
+    W, H, D = (float(np.float32(v)) for v in (2e-3, 1.5e-3, 1e-3))
+    dist, lam = float(np.float32(0.1)), float(np.float32(1e-9))
+    cx, cy, cz = (float(np.float32(v)) for v in (1e-5, -2e-5, 0.0))
+    q = oracle.hkl_3d(nx, ny, nz, W, H, D, dist, lam, cx, cy, cz).reshape(nz, ny, nx, 3)
+    dl = mp.mpf(float(np.float32(np.float32(dist) * np.float32(lam))))
+    for (z, y, x) in [(0, 0, 0), (4, 5, 6), (2, 3, 3), (1, 2, 5)]:
+        p = [((mp.mpf(i) - (mp.mpf(n) - 1) / 2) / n) * e - c
+             for i, n, e, c in ((x, nx, W, cx), (y, ny, H, cy), (z, nz, D, cz))]
+        want = [float(np.float32(float(v / dl))) for v in p]
+        assert list(q[z, y, x]) == [np.float32(v) for v in want], (z, y, x)
+    # uniform spacing extent / (n d lambda) along each axis
+    sx = np.diff(q[..., 0].astype(np.float64), axis=2)
+    np.testing.assert_allclose(sx, W / nx / float(dl), rtol=1e-6)
+    sz = np.diff(q[..., 2].astype(np.float64), axis=0)
+    np.testing.assert_allclose(sz, D / nz / float(dl), rtol=1e-6)
+
+
+def test_hkl_3d_centred_is_antisymmetric():
+    q = oracle.hkl_3d(6, 5, 4, 1e-3, 1e-3, 1e-3, 0.05, 2e-10)
+    np.testing.assert_array_equal(q[::-1], -q)
+    q = oracle.hkl_3d(5, 5, 5, 1e-3, 1e-3, 1e-3, 0.05, 2e-10)
+    assert np.all(q[q.shape[0] // 2] == 0)
