@@ -221,15 +221,19 @@ typedef struct {
   float readout_noise;    /* e- rms per binned pixel */
   float linear_full_well; /* e- */
   float maximum_value;    /* ADC counts */
+  int nz, bin_z;          /* 3-D detector volume: raw voxels nx*ny*nz, binned (nz/bin_z) deep;
+                             0 means 1 (a 2-D detector) */
 } spsim_hip_detector;
 
-/* For each of npatterns intensity images d_I[p*npix .. ] (npix = nx*ny) runs
+/* For each of npatterns intensity images d_I[p*npix .. ] (npix = nx*ny*nz) runs
    Thomson x solid angle x beam -> photons, Poisson(photons*QE) -> count,
    electrons, binned readout with dark current + Gaussian readout noise clamped
    at 0 and the ADC factor, and the noiseless binned output. Randomness comes
    from Philox-4x32-10 keyed by seed, with counters (pattern_base + p, pixel),
    so results do not depend on launch shape. Any output pointer may be NULL.
-   Raw outputs are [npatterns][npix], binned ones [npatterns][nbinned]. */
+   Raw outputs are [npatterns][npix], binned ones [npatterns][nbinned] with
+   nbinned = (nx/bin_x)*(ny/bin_y)*(nz/bin_z), z slowest. Thomson and solid
+   angle depend on (x, y) only, as in the reference's loops. */
 int spsim_hip_detector_chain(const spsim_hip_detector *det, const float *d_I, int npatterns, uint64_t seed,
                              uint64_t pattern_base, float *d_thomson, float *d_solid_angle, float *d_photons,
                              float *d_count, float *d_electrons, float *d_real_output,

@@ -62,10 +62,10 @@ def lib() -> C.CDLL:
         L.oracle_photons.argtypes = [_f32p, _f32p, _f32p, C.c_float, C.c_size_t, _f32p]
         L.oracle_electrons_per_photon.restype = C.c_float
         L.oracle_electrons_per_photon.argtypes = [C.c_float, C.c_float, C.c_int]
-        L.oracle_noiseless_output.argtypes = [_f32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float,
-                                              C.c_float, C.c_float, _f32p]
-        L.oracle_real_output.argtypes = [_f32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, _f32p,
-                                         C.c_float, C.c_float, _f32p]
+        L.oracle_noiseless_output.argtypes = [_f32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                              C.c_float, C.c_float, C.c_float, C.c_float, _f32p]
+        L.oracle_real_output.argtypes = [_f32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float,
+                                         C.c_float, _f32p, C.c_float, C.c_float, _f32p]
         L.oracle_philox4x32_10.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
         L.oracle_std_normal.restype = C.c_double
         L.oracle_std_normal.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32]
@@ -230,18 +230,18 @@ def electrons_per_photon(wavelength, e_h, noiseless=False):
     return lib().oracle_electrons_per_photon(wavelength, e_h, int(noiseless))
 
 
-def noiseless_output(ph, nx, ny, bx, by, qe, epp, full_well, max_value):
+def noiseless_output(ph, nx, ny, bx, by, qe, epp, full_well, max_value, nz=1, bz=1):
     ph = np.ascontiguousarray(ph, np.float32)
-    out = np.zeros((nx // bx) * (ny // by), np.float32)
-    lib().oracle_noiseless_output(ph, nx, ny, bx, by, qe, epp, full_well, max_value, out)
+    out = np.zeros((nx // bx) * (ny // by) * (nz // bz), np.float32)
+    lib().oracle_noiseless_output(ph, nx, ny, nz, bx, by, bz, qe, epp, full_well, max_value, out)
     return out
 
 
-def real_output(el, nx, ny, bx, by, dark, exposure, noise, full_well, max_value):
+def real_output(el, nx, ny, bx, by, dark, exposure, noise, full_well, max_value, nz=1, bz=1):
     el = np.ascontiguousarray(el, np.float32)
     noise = np.ascontiguousarray(noise, np.float32)
-    out = np.zeros((nx // bx) * (ny // by), np.float32)
-    lib().oracle_real_output(el, nx, ny, bx, by, dark, exposure, noise, full_well, max_value, out)
+    out = np.zeros((nx // bx) * (ny // by) * (nz // bz), np.float32)
+    lib().oracle_real_output(el, nx, ny, nz, bx, by, bz, dark, exposure, noise, full_well, max_value, out)
     return out
 
 

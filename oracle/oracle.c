@@ -473,36 +473,43 @@ static float adc_constant(float full_well, float max_value) {
   return adc;
 }
 
-/* src/amplification.c:119-153 (2-D) */
-void oracle_noiseless_output(const float *photons, int nx, int ny, int bx, int by, float qe, float epp,
-                             float full_well, float max_value, float *out) {
+/* src/amplification.c:119-153; nz = bz = 1 for a 2-D detector */
+void oracle_noiseless_output(const float *photons, int nx, int ny, int nz, int bx, int by, int bz, float qe,
+                             float epp, float full_well, float max_value, float *out) {
   const float adc = adc_constant(full_well, max_value);
   int i = 0;
-  for (int y = 0; y < ny / by; ++y)
-    for (int x = 0; x < nx / bx; ++x) {
-      float acc = 0;
-      for (int yi = 0; yi < by; ++yi)
-        for (int xi = 0; xi < bx; ++xi) acc += photons[(y * by + yi) * nx + (x * bx + xi)] * qe * epp * adc;
-      out[i++] = acc;
-    }
+  for (int z = 0; z < nz / bz; ++z)
+    for (int y = 0; y < ny / by; ++y)
+      for (int x = 0; x < nx / bx; ++x) {
+        float acc = 0;
+        for (int zi = 0; zi < bz; ++zi)
+          for (int yi = 0; yi < by; ++yi)
+            for (int xi = 0; xi < bx; ++xi)
+              acc += photons[((size_t)(z * bz + zi) * ny + (y * by + yi)) * nx + (x * bx + xi)] * qe * epp * adc;
+        out[i++] = acc;
+      }
 }
 
 /* src/amplification.c:77-114 (2-D) with the per-binned-pixel readout noise
    supplied by the caller (noise[i] = the Gaussian draw in electrons). */
-void oracle_real_output(const float *electrons, int nx, int ny, int bx, int by, float dark_current, float exposure,
-                        const float *noise, float full_well, float max_value, float *out) {
+void oracle_real_output(const float *electrons, int nx, int ny, int nz, int bx, int by, int bz, float dark_current,
+                        float exposure, const float *noise, float full_well, float max_value, float *out) {
   const float adc = adc_constant(full_well, max_value);
   int i = 0;
-  for (int y = 0; y < ny / by; ++y)
-    for (int x = 0; x < nx / bx; ++x) {
-      float acc = 0;
-      for (int yi = 0; yi < by; ++yi)
-        for (int xi = 0; xi < bx; ++xi) acc += (electrons[(y * by + yi) * nx + (x * bx + xi)] + dark_current * exposure);
-      acc += noise[i];
-      if (acc < 0) acc = 0;
-      acc *= adc;
-      out[i++] = acc;
-    }
+  for (int z = 0; z < nz / bz; ++z)
+    for (int y = 0; y < ny / by; ++y)
+      for (int x = 0; x < nx / bx; ++x) {
+        float acc = 0;
+        for (int zi = 0; zi < bz; ++zi)
+          for (int yi = 0; yi < by; ++yi)
+            for (int xi = 0; xi < bx; ++xi)
+              acc += (electrons[((size_t)(z * bz + zi) * ny + (y * by + yi)) * nx + (x * bx + xi)] +
+                      dark_current * exposure);
+        acc += noise[i];
+        if (acc < 0) acc = 0;
+        acc *= adc;
+        out[i++] = acc;
+      }
 }
 
 /* ---------------------------------------------------------------- random numbers */

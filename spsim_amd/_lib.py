@@ -43,7 +43,7 @@ class DetectorParams(C.Structure):
         ("beam_intensity", C.c_float), ("quantum_efficiency", C.c_float), ("wavelength", C.c_float),
         ("electron_hole_production_energy", C.c_float), ("dark_current", C.c_float),
         ("exposure_time", C.c_float), ("readout_noise", C.c_float), ("linear_full_well", C.c_float),
-        ("maximum_value", C.c_float),
+        ("maximum_value", C.c_float), ("nz", C.c_int), ("bin_z", C.c_int),
     ]
 
 

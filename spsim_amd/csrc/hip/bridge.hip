@@ -72,10 +72,13 @@ int run_stage(const spsim_hip_detector *det, Stage st, const float *h0, const fl
   return SPSIM_HIP_OK;
 }
 
-size_t raw_count(const spsim_hip_detector *det) { return det ? (size_t)det->nx * det->ny : 0; }
+size_t raw_count(const spsim_hip_detector *det) {
+  return det ? (size_t)det->nx * det->ny * (det->nz > 0 ? det->nz : 1) : 0;
+}
 size_t bin_count(const spsim_hip_detector *det) {
   if (!det || det->bin_x < 1 || det->bin_y < 1) return 0;
-  return (size_t)(det->nx / det->bin_x) * (det->ny / det->bin_y);
+  const int nz = det->nz > 0 ? det->nz : 1, bz = det->bin_z > 0 ? det->bin_z : 1;
+  return (size_t)(det->nx / det->bin_x) * (det->ny / det->bin_y) * (nz / bz);
 }
 
 }  // namespace

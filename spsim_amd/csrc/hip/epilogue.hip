@@ -68,7 +68,8 @@ struct RawPixel {
 
 // Stages of one raw pixel: photons (src/diffraction.c:1202), Poisson count
 // (src/noise.c:42-46), electrons (src/amplification.c:66-68).
-__device__ __forceinline__ RawPixel raw_pixel(const DetConst &d, int x, int y, float I, uint64_t seed,
+// idx is the raw pixel's flat index (z * ny + y) * nx + x, the Poisson draw's counter.
+__device__ __forceinline__ RawPixel raw_pixel(const DetConst &d, int x, int y, uint32_t idx, float I, uint64_t seed,
                                               uint32_t pattern) {
 #pragma clang fp contract(off)
   RawPixel r;
@@ -76,7 +77,7 @@ __device__ __forceinline__ RawPixel raw_pixel(const DetConst &d, int x, int y, f
   r.om = solid_angle_at(d, x, y);
   r.ph = r.th * r.om * I * d.beam_intensity;
   const float mean = r.ph * d.qe;
-  r.cnt = (float)poisson_sample((double)mean, seed, (uint32_t)(y * d.nx + x), pattern);
+  r.cnt = (float)poisson_sample((double)mean, seed, idx, pattern);
   r.el = r.cnt * d.e_per_photon;
   return r;
 }
@@ -102,31 +103,34 @@ struct ChainOut {
 __global__ __launch_bounds__(kT) void chain_binned_kernel(DetConst d, const float *__restrict__ I, int npatterns,
                                                           uint64_t seed, uint64_t pattern_base, ChainOut o) {
 #pragma clang fp contract(off)
-  const int nbx = d.nx / d.bin_x, nby = d.ny / d.bin_y;
-  const long long nb = (long long)nbx * nby;
+  const int nbx = d.nx / d.bin_x, nby = d.ny / d.bin_y, nbz = d.nz / d.bin_z;
+  const long long nb = (long long)nbx * nby * nbz;
   const long long gid = (long long)blockIdx.x * kT + threadIdx.x;
   if (gid >= nb * npatterns) return;
   const int p = (int)(gid / nb);
   const int bin = (int)(gid % nb);
-  const int bx = bin % nbx, by = bin / nbx;
+  const int bx = bin % nbx, by = (bin / nbx) % nby, bz = bin / (nbx * nby);
   const uint32_t pat = (uint32_t)(pattern_base + p);
-  const size_t npix = (size_t)d.nx * d.ny;
+  const size_t npix = (size_t)d.nx * d.ny * d.nz;
   const float *Ip = I + (size_t)p * npix;
   const float dark = d.dark_current * d.exposure_time;
   float acc = 0.0f, accn = 0.0f;
-  for (int yi = 0; yi < d.bin_y; ++yi) {
-    for (int xi = 0; xi < d.bin_x; ++xi) {
-      const int x = bx * d.bin_x + xi, y = by * d.bin_y + yi;
-      const size_t i = (size_t)y * d.nx + x;
-      const RawPixel r = raw_pixel(d, x, y, Ip[i], seed, pat);
-      acc += (r.el + dark);
-      accn += r.ph * d.qe * d.e_per_photon_noiseless * d.adc;
-      const size_t oi = (size_t)p * npix + i;
-      if (o.th) o.th[oi] = r.th;
-      if (o.om) o.om[oi] = r.om;
-      if (o.ph) o.ph[oi] = r.ph;
-      if (o.cnt) o.cnt[oi] = r.cnt;
-      if (o.el) o.el[oi] = r.el;
+  // bin sums in (z, y, x) order (src/amplification.c:94-102, 139-147)
+  for (int zi = 0; zi < d.bin_z; ++zi) {
+    for (int yi = 0; yi < d.bin_y; ++yi) {
+      for (int xi = 0; xi < d.bin_x; ++xi) {
+        const int x = bx * d.bin_x + xi, y = by * d.bin_y + yi, z = bz * d.bin_z + zi;
+        const size_t i = ((size_t)z * d.ny + y) * d.nx + x;
+        const RawPixel r = raw_pixel(d, x, y, (uint32_t)i, Ip[i], seed, pat);
+        acc += (r.el + dark);
+        accn += r.ph * d.qe * d.e_per_photon_noiseless * d.adc;
+        const size_t oi = (size_t)p * npix + i;
+        if (o.th) o.th[oi] = r.th;
+        if (o.om) o.om[oi] = r.om;
+        if (o.ph) o.ph[oi] = r.ph;
+        if (o.cnt) o.cnt[oi] = r.cnt;
+        if (o.el) o.el[oi] = r.el;
+      }
     }
   }
   const size_t ob = (size_t)p * nb + bin;
@@ -134,19 +138,19 @@ __global__ __launch_bounds__(kT) void chain_binned_kernel(DetConst d, const floa
   if (o.noiseless) o.noiseless[ob] = accn;
 }
 
-// Raw outputs for pixels outside complete bins (only when nx or ny is not a
-// multiple of the binning and raw outputs were requested).
+// Raw outputs for pixels outside complete bins (only when nx, ny or nz is not
+// a multiple of the binning and raw outputs were requested).
 __global__ __launch_bounds__(kT) void chain_leftover_kernel(DetConst d, const float *__restrict__ I, int npatterns,
                                                             uint64_t seed, uint64_t pattern_base, ChainOut o) {
-  const size_t npix = (size_t)d.nx * d.ny;
+  const size_t npix = (size_t)d.nx * d.ny * d.nz;
   const long long gid = (long long)blockIdx.x * kT + threadIdx.x;
   if (gid >= (long long)npix * npatterns) return;
   const int p = (int)(gid / (long long)npix);
   const size_t i = (size_t)(gid % (long long)npix);
-  const int x = (int)(i % d.nx), y = (int)(i / d.nx);
-  const int nbx = d.nx / d.bin_x, nby = d.ny / d.bin_y;
-  if (x < nbx * d.bin_x && y < nby * d.bin_y) return;  // covered by chain_binned_kernel
-  const RawPixel r = raw_pixel(d, x, y, I[(size_t)p * npix + i], seed, (uint32_t)(pattern_base + p));
+  const int x = (int)(i % d.nx), y = (int)((i / d.nx) % d.ny), z = (int)(i / ((size_t)d.nx * d.ny));
+  const int nbx = d.nx / d.bin_x, nby = d.ny / d.bin_y, nbz = d.nz / d.bin_z;
+  if (x < nbx * d.bin_x && y < nby * d.bin_y && z < nbz * d.bin_z) return;  // covered by chain_binned_kernel
+  const RawPixel r = raw_pixel(d, x, y, (uint32_t)i, I[(size_t)p * npix + i], seed, (uint32_t)(pattern_base + p));
   const size_t oi = (size_t)p * npix + i;
   if (o.th) o.th[oi] = r.th;
   if (o.om) o.om[oi] = r.om;
@@ -157,16 +161,18 @@ __global__ __launch_bounds__(kT) void chain_leftover_kernel(DetConst d, const fl
 
 // ---- stage kernels (one raw pixel or one binned pixel per lane) ----
 
+__device__ __forceinline__ int raw_count(const DetConst &d) { return d.nx * d.ny * d.nz; }
+
 __global__ __launch_bounds__(kT) void thomson_kernel(DetConst d, float *__restrict__ out) {
   const int i = blockIdx.x * kT + threadIdx.x;
-  if (i >= d.nx * d.ny) return;
-  out[i] = thomson_at(d, i % d.nx, i / d.nx);
+  if (i >= raw_count(d)) return;
+  out[i] = thomson_at(d, i % d.nx, (i / d.nx) % d.ny);
 }
 
 __global__ __launch_bounds__(kT) void solid_angle_kernel(DetConst d, float *__restrict__ out) {
   const int i = blockIdx.x * kT + threadIdx.x;
-  if (i >= d.nx * d.ny) return;
-  out[i] = solid_angle_at(d, i % d.nx, i / d.nx);
+  if (i >= raw_count(d)) return;
+  out[i] = solid_angle_at(d, i % d.nx, (i / d.nx) % d.ny);
 }
 
 __global__ __launch_bounds__(kT) void photons_kernel(DetConst d, const float *__restrict__ th,
@@ -174,7 +180,7 @@ __global__ __launch_bounds__(kT) void photons_kernel(DetConst d, const float *__
                                                      float *__restrict__ out) {
 #pragma clang fp contract(off)
   const int i = blockIdx.x * kT + threadIdx.x;
-  if (i >= d.nx * d.ny) return;
+  if (i >= raw_count(d)) return;
   out[i] = th[i] * om[i] * I[i] * d.beam_intensity;
 }
 
@@ -182,7 +188,7 @@ __global__ __launch_bounds__(kT) void poisson_kernel(DetConst d, const float *__
                                                      uint32_t pattern, float *__restrict__ out) {
 #pragma clang fp contract(off)
   const int i = blockIdx.x * kT + threadIdx.x;
-  if (i >= d.nx * d.ny) return;
+  if (i >= raw_count(d)) return;
   const float mean = ph[i] * d.qe;
   out[i] = (float)poisson_sample((double)mean, seed, (uint32_t)i, pattern);
 }
@@ -190,35 +196,39 @@ __global__ __launch_bounds__(kT) void poisson_kernel(DetConst d, const float *__
 __global__ __launch_bounds__(kT) void electrons_kernel(DetConst d, const float *__restrict__ cnt,
                                                        float *__restrict__ out) {
   const int i = blockIdx.x * kT + threadIdx.x;
-  if (i >= d.nx * d.ny) return;
+  if (i >= raw_count(d)) return;
   out[i] = cnt[i] * d.e_per_photon;
 }
 
 __global__ __launch_bounds__(kT) void real_output_kernel(DetConst d, const float *__restrict__ el, uint64_t seed,
                                                          uint32_t pattern, float *__restrict__ out) {
 #pragma clang fp contract(off)
-  const int nbx = d.nx / d.bin_x, nby = d.ny / d.bin_y;
+  const int nbx = d.nx / d.bin_x, nby = d.ny / d.bin_y, nbz = d.nz / d.bin_z;
   const int bin = blockIdx.x * kT + threadIdx.x;
-  if (bin >= nbx * nby) return;
-  const int bx = bin % nbx, by = bin / nbx;
+  if (bin >= nbx * nby * nbz) return;
+  const int bx = bin % nbx, by = (bin / nbx) % nby, bz = bin / (nbx * nby);
   const float dark = d.dark_current * d.exposure_time;
   float acc = 0.0f;
-  for (int yi = 0; yi < d.bin_y; ++yi)
-    for (int xi = 0; xi < d.bin_x; ++xi) acc += (el[(by * d.bin_y + yi) * d.nx + (bx * d.bin_x + xi)] + dark);
+  for (int zi = 0; zi < d.bin_z; ++zi)
+    for (int yi = 0; yi < d.bin_y; ++yi)
+      for (int xi = 0; xi < d.bin_x; ++xi)
+        acc += (el[((bz * d.bin_z + zi) * d.ny + (by * d.bin_y + yi)) * d.nx + (bx * d.bin_x + xi)] + dark);
   out[bin] = finish_real(d, acc, seed, (uint32_t)bin, pattern);
 }
 
 __global__ __launch_bounds__(kT) void noiseless_output_kernel(DetConst d, const float *__restrict__ ph,
                                                               float *__restrict__ out) {
 #pragma clang fp contract(off)
-  const int nbx = d.nx / d.bin_x, nby = d.ny / d.bin_y;
+  const int nbx = d.nx / d.bin_x, nby = d.ny / d.bin_y, nbz = d.nz / d.bin_z;
   const int bin = blockIdx.x * kT + threadIdx.x;
-  if (bin >= nbx * nby) return;
-  const int bx = bin % nbx, by = bin / nbx;
+  if (bin >= nbx * nby * nbz) return;
+  const int bx = bin % nbx, by = (bin / nbx) % nby, bz = bin / (nbx * nby);
   float acc = 0.0f;
-  for (int yi = 0; yi < d.bin_y; ++yi)
-    for (int xi = 0; xi < d.bin_x; ++xi)
-      acc += ph[(by * d.bin_y + yi) * d.nx + (bx * d.bin_x + xi)] * d.qe * d.e_per_photon_noiseless * d.adc;
+  for (int zi = 0; zi < d.bin_z; ++zi)
+    for (int yi = 0; yi < d.bin_y; ++yi)
+      for (int xi = 0; xi < d.bin_x; ++xi)
+        acc += ph[((bz * d.bin_z + zi) * d.ny + (by * d.bin_y + yi)) * d.nx + (bx * d.bin_x + xi)] * d.qe *
+               d.e_per_photon_noiseless * d.adc;
   out[bin] = acc;
 }
 
@@ -228,13 +238,16 @@ inline unsigned blocks_for(long long n) { return (unsigned)((n + kT - 1) / kT); 
 
 int make_det_const(const spsim_hip_detector *p, DetConst *d) {
   if (!p || !d) return fail(SPSIM_HIP_ERR_INVALID, "detector is NULL");
-  if (p->nx < 0 || p->ny < 0 || (long long)p->nx * p->ny > (1LL << 28))
-    return fail(SPSIM_HIP_ERR_INVALID, "bad detector size %d x %d", p->nx, p->ny);
-  if (p->bin_x < 1 || p->bin_y < 1) return fail(SPSIM_HIP_ERR_INVALID, "binning must be >= 1");
+  const int nz = p->nz > 0 ? p->nz : 1, bin_z = p->bin_z > 0 ? p->bin_z : 1;
+  if (p->nx < 0 || p->ny < 0 || p->nz < 0 || (long long)p->nx * p->ny * nz > (1LL << 28))
+    return fail(SPSIM_HIP_ERR_INVALID, "bad detector size %d x %d x %d", p->nx, p->ny, p->nz);
+  if (p->bin_x < 1 || p->bin_y < 1 || p->bin_z < 0) return fail(SPSIM_HIP_ERR_INVALID, "binning must be >= 1");
   d->nx = p->nx;
   d->ny = p->ny;
+  d->nz = nz;
   d->bin_x = p->bin_x;
   d->bin_y = p->bin_y;
+  d->bin_z = bin_z;
   d->width = p->width;
   d->height = p->height;
   d->center_x = p->center_x;
@@ -265,8 +278,8 @@ int make_det_const(const spsim_hip_detector *p, DetConst *d) {
 
 int launch_stage(Stage st, const DetConst &d, const float *in0, const float *in1, const float *in2, uint64_t seed,
                  uint32_t pattern, float *out, hipStream_t s) {
-  const long long npix = (long long)d.nx * d.ny;
-  const long long nbin = (long long)(d.nx / d.bin_x) * (d.ny / d.bin_y);
+  const long long npix = (long long)d.nx * d.ny * d.nz;
+  const long long nbin = (long long)(d.nx / d.bin_x) * (d.ny / d.bin_y) * (d.nz / d.bin_z);
   switch (st) {
     case Stage::kThomson:
       if (npix) hipLaunchKernelGGL(thomson_kernel, dim3(blocks_for(npix)), dim3(kT), 0, s, d, out);
@@ -298,15 +311,15 @@ int launch_chain(const DetConst &d, const float *I, int npatterns, uint64_t seed
                  float *om, float *ph, float *cnt, float *el, float *real, float *noiseless, hipStream_t s) {
   if (npatterns <= 0) return SPSIM_HIP_OK;
   ChainOut o{th, om, ph, cnt, el, real, noiseless};
-  const long long nb = (long long)(d.nx / d.bin_x) * (d.ny / d.bin_y);
+  const long long nb = (long long)(d.nx / d.bin_x) * (d.ny / d.bin_y) * (d.nz / d.bin_z);
   if (nb > 0) {
     hipLaunchKernelGGL(chain_binned_kernel, dim3(blocks_for(nb * npatterns)), dim3(kT), 0, s, d, I, npatterns, seed,
                        pattern_base, o);
     SPSIM_HIP_TRY(hipGetLastError());
   }
   const bool raw = th || om || ph || cnt || el;
-  const bool ragged = (d.nx % d.bin_x) != 0 || (d.ny % d.bin_y) != 0;
-  const long long npix = (long long)d.nx * d.ny;
+  const bool ragged = (d.nx % d.bin_x) != 0 || (d.ny % d.bin_y) != 0 || (d.nz % d.bin_z) != 0;
+  const long long npix = (long long)d.nx * d.ny * d.nz;
   if (raw && ragged && npix > 0) {
     hipLaunchKernelGGL(chain_leftover_kernel, dim3(blocks_for(npix * npatterns)), dim3(kT), 0, s, d, I, npatterns, seed,
                        pattern_base, o);
@@ -325,9 +338,10 @@ extern "C" int spsim_hip_detector_chain(const spsim_hip_detector *det, const flo
   DetConst d;
   SPSIM_TRY(make_det_const(det, &d));
   if (npatterns < 0) return fail(SPSIM_HIP_ERR_INVALID, "negative npatterns");
-  if ((long long)npatterns * d.nx * d.ny > (1LL << 31))
+  if ((long long)npatterns * d.nx * d.ny * d.nz > (1LL << 31))
     return fail(SPSIM_HIP_ERR_INVALID, "npatterns * npix exceeds 2^31");
-  if (npatterns > 0 && (long long)d.nx * d.ny > 0 && !d_I) return fail(SPSIM_HIP_ERR_INVALID, "d_I is NULL");
+  if (npatterns > 0 && (long long)d.nx * d.ny * d.nz > 0 && !d_I)
+    return fail(SPSIM_HIP_ERR_INVALID, "d_I is NULL");
   SPSIM_TRY(ensure_device(nullptr));
   return launch_chain(d, d_I, npatterns, seed, pattern_base, d_thomson, d_solid_angle, d_photons, d_count, d_electrons,
                       d_real_output, d_noiseless_output, (hipStream_t)stream);

@@ -8,6 +8,7 @@ namespace spsim {
 // Detector constants in the precision the reference computes them.
 struct DetConst {
   int nx, ny, bin_x, bin_y;
+  int nz, bin_z;  // 1 for a 2-D detector
   double width, height, center_x, center_y;  // float inputs widened
   double distance;
   double dist2;       // (double)(float distance * float distance)

@@ -13,17 +13,15 @@
 #include "noise.h"
 #include "spsim_internal.h"
 
-static size_t binned_pixels(const CCD *det) {
-  const int bx = det->binning_x > 0 ? det->binning_x : 1, by = det->binning_y > 0 ? det->binning_y : 1;
-  return (size_t)(det->nx / bx) * (det->ny / by);
+static size_t raw_pixels(const CCD *det) {
+  return (size_t)det->nx * det->ny * (det->nz > 0 ? det->nz : 1);
 }
 
-static int check_2d(const CCD *det, const char *what) {
-  if (det->nz > 1) {
-    fprintf(stderr, "spsim: %s: 3-D detectors (nz > 1) are not supported\n", what);
-    return 0;
-  }
-  return 1;
+/* (nx/bx)*(ny/by)*(nz/bz), src/amplification.c:88,133 */
+static size_t binned_pixels(const CCD *det) {
+  const int bx = det->binning_x > 0 ? det->binning_x : 1, by = det->binning_y > 0 ? det->binning_y : 1;
+  const int nz = det->nz > 0 ? det->nz : 1, bz = det->binning_z > 0 ? det->binning_z : 1;
+  return (size_t)(det->nx / bx) * (det->ny / by) * (nz / bz);
 }
 
 void generate_poisson_noise(CCD *det) {
@@ -33,9 +31,8 @@ void generate_poisson_noise(CCD *det) {
     fprintf(stderr, "Please calculate photons per pixel first\n");
     return;
   }
-  if (!check_2d(det, "generate_poisson_noise")) return;
   spsim_detector_params(det, NULL, &p);
-  float *out = (float *)malloc(sizeof(float) * (size_t)det->nx * det->ny + 1);
+  float *out = (float *)malloc(sizeof(float) * raw_pixels(det) + 1);
   int rc = spsim_bridge_poisson(&p, det->photons_per_pixel, spsim_noise_seed(), spsim_next_poisson_pattern(), out);
   if (rc != SPSIM_HIP_OK) {
     spsim_report("generate_poisson_noise", rc);
@@ -52,9 +49,8 @@ void calculate_electrons_per_pixel(CCD *det, Experiment *experiment) {
     fprintf(stderr, "Calculate photon_count first\n");
     return;
   }
-  if (!check_2d(det, "calculate_electrons_per_pixel")) return;
   spsim_detector_params(det, experiment, &p);
-  float *out = (float *)malloc(sizeof(float) * (size_t)det->nx * det->ny + 1);
+  float *out = (float *)malloc(sizeof(float) * raw_pixels(det) + 1);
   int rc = spsim_bridge_electrons(&p, det->photon_count, out);
   if (rc != SPSIM_HIP_OK) {
     spsim_report("calculate_electrons_per_pixel", rc);
@@ -71,7 +67,6 @@ void calculate_real_detector_output(CCD *det, Experiment *experiment) {
     fprintf(stderr, "Calculate electrons_per_pixel first\n");
     return;
   }
-  if (!check_2d(det, "calculate_real_detector_output")) return;
   spsim_detector_params(det, experiment, &p);
   float *out = (float *)malloc(sizeof(float) * binned_pixels(det) + 1);
   int rc = spsim_bridge_real_output(&p, det->electrons_per_pixel, spsim_noise_seed(), spsim_next_readout_pattern(), out);
@@ -90,7 +85,6 @@ void calculate_noiseless_detector_output(CCD *det, Experiment *experiment) {
     fprintf(stderr, "Calculate photons_per_pixel first\n");
     return;
   }
-  if (!check_2d(det, "calculate_noiseless_detector_output")) return;
   spsim_detector_params(det, experiment, &p);
   float *out = (float *)malloc(sizeof(float) * binned_pixels(det) + 1);
   int rc = spsim_bridge_noiseless_output(&p, det->photons_per_pixel, out);

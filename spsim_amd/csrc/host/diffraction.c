@@ -276,6 +276,8 @@ void spsim_detector_params(const CCD *det, const Experiment *exp, spsim_hip_dete
   p->readout_noise = det->readout_noise;
   p->linear_full_well = det->linear_full_well;
   p->maximum_value = det->maximum_value;
+  p->nz = det->nz > 0 ? det->nz : 1;
+  p->bin_z = det->binning_z > 0 ? det->binning_z : 1;
   if (exp) {
     p->polarization = exp->polarization;
     p->beam_intensity = exp->beam_intensity;
@@ -289,19 +291,10 @@ static size_t raw_pixels(const CCD *det) {
   return (size_t)(det->nx > 0 ? det->nx : 0) * (det->ny > 0 ? det->ny : 0) * nz;
 }
 
-/* The 2-D stage result of one z slice is repeated across z, as the reference's
-   loops compute it from (x, y) only. */
-static void replicate_z(const CCD *det, float *a) {
-  const int nz = det->nz > 0 ? det->nz : 1;
-  const size_t slice = (size_t)det->nx * det->ny;
-  for (int z = 1; z < nz; ++z) memcpy(a + z * slice, a, slice * sizeof(float));
-}
-
 void calculate_thomson_correction(CCD *det, Experiment *exp) {
   spsim_hip_detector p;
   if (!det) return;
   spsim_detector_params(det, exp, &p);
-  if (det->nz > 1) p.polarization = POLARIZATION_IGNORE;
   float *out = (float *)malloc(sizeof(float) * raw_pixels(det) + 1);
   int rc = spsim_bridge_thomson(&p, out);
   if (rc != SPSIM_HIP_OK) {
@@ -309,7 +302,6 @@ void calculate_thomson_correction(CCD *det, Experiment *exp) {
     free(out);
     return;
   }
-  replicate_z(det, out);
   det->thomson_correction = out;
 }
 
@@ -324,7 +316,6 @@ void calculate_pixel_solid_angle(CCD *det) {
     free(out);
     return;
   }
-  replicate_z(det, out);
   det->solid_angle = out;
 }
 
@@ -336,8 +327,6 @@ void calculate_photons_per_pixel(Diffraction_Pattern *pattern, Options *opts) {
   if (!det->solid_angle) calculate_pixel_solid_angle(det);
   if (!det->thomson_correction || !det->solid_angle) return;
   spsim_detector_params(det, opts->experiment, &p);
-  /* all raw pixels (nz slices included) are one flat list for this stage */
-  p.ny = det->ny * (det->nz > 0 ? det->nz : 1);
   float *out = (float *)malloc(sizeof(float) * raw_pixels(det) + 1);
   int rc = spsim_bridge_photons(&p, det->thomson_correction, det->solid_angle, pattern->ints, out);
   if (rc != SPSIM_HIP_OK) {

@@ -147,8 +147,9 @@ Diffraction_Pattern *simulate_shot(Molecule *mol, Options *opts) {
   CCD *det = opts->detector;
   spsim_hip_detector p;
   spsim_detector_params(det, opts->experiment, &p);
-  const size_t n = (size_t)det->nx * det->ny;
-  const size_t nb = (size_t)(det->nx / p.bin_x) * (det->ny / p.bin_y);
+  const int nz = p.nz > 0 ? p.nz : 1, bz = p.bin_z > 0 ? p.bin_z : 1;
+  const size_t n = (size_t)det->nx * det->ny * nz;
+  const size_t nb = (size_t)(det->nx / p.bin_x) * (det->ny / p.bin_y) * (nz / bz);
   float *th = (float *)malloc(sizeof(float) * n + 1), *om = (float *)malloc(sizeof(float) * n + 1);
   float *ph = (float *)malloc(sizeof(float) * n + 1), *cnt = (float *)malloc(sizeof(float) * n + 1);
   float *el = (float *)malloc(sizeof(float) * n + 1);

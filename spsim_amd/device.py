@@ -210,7 +210,8 @@ def rotate_hkl(hkl: torch.Tensor, R, out: torch.Tensor | None = None, stream=Non
 
 def detector_params(**kw) -> DetectorParams:
     p = DetectorParams()
-    p.bin_x = p.bin_y = 1
+    p.bin_x = p.bin_y = p.bin_z = 1
+    p.nz = 1
     for k, v in kw.items():
         setattr(p, k, v)
     return p
@@ -219,8 +220,9 @@ def detector_params(**kw) -> DetectorParams:
 def detector_chain(det: DetectorParams, I: torch.Tensor, npatterns: int = 1, seed: int = 0, pattern_base: int = 0,
                    outputs=("real_output", "noiseless_output"), stream=None) -> dict:
     """Runs the fused detector chain; returns the requested outputs as tensors."""
-    npix = det.nx * det.ny
-    nb = (det.nx // det.bin_x) * (det.ny // det.bin_y)
+    nz, bz = max(det.nz, 1), max(det.bin_z, 1)
+    npix = det.nx * det.ny * nz
+    nb = (det.nx // det.bin_x) * (det.ny // det.bin_y) * (nz // bz)
     names = ["thomson", "solid_angle", "photons", "count", "electrons", "real_output", "noiseless_output"]
     sizes = [npix] * 5 + [nb] * 2
     res = {}

@@ -137,3 +137,36 @@ def test_ragged_binning_raw_outputs(gpu):
     th = oracle.thomson(30, 21, p["width"], p["height"], p["distance"], p["center_x"], p["center_y"], 0)
     assert np.array_equal(o["thomson"][0], th)  # every raw pixel, including those outside full bins
     assert o["real_output"].shape[1] == (30 // 4) * (21 // 4)
+
+
+@pytest.mark.parametrize("shape,bins", [((16, 12, 6), (4, 4, 2)), ((15, 10, 7), (2, 3, 3))])
+def test_three_d_volume_chain(gpu, shape, bins):
+    """3-D detector (src/amplification.c:88-110, 133-150): Thomson and solid angle
+    per (x, y) repeated over z, Poisson counters over the flat voxel index, bins
+    summed in (z, y, x) order; ragged volumes still give every raw voxel."""
+    nx, ny, nz = shape
+    bx, by, bz = bins
+    p = params(nx=nx, ny=ny, bx=bx, by=by, nz=nz, bin_z=bz, spherical=1, beam_intensity=1e32)
+    rng = np.random.default_rng(12)
+    I = (10.0 ** rng.uniform(4.0, 8.0, nx * ny * nz)).astype(np.float32)
+    seed, base = 31, 2
+    o = run_chain(gpu, p, I, seed=seed, base=base)
+    th2 = oracle.thomson(nx, ny, p["width"], p["height"], p["distance"], p["center_x"], p["center_y"], 0)
+    om2 = oracle.solid_angle(nx, ny, p["width"], p["height"], p["distance"], p["pixel_width"], p["pixel_height"],
+                             p["center_x"], p["center_y"], 1)
+    assert np.array_equal(o["thomson"][0], np.tile(th2, nz))
+    assert np.array_equal(o["solid_angle"][0], np.tile(om2, nz))
+    ph = oracle.photons(o["thomson"][0], o["solid_angle"][0], I, p["beam_intensity"])
+    assert np.array_equal(o["photons"][0], ph)
+    lam = (ph * np.float32(p["quantum_efficiency"])).astype(np.float64)
+    ref = np.array([oracle.poisson(l, seed, i, base) for i, l in enumerate(lam)])
+    assert np.mean(o["count"][0] == ref) > 0.999
+    epp_nl = oracle.electrons_per_photon(p["wavelength"], p["electron_hole_production_energy"], noiseless=True)
+    nl = oracle.noiseless_output(ph, nx, ny, bx, by, p["quantum_efficiency"], epp_nl, p["linear_full_well"],
+                                 p["maximum_value"], nz=nz, bz=bz)
+    assert np.array_equal(o["noiseless_output"][0], nl)
+    nb = (nx // bx) * (ny // by) * (nz // bz)
+    noise = np.array([p["readout_noise"] * oracle.std_normal(seed, b, base, 2) for b in range(nb)], np.float32)
+    ro = oracle.real_output(o["electrons"][0], nx, ny, bx, by, p["dark_current"], p["exposure_time"], noise,
+                            p["linear_full_well"], p["maximum_value"], nz=nz, bz=bz)
+    np.testing.assert_allclose(o["real_output"][0], ro, rtol=1e-6, atol=1e-3)

@@ -119,3 +119,35 @@ def test_std_normal_moments():
     assert abs(x.mean()) < 5 / math.sqrt(n)
     assert abs(x.var() - 1) < 5 * math.sqrt(2 / n)
     assert abs(np.mean(x < -1.0) - 0.158655) < 0.01
+
+
+def test_three_d_binning_hand_sums():
+    """Binned 3-D outputs (src/amplification.c:88-110, 133-150): each output voxel
+    is the float sum over its bx x by x bz block in (z, y, x) order."""
+    nx, ny, nz, bx, by, bz = 5, 4, 6, 2, 2, 3
+    rng = np.random.default_rng(3)
+    ph = rng.uniform(0, 10, nx * ny * nz).astype(np.float32)
+    out = oracle.noiseless_output(ph, nx, ny, bx, by, 1.0, 1.0, 0.0, 0.0, nz=nz, bz=bz)
+    vol = ph.reshape(nz, ny, nx)
+    want = []
+    for z in range(nz // bz):
+        for y in range(ny // by):
+            for x in range(nx // bx):
+                acc = np.float32(0)
+                for zi in range(bz):
+                    for yi in range(by):
+                        for xi in range(bx):
+                            acc = np.float32(acc + vol[z * bz + zi, y * by + yi, x * bx + xi])
+                want.append(acc)
+    assert np.array_equal(out, np.array(want, np.float32))
+    # real output: + dark * exposure per voxel, + noise per bin, clamp, ADC; a 2-D call is nz = bz = 1
+    el = rng.uniform(0, 5, nx * ny * nz).astype(np.float32)
+    noise = rng.normal(0, 3, (nx // bx) * (ny // by) * (nz // bz)).astype(np.float32)
+    ro = oracle.real_output(el, nx, ny, bx, by, 0.5, 2.0, noise, 100.0, 50.0, nz=nz, bz=bz)
+    ev = el.reshape(nz, ny, nx)[: nz // bz * bz, : ny // by * by, : nx // bx * bx]  # whole bins only
+    blocks = ev.reshape(nz // bz, bz, ny // by, by, nx // bx, bx).transpose(0, 2, 4, 1, 3, 5).reshape(len(noise), -1)
+    np.testing.assert_allclose(ro, np.maximum(blocks.sum(1) + 1.0 * bx * by * bz + noise, 0) * 0.5, rtol=1e-5)
+    # the first z slab of a bz = 1 volume is the 2-D result of the first slice
+    two_d = oracle.noiseless_output(ph[: nx * ny], nx, ny, bx, by, 1.0, 1.0, 0.0, 0.0)
+    slab = oracle.noiseless_output(ph, nx, ny, bx, by, 1.0, 1.0, 0.0, 0.0, nz=nz, bz=1)
+    assert np.array_equal(slab[: two_d.size], two_d)
