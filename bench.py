@@ -67,6 +67,12 @@ def rel_l2(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
+def kernel_name(kind, tile):
+    if kind == "grid":
+        return "sf_octant_kernel" if tile.pixels_per_lane == 8 else "sf_grid_kernel"
+    return "sf_pattern_kernel"
+
+
 def headline(args, torch, dv, wl, _lib):
     dev = torch.device("cuda", 0)
     det, Z, pos, B = wl.config2()
@@ -88,7 +94,8 @@ def headline(args, torch, dv, wl, _lib):
             dv.structure_factor(q, atoms, ff, F=F, I=I, tile=t)
 
     candidates = [("pattern", dv.make_tile(k, c, 1)) for k in (2, 4) for c in (128, 256)]
-    candidates += [("grid", dv.make_tile(k, c, 1)) for k in (1, 2) for c in (128, 256)]
+    # grid kernels: 1 or 2 mirror quads per lane, or one octant (8 pixels) per lane
+    candidates += [("grid", dv.make_tile(k, c, 1)) for k in (1, 2, 8) for c in (128, 256)]
     if args.kernel != "auto":
         candidates = [c for c in candidates if c[0] == args.kernel]
     best, best_ms, sweep = None, 1e30, []
@@ -143,13 +150,19 @@ def headline(args, torch, dv, wl, _lib):
                  "chunk_atoms": tile.chunk_atoms, "atom_splits": tile.atom_splits, "sweep": sweep},
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
-                     "kernel": "sf_grid_kernel" if kind == "grid" else "sf_pattern_kernel",
+                     "kernel": kernel_name(kind, tile),
                      "w_eq_flops_per_term": W_EQ_DW,
                      "terms_per_launch": terms, "kernel_ms": round(kernel_ms, 4)},
         "rel_err_vs_fp64": err,
         "natoms": atoms.natoms, "npix": npix, "Z": Z, "pos": pos, "B": B, "hkl": q,
     }
     return res
+
+
+def batch_kernel_name(L, plan, tile):
+    if not L.spsim_hip_batch_uses_grid(plan):
+        return "sf_batch_kernel"
+    return "sf_batch_octant_kernel" if tile.pixels_per_lane == 8 else "sf_batch_grid_kernel"
 
 
 def batched(args, torch, dv, wl, _lib):
@@ -205,8 +218,8 @@ def batched(args, torch, dv, wl, _lib):
             "speedup_batched_vs_plain": round(ms_plain / ms_batched, 3),
             "gterms_per_s_batched": round(terms.value / (ms_batched * 1e-3) / 1e9, 1),
             "roofline_frac_batched": round(terms.value * W_EQ_PLAIN / (ms_batched * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, 4),
-            "tile": {"kernel": "sf_batch_grid_kernel" if L.spsim_hip_batch_uses_grid(plan) else "sf_batch_kernel",
-                     "per_lane": best.pixels_per_lane, "chunk_atoms": best.chunk_atoms},
+            "tile": {"kernel": batch_kernel_name(L, plan, best), "per_lane": best.pixels_per_lane,
+                     "chunk_atoms": best.chunk_atoms},
             "batched_vs_plain_rel_l2": agree,
             "graph": True,
         }
@@ -283,7 +296,7 @@ def pipeline(args, torch, dv, wl, _lib):
             "detector_chain_hbm_frac": round(chain_bytes / (ms_chain * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "plain_structure_factor_ms": round(ms_plain, 2),
             "speedup_batched_vs_plain": round(ms_plain / ms_sf, 3),
-            "kernel": "sf_batch_grid_kernel" if L.spsim_hip_batch_uses_grid(plan) else "sf_batch_kernel",
+            "kernel": batch_kernel_name(L, plan, best),
             "tile": {"per_lane": best.pixels_per_lane, "chunk_atoms": best.chunk_atoms},
         }
     finally:

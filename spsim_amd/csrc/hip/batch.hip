@@ -14,7 +14,8 @@ using spsim::MolInfo;
 
 struct spsim_hip_batch {
   int npix = 0, nx = 0, ny = 0, npatterns = 0, nmol = 0;
-  bool grid = false;  // mirror-symmetric detector: quad work items
+  bool grid = false;    // mirror-symmetric detector: quad work items
+  bool square = false;  // also n x n and transpose-symmetric: octant work items allowed
   spsim_hip_tile tile{};
   int blocks = 0;
   int cus = 0;
@@ -75,7 +76,8 @@ void free_batch(spsim_hip_batch *b) {
 
 // Work items (pattern, tile) for tile size `tp`, heaviest pattern first so the
 // persistent workgroups finish together (longest-processing-time order).
-std::vector<int2> build_items(const spsim_hip_batch &b, int tile_units_, const std::vector<int> *only_patterns = nullptr,
+std::vector<int2> build_items(const spsim_hip_batch &b, const spsim_hip_tile &t,
+                              const std::vector<int> *only_patterns = nullptr,
                               int max_tiles = -1) {
   std::vector<int> pats;
   if (only_patterns) {
@@ -86,8 +88,9 @@ std::vector<int2> build_items(const spsim_hip_batch &b, int tile_units_, const s
   }
   auto natoms = [&](int p) { const MolInfo &m = b.h_mols[b.h_pattern_mol[p]]; return m.atom_end - m.atom_begin; };
   std::stable_sort(pats.begin(), pats.end(), [&](int a, int c) { return natoms(a) > natoms(c); });
-  const int units = b.grid ? grid_quads(b.nx, b.ny) : b.npix;
-  int tiles = (units + tile_units_ - 1) / tile_units_;
+  const int units = b.grid ? grid_units(t, b.nx, b.ny) : b.npix;
+  const int per_tile = b.grid ? grid_units_per_tile(t) : tile_units(t);
+  int tiles = (units + per_tile - 1) / per_tile;
   if (max_tiles > 0) tiles = std::min(tiles, max_tiles);
   std::vector<int2> items;
   items.reserve((size_t)pats.size() * tiles);
@@ -119,10 +122,11 @@ BatchLaunch launch_args(const spsim_hip_batch *b, const int2 *items, int nitems,
 }
 
 int set_tile(spsim_hip_batch *b, const spsim_hip_tile &t) {
-  if (!tile_supported(t, b->grid)) return fail(SPSIM_HIP_ERR_INVALID, "unsupported tile");
+  if (!tile_supported(t, b->grid) || (t.pixels_per_lane == kOctantLane && !b->square))
+    return fail(SPSIM_HIP_ERR_INVALID, "unsupported tile (octants need a square, transpose-symmetric q list)");
   int per_cu = 1;
   SPSIM_TRY(batch_blocks_per_cu(t, b->has_b, b->grid, &per_cu));
-  std::vector<int2> items = build_items(*b, tile_units(t));
+  std::vector<int2> items = build_items(*b, t);
   if (b->items) SPSIM_HIP_TRY(hipFree(b->items));
   b->items = nullptr;
   SPSIM_TRY(dupload(&b->items, items.data(), items.size()));
@@ -168,6 +172,7 @@ int spsim_hip_batch_create(const float *hkl, int nx, int ny, int nmol, const int
   b->nx = nx;
   b->ny = ny;
   b->grid = host_mirror_symmetric(hkl, nx, ny);
+  b->square = b->grid && nx == ny && host_transpose_symmetric(hkl, nx);
   b->npatterns = npatterns;
   b->nmol = nmol;
   b->cus = cus;
@@ -221,6 +226,7 @@ int spsim_hip_batch_create(const float *hkl, int nx, int ny, int nmol, const int
   }
   const int avg_atoms = packed_total / std::max(nmol, 1);
   spsim_hip_tile t = tile ? *tile : default_tile(avg_atoms, npix, b->grid);
+  if (!tile && b->square) t.pixels_per_lane = kOctantLane;
   if ((rc = set_tile(b, t)) != SPSIM_HIP_OK) {
     free_batch(b);
     return rc;
@@ -323,8 +329,10 @@ int spsim_hip_batch_autotune(spsim_hip_batch *b, void *stream, spsim_hip_tile *b
   std::vector<int> sample;
   const int want = std::min(b->npatterns, 6);
   for (int i = 0; i < want; ++i) sample.push_back(order[(size_t)i * b->npatterns / want]);
-  const int ks[4] = {1, 2, 4, 8};
-  const int nks = b->grid ? 2 : 4;  // quads per lane {1, 2} or pixels per lane {1, 2, 4, 8}
+  // quads per lane {1, 2} (+ octants on a square list) or pixels per lane {1, 2, 4, 8}
+  const int ks_grid[3] = {1, 2, kOctantLane}, ks_pix[4] = {1, 2, 4, 8};
+  const int *ks = b->grid ? ks_grid : ks_pix;
+  const int nks = b->grid ? (b->square ? 3 : 2) : 4;
   const int cs[3] = {64, 128, 256};
   float *dI = nullptr;
   unsigned *q = nullptr;
@@ -342,20 +350,20 @@ int spsim_hip_batch_autotune(spsim_hip_batch *b, void *stream, spsim_hip_tile *b
   for (int ki = 0; ki < nks && rc == SPSIM_HIP_OK; ++ki) {
     for (int ci = 0; ci < 3 && rc == SPSIM_HIP_OK; ++ci) {
       spsim_hip_tile t{ks[ki], 4, cs[ci], 0};
-      const int tu = tile_units(t);
+      const int tu = b->grid ? grid_units_per_tile(t) : tile_units(t);
       int per_cu = 1;
       if ((rc = batch_blocks_per_cu(t, b->has_b, b->grid, &per_cu)) != SPSIM_HIP_OK) break;
       const int blocks = b->cus * per_cu;
-      const int units = b->grid ? grid_quads(b->nx, b->ny) : b->npix;
+      const int units = b->grid ? grid_units(t, b->nx, b->ny) : b->npix;
       // enough tiles per sampled pattern for ~4 items per workgroup
       const int tiles = std::max(1, std::min((units + tu - 1) / tu, (4 * blocks + want - 1) / std::max(want, 1)));
-      std::vector<int2> items = build_items(*b, tu, &sample, tiles);
+      std::vector<int2> items = build_items(*b, t, &sample, tiles);
       double terms = 0.0;
       for (const int2 &it : items) {
-        // pixel-atom terms covered by the item (a quad counts 4 pixels; the
-        // duplicated middle row/column of odd sizes is ignored here)
+        // pixel-atom terms covered by the item (a quad counts 4 pixels, an
+        // octant 8; the duplicated middle rows and diagonal are ignored here)
         const int n_units = std::min(tu, units - it.y * tu);
-        terms += (double)natoms(it.x) * n_units * (b->grid ? 4 : 1);
+        terms += (double)natoms(it.x) * n_units * (b->grid ? grid_unit_pixels(t) : 1);
       }
       int2 *ditems = nullptr;
       if ((rc = dupload(&ditems, items.data(), items.size())) != SPSIM_HIP_OK) break;

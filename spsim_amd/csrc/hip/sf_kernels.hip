@@ -407,6 +407,195 @@ __device__ __forceinline__ void grid_tile(const float *__restrict__ hkl, int nx,
   }
 }
 
+// ---- Square mirror-symmetric detectors: octants -----------------------------
+//
+// On a square detector centred on the beam with square pixels, the transposed
+// pixel (row and column swapped) holds (qy, qx, qz) bit for bit: the
+// geometry forms px and py by the same expression, and |p| adds the same
+// squares. One lane then owns an octant, the quad of (x, y) and the quad of
+// its transpose (y, x), 8 pixels that share c = qz rz, the form factor and the
+// Debye-Waller factor. With a' = qy rx, b' = qx ry for the transposed quad,
+// the phases c, a + b, a - b, a' + b', a' - b' need 10 sines and cosines for 8
+// terms (12 with two separate quads). Octants cover x <= y < (n + 1) / 2; on
+// the diagonal both quads are the same pixels and store the same values.
+// Workgroups whose octants are not symmetric fall back to two independent
+// quads, and without mirror symmetry to the per-pixel sum.
+
+// Octant index t -> (x, y), x <= y, t = y (y + 1) / 2 + x.
+__device__ __forceinline__ void octant_xy(int t, int &x, int &y) {
+  int yy = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+  while ((yy + 1) * (yy + 2) / 2 <= t) ++yy;
+  while (yy * (yy + 1) / 2 > t) --yy;
+  y = yy;
+  x = t - yy * (yy + 1) / 2;
+}
+
+// pix[0..3]: quad of (x, y) in quad_pixels order; pix[4..7]: quad of (y, x).
+__device__ __forceinline__ void octant_pixels(int t, int n, int (&pix)[8]) {
+  int x, y;
+  octant_xy(t, x, y);
+  const int xm = n - 1 - x, ym = n - 1 - y;
+  pix[0] = y * n + x;
+  pix[1] = y * n + xm;
+  pix[2] = ym * n + x;
+  pix[3] = ym * n + xm;
+  pix[4] = x * n + y;
+  pix[5] = x * n + ym;
+  pix[6] = xm * n + y;
+  pix[7] = xm * n + ym;
+}
+
+// q at pixel p equals (sx * q0.y, sy * q0.x, q0.z) bit for bit.
+__device__ __forceinline__ bool transposed(const float *__restrict__ hkl, int p0, int p, float sx, float sy) {
+  return hkl[3 * p + 0] == sx * hkl[3 * p0 + 1] && hkl[3 * p + 1] == sy * hkl[3 * p0 + 0] &&
+         hkl[3 * p + 2] == hkl[3 * p0 + 2];
+}
+
+template <int CHUNK, bool HAS_B>
+__device__ __forceinline__ void octant_sum(double qx, double qy, double qz, float s2,
+                                           const spsim_hip_atom *__restrict__ atoms, const float *__restrict__ atom_b,
+                                           const spsim_hip_segment *__restrict__ segs, int nsegs, int a_begin,
+                                           int a_end, const float *__restrict__ ff, float b_global, const double *R,
+                                           AtomLds *__restrict__ sm, double (&Fr)[8], double (&Fi)[8]) {
+  const float nl2 = -s2 * (float)kLog2e;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    Fr[i] = 0.0;
+    Fi[i] = 0.0;
+  }
+  for (int sg = 0; sg < nsegs; ++sg) {
+    const int s_start = segs[sg].start;
+    const int s_end = s_start + segs[sg].count;
+    const int species = segs[sg].species;
+    const int lo = max(s_start, a_begin);
+    const int hi = min(s_end, a_end);
+    if (lo >= hi) continue;
+    const float Bs = (species == SPSIM_HIP_UNIT_SPECIES) ? 0.0f : b_global;
+    const float f = form_factor(ff + species * SPSIM_HIP_FF_STRIDE, s2, Bs);
+    for (int c0 = lo; c0 < hi; c0 += CHUNK) {
+      const int n = min(CHUNK, hi - c0);
+      stage_chunk<HAS_B>(atoms, atom_b, c0, n, R, sm);
+      // [0]: quad (x, y), [1]: transposed quad; P/Q over a+b, P2/Q2 over a-b
+      float pr[2] = {0.f, 0.f}, pi[2] = {0.f, 0.f}, qr[2] = {0.f, 0.f}, qi[2] = {0.f, 0.f};
+      float p2r[2] = {0.f, 0.f}, p2i[2] = {0.f, 0.f}, q2r[2] = {0.f, 0.f}, q2i[2] = {0.f, 0.f};
+      for (int j = 0; j < n; ++j) {
+        const AtomLds at = sm[j];
+        const double tc = qz * at.z;
+        const double ta[2] = {qx * at.x, qy * at.x};
+        const double tb[2] = {qy * at.y, qx * at.y};
+        const float fc = (float)__builtin_amdgcn_fract(tc);
+        float w;
+        if constexpr (HAS_B) {
+          w = __builtin_amdgcn_exp2f(__builtin_fmaf(at.b, nl2, at.w));
+        } else {
+          w = at.w;
+        }
+        const float ur = w * __builtin_amdgcn_cosf(fc);
+        const float ui = w * __builtin_amdgcn_sinf(fc);
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const float fs = (float)__builtin_amdgcn_fract(ta[h] + tb[h]);
+          const float fd = (float)__builtin_amdgcn_fract(ta[h] - tb[h]);
+          const float cs = __builtin_amdgcn_cosf(fs), ss = __builtin_amdgcn_sinf(fs);
+          const float cd = __builtin_amdgcn_cosf(fd), sd = __builtin_amdgcn_sinf(fd);
+          pr[h] = __builtin_fmaf(ur, cs, pr[h]);
+          pi[h] = __builtin_fmaf(ui, cs, pi[h]);
+          qr[h] = __builtin_fmaf(ur, ss, qr[h]);
+          qi[h] = __builtin_fmaf(ui, ss, qi[h]);
+          p2r[h] = __builtin_fmaf(ur, cd, p2r[h]);
+          p2i[h] = __builtin_fmaf(ui, cd, p2i[h]);
+          q2r[h] = __builtin_fmaf(ur, sd, q2r[h]);
+          q2i[h] = __builtin_fmaf(ui, sd, q2i[h]);
+        }
+      }
+      const double fk = f;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {  // same assembly as quad_sum
+        Fr[4 * h + 0] = __builtin_fma(fk, (double)pr[h] - (double)qi[h], Fr[4 * h + 0]);
+        Fi[4 * h + 0] = __builtin_fma(fk, (double)pi[h] + (double)qr[h], Fi[4 * h + 0]);
+        Fr[4 * h + 1] = __builtin_fma(fk, (double)p2r[h] + (double)q2i[h], Fr[4 * h + 1]);
+        Fi[4 * h + 1] = __builtin_fma(fk, (double)p2i[h] - (double)q2r[h], Fi[4 * h + 1]);
+        Fr[4 * h + 2] = __builtin_fma(fk, (double)p2r[h] - (double)q2i[h], Fr[4 * h + 2]);
+        Fi[4 * h + 2] = __builtin_fma(fk, (double)p2i[h] + (double)q2r[h], Fi[4 * h + 2]);
+        Fr[4 * h + 3] = __builtin_fma(fk, (double)pr[h] + (double)qi[h], Fr[4 * h + 3]);
+        Fi[4 * h + 3] = __builtin_fma(fk, (double)pi[h] - (double)qr[h], Fi[4 * h + 3]);
+      }
+    }
+  }
+}
+
+// One quad of an octant (pixels pix[first .. first + 3]) by the quad sum, or
+// pixel by pixel when the list is not mirror-symmetric.
+template <int CHUNK, bool HAS_B>
+__device__ __forceinline__ void octant_half(const float *__restrict__ hkl, const int (&pix)[8], int first, bool mirror,
+                                            const spsim_hip_atom *__restrict__ atoms, const float *__restrict__ atom_b,
+                                            const spsim_hip_segment *__restrict__ segs, int nsegs, int a_begin,
+                                            int a_end, const float *__restrict__ ff, float b_global, const double *R,
+                                            AtomLds *__restrict__ sm, double (&Fr)[4], double (&Fi)[4]) {
+  if (mirror) {
+    double qx[1], qy[1], qz[1];
+    float s2[1];
+    load_q(hkl, pix[first], pix[first] >= 0, qx[0], qy[0], qz[0], s2[0]);
+    quad_sum<1, CHUNK, HAS_B>(qx, qy, qz, s2, atoms, atom_b, segs, nsegs, a_begin, a_end, ff, b_global, R, sm, Fr, Fi);
+  } else {
+    double qx[4], qy[4], qz[4];
+    float s2[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) load_q(hkl, pix[first + i], pix[first + i] >= 0, qx[i], qy[i], qz[i], s2[i]);
+    tile_sum<4, CHUNK, HAS_B>(qx, qy, qz, s2, atoms, atom_b, segs, nsegs, a_begin, a_end, ff, b_global, R, sm, Fr, Fi);
+  }
+}
+
+// One octant per lane of an n x n detector list: workgroup votes on mirror and
+// transpose symmetry, then the octant sum, two quads, or eight pixels.
+template <int CHUNK, bool HAS_B>
+__device__ __forceinline__ void octant_tile(const float *__restrict__ hkl, int n, int oct_base,
+                                            const spsim_hip_atom *__restrict__ atoms, const float *__restrict__ atom_b,
+                                            const spsim_hip_segment *__restrict__ segs, int nsegs, int a_begin,
+                                            int a_end, const float *__restrict__ ff, float b_global, const double *R,
+                                            AtomLds *__restrict__ sm, int (&pix)[8], double (&Fr)[8],
+                                            double (&Fi)[8]) {
+  const int t = oct_base + threadIdx.x;
+  bool mir = true, tra = true;
+  if (t < grid_octants(n)) {
+    octant_pixels(t, n, pix);
+    const int p0 = pix[0], t0 = pix[4];
+    mir = mirrored(hkl, p0, pix[1], -1.0f, 1.0f) && mirrored(hkl, p0, pix[2], 1.0f, -1.0f) &&
+          mirrored(hkl, p0, pix[3], -1.0f, -1.0f) && mirrored(hkl, t0, pix[5], -1.0f, 1.0f) &&
+          mirrored(hkl, t0, pix[6], 1.0f, -1.0f) && mirrored(hkl, t0, pix[7], -1.0f, -1.0f);
+    tra = transposed(hkl, p0, t0, 1.0f, 1.0f);
+  } else {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) pix[i] = -1;
+  }
+  const bool no_mirror = __syncthreads_or(!mir);
+  const bool no_transpose = __syncthreads_or(!tra);
+  if (!no_mirror && !no_transpose) {
+    double qx, qy, qz;
+    float s2;
+    load_q(hkl, pix[0], pix[0] >= 0, qx, qy, qz, s2);
+    octant_sum<CHUNK, HAS_B>(qx, qy, qz, s2, atoms, atom_b, segs, nsegs, a_begin, a_end, ff, b_global, R, sm, Fr, Fi);
+  } else {
+    // Fallbacks run the two quads one after the other, so the kernel's
+    // register budget is the octant path's.
+    double hr[4], hi[4];
+    octant_half<CHUNK, HAS_B>(hkl, pix, 0, !no_mirror, atoms, atom_b, segs, nsegs, a_begin, a_end, ff, b_global, R, sm,
+                              hr, hi);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      Fr[i] = hr[i];
+      Fi[i] = hi[i];
+    }
+    octant_half<CHUNK, HAS_B>(hkl, pix, 4, !no_mirror, atoms, atom_b, segs, nsegs, a_begin, a_end, ff, b_global, R, sm,
+                              hr, hi);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      Fr[4 + i] = hr[i];
+      Fi[4 + i] = hi[i];
+    }
+  }
+}
+
 // ---- single-pattern kernels (blockIdx.x = tile, blockIdx.y = atom split) ----
 
 template <int K, int CHUNK, bool HAS_B>
@@ -446,6 +635,22 @@ __global__ __launch_bounds__(kThreads) void sf_grid_kernel(PatternLaunch a, int 
                              a_end, a.ff, a.b_global, a.has_rot ? a.rot.r : nullptr, sm, pix, Fr, Fi);
 #pragma unroll
   for (int i = 0; i < 4 * K; ++i)
+    if (pix[i] >= 0) store_pixel(pix[i], Fr[i], Fi[i], a.npix, split, partial, a.F, a.I);
+}
+
+template <int CHUNK, bool HAS_B>
+__global__ __launch_bounds__(kThreads) void sf_octant_kernel(PatternLaunch a, int n, int ny_unused,
+                                                             double2 *__restrict__ partial) {
+  __shared__ AtomLds sm[CHUNK];
+  const int split = blockIdx.y, nsplit = gridDim.y;
+  const int a_begin = (int)(((long long)a.natoms * split) / nsplit);
+  const int a_end = (int)(((long long)a.natoms * (split + 1)) / nsplit);
+  int pix[8];
+  double Fr[8], Fi[8];
+  octant_tile<CHUNK, HAS_B>(a.hkl, n, blockIdx.x * kThreads, a.atoms, a.atom_b, a.segs, a.nsegs, a_begin, a_end, a.ff,
+                            a.b_global, a.has_rot ? a.rot.r : nullptr, sm, pix, Fr, Fi);
+#pragma unroll
+  for (int i = 0; i < 8; ++i)
     if (pix[i] >= 0) store_pixel(pix[i], Fr[i], Fi[i], a.npix, split, partial, a.F, a.I);
 }
 
@@ -559,6 +764,28 @@ __global__ __launch_bounds__(kThreads) void sf_batch_grid_kernel(BatchLaunch a) 
   }
 }
 
+template <int CHUNK, bool HAS_B>
+__global__ __launch_bounds__(kThreads) void sf_batch_octant_kernel(BatchLaunch a) {
+  __shared__ AtomLds sm[CHUNK];
+  __shared__ int s_item;
+  for (;;) {
+    const int item = next_item(a.queue, &s_item);
+    if (item >= a.nitems) break;
+    const int2 it = a.items[item];
+    const int p = it.x;
+    const MolInfo m = a.mols[a.pattern_mol[p]];
+    int pix[8];
+    double Fr[8], Fi[8];
+    octant_tile<CHUNK, HAS_B>(a.hkl, a.nx, it.y * kThreads, a.atoms + m.atom_begin,
+                              HAS_B ? a.atom_b + m.atom_begin : nullptr, a.segs + m.seg_begin, m.nseg, 0,
+                              m.atom_end - m.atom_begin, a.ff, a.b_global, a.rot + 9 * (size_t)p, sm, pix, Fr, Fi);
+    const size_t off = (size_t)p * a.npix;
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+      if (pix[i] >= 0) store_pixel(pix[i], Fr[i], Fi[i], a.npix, 0, nullptr, a.F ? a.F + off : nullptr, a.I + off);
+  }
+}
+
 // ---- instantiation tables ----
 
 #define SPSIM_PIXEL_TILES(X) \
@@ -577,6 +804,12 @@ const void *kernel_ptr(Kind kind, int K, int C) {
                                                        : (const void *)&sf_batch_grid_kernel<k_, c_, HAS_B>;
 #define S(k_, c_) \
   if (K == k_ && C == c_) return (const void *)&sf_spectrum_kernel<k_, c_, HAS_B>;
+#define O(c_) \
+  if (K == kOctantLane && C == c_) return kind == Kind::kGrid ? (const void *)&sf_octant_kernel<c_, HAS_B> \
+                                                              : (const void *)&sf_batch_octant_kernel<c_, HAS_B>;
+  if (kind == Kind::kGrid || kind == Kind::kBatchGrid) {
+    O(64) O(128) O(256)
+  }
   if (kind == Kind::kPattern || kind == Kind::kBatch) {
     SPSIM_PIXEL_TILES(P)
   } else if (kind == Kind::kSpectrum) {
@@ -587,6 +820,7 @@ const void *kernel_ptr(Kind kind, int K, int C) {
 #undef P
 #undef Q
 #undef S
+#undef O
   return nullptr;
 }
 
@@ -601,8 +835,9 @@ int launch_single(Kind kind, const PatternLaunch &a, int nx, int ny, const spsim
   if (a.npix <= 0) return SPSIM_HIP_OK;
   const void *k = kernel_for(kind, t, a.atom_b != nullptr);
   if (!k) return fail(SPSIM_HIP_ERR_INVALID, "unsupported tile (per lane %d, chunk %d)", t.pixels_per_lane, t.chunk_atoms);
-  const int units = kind == Kind::kGrid ? grid_quads(nx, ny) : a.npix;
-  const int tiles = (units + tile_units(t) - 1) / tile_units(t);
+  const int units = kind == Kind::kGrid ? grid_units(t, nx, ny) : a.npix;
+  const int per_tile = kind == Kind::kGrid ? grid_units_per_tile(t) : tile_units(t);
+  const int tiles = (units + per_tile - 1) / per_tile;
   if (splits < 1) splits = 1;
   double2 *partial = nullptr;
   if (splits > 1) {
@@ -689,6 +924,16 @@ int auto_splits(int tiles, int natoms, int chunk_atoms, int cus) {
   return (int)s;
 }
 
+bool host_transpose_symmetric(const float *hkl, int n) {
+  for (int y = 0; y < n; ++y)
+    for (int x = 0; x < y; ++x) {
+      const float *q = hkl + 3 * ((size_t)y * n + x);
+      const float *t = hkl + 3 * ((size_t)x * n + y);
+      if (!(t[0] == q[1] && t[1] == q[0] && t[2] == q[2])) return false;
+    }
+  return true;
+}
+
 bool host_mirror_symmetric(const float *hkl, int nx, int ny) {
   for (int y = 0; y < ny; ++y) {
     for (int x = 0; x < nx; ++x) {
@@ -768,11 +1013,13 @@ extern "C" int spsim_hip_sf_grid(const float *d_hkl, int nx, int ny, const spsim
   int cus = 0;
   SPSIM_TRY(ensure_device(&cus));
   if (npix == 0) return SPSIM_HIP_OK;
-  const spsim_hip_tile t = tile ? *tile : default_tile(natoms, (int)npix, true);
-  if (!tile_supported(t, true))
-    return fail(SPSIM_HIP_ERR_INVALID, "unsupported grid tile (quads_per_lane=%d waves=%d chunk_atoms=%d)",
+  spsim_hip_tile t = tile ? *tile : default_tile(natoms, (int)npix, true);
+  if (!tile && nx == ny) t.pixels_per_lane = kOctantLane;
+  if (!tile_supported(t, true) || (t.pixels_per_lane == kOctantLane && nx != ny))
+    return fail(SPSIM_HIP_ERR_INVALID,
+                "unsupported grid tile (quads_per_lane=%d waves=%d chunk_atoms=%d; octants need nx == ny)",
                 t.pixels_per_lane, t.waves, t.chunk_atoms);
-  const int tiles = (grid_quads(nx, ny) + tile_units(t) - 1) / tile_units(t);
+  const int tiles = (grid_units(t, nx, ny) + grid_units_per_tile(t) - 1) / grid_units_per_tile(t);
   const int splits = t.atom_splits > 0 ? t.atom_splits : auto_splits(tiles, natoms, t.chunk_atoms, cus);
   const PatternLaunch a =
       make_launch(d_hkl, (int)npix, d_atoms, d_atom_b, d_segs, nsegs, natoms, d_ff, b_global, rot9, d_F, d_I);

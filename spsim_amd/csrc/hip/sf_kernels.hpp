@@ -61,12 +61,28 @@ struct BatchLaunch {
 };
 
 // Valid tiles: per-pixel kernels take pixels_per_lane in {1,2,4,8}; the
-// mirror-symmetric (grid) kernels take quads per lane in {1,2}; waves == 4;
-// chunk_atoms in {64,128,256}.
+// mirror-symmetric (grid) kernels take quads per lane in {1,2}, or
+// kOctantLane for one octant (8 pixels) per lane on a square detector;
+// waves == 4; chunk_atoms in {64,128,256}.
+constexpr int kOctantLane = 8;
 bool tile_supported(const spsim_hip_tile &t, bool grid);
 // Pixels (per-pixel kernels) or quads (grid kernels) covered by one tile.
 inline int tile_units(const spsim_hip_tile &t) { return t.pixels_per_lane * t.waves * 64; }
 __host__ __device__ inline int grid_quads(int nx, int ny) { return ((nx + 1) / 2) * ((ny + 1) / 2); }
+// Octants x <= y of the lower quadrant of an n x n detector.
+__host__ __device__ inline int grid_octants(int n) {
+  const int h = (n + 1) / 2;
+  return h * (h + 1) / 2;
+}
+// Work units of a grid tile (quads, or octants for kOctantLane) and their count.
+inline int grid_units_per_tile(const spsim_hip_tile &t) {
+  return t.pixels_per_lane == kOctantLane ? t.waves * 64 : tile_units(t);
+}
+inline int grid_units(const spsim_hip_tile &t, int nx, int ny) {
+  return t.pixels_per_lane == kOctantLane ? grid_octants(nx) : grid_quads(nx, ny);
+}
+// Pixels per grid work unit.
+inline int grid_unit_pixels(const spsim_hip_tile &t) { return t.pixels_per_lane == kOctantLane ? 8 : 4; }
 
 // One pattern. splits >= 1; with splits > 1 the library workspace holds the partial sums.
 int launch_pattern(const PatternLaunch &a, const spsim_hip_tile &t, int splits, hipStream_t s);
@@ -84,5 +100,7 @@ spsim_hip_tile default_tile(int natoms, int npix, bool grid);
 int auto_splits(int tiles, int natoms, int chunk_atoms, int cus);
 // True when the [ny][nx][3] host list is exactly mirror-symmetric in x and y.
 bool host_mirror_symmetric(const float *hkl, int nx, int ny);
+// True when the n x n host list is also transpose-symmetric: q(y, x) = (qy, qx, qz)(x, y).
+bool host_transpose_symmetric(const float *hkl, int n);
 
 }  // namespace spsim

@@ -118,10 +118,11 @@ def test_autotune_and_every_tile(gpu, setup):
         best = _lib.Tile()
         rate = C.c_double()
         _lib.check(L.spsim_hip_batch_autotune(plan, sp, C.byref(best), C.byref(rate)), "autotune")
-        assert best.pixels_per_lane in ((1, 2) if centred else (1, 2, 4, 8)) and best.chunk_atoms in (64, 128, 256)
+        # centred square detector: quads per lane 1, 2 or one octant (8) per lane
+        assert best.pixels_per_lane in ((1, 2, 8) if centred else (1, 2, 4, 8)) and best.chunk_atoms in (64, 128, 256)
         assert rate.value > 0
         I = torch.zeros((len(pattern_mol), q.shape[0]), dtype=torch.float32, device=gpu)
-        for k in ((1, 2) if centred else (1, 2, 4, 8)):
+        for k in ((1, 2, 8) if centred else (1, 2, 4, 8)):
             for c in (64, 128, 256):
                 t = _lib.Tile(k, 4, c, 0)
                 _lib.check(L.spsim_hip_batch_set_tile(plan, C.byref(t)), "set_tile")

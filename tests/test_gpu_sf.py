@@ -193,9 +193,12 @@ def test_sf_grid_mirror_kernel_vs_f64(gpu, nx, ny):
     Z, pos, B = wl.random_molecule(900, 4e-9, rng, atom_b_range=(10.0, 40.0))
     hkl = oracle.hkl_detector(nx, ny, np.float32(nx * 3e-4).item(), np.float32(ny * 3e-4).item(), 0.05, 0.0, 0.0, 0,
                               0.15e-9)
+    tiles = [None, dv.make_tile(1, 64, 1), dv.make_tile(2, 256, 3)]
+    if nx == ny:  # square: octant tiles (8 pixels per lane), also with atom splits
+        tiles += [dv.make_tile(8, 128, 1), dv.make_tile(8, 64, 3)]
     for has_b in (False, True):
         F_ref, I_ref = oracle.sf_f64(hkl, Z, pos, atom_b=B if has_b else None, B=3.0, threads=oracle_threads())
-        for tile in (None, dv.make_tile(1, 64, 1), dv.make_tile(2, 256, 3)):
+        for tile in tiles:
             F, I = _gpu_grid(gpu, hkl, nx, ny, Z, pos, atom_b=B if has_b else None, B=3.0, tile=tile)
             assert rel_l2(I, I_ref) < TOL_F64, (has_b, tile)
             assert rel_l2(np.c_[F.real, F.imag], np.c_[F_ref.real, F_ref.imag]) < TOL_F64
@@ -222,6 +225,33 @@ def test_sf_grid_falls_back_without_symmetry(gpu):
     _, I_ref3 = oracle.sf_f64(hkl3, Z, pos)
     _, I3 = _gpu_grid(gpu, hkl3, n, n, Z, pos)
     assert rel_l2(I3, I_ref3) < TOL_F64
+    # square and centred but with rectangular pixels: mirror-symmetric, not
+    # transpose-symmetric, so octant workgroups fall back to two quads
+    hkl4 = oracle.hkl_detector(n, n, np.float32(n * 5e-4).item(), np.float32(n * 4e-4).item(), 0.1, 0.0, 0.0, 0,
+                               0.2e-9)
+    _, I_ref4 = oracle.sf_f64(hkl4, Z, pos)
+    for tile in (None, dv.make_tile(8, 128, 1)):
+        _, I4 = _gpu_grid(gpu, hkl4, n, n, Z, pos, tile=tile)
+        assert rel_l2(I4, I_ref4) < TOL_F64
+    # one transposed pixel disturbed
+    hkl5 = _det_hkl(wl.Detector(n, 5e-4, 0.1, 0.2e-9)).copy()
+    hkl5[3 * n + 11, 1] *= np.float32(1.0001)
+    _, I_ref5 = oracle.sf_f64(hkl5, Z, pos)
+    _, I5 = _gpu_grid(gpu, hkl5, n, n, Z, pos, tile=dv.make_tile(8, 256, 1))
+    assert rel_l2(I5, I_ref5) < TOL_F64
+
+
+def test_sf_grid_octant_tile_needs_square(gpu):
+    import torch
+
+    from spsim_amd import _lib
+
+    rng = np.random.default_rng(32)
+    Z, pos, _ = wl.random_molecule(50, 2e-9, rng)
+    hkl = oracle.hkl_detector(8, 6, 8 * 5e-4, 6 * 5e-4, 0.1, 0.0, 0.0, 0, 0.2e-9)
+    with pytest.raises(RuntimeError, match="octants need nx == ny"):
+        _gpu_grid(gpu, hkl, 8, 6, Z, pos, tile=dv.make_tile(8, 128, 1))
+    assert "octants" in _lib.last_error()
 
 
 def test_sf_grid_config2_subset_vs_f64(gpu):
