@@ -27,6 +27,16 @@ Image *make_cimage(Complex *array, SpRotation *rot, Options *opts);
 void array_to_image(float *array, Image *img);
 void iarray_to_image(int *array, Image *img);
 
+/* src/spsim.c:146-152, 176-182 (include/spsim.h:51-52): one image per file,
+   "<basename>-%05d.cxi" (HDF5, CXI 1.5 layout; HDF5 is loaded at the first
+   write). */
+void output_array(char *basename, int index, float *array, SpRotation *rot, Options *opts);
+void output_carray(char *basename, int index, Complex *array, SpRotation *rot, Options *opts);
+/* src/spsim.c:230-279 (include/spsim.h:53): the files the output_* flags select. */
+void output_files(Diffraction_Pattern *pattern, Options *opts, int index);
+/* src/io.c:63-93 (include/io.h:23): VTK legacy structured points. */
+void write_3D_array_to_vtk(float *F, int nx, int ny, int nz, char *filename);
+
 /* src/spsim.c:57-81 */
 void gaussian_blur_pattern(Options *opts, Diffraction_Pattern *pattern);
 

@@ -76,7 +76,7 @@ def build(verbose: bool = False, jobs: int | None = None) -> Path:
             if verbose:
                 print("compiled", cmd[-3])
     tmp = LIB.with_suffix(".so.tmp")
-    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", str(tmp), *map(str, objs), "-lm"])
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", str(tmp), *map(str, objs), "-lm", "-ldl"])
     os.replace(tmp, LIB)
     if verbose:
         print("linked", LIB)
