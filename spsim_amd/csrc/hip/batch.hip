@@ -122,7 +122,7 @@ BatchLaunch launch_args(const spsim_hip_batch *b, const int2 *items, int nitems,
 }
 
 int set_tile(spsim_hip_batch *b, const spsim_hip_tile &t) {
-  if (!tile_supported(t, b->grid) || (t.pixels_per_lane == kOctantLane && !b->square))
+  if (!tile_supported(t, b->grid) || (b->grid && t.pixels_per_lane == kOctantLane && !b->square))
     return fail(SPSIM_HIP_ERR_INVALID, "unsupported tile (octants need a square, transpose-symmetric q list)");
   int per_cu = 1;
   SPSIM_TRY(batch_blocks_per_cu(t, b->has_b, b->grid, &per_cu));

@@ -93,11 +93,11 @@ def test_compute_sf_takes_the_spectrum_path(gpu):
     d = opts.detector
     d.nx = d.ny = 24
     d.nz = 1
-    d.pixel_width = d.pixel_height = 2e-4
-    d.width = d.height = float(np.float32(24 * 2e-4))
-    d.distance = 0.08
-    opts.experiment.wavelength = 0.4e-9
-    opts.experiment.bandwidth = 0.03
+    d.pixel_width = d.pixel_height = 1e-3
+    d.width = d.height = float(np.float32(24 * 1e-3))
+    d.distance = 0.05
+    opts.experiment.wavelength = 0.2e-9
+    opts.experiment.bandwidth = 0.05
     opts.wavelength_samples = 4
     opts.use_cuda = 1
     opts.verbosity_level = 0
@@ -109,7 +109,7 @@ def test_compute_sf_takes_the_spectrum_path(gpu):
     q = np.ctypeslib.as_array(ptr, shape=(n.value * 3,)).reshape(-1, 3).copy()
     pat = spsim.compute_sf(mol, ptr, n.value, opts)
     assert opts.wavelength_samples == 5
-    sc, w = oracle.spectrum_samples(float(np.float32(0.4e-9)), float(np.float32(0.03)), 5)
+    sc, w = oracle.spectrum_samples(float(np.float32(0.2e-9)), float(np.float32(0.05)), 5)
     _, I_ref = oracle.sf_spectrum_f64(q, Z, pos, sc, w)
     assert rel_l2(spsim.float_array(pat.ints, n.value), I_ref) < TOL
     _, I_plain = oracle.sf_f64(q, Z, pos)
