@@ -95,8 +95,11 @@ def headline(args, torch, dv, wl, _lib):
             dv.structure_factor(q, atoms, ff, F=F, I=I, tile=t)
 
     candidates = [("pattern", dv.make_tile(k, c, 1)) for k in (2, 4) for c in (128, 256)]
-    # grid kernels: 1 or 2 mirror quads per lane, or one octant (8 pixels) per lane
-    candidates += [("grid", dv.make_tile(k, c, 1)) for k in (1, 2, 8) for c in (128, 256)]
+    # grid kernels: 1 or 2 mirror quads per lane, or one octant (8 pixels) per
+    # lane; an octant tile covers 8 pixels per lane, so it needs 2-4 atom
+    # ranges (a second, reducing launch) to give each SIMD 4 waves
+    candidates += [("grid", dv.make_tile(k, c, 1)) for k in (1, 2) for c in (128, 256)]
+    candidates += [("grid", dv.make_tile(8, c, s)) for c in (128, 256) for s in (2, 4)]
     if args.kernel != "auto":
         candidates = [c for c in candidates if c[0] == args.kernel]
     best, best_ms, sweep = None, 1e30, []
@@ -109,7 +112,8 @@ def headline(args, torch, dv, wl, _lib):
         e1.record()
         e1.synchronize()
         ms = e0.elapsed_time(e1)
-        sweep.append({"kernel": kind, "lane": t.pixels_per_lane, "chunk": t.chunk_atoms, "ms": round(ms, 3)})
+        sweep.append({"kernel": kind, "lane": t.pixels_per_lane, "chunk": t.chunk_atoms, "splits": t.atom_splits,
+                      "ms": round(ms, 3)})
         if ms < best_ms:
             best, best_ms = (kind, t), ms
     kind, tile = best
