@@ -68,6 +68,22 @@ def rel_l2(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
+def recorded_traffic(kernel, tile):
+    """HBM bytes per launch of the headline kernel from the FETCH_SIZE /
+    WRITE_SIZE counter runs recorded in profiles/ (scaled by the copy
+    calibration recorded with them); None when the chosen kernel or tile is not
+    the one that was profiled."""
+    try:
+        rec = json.loads((ROOT / "profiles" / "traffic_large_single_pattern.json").read_text())
+    except (OSError, ValueError):
+        return None
+    if (rec.get("kernel"), rec.get("per_lane"), rec.get("chunk_atoms"), rec.get("atom_splits")) != \
+            (kernel, tile.pixels_per_lane, tile.chunk_atoms, tile.atom_splits):
+        return None
+    return round(1024 * (rec["fetch_kib_per_launch"] * rec["fetch_scale"] +
+                         rec["write_kib_per_launch"] * rec["write_scale"]))
+
+
 def kernel_name(kind, tile):
     if kind == "grid":
         return "sf_octant_kernel" if tile.pixels_per_lane == 8 else "sf_grid_kernel"
@@ -146,6 +162,7 @@ def headline(args, torch, dv, wl, _lib):
         print(f"accuracy check skipped: {e}", file=sys.stderr)
 
     achieved = terms * W_EQ_DW / (kernel_ms * 1e-3) / 1e12
+    traffic = recorded_traffic(kernel_name(kind, tile), tile)
     res = {
         "terms": terms,
         "kernel_ms": kernel_ms,
@@ -154,7 +171,7 @@ def headline(args, torch, dv, wl, _lib):
         "tile": {"kernel": kind, "per_lane": tile.pixels_per_lane, "waves": tile.waves,
                  "chunk_atoms": tile.chunk_atoms, "atom_splits": tile.atom_splits, "sweep": sweep},
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                     "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": traffic,
                      "kernel": kernel_name(kind, tile),
                      "w_eq_flops_per_term": W_EQ_DW,
                      "terms_per_launch": terms, "kernel_ms": round(kernel_ms, 4)},
