@@ -127,8 +127,13 @@ int spsim_hip_sf_pattern(const float *d_hkl, int npix, const spsim_hip_atom *d_a
    mirrored in x has q = (-qx, qy, qz), in y (qx, -qy, qz), bit for bit), each
    lane evaluates the four mirror pixels together and shares their sines,
    cosines, form factor and Debye-Waller factor; workgroups whose pixels are not
-   symmetric fall back to the per-pixel sum. tile->pixels_per_lane counts quads
-   per lane here (1 or 2); tile may be NULL. */
+   symmetric fall back to the per-pixel sum. On a square detector (nx == ny)
+   whose list is also transpose-symmetric (q(y, x) = (qy, qx, qz)(x, y)), one
+   lane evaluates an octant, a quad and its transpose (8 pixels sharing qz r_z);
+   workgroups without that symmetry fall back to two quads. tile->pixels_per_lane
+   is the quads per lane here (1 or 2), or 8 for one octant per lane (nx == ny
+   only; the default for square detectors, with atom_splits chosen to fill the
+   GPU). tile may be NULL. */
 int spsim_hip_sf_grid(const float *d_hkl, int nx, int ny, const spsim_hip_atom *d_atoms, const float *d_atom_b,
                       const spsim_hip_segment *d_segs, int nsegs, int natoms, const float *d_ff, float b_global,
                       const double *rot9, float *d_F, float *d_I, const spsim_hip_tile *tile, void *stream);
