@@ -177,8 +177,9 @@ typedef struct spsim_hip_batch spsim_hip_batch;
    molecule pattern_mol[p] with q rotated by rot9[p] (row-major 3x3, q' = R q,
    applied exactly in FP64, as in spsim_hip_sf_pattern). hkl is the shared
    [ny][nx][3] host q list. When that list is mirror-symmetric the plan runs the
-   quad kernel (tile->pixels_per_lane = quads per lane), otherwise the per-pixel
-   kernel. The plan copies everything to the device, builds the cost-sorted
+   quad kernel (tile->pixels_per_lane = quads per lane), or, when it is also
+   square and transpose-symmetric, the octant kernel (pixels_per_lane = 8, the
+   default there); otherwise the per-pixel kernel. The plan copies everything to the device, builds the cost-sorted
    (pattern, tile) work list and owns its device memory until
    spsim_hip_batch_destroy. tile may be NULL (default tiling). */
 int spsim_hip_batch_create(const float *hkl, int nx, int ny, int nmol, const int *mol_offsets, const int *Z,
