@@ -53,6 +53,7 @@ def parse():
     p.add_argument("--no-pipeline", action="store_true", help="skip the full-pipeline measurement")
     p.add_argument("--no-adjoint", action="store_true", help="skip the differentiation measurement")
     p.add_argument("--no-spectrum", action="store_true", help="skip the wavelength-spectrum measurement")
+    p.add_argument("--no-nfft", action="store_true", help="skip the non-uniform FFT measurement")
     p.add_argument("--profile", action="store_true", help="headline workload only (for rocprofv3)")
     p.add_argument("--kernel", choices=("auto", "pattern", "grid"), default="auto",
                    help="restrict the headline sweep to one structure-factor kernel")
@@ -420,6 +421,71 @@ def spectrum(args, torch, dv, wl, _lib):
     }
 
 
+def nfft(args, torch, dv, wl, _lib, h):
+    """Non-uniform FFT structure factor (spsim_hip_sf_nfft, the use_nfft_for_sf
+    path) against the direct mirror-symmetric kernel on the large-single-pattern
+    geometry with that path's semantics (global B only, unit strengths):
+    the same 100,000-atom molecule and 1024 x 1024 q list. One NUFFT call
+    synchronises once to size its grids from the data bounds, so it is timed
+    end to end (HIP events and host wall clock around whole calls)."""
+    dev = torch.device("cuda", 0)
+    Z, pos, q = h["Z"], h["pos"], h["hkl"]
+    n = int(round(np.sqrt(h["npix"])))
+    ff = dv.form_factor_table(device=dev)
+    atoms_n = dv.DeviceAtoms(dv.pack_atoms(Z, pos, mode=_lib.PACK_NFFT), dev)
+    atoms_d = dv.DeviceAtoms(dv.pack_atoms(Z, pos), dev)
+    I_n = torch.empty(h["npix"], dtype=torch.float32, device=dev)
+    I_d = torch.empty(h["npix"], dtype=torch.float32, device=dev)
+    eps = 1e-6
+    info = {}
+
+    def run_n():
+        info.update(dv.structure_factor_nfft(q, atoms_n, ff, eps=eps, I=I_n, want_F=False)[2])
+
+    def run_d():
+        dv.structure_factor_grid(q, n, n, atoms_d, ff, I=I_d, want_F=False, tile=dv.make_tile(8, 256, 4))
+
+    def timed(fn, reps=5):
+        fn()
+        torch.cuda.synchronize()
+        ms, wall = [], []
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            t0 = time.perf_counter()
+            e0.record()
+            fn()
+            e1.record()
+            e1.synchronize()
+            wall.append((time.perf_counter() - t0) * 1e3)
+            ms.append(e0.elapsed_time(e1))
+        return float(np.median(ms)), float(np.median(wall))
+
+    ms_n, wall_n = timed(run_n)
+    ms_d, wall_d = timed(run_d)
+    terms = float(h["npix"]) * atoms_n.natoms
+    err_n = err_d = None
+    try:
+        import oracle
+
+        rng = np.random.default_rng(5)
+        sub = np.sort(rng.choice(h["npix"], 256, replace=False))
+        _, I_ref = oracle.sf_nfft_f64(q.cpu().numpy()[sub], Z, pos, threads=min(16, os.cpu_count() or 1))
+        err_n = rel_l2(I_n.cpu().numpy()[sub], I_ref)
+        err_d = rel_l2(I_d.cpu().numpy()[sub], I_ref)
+    except Exception as e:  # the oracle is a checker only
+        print(f"nfft accuracy check skipped: {e}", file=sys.stderr)
+    return {
+        "workload": "100000-atom C/N/O/S molecule (radius 12 nm), 1024x1024 detector 75 um at 0.1 m, lambda 0.15 nm, "
+                    "global B = 0, unit strengths (use_nfft_for_sf semantics)",
+        "eps": eps, "grid": info,
+        "ms": round(ms_n, 3), "wall_ms": round(wall_n, 3),
+        "ms_direct": round(ms_d, 3), "wall_ms_direct": round(wall_d, 3),
+        "speedup_vs_direct": round(ms_d / ms_n, 2),
+        "effective_gterms_per_s": round(terms / (ms_n * 1e-3) / 1e9, 1),
+        "rel_err_vs_fp64": err_n, "rel_err_direct_vs_fp64": err_d,
+    }
+
+
 def cpu_baseline(h):
     import oracle
 
@@ -506,6 +572,9 @@ def main():
     if not args.profile and not args.no_spectrum:
         log("spectrum: wavelength-sampled pattern")
         out["spectrum"] = spectrum(args, torch, dv, wl, _lib)
+    if not args.profile and not args.no_nfft:
+        log("nfft: non-uniform FFT vs direct sum")
+        out["nfft"] = nfft(args, torch, dv, wl, _lib, h)
     if not args.profile and not args.no_cpu:
         log("cpu baseline: FP64 oracle on a pixel sample")
         out["cpu_baseline"] = cpu_baseline(h)

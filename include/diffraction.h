@@ -57,6 +57,18 @@ Diffraction_Pattern *vector_compute_pattern_on_list(Molecule *mol, float *HKL_li
 Diffraction_Pattern *spsim_spectrum_pattern_on_list(Molecule *mol, float *HKL_list, int HKL_list_size, float B,
                                                     Experiment *exp, Options *opts);
 
+/* Reference src/diffraction.c:528-675 (include/diffraction.h:101, declared there
+   only with NFFT_SUPPORT), the structure factor of the use_nfft_for_sf option:
+   F_i = sum over the elements present (Z == 0 included, with the form factor of
+   table row 0) of f_Z(|q_i|) sum_{j in Z} exp(-2 pi i q_i.r_j), unit strengths
+   (no illumination), delta_atoms -> f = 1 for every atom, ints_i = |F_i|^2.
+   The reference evaluates the sums with NFFT3's NNFFT (window parameter m = 2);
+   this one with its own GPU type-3 NUFFT at relative accuracy 1e-6
+   (spsim_hip_sf_nfft). det is unused, as in the reference. HKL_list is copied
+   into the result (the reference leaves that field unset). */
+Diffraction_Pattern *compute_pattern_on_list_by_nfft(Molecule *mol, float *HKL_list, int HKL_list_size, CCD *det,
+                                                     float B, Options *opts);
+
 /* Reference src/diffraction.c:1146-1187 */
 void calculate_pixel_solid_angle(CCD *det);
 /* Reference src/diffraction.c:1109-1144 */

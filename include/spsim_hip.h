@@ -16,6 +16,7 @@
  *   spsim_hip_detector_chain    <- calculate_thomson_correction .. calculate_noiseless_detector_output
  *                                  src/diffraction.c:1109-1204, src/noise.c:32-51,
  *                                  src/amplification.c:54-153 (fused)
+ *   spsim_hip_sf_nfft           <- compute_pattern_on_list_by_nfft src/diffraction.c:528-675 (NNFFT path)
  *   spsim_hip_sf_adjoint        <- (no reference counterpart) gradients of a Poisson
  *                                  negative log-likelihood w.r.t. atom positions and B factors
  *
@@ -50,6 +51,8 @@ extern "C" {
 /* How raw (Z, pos) atoms are turned into packed records. */
 #define SPSIM_HIP_PACK_PLAIN 0  /* compute_pattern_on_list: skip Z==0, delta -> unit species */
 #define SPSIM_HIP_PACK_VECTOR 1 /* vector_compute_pattern_on_list quirks (see diffraction.h) */
+#define SPSIM_HIP_PACK_NFFT 2   /* compute_pattern_on_list_by_nfft: Z==0 kept (f = row 0), delta -> unit
+                                   species for every atom, amplitudes ignored */
 
 /* Packed atom record in HBM (16 bytes): position in metres and the real
    amplitude weight (sqrt of the beam illumination at the atom, 1 for a flat beam). */
@@ -164,6 +167,33 @@ int spsim_hip_sf_spectrum(const float *d_hkl, int npix, const spsim_hip_atom *d_
                           const spsim_hip_segment *d_segs, int nsegs, int natoms, const float *d_ff,
                           float b_global, const double *rot9, int nsamples, const double *scales,
                           const double *weights, float *d_F, float *d_I, const spsim_hip_tile *tile, void *stream);
+
+/* ---- Non-uniform FFT structure factor (replaces compute_pattern_on_list_by_nfft,
+   src/diffraction.c:528-675, the NNFFT path behind the use_nfft_for_sf option) ---- */
+
+/* F[i] = sum_s f_s(|q_i|) sum_{j in s} w_j exp(-2 pi i q_i.r_j), I = |F|^2, over the
+   species segments of the packed atoms (f_s with the global B, 1 for the unit
+   species), computed by a type-3 non-uniform FFT instead of the direct sum:
+   atoms are spread onto per-species grids with an exponential-of-semicircle
+   window, the grids go through a pruned 3-D FFT, and each q interpolates from
+   them. eps sets the window width W = ceil(-log10 eps) + 1 in [4, 10] (eps <= 0:
+   1e-6); the error of F is about eps times sum_j |w_j| f_s. No per-atom B.
+   The grids scale with (atom box x q range) per axis; lines longer than 4096
+   points or grids above 64 GiB are rejected. info (host, may be NULL) receives
+   {W, Mx, My, Mz, nfx, nfy, nfz, Lx, Ly, Lz}: spread grid, FFT and kept-output
+   sizes per axis. Uses the library workspace; synchronises `stream` once (to
+   read the bounds that size the grids), the rest is stream-ordered. */
+int spsim_hip_sf_nfft(const float *d_hkl, int npix, const spsim_hip_atom *d_atoms, const spsim_hip_segment *d_segs,
+                      int nsegs, int natoms, const float *d_ff, float b_global, double eps, float *d_F, float *d_I,
+                      int *info, void *stream);
+
+/* The pruned FFT stage of spsim_hip_sf_nfft on its own: every line of the
+   complex input view [outer][M][inner] is placed at points m = m_lo + i
+   (i < M) of a zero line of nf points (slot m mod nf), transformed forward
+   (exp(-2 pi i l m / nf)), and the values at l = l_lo + i (i < L) are written
+   to the view [outer][L][inner]. nf must be 2,3,5-smooth, in [2, 4096], and >= M. */
+int spsim_hip_fft_pruned(const float *d_in, long long outer, long long inner, int M, int m_lo, int nf, int L,
+                         int l_lo, float *d_out, void *stream);
 
 /* Free the library's cached device workspace and graphs on the current device. */
 int spsim_hip_release(void);

@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "form_factor.hpp"
 
 namespace spsim {
 namespace {
@@ -43,16 +44,6 @@ struct AtomTile {
   int species;
   int pad;
 };
-
-__device__ __forceinline__ float form_factor(const float *__restrict__ c, float s2, float B) {
-  float f = 0.0f;
-  f += c[0] * expf(-(c[4] + B) * s2);
-  f += c[1] * expf(-(c[5] + B) * s2);
-  f += c[2] * expf(-(c[6] + B) * s2);
-  f += c[3] * expf(-(c[7] + B) * s2);
-  f += c[8] * expf(-B * s2);
-  return f;
-}
 
 struct AdjArgs {
   const float *hkl;

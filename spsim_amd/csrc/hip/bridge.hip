@@ -191,6 +191,31 @@ int spsim_bridge_structure_factor_grid(const float *hkl, int nx, int ny, const s
   return structure_factor_impl(hkl, nx * ny, nx, ny, atoms, atom_b, segs, nsegs, natoms, ff, b_global, F, I);
 }
 
+int spsim_bridge_structure_factor_nfft(const float *hkl, int npix, const spsim_hip_atom *atoms,
+                                       const spsim_hip_segment *segs, int nsegs, int natoms, const float *ff,
+                                       float b_global, double eps, Complex *F, float *I) {
+  using namespace spsim;
+  SPSIM_TRY(ensure_device(nullptr));
+  if (npix <= 0) return SPSIM_HIP_OK;
+  Stream st;
+  SPSIM_TRY(st.create());
+  DevBufs b;
+  float *dq = nullptr, *dff = nullptr, *dF = nullptr, *dI = nullptr;
+  spsim_hip_atom *da = nullptr;
+  spsim_hip_segment *ds = nullptr;
+  SPSIM_TRY(b.upload(&dq, hkl, (size_t)npix * 3, st.s));
+  SPSIM_TRY(b.upload(&da, atoms, (size_t)natoms, st.s));
+  SPSIM_TRY(b.upload(&ds, segs, (size_t)nsegs, st.s));
+  SPSIM_TRY(b.upload(&dff, ff, (size_t)SPSIM_HIP_NSPECIES * SPSIM_HIP_FF_STRIDE, st.s));
+  if (F) SPSIM_TRY(b.alloc(&dF, (size_t)npix * 2));
+  SPSIM_TRY(b.alloc(&dI, (size_t)npix));
+  SPSIM_TRY(spsim_hip_sf_nfft(dq, npix, da, ds, nsegs, natoms, dff, b_global, eps, dF, dI, nullptr, st.s));
+  if (F) SPSIM_TRY(download((float *)F, (const float *)dF, (size_t)npix * 2, st.s));
+  SPSIM_TRY(download(I, (const float *)dI, (size_t)npix, st.s));
+  SPSIM_HIP_TRY(hipStreamSynchronize(st.s));
+  return SPSIM_HIP_OK;
+}
+
 int spsim_bridge_thomson(const spsim_hip_detector *det, float *out) {
   using namespace spsim;
   return run_stage(det, Stage::kThomson, nullptr, nullptr, nullptr, 0, 0, 0, out, raw_count(det));

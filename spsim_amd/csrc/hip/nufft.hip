@@ -1,0 +1,693 @@
+// Type-3 non-uniform FFT structure factor for gfx950 (MI355X): the
+// asymptotic alternative to the direct sum, replacing the NNFFT path of
+// compute_pattern_on_list_by_nfft (src/diffraction.c:528-675).
+//
+// The reference runs one NNFFT (non-equispaced nodes in space and frequency,
+// NFFT3 library) per element present, with unit strengths at the atoms,
+// multiplies each element's result by f_Z(|q|) (global B only) and sums:
+//   F(q) = sum_Z f_Z(|q|) sum_{j in Z} exp(-2 pi i q.r_j),  I = |F|^2.
+// This file computes the same sums with its own type-3 NUFFT (the method of
+// Lee & Greengard, SIAM J. Sci. Comput. 2005, with the "exponential of
+// semicircle" window of Barnett, Magland & af Klinteberg, SIAM J. Sci.
+// Comput. 41 (2019) C479-C504), per axis d and with window width W:
+//
+//  1. Centre: x' = x - cx, s' = q - cs (cx, cs = centres of the atom and q
+//     boxes; half-widths X, S). exp(-2 pi i q.x) = e^{-2 pi i cs.x'}
+//     e^{-2 pi i s'.x'} e^{-2 pi i q.cx}: a phase per atom, the centred sum,
+//     a phase per q.
+//  2. Spread: grid spacing h = 1/(2 sigma S) (sigma = 2); every atom adds
+//     c_j phi((m - x'_j/h) 2/W) D_m to W^3 grid points m of its species'
+//     grid, where phi(z) = exp(beta (sqrt(1 - z^2) - 1)) and D_m deconvolves
+//     the second window in advance.
+//  3. Pruned FFT: along x, y, then z, each grid line (M points) is zero-padded
+//     to nf >= 2M points, transformed in LDS (Stockham, radices 4/2/3/5) and
+//     only the L outputs the targets can reach are kept.
+//  4. Interpolate: each q gathers W^3 FFT values per species with the same
+//     window, divides by the first window's transform at s', multiplies by
+//     f_Z(|q|), sums the species and applies the q phase.
+// The error of the centred sum is about 10^{1-W} relative to the number of
+// atoms (W = 7 by default); the grids and FFT run in FP32, the phases in FP64.
+// Atom contributions reach the grid through FP32 atomics, so the last bits
+// of F can vary from run to run.
+#include <cmath>
+#include <vector>
+
+#include "common.hpp"
+#include "form_factor.hpp"
+
+namespace spsim {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kMinW = 4, kMaxW = 10;
+constexpr int kMaxQuad = 2 * kMaxW + 6;
+constexpr int kMaxNf = 4096;      // longest FFT line (LDS: two buffers of kFftLds values)
+constexpr int kFftLds = 4096;     // complex values per LDS buffer per workgroup
+constexpr int kMaxStages = 16;
+constexpr double kSigma = 2.0;     // oversampling of both windows
+constexpr size_t kMaxGridBytes = (size_t)64 << 30;
+
+struct NufftDim {
+  double cx, cs, h;  // atom centre, q centre (1/m), grid spacing (m)
+  int m_lo, M;       // spread grid: m in [m_lo, m_lo + M)
+  int nf;            // FFT length
+  int l_lo, L;       // kept FFT outputs: l in [l_lo, l_lo + L)
+};
+
+struct NufftGeom {
+  NufftDim d[3];
+  float beta;
+  int nq;
+  float qz[kMaxQuad];  // Gauss-Legendre nodes on [0, 1]
+  float qc[kMaxQuad];  // 2 * weight * phi(node): phi_hat(xi) = sum qc cos(2 pi xi qz)
+};
+
+struct FftPass {
+  long long nlines;  // lines = outer * inner
+  int inner;         // stride between consecutive points of a line (1: lines are contiguous)
+  int M, nf, L, m_lo, l_lo;
+  int T;             // lines per workgroup
+  int nstages;
+  int radix[kMaxStages];
+};
+
+__device__ __forceinline__ float es_window(float z, float beta) {
+  const float r = 1.0f - z * z;
+  return r > 0.0f ? expf(beta * (sqrtf(r) - 1.0f)) : 0.0f;
+}
+
+// Fourier transform of the window at xi (cycles per half-width), by quadrature.
+__device__ __forceinline__ float es_hat(const NufftGeom &g, float xi) {
+  float s = 0.0f;
+  for (int i = 0; i < g.nq; ++i) s += g.qc[i] * cospif(2.0f * xi * g.qz[i]);
+  return s;
+}
+
+__device__ __forceinline__ double nan_min(double a, double b) { return (a != a || b != b) ? a + b : fmin(a, b); }
+__device__ __forceinline__ double nan_max(double a, double b) { return (a != a || b != b) ? a + b : fmax(a, b); }
+
+// Per-block minima and maxima of the atom positions (0..2) and q (3..5);
+// part[blk * 12 + v] for v < 6 holds minima, 6..11 maxima. NaN propagates.
+__global__ __launch_bounds__(kThreads) void bounds_partial(const spsim_hip_atom *__restrict__ atoms, int natoms,
+                                                          const float *__restrict__ hkl, int npix,
+                                                          double *__restrict__ part) {
+  double lo[6], hi[6];
+  for (int v = 0; v < 6; ++v) {
+    lo[v] = INFINITY;
+    hi[v] = -INFINITY;
+  }
+  const int stride = gridDim.x * kThreads;
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < natoms; i += stride) {
+    const spsim_hip_atom a = atoms[i];
+    const double p[3] = {a.x, a.y, a.z};
+    for (int v = 0; v < 3; ++v) {
+      lo[v] = nan_min(lo[v], p[v]);
+      hi[v] = nan_max(hi[v], p[v]);
+    }
+  }
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < npix; i += stride) {
+    for (int v = 0; v < 3; ++v) {
+      const double q = hkl[3 * (size_t)i + v];
+      lo[3 + v] = nan_min(lo[3 + v], q);
+      hi[3 + v] = nan_max(hi[3 + v], q);
+    }
+  }
+  __shared__ double red[12][kThreads];
+  for (int v = 0; v < 6; ++v) {
+    red[v][threadIdx.x] = lo[v];
+    red[6 + v][threadIdx.x] = hi[v];
+  }
+  __syncthreads();
+  for (int w = kThreads / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      for (int v = 0; v < 6; ++v) {
+        red[v][threadIdx.x] = nan_min(red[v][threadIdx.x], red[v][threadIdx.x + w]);
+        red[6 + v][threadIdx.x] = nan_max(red[6 + v][threadIdx.x], red[6 + v][threadIdx.x + w]);
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 12) part[(size_t)blockIdx.x * 12 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// One workgroup folds nblk partial records into out[12].
+__global__ __launch_bounds__(kThreads) void bounds_final(const double *__restrict__ part, int nblk,
+                                                        double *__restrict__ out) {
+  __shared__ double red[12][kThreads];
+  for (int v = 0; v < 12; ++v) {
+    double acc = v < 6 ? INFINITY : -INFINITY;
+    for (int b = threadIdx.x; b < nblk; b += kThreads) {
+      const double x = part[(size_t)b * 12 + v];
+      acc = v < 6 ? nan_min(acc, x) : nan_max(acc, x);
+    }
+    red[v][threadIdx.x] = acc;
+  }
+  __syncthreads();
+  for (int w = kThreads / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      for (int v = 0; v < 12; ++v)
+        red[v][threadIdx.x] = v < 6 ? nan_min(red[v][threadIdx.x], red[v][threadIdx.x + w])
+                                    : nan_max(red[v][threadIdx.x], red[v][threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 12) out[threadIdx.x] = red[threadIdx.x][0];
+}
+
+// D_m = 2 / (W phi_hat(m W / (2 nf))) for the spread grid of every axis,
+// stored back to back: dec[0 .. Mx) | dec[Mx .. Mx+My) | dec[.. + Mz).
+__global__ __launch_bounds__(kThreads) void deconv_table(NufftGeom g, int W, float *__restrict__ dec) {
+  const int i = blockIdx.x * kThreads + threadIdx.x;
+  int off = 0;
+  for (int d = 0; d < 3; ++d) {
+    const NufftDim &D = g.d[d];
+    if (i < D.M) {
+      const float xi = (float)((double)(D.m_lo + i) * W / (2.0 * D.nf));
+      dec[off + i] = 2.0f / ((float)W * es_hat(g, xi));
+    }
+    off += D.M;
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void fill_zero(float4 *__restrict__ p, size_t n4) {
+  for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n4; i += (size_t)gridDim.x * kThreads)
+    p[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+// One thread per packed atom: adds the atom's windowed, phased unit strength
+// to W^3 points of its species' grid [ch][Mz][My][Mx].
+template <int W>
+__global__ __launch_bounds__(kThreads) void nufft_spread(const spsim_hip_atom *__restrict__ atoms, int natoms,
+                                                        const spsim_hip_segment *__restrict__ segs, int nsegs,
+                                                        NufftGeom g, const float *__restrict__ dec,
+                                                        float2 *__restrict__ grid) {
+  const int j = blockIdx.x * kThreads + threadIdx.x;
+  if (j >= natoms) return;
+  int ch = 0;
+  for (int s = 1; s < nsegs; ++s)
+    if (segs[s].start <= j) ch = s;
+  const spsim_hip_atom a = atoms[j];
+  const double xc[3] = {(double)a.x - g.d[0].cx, (double)a.y - g.d[1].cx, (double)a.z - g.d[2].cx};
+  double ph = -(xc[0] * g.d[0].cs + xc[1] * g.d[1].cs + xc[2] * g.d[2].cs);
+  ph -= rint(ph);
+  float sn, cs;
+  sincospif((float)(2.0 * ph), &sn, &cs);
+  const float cr = a.w * cs, ci = a.w * sn;
+
+  float wt[3][W];
+  int base[3];
+  int off = 0;
+  for (int d = 0; d < 3; ++d) {
+    const NufftDim &D = g.d[d];
+    const double u = xc[d] / D.h;
+    const double m0 = ceil(u - 0.5 * W);
+    const double i0 = m0 - D.m_lo;
+    if (!(i0 >= 0.0 && i0 + W <= (double)D.M)) return;  // outside the grid: excluded by the bounds
+    base[d] = (int)i0;
+    const float fb = (float)(m0 - u);
+#pragma unroll
+    for (int i = 0; i < W; ++i) wt[d][i] = es_window((fb + (float)i) * (2.0f / W), g.beta) * dec[off + base[d] + i];
+    off += D.M;
+  }
+  const int Mx = g.d[0].M, My = g.d[1].M, Mz = g.d[2].M;
+  float2 *gc = grid + (size_t)ch * Mz * My * Mx;
+  for (int iz = 0; iz < W; ++iz) {
+    for (int iy = 0; iy < W; ++iy) {
+      const float wzy = wt[2][iz] * wt[1][iy];
+      const float vr = cr * wzy, vi = ci * wzy;
+      float *row = reinterpret_cast<float *>(gc + ((size_t)(base[2] + iz) * My + base[1] + iy) * Mx + base[0]);
+#pragma unroll
+      for (int ix = 0; ix < W; ++ix) {
+        unsafeAtomicAdd(row + 2 * ix, vr * wt[0][ix]);
+        unsafeAtomicAdd(row + 2 * ix + 1, vi * wt[0][ix]);
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+// -i * a
+__device__ __forceinline__ float2 mul_mi(float2 a) { return make_float2(a.y, -a.x); }
+
+// One Stockham stage of radix R on a line: butterfly i (of nb = n/R) reads
+// x[i + t nb], twiddles by exp(-2 pi i t k / (span R)) with k = i mod span,
+// applies the R-point DFT and writes y[(i - k) R + k + s span].
+template <int R>
+__device__ __forceinline__ void stockham_bfly(const float2 *__restrict__ x, float2 *__restrict__ y, int i, int nb,
+                                              int span) {
+  const int k = i % span;
+  float2 u[R];
+#pragma unroll
+  for (int t = 0; t < R; ++t) u[t] = x[i + t * nb];
+  if (k) {
+    const float a = -2.0f * (float)k / (float)(span * R);
+#pragma unroll
+    for (int t = 1; t < R; ++t) {
+      float s, c;
+      sincospif(a * (float)t, &s, &c);
+      u[t] = cmul(u[t], make_float2(c, s));
+    }
+  }
+  float2 v[R];
+  if constexpr (R == 2) {
+    v[0] = cadd(u[0], u[1]);
+    v[1] = csub(u[0], u[1]);
+  } else if constexpr (R == 4) {
+    const float2 a = cadd(u[0], u[2]), b = csub(u[0], u[2]);
+    const float2 c = cadd(u[1], u[3]), d = mul_mi(csub(u[1], u[3]));
+    v[0] = cadd(a, c);
+    v[2] = csub(a, c);
+    v[1] = cadd(b, d);
+    v[3] = csub(b, d);
+  } else if constexpr (R == 3) {
+    constexpr float s3 = 0.86602540378443864676f;  // sin(2 pi / 3)
+    const float2 t1 = cadd(u[1], u[2]), t2 = csub(u[1], u[2]);
+    const float2 m = make_float2(u[0].x - 0.5f * t1.x, u[0].y - 0.5f * t1.y);
+    const float2 r = mul_mi(make_float2(s3 * t2.x, s3 * t2.y));
+    v[0] = cadd(u[0], t1);
+    v[1] = cadd(m, r);
+    v[2] = csub(m, r);
+  } else {
+    static_assert(R == 5, "radix");
+    constexpr float c1 = 0.30901699437494742410f;   // cos(2 pi / 5)
+    constexpr float c2 = -0.80901699437494742410f;  // cos(4 pi / 5)
+    constexpr float s1 = 0.95105651629515357212f;   // sin(2 pi / 5)
+    constexpr float s2 = 0.58778525229247312917f;   // sin(4 pi / 5)
+    const float2 a1 = cadd(u[1], u[4]), b1 = csub(u[1], u[4]);
+    const float2 a2 = cadd(u[2], u[3]), b2 = csub(u[2], u[3]);
+    const float2 m1 = make_float2(u[0].x + c1 * a1.x + c2 * a2.x, u[0].y + c1 * a1.y + c2 * a2.y);
+    const float2 m2 = make_float2(u[0].x + c2 * a1.x + c1 * a2.x, u[0].y + c2 * a1.y + c1 * a2.y);
+    const float2 r1 = mul_mi(make_float2(s1 * b1.x + s2 * b2.x, s1 * b1.y + s2 * b2.y));
+    const float2 r2 = mul_mi(make_float2(s2 * b1.x - s1 * b2.x, s2 * b1.y - s1 * b2.y));
+    v[0] = make_float2(u[0].x + a1.x + a2.x, u[0].y + a1.y + a2.y);
+    v[1] = cadd(m1, r1);
+    v[4] = csub(m1, r1);
+    v[2] = cadd(m2, r2);
+    v[3] = csub(m2, r2);
+  }
+  const int j = (i - k) * R + k;
+#pragma unroll
+  for (int s = 0; s < R; ++s) y[j + s * span] = v[s];
+}
+
+// Pruned forward FFT of T lines per workgroup. A line of the input view
+// [outer][M][inner] holds points m = m_lo + i at FFT slot m mod nf (the rest
+// zero); the output view [outer][L][inner] receives FFT values at l = l_lo + i
+// (slot l mod nf).
+__global__ __launch_bounds__(kThreads) void fft_lines(FftPass p, const float2 *__restrict__ in,
+                                                     float2 *__restrict__ out) {
+  extern __shared__ float2 lds[];
+  const int nf = p.nf, T = p.T;
+  float2 *x = lds;
+  float2 *y = lds + T * nf;
+  const long long line0 = (long long)blockIdx.x * T;
+  for (int e = threadIdx.x; e < T * nf; e += kThreads) x[e] = make_float2(0.0f, 0.0f);
+  __syncthreads();
+  for (int e = threadIdx.x; e < T * p.M; e += kThreads) {
+    int t, i;
+    if (p.inner == 1) {
+      t = e / p.M;
+      i = e - t * p.M;
+    } else {
+      i = e / T;
+      t = e - i * T;
+    }
+    const long long line = line0 + t;
+    if (line >= p.nlines) continue;
+    const long long o = line / p.inner;
+    const long long c = line - o * p.inner;
+    int slot = (p.m_lo + i) % nf;
+    if (slot < 0) slot += nf;
+    x[t * nf + slot] = in[(o * p.M + i) * p.inner + c];
+  }
+  __syncthreads();
+  int span = 1;
+  for (int s = 0; s < p.nstages; ++s) {
+    const int R = p.radix[s];
+    const int nb = nf / R;
+    for (int b = threadIdx.x; b < T * nb; b += kThreads) {
+      const int t = b / nb, i = b - t * nb;
+      const float2 *xs = x + t * nf;
+      float2 *ys = y + t * nf;
+      switch (R) {
+        case 4: stockham_bfly<4>(xs, ys, i, nb, span); break;
+        case 2: stockham_bfly<2>(xs, ys, i, nb, span); break;
+        case 3: stockham_bfly<3>(xs, ys, i, nb, span); break;
+        default: stockham_bfly<5>(xs, ys, i, nb, span); break;
+      }
+    }
+    __syncthreads();
+    float2 *tmp = x;
+    x = y;
+    y = tmp;
+    span *= R;
+  }
+  for (int e = threadIdx.x; e < T * p.L; e += kThreads) {
+    int t, i;
+    if (p.inner == 1) {
+      t = e / p.L;
+      i = e - t * p.L;
+    } else {
+      i = e / T;
+      t = e - i * T;
+    }
+    const long long line = line0 + t;
+    if (line >= p.nlines) continue;
+    const long long o = line / p.inner;
+    const long long c = line - o * p.inner;
+    int slot = (p.l_lo + i) % nf;
+    if (slot < 0) slot += nf;
+    out[(o * p.L + i) * p.inner + c] = x[t * nf + slot];
+  }
+}
+
+// One thread per q: W^3 gathers per species grid [ch][Lz][Ly][Lx], window
+// correction, form factor, species sum and the q phase.
+template <int W>
+__global__ __launch_bounds__(kThreads) void nufft_interp(const float *__restrict__ hkl, int npix, NufftGeom g,
+                                                        const float2 *__restrict__ H, int nch,
+                                                        const spsim_hip_segment *__restrict__ segs,
+                                                        const float *__restrict__ ff, float b_global,
+                                                        float2 *__restrict__ F, float *__restrict__ I) {
+  const int k = blockIdx.x * kThreads + threadIdx.x;
+  if (k >= npix) return;
+  const double q[3] = {hkl[3 * (size_t)k], hkl[3 * (size_t)k + 1], hkl[3 * (size_t)k + 2]};
+  float wt[3][W];
+  int base[3];
+  float corr = 1.0f;
+  bool ok = true;
+  for (int d = 0; d < 3; ++d) {
+    const NufftDim &D = g.d[d];
+    const double t = (q[d] - D.cs) * D.h;
+    const double v = t * D.nf;
+    const double l0 = ceil(v - 0.5 * W);
+    const double i0 = l0 - D.l_lo;
+    ok = ok && (i0 >= 0.0 && i0 + W <= (double)D.L);
+    base[d] = ok ? (int)i0 : 0;
+    const float fb = (float)(l0 - v);
+#pragma unroll
+    for (int i = 0; i < W; ++i) wt[d][i] = es_window((fb + (float)i) * (2.0f / W), g.beta);
+    corr *= 2.0f / ((float)W * es_hat(g, (float)t * (0.5f * W)));
+  }
+  float Fr, Fi;
+  if (!ok) {
+    // Unreachable for q inside the bounds the grids were sized for.
+    Fr = Fi = __builtin_nanf("");
+  } else {
+    const float s2 = (float)((q[0] * q[0] + q[1] * q[1] + q[2] * q[2]) * 2.5e-21);
+    const int Lx = g.d[0].L, Ly = g.d[1].L, Lz = g.d[2].L;
+    const size_t chs = (size_t)Lz * Ly * Lx;
+    float fr = 0.0f, fi = 0.0f;
+    for (int ch = 0; ch < nch; ++ch) {
+      const int sp = segs[ch].species;
+      const float f = form_factor(ff + sp * SPSIM_HIP_FF_STRIDE, s2, sp == SPSIM_HIP_UNIT_SPECIES ? 0.0f : b_global);
+      const float2 *Hc = H + ch * chs;
+      float ar = 0.0f, ai = 0.0f;
+      for (int iz = 0; iz < W; ++iz) {
+        for (int iy = 0; iy < W; ++iy) {
+          const float2 *row = Hc + ((size_t)(base[2] + iz) * Ly + base[1] + iy) * Lx + base[0];
+          float rr = 0.0f, ri = 0.0f;
+#pragma unroll
+          for (int ix = 0; ix < W; ++ix) {
+            const float2 h = row[ix];
+            rr += h.x * wt[0][ix];
+            ri += h.y * wt[0][ix];
+          }
+          const float wzy = wt[2][iz] * wt[1][iy];
+          ar += wzy * rr;
+          ai += wzy * ri;
+        }
+      }
+      fr += f * ar;
+      fi += f * ai;
+    }
+    fr *= corr;
+    fi *= corr;
+    double ph = -(q[0] * g.d[0].cx + q[1] * g.d[1].cx + q[2] * g.d[2].cx);
+    ph -= rint(ph);
+    float sn, cs;
+    sincospif((float)(2.0 * ph), &sn, &cs);
+    Fr = fr * cs - fi * sn;
+    Fi = fr * sn + fi * cs;
+  }
+  if (F) F[k] = make_float2(Fr, Fi);
+  if (I) I[k] = Fr * Fr + Fi * Fi;
+}
+
+__global__ __launch_bounds__(kThreads) void zero_pattern(int npix, float2 *__restrict__ F, float *__restrict__ I) {
+  const int k = blockIdx.x * kThreads + threadIdx.x;
+  if (k >= npix) return;
+  if (F) F[k] = make_float2(0.0f, 0.0f);
+  if (I) I[k] = 0.0f;
+}
+
+// ---- host side ----
+
+// Gauss-Legendre nodes and weights on [0, 1] (Newton iteration on P_n).
+void gauss_legendre01(int n, double *z, double *w) {
+  for (int i = 0; i < n; ++i) {
+    double x = cos(M_PI * (i + 0.75) / (n + 0.5));
+    double dp = 1.0;
+    for (int it = 0; it < 100; ++it) {
+      double p0 = 1.0, p1 = x;
+      for (int k = 2; k <= n; ++k) {
+        const double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
+        p0 = p1;
+        p1 = p2;
+      }
+      const double pn = n == 1 ? x : p1, pm = n == 1 ? 1.0 : p0;
+      dp = n * (x * pn - pm) / (x * x - 1.0);
+      const double dx = pn / dp;
+      x -= dx;
+      if (fabs(dx) < 1e-16) break;
+    }
+    z[i] = 0.5 * (1.0 - x);
+    w[i] = 1.0 / ((1.0 - x * x) * dp * dp);  // 2/((1-x^2) P'^2), halved for [0, 1]
+  }
+}
+
+bool smooth_length(int n) {
+  for (int p : {2, 3, 5})
+    while (n % p == 0) n /= p;
+  return n == 1;
+}
+
+int fft_length(int at_least) {
+  int n = at_least + (at_least & 1);
+  while (!smooth_length(n)) n += 2;
+  return n;
+}
+
+int radices(int n, int *r) {
+  int k = 0;
+  while (n % 4 == 0) r[k++] = 4, n /= 4;
+  while (n % 2 == 0) r[k++] = 2, n /= 2;
+  while (n % 3 == 0) r[k++] = 3, n /= 3;
+  while (n % 5 == 0) r[k++] = 5, n /= 5;
+  return k;
+}
+
+int width_for(double eps) {
+  if (!(eps > 0.0)) eps = 1e-6;
+  int W = (int)ceil(-log10(eps)) + 1;
+  return W < kMinW ? kMinW : (W > kMaxW ? kMaxW : W);
+}
+
+// Grid of one axis for atoms in [xmin, xmax] and q in [smin, smax].
+int plan_axis(double xmin, double xmax, double smin, double smax, int W, NufftDim *D) {
+  D->cx = 0.5 * (xmin + xmax);
+  D->cs = 0.5 * (smin + smax);
+  const double X = 0.5 * (xmax - xmin), S = 0.5 * (smax - smin);
+  D->h = S > 0.0 ? 1.0 / (2.0 * kSigma * S) : (X > 0.0 ? 2.0 * X : 1e-9);
+  const double umax = X / D->h;
+  if (!(umax < 0.25 * kMaxNf)) return fail(SPSIM_HIP_ERR_INVALID, "NUFFT grid too large (box x q-range %g)", umax);
+  D->m_lo = (int)floor(-umax - 0.5 * W) - 1;
+  const int m_hi = (int)ceil(umax + 0.5 * W) + 1;
+  D->M = m_hi - D->m_lo + 1;
+  D->nf = fft_length((int)ceil(kSigma * D->M));
+  if (D->nf > kMaxNf) return fail(SPSIM_HIP_ERR_INVALID, "NUFFT line of %d points exceeds %d", D->nf, kMaxNf);
+  const double vmax = S * D->h * D->nf;
+  D->l_lo = (int)floor(-vmax - 0.5 * W) - 1;
+  const int l_hi = (int)ceil(vmax + 0.5 * W) + 1;
+  D->L = l_hi - D->l_lo + 1;
+  return SPSIM_HIP_OK;
+}
+
+FftPass make_pass(const NufftDim &D, long long outer, long long inner) {
+  FftPass p{};
+  p.nlines = outer * inner;
+  p.inner = (int)inner;
+  p.M = D.M;
+  p.nf = D.nf;
+  p.L = D.L;
+  p.m_lo = D.m_lo;
+  p.l_lo = D.l_lo;
+  p.T = kFftLds / D.nf;
+  if (p.T < 1) p.T = 1;
+  p.nstages = radices(D.nf, p.radix);
+  return p;
+}
+
+int launch_fft(const FftPass &p, const float2 *in, float2 *out, hipStream_t stream) {
+  const long long blocks = (p.nlines + p.T - 1) / p.T;
+  if (blocks <= 0) return SPSIM_HIP_OK;
+  if (blocks > 0x7fffffffLL) return fail(SPSIM_HIP_ERR_INVALID, "too many FFT lines");
+  const size_t lds = 2 * (size_t)p.T * p.nf * sizeof(float2);
+  hipLaunchKernelGGL(fft_lines, dim3((unsigned)blocks), dim3(kThreads), lds, stream, p, in, out);
+  SPSIM_HIP_TRY(hipGetLastError());
+  return SPSIM_HIP_OK;
+}
+
+template <int W>
+int launch_spread_interp(bool spread, const NufftGeom &g, const spsim_hip_atom *atoms, int natoms,
+                         const spsim_hip_segment *segs, int nsegs, const float *dec, float2 *grid, const float *hkl,
+                         int npix, const float2 *H, const float *ff, float b_global, float2 *F, float *I,
+                         hipStream_t stream) {
+  if (spread) {
+    hipLaunchKernelGGL(nufft_spread<W>, dim3((natoms + kThreads - 1) / kThreads), dim3(kThreads), 0, stream, atoms,
+                       natoms, segs, nsegs, g, dec, grid);
+  } else {
+    hipLaunchKernelGGL(nufft_interp<W>, dim3((npix + kThreads - 1) / kThreads), dim3(kThreads), 0, stream, hkl, npix,
+                       g, H, nsegs, segs, ff, b_global, F, I);
+  }
+  SPSIM_HIP_TRY(hipGetLastError());
+  return SPSIM_HIP_OK;
+}
+
+int dispatch_w(int W, bool spread, const NufftGeom &g, const spsim_hip_atom *atoms, int natoms,
+               const spsim_hip_segment *segs, int nsegs, const float *dec, float2 *grid, const float *hkl, int npix,
+               const float2 *H, const float *ff, float b_global, float2 *F, float *I, hipStream_t s) {
+#define SPSIM_NUFFT_W(w) \
+  case w: return launch_spread_interp<w>(spread, g, atoms, natoms, segs, nsegs, dec, grid, hkl, npix, H, ff, b_global, F, I, s)
+  switch (W) {
+    SPSIM_NUFFT_W(4);
+    SPSIM_NUFFT_W(5);
+    SPSIM_NUFFT_W(6);
+    SPSIM_NUFFT_W(7);
+    SPSIM_NUFFT_W(8);
+    SPSIM_NUFFT_W(9);
+    SPSIM_NUFFT_W(10);
+    default: return fail(SPSIM_HIP_ERR_INVALID, "window width %d", W);
+  }
+#undef SPSIM_NUFFT_W
+}
+
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+}  // namespace
+}  // namespace spsim
+
+extern "C" int spsim_hip_sf_nfft(const float *d_hkl, int npix, const spsim_hip_atom *d_atoms,
+                                 const spsim_hip_segment *d_segs, int nsegs, int natoms, const float *d_ff,
+                                 float b_global, double eps, float *d_F, float *d_I, int *info, void *stream) {
+  using namespace spsim;
+  if (npix < 0 || natoms < 0 || nsegs < 0 || nsegs > SPSIM_HIP_NSPECIES)
+    return fail(SPSIM_HIP_ERR_INVALID, "negative size or too many segments");
+  if ((npix > 0 && !d_hkl) || (natoms > 0 && (!d_atoms || !d_segs || nsegs < 1)) || !d_ff)
+    return fail(SPSIM_HIP_ERR_INVALID, "NULL input pointer");
+  SPSIM_TRY(ensure_device(nullptr));
+  hipStream_t s = (hipStream_t)stream;
+  const int W = width_for(eps);
+  if (info)
+    for (int i = 0; i < 10; ++i) info[i] = 0;
+  if (info) info[0] = W;
+  if (npix == 0) return SPSIM_HIP_OK;
+  if (!d_F && !d_I) return fail(SPSIM_HIP_ERR_INVALID, "both outputs are NULL");
+  const unsigned pblocks = (unsigned)((npix + kThreads - 1) / kThreads);
+  if (natoms == 0) {
+    hipLaunchKernelGGL(zero_pattern, dim3(pblocks), dim3(kThreads), 0, s, npix, (float2 *)d_F, d_I);
+    SPSIM_HIP_TRY(hipGetLastError());
+    return SPSIM_HIP_OK;
+  }
+
+  // 1. Bounds of the atoms and of q (one host synchronisation).
+  const int most = natoms > npix ? natoms : npix;
+  int nblk = (most + kThreads - 1) / kThreads;
+  if (nblk > 1024) nblk = 1024;
+  void *ws = nullptr;
+  SPSIM_TRY(workspace(align256(sizeof(double) * 12 * (nblk + 1)), &ws));
+  double *part = (double *)ws;
+  double *dbounds = part + (size_t)12 * nblk;
+  hipLaunchKernelGGL(bounds_partial, dim3(nblk), dim3(kThreads), 0, s, d_atoms, natoms, d_hkl, npix, part);
+  SPSIM_HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(bounds_final, dim3(1), dim3(kThreads), 0, s, part, nblk, dbounds);
+  SPSIM_HIP_TRY(hipGetLastError());
+  double hb[12];
+  SPSIM_HIP_TRY(hipMemcpyAsync(hb, dbounds, sizeof hb, hipMemcpyDeviceToHost, s));
+  SPSIM_HIP_TRY(hipStreamSynchronize(s));
+  for (double v : hb)
+    if (!std::isfinite(v)) return fail(SPSIM_HIP_ERR_INVALID, "non-finite atom position or q");
+
+  // 2. Plan the grids.
+  NufftGeom g{};
+  for (int d = 0; d < 3; ++d) SPSIM_TRY(plan_axis(hb[d], hb[6 + d], hb[3 + d], hb[9 + d], W, &g.d[d]));
+  g.beta = (float)(2.30 * W);
+  g.nq = 2 * W + 6;
+  {
+    double z[kMaxQuad], w[kMaxQuad];
+    gauss_legendre01(g.nq, z, w);
+    for (int i = 0; i < g.nq; ++i) {
+      g.qz[i] = (float)z[i];
+      g.qc[i] = (float)(2.0 * w[i] * exp(g.beta * (sqrt(1.0 - z[i] * z[i]) - 1.0)));
+    }
+  }
+  const NufftDim &X = g.d[0], &Y = g.d[1], &Z = g.d[2];
+  if (info) {
+    const int v[9] = {X.M, Y.M, Z.M, X.nf, Y.nf, Z.nf, X.L, Y.L, Z.L};
+    for (int i = 0; i < 9; ++i) info[1 + i] = v[i];
+  }
+  const size_t nch = (size_t)nsegs;
+  const size_t nG = nch * Z.M * Y.M * X.M, nA = nch * Z.M * Y.M * X.L;
+  const size_t nB = nch * Z.M * Y.L * X.L, nH = nch * Z.L * Y.L * X.L;
+  const size_t n1 = nG > nB ? nG : nB, n2 = nA > nH ? nA : nH;
+  const size_t b_dec = align256(sizeof(float) * (X.M + Y.M + Z.M));
+  const size_t b1 = align256(sizeof(float2) * n1), b2 = align256(sizeof(float2) * n2);
+  if (b1 + b2 > kMaxGridBytes) return fail(SPSIM_HIP_ERR_INVALID, "NUFFT grids need %zu bytes", b1 + b2);
+  SPSIM_TRY(workspace(b_dec + b1 + b2, &ws));
+  float *dec = (float *)ws;
+  float2 *buf1 = (float2 *)((char *)ws + b_dec);
+  float2 *buf2 = (float2 *)((char *)ws + b_dec + b1);
+
+  // 3. Deconvolution table, zeroed grids, spread.
+  const int Mmax = X.M > Y.M ? (X.M > Z.M ? X.M : Z.M) : (Y.M > Z.M ? Y.M : Z.M);
+  hipLaunchKernelGGL(deconv_table, dim3((Mmax + kThreads - 1) / kThreads), dim3(kThreads), 0, s, g, W, dec);
+  SPSIM_HIP_TRY(hipGetLastError());
+  const size_t n4 = (sizeof(float2) * nG + 15) / 16;
+  hipLaunchKernelGGL(fill_zero, dim3((unsigned)((n4 + kThreads - 1) / kThreads < 4096 ? (n4 + kThreads - 1) / kThreads : 4096)),
+                     dim3(kThreads), 0, s, (float4 *)buf1, n4);
+  SPSIM_HIP_TRY(hipGetLastError());
+  SPSIM_TRY(dispatch_w(W, true, g, d_atoms, natoms, d_segs, nsegs, dec, buf1, nullptr, 0, nullptr, d_ff, b_global,
+                       nullptr, nullptr, s));
+
+  // 4. Pruned FFT along x, y, z: G (buf1) -> A (buf2) -> B (buf1) -> H (buf2).
+  SPSIM_TRY(launch_fft(make_pass(X, (long long)nch * Z.M * Y.M, 1), buf1, buf2, s));
+  SPSIM_TRY(launch_fft(make_pass(Y, (long long)nch * Z.M, X.L), buf2, buf1, s));
+  SPSIM_TRY(launch_fft(make_pass(Z, (long long)nch, (long long)Y.L * X.L), buf1, buf2, s));
+
+  // 5. Interpolate at every q.
+  SPSIM_TRY(dispatch_w(W, false, g, nullptr, 0, d_segs, nsegs, nullptr, nullptr, d_hkl, npix, buf2, d_ff, b_global,
+                       (float2 *)d_F, d_I, s));
+  return SPSIM_HIP_OK;
+}
+
+extern "C" int spsim_hip_fft_pruned(const float *d_in, long long outer, long long inner, int M, int m_lo, int nf,
+                                    int L, int l_lo, float *d_out, void *stream) {
+  using namespace spsim;
+  if (outer < 0 || inner < 1 || M < 1 || L < 1 || !d_in || !d_out)
+    return fail(SPSIM_HIP_ERR_INVALID, "bad FFT shape or NULL pointer");
+  if (nf < 2 || nf > kMaxNf || !smooth_length(nf) || M > nf)
+    return fail(SPSIM_HIP_ERR_INVALID, "FFT length %d must be a 2,3,5-smooth number in [2, %d] and >= M", nf, kMaxNf);
+  if (inner > 0x7fffffffLL) return fail(SPSIM_HIP_ERR_INVALID, "inner stride too large");
+  SPSIM_TRY(ensure_device(nullptr));
+  NufftDim D{};
+  D.M = M;
+  D.m_lo = m_lo;
+  D.nf = nf;
+  D.L = L;
+  D.l_lo = l_lo;
+  return launch_fft(make_pass(D, outer, inner), (const float2 *)d_in, (float2 *)d_out, (hipStream_t)stream);
+}

@@ -21,6 +21,7 @@
 //  * Per-atom Debye-Waller factor exp(-B_j s^2) (optional) is folded with the
 //    amplitude into one exp2: w_j e^{-B_j s^2} = 2^(log2 w_j - B_j s^2 log2 e).
 #include "sf_kernels.hpp"
+#include "form_factor.hpp"
 
 #include <algorithm>
 
@@ -37,18 +38,6 @@ struct __align__(16) AtomLds {
   float b;  // per-atom B factor (A^2)
 };
 static_assert(sizeof(AtomLds) == 32, "LDS atom record must be 32 bytes");
-
-// Cromer-Mann form factor times the global Debye-Waller factor, with s2 = s^2 in
-// A^-2. Terms are added in the reference's order (src/diffraction.c:183-193).
-__device__ __forceinline__ float form_factor(const float *__restrict__ c, float s2, float B) {
-  float f = 0.0f;
-  f += c[0] * expf(-(c[4] + B) * s2);
-  f += c[1] * expf(-(c[5] + B) * s2);
-  f += c[2] * expf(-(c[6] + B) * s2);
-  f += c[3] * expf(-(c[7] + B) * s2);
-  f += c[8] * expf(-B * s2);
-  return f;
-}
 
 // Loads q for one pixel (zero for pixels past the end) and s^2 = (|q| 1e-10 / 2)^2
 // in A^-2. |q| is rotation invariant, so s^2 is the same with or without R.

@@ -41,6 +41,11 @@ int spsim_bridge_structure_factor_grid(const float *hkl, int nx, int ny, const s
                                        const float *atom_b, const spsim_hip_segment *segs, int nsegs, int natoms,
                                        const float *ff, float b_global, Complex *F, float *I);
 
+/* Same by the non-uniform FFT (spsim_hip_sf_nfft) with relative accuracy eps. */
+int spsim_bridge_structure_factor_nfft(const float *hkl, int npix, const spsim_hip_atom *atoms,
+                                       const spsim_hip_segment *segs, int nsegs, int natoms, const float *ff,
+                                       float b_global, double eps, Complex *F, float *I);
+
 /* Detector stages. Inputs are host arrays of nx*ny raw pixels; binned outputs
    have (nx/bin_x)*(ny/bin_y) entries. */
 int spsim_bridge_thomson(const spsim_hip_detector *det, float *out);

@@ -255,6 +255,49 @@ Diffraction_Pattern *spsim_spectrum_pattern_on_list(Molecule *mol, float *HKL_li
   return compute_pattern(mol, HKL_list, HKL_list_size, B, exp, opts, SPSIM_HIP_PACK_PLAIN, n, scales, weights);
 }
 
+Diffraction_Pattern *compute_pattern_on_list_by_nfft(Molecule *mol, float *HKL_list, int HKL_list_size, CCD *det,
+                                                     float B, Options *opts) {
+  (void)det;
+  if (!mol || HKL_list_size < 0 || (HKL_list_size > 0 && !HKL_list) || mol->natoms < 0) return NULL;
+  const int N = mol->natoms;
+  const int delta = opts ? opts->delta_atoms : 0;
+  Diffraction_Pattern *res = (Diffraction_Pattern *)calloc(1, sizeof(Diffraction_Pattern));
+  spsim_hip_atom *atoms = (spsim_hip_atom *)malloc(sizeof(spsim_hip_atom) * (N > 0 ? N : 1));
+  spsim_hip_segment segs[SPSIM_HIP_NSPECIES];
+  if (!res || !atoms) goto fail;
+  res->F = (Complex *)malloc(sizeof(Complex) * (size_t)HKL_list_size + 1);
+  res->ints = (float *)malloc(sizeof(float) * (size_t)HKL_list_size + 1);
+  res->HKL_list = (float *)malloc(sizeof(float) * 3 * (size_t)HKL_list_size + 1);
+  if (!res->F || !res->ints || !res->HKL_list) goto fail;
+  if (HKL_list_size) memcpy(res->HKL_list, HKL_list, sizeof(float) * 3 * (size_t)HKL_list_size);
+  res->HKL_list_size = HKL_list_size;
+  res->rot = NULL;
+  int natoms = 0, nsegs = 0;
+  int rc = spsim_hip_pack_atoms(mol->atomic_number, mol->pos, NULL, NULL, N, delta, SPSIM_HIP_PACK_NFFT, atoms, NULL,
+                                NULL, segs, &natoms, &nsegs);
+  if (rc != SPSIM_HIP_OK) {
+    fprintf(stderr, "spsim: compute_pattern_on_list_by_nfft: atomic numbers must lie in [0, 99]\n");
+    goto fail;
+  }
+  rc = spsim_bridge_structure_factor_nfft(HKL_list, HKL_list_size, atoms, segs, nsegs, natoms,
+                                          spsim_form_factor_table(), B, 1e-6, res->F, res->ints);
+  if (rc != SPSIM_HIP_OK) {
+    spsim_report("compute_pattern_on_list_by_nfft", rc);
+    goto fail;
+  }
+  free(atoms);
+  return res;
+fail:
+  if (res) {
+    free(res->F);
+    free(res->ints);
+    free(res->HKL_list);
+    free(res);
+  }
+  free(atoms);
+  return NULL;
+}
+
 void spsim_detector_params(const CCD *det, const Experiment *exp, spsim_hip_detector *p) {
   memset(p, 0, sizeof *p);
   p->nx = det->nx;

@@ -19,6 +19,12 @@ static int species_of(int z, int j, int natoms, int delta_atoms, int mode) {
     if (delta_atoms && j < 4 * (natoms / 4)) return SPSIM_HIP_UNIT_SPECIES;
     return z;
   }
+  if (mode == SPSIM_HIP_PACK_NFFT) {
+    /* src/diffraction.c:556-563, 599-624: every element present gets its own
+       transform, Z == 0 included; delta_atoms skips the form factor
+       (:659-661), so every atom scatters with f = 1. */
+    return delta_atoms ? SPSIM_HIP_UNIT_SPECIES : z;
+  }
   /* src/diffraction.c:830-843 */
   if (z == 0) return -1;
   if (delta_atoms) return SPSIM_HIP_UNIT_SPECIES;
@@ -33,7 +39,8 @@ int spsim_hip_pack_atoms(const int *Z, const float *pos, const float *amp, const
   int j, s, n, nseg;
   if (natoms < 0 || !out_atoms || !out_segs || !out_natoms || !out_nsegs) return SPSIM_HIP_ERR_INVALID;
   if (natoms > 0 && (!Z || !pos)) return SPSIM_HIP_ERR_INVALID;
-  if (mode != SPSIM_HIP_PACK_PLAIN && mode != SPSIM_HIP_PACK_VECTOR) return SPSIM_HIP_ERR_INVALID;
+  if (mode != SPSIM_HIP_PACK_PLAIN && mode != SPSIM_HIP_PACK_VECTOR && mode != SPSIM_HIP_PACK_NFFT)
+    return SPSIM_HIP_ERR_INVALID;
   for (s = 0; s < SPSIM_HIP_NSPECIES; ++s) count[s] = 0;
   for (j = 0; j < natoms; ++j) {
     /* Z indexes a 100-row table; anything outside it is a caller error (the
@@ -63,8 +70,9 @@ int spsim_hip_pack_atoms(const int *Z, const float *pos, const float *amp, const
     out_atoms[k].x = pos[3 * j + 0];
     out_atoms[k].y = pos[3 * j + 1];
     out_atoms[k].z = pos[3 * j + 2];
-    /* Delta atoms ignore the illumination weight (src/diffraction.c:840-842). */
-    out_atoms[k].w = (s == SPSIM_HIP_UNIT_SPECIES || !amp) ? 1.0f : amp[j];
+    /* Delta atoms ignore the illumination weight (src/diffraction.c:840-842);
+       the NFFT path has unit strengths (:619-620). */
+    out_atoms[k].w = (s == SPSIM_HIP_UNIT_SPECIES || !amp || mode == SPSIM_HIP_PACK_NFFT) ? 1.0f : amp[j];
     if (out_b) out_b[k] = b ? b[j] : 0.0f;
     if (out_perm) out_perm[k] = j;
   }

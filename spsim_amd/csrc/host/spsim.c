@@ -41,9 +41,15 @@ float *get_HKL_list(Options *opts, int *HKL_list_size) {
 Diffraction_Pattern *compute_sf(Molecule *mol, float *HKL_list, int HKL_list_size, Options *opts) {
   Diffraction_Pattern *pattern = NULL;
   if (!opts) return NULL;
-  if (opts->use_fft_for_sf || opts->use_nfft_for_sf) {
-    fprintf(stderr, "spsim: compute_sf: FFT/NFFT structure factors are not supported\n");
+  if (opts->use_fft_for_sf) {
+    fprintf(stderr, "spsim: compute_sf: the real-space FFT structure factor is not supported\n");
     return NULL;
+  }
+  /* src/spsim.c:119-122 (a build with NFFT_SUPPORT) */
+  if (opts->use_nfft_for_sf) {
+    pattern = compute_pattern_on_list_by_nfft(mol, HKL_list, HKL_list_size, opts->detector, opts->b_factor, opts);
+    if (pattern) pattern->rot = NULL;
+    return pattern;
   }
   /* use_cuda / vectorize select the same GPU kernel; vectorize keeps the
      vector path's semantics (src/spsim.c:127-140). With use_cuda, a non-zero

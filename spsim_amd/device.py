@@ -179,6 +179,41 @@ def structure_factor_spectrum(hkl: torch.Tensor, atoms: DeviceAtoms, ff: torch.T
     return (F if want_F else None), I
 
 
+def structure_factor_nfft(hkl: torch.Tensor, atoms: DeviceAtoms, ff: torch.Tensor, B: float = 0.0,
+                          eps: float = 1e-6, F: torch.Tensor | None = None, I: torch.Tensor | None = None,
+                          stream=None, want_F: bool = True):
+    """One pattern by the type-3 non-uniform FFT (spsim_hip_sf_nfft); atoms
+    packed with PACK_NFFT give the reference's NFFT semantics. Returns
+    (F or None, I, info) with info = {W, M, nf, L} (per-axis sizes as tuples)."""
+    if atoms.b is not None:
+        raise ValueError("the NUFFT path takes no per-atom B factors")
+    npix = int(hkl.shape[0])
+    if want_F and F is None:
+        F = torch.empty((npix, 2), dtype=torch.float32, device=hkl.device)
+    if I is None:
+        I = torch.empty((npix,), dtype=torch.float32, device=hkl.device)
+    info = (C.c_int * 10)()
+    rc = _lib.load().spsim_hip_sf_nfft(_ptr(hkl), npix, _ptr(atoms.atoms), _ptr(atoms.segs), atoms.nsegs,
+                                       atoms.natoms, _ptr(ff), float(B), float(eps), _ptr(F if want_F else None),
+                                       _ptr(I), C.cast(info, C.c_void_p), _stream(stream))
+    check(rc, "spsim_hip_sf_nfft")
+    v = list(info)
+    return (F if want_F else None), I, {"W": v[0], "M": tuple(v[1:4]), "nf": tuple(v[4:7]), "L": tuple(v[7:10])}
+
+
+def fft_pruned(x: torch.Tensor, M: int, m_lo: int, nf: int, L: int, l_lo: int, stream=None) -> torch.Tensor:
+    """Pruned forward FFT along axis 1 of a complex64 device tensor [outer, M, inner]
+    (spsim_hip_fft_pruned); returns [outer, L, inner]."""
+    if x.dtype != torch.complex64 or x.dim() != 3 or x.shape[1] != M:
+        raise ValueError("x must be complex64 [outer, M, inner]")
+    x = x.contiguous()
+    outer, _, inner = x.shape
+    out = torch.empty((outer, L, inner), dtype=torch.complex64, device=x.device)
+    check(_lib.load().spsim_hip_fft_pruned(_ptr(x), outer, inner, M, m_lo, nf, L, l_lo, _ptr(out), _stream(stream)),
+          "spsim_hip_fft_pruned")
+    return out
+
+
 def hkl_detector(nx, ny, width, height, distance, wavelength, center_x=0.0, center_y=0.0, spherical=0, device=None,
                  stream=None) -> torch.Tensor:
     dev = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())

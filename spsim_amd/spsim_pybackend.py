@@ -241,6 +241,8 @@ _SIGS = {
                                                                   _P(Experiment), _P(Options)]),
     "spsim_spectrum_pattern_on_list": (_P(Diffraction_Pattern), [_P(Molecule), _P(C.c_float), C.c_int, C.c_float,
                                                                   _P(Experiment), _P(Options)]),
+    "compute_pattern_on_list_by_nfft": (_P(Diffraction_Pattern), [_P(Molecule), _P(C.c_float), C.c_int, _P(CCD),
+                                                                C.c_float, _P(Options)]),
     "calculate_pixel_solid_angle": (None, [_P(CCD)]),
     "calculate_thomson_correction": (None, [_P(CCD), _P(Experiment)]),
     "calculate_photons_per_pixel": (None, [_P(Diffraction_Pattern), _P(Options)]),

@@ -1,0 +1,209 @@
+"""The non-uniform FFT structure factor on the GPU (spsim_hip_sf_nfft,
+spsim_hip_fft_pruned and the compute_pattern_on_list_by_nfft drop-in) against
+numpy's FFT and the FP64 oracle of the sums the reference's NNFFT path
+approximates (oracle.sf_nfft_f64)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle
+from spsim_amd import _lib
+from spsim_amd import device as dv
+from spsim_amd import spsim
+from spsim_amd import workloads as wl
+from tests.conftest import oracle_threads, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def fft_reference(x, m_lo, nf, L, l_lo):
+    outer, M, inner = x.shape
+    line = np.zeros((outer, nf, inner), np.complex128)
+    line[:, (m_lo + np.arange(M)) % nf, :] = x
+    f = np.fft.fft(line, axis=1)
+    return f[:, (l_lo + np.arange(L)) % nf, :]
+
+
+@pytest.mark.parametrize("nf,M,m_lo,L,l_lo,outer,inner", [
+    (2, 1, 0, 2, 0, 3, 1), (3, 2, -1, 3, -1, 2, 5), (4, 4, -2, 4, 0, 4, 1), (5, 3, -1, 5, -2, 3, 2),
+    (6, 3, -1, 7, -3, 2, 1), (12, 6, -3, 8, -4, 5, 3), (30, 15, -7, 20, -10, 4, 1), (60, 30, -15, 35, -17, 3, 17),
+    (96, 48, -24, 60, -30, 6, 1), (100, 50, -25, 60, -30, 2, 9), (125, 60, -30, 70, -35, 3, 1),
+    (480, 240, -120, 250, -125, 4, 1), (480, 240, -120, 250, -125, 2, 33), (1000, 500, -250, 520, -260, 2, 1),
+    (2048, 1000, -500, 1030, -515, 2, 3), (4096, 2048, -1024, 2060, -1030, 1, 2)])
+def test_fft_pruned_matches_numpy(gpu, nf, M, m_lo, L, l_lo, outer, inner):
+    import torch
+
+    rng = np.random.default_rng(nf * 7 + inner)
+    x = (rng.normal(size=(outer, M, inner)) + 1j * rng.normal(size=(outer, M, inner))).astype(np.complex64)
+    out = dv.fft_pruned(torch.from_numpy(x).to(gpu), M, m_lo, nf, L, l_lo).cpu().numpy()
+    ref = fft_reference(x.astype(np.complex128), m_lo, nf, L, l_lo)
+    assert rel_l2(np.c_[out.real.ravel(), out.imag.ravel()], np.c_[ref.real.ravel(), ref.imag.ravel()]) < 2e-6
+
+
+def test_fft_pruned_rejects_bad_lengths(gpu):
+    import torch
+
+    x = torch.zeros((1, 4, 1), dtype=torch.complex64, device=gpu)
+    for nf in (7, 4, 8192):  # not smooth / shorter than M... / too long
+        M = 4 if nf != 4 else 5
+        xx = x if M == 4 else torch.zeros((1, 5, 1), dtype=torch.complex64, device=gpu)
+        with pytest.raises(RuntimeError):
+            dv.fft_pruned(xx, M, 0, nf, 4, 0)
+
+
+def molecule(n, radius, seed, z0=3):
+    rng = np.random.default_rng(seed)
+    Z, pos, _ = wl.random_molecule(n, radius, rng)
+    Z[:z0] = 0
+    return Z, pos
+
+
+def detector_q(n=40, lam=0.3e-9, pix=1e-4, dist=0.05, cx=0.0, cy=0.0):
+    return oracle.hkl_detector(n, n, n * pix, n * pix, dist, cx, cy, 0, lam)
+
+
+def run_nfft(gpu, q, Z, pos, B=0.0, eps=1e-6, delta=False):
+    import torch
+
+    atoms = dv.DeviceAtoms(dv.pack_atoms(Z, pos, delta_atoms=delta, mode=_lib.PACK_NFFT), gpu)
+    F, I, info = dv.structure_factor_nfft(torch.from_numpy(np.ascontiguousarray(q, np.float32)).to(gpu), atoms,
+                                          dv.form_factor_table(device=gpu), B=B, eps=eps)
+    torch.cuda.synchronize()
+    Fh = F.cpu().numpy()
+    return Fh[:, 0] + 1j * Fh[:, 1], I.cpu().numpy(), info
+
+
+def cplx_err(F, F_ref):
+    return rel_l2(np.c_[F.real, F.imag], np.c_[F_ref.real, F_ref.imag])
+
+
+@pytest.mark.parametrize("B", [0.0, 12.0])
+def test_nfft_vs_exact_detector(gpu, B):
+    Z, pos = molecule(900, 4e-9, seed=1)
+    q = detector_q(48, cx=1e-4, cy=-2e-4)
+    F_ref, I_ref = oracle.sf_nfft_f64(q, Z, pos, B=B, threads=oracle_threads())
+    F, I, info = run_nfft(gpu, q, Z, pos, B=B)
+    assert info["W"] == 7
+    assert rel_l2(I, I_ref) < 1e-5, info
+    assert cplx_err(F, F_ref) < 1e-5
+
+
+def test_nfft_error_follows_eps(gpu):
+    Z, pos = molecule(600, 3e-9, seed=2)
+    q = detector_q(40, lam=0.2e-9)
+    F_ref, _ = oracle.sf_nfft_f64(q, Z, pos, threads=oracle_threads())
+    errs = []
+    for eps in (1e-2, 1e-3, 1e-4, 1e-5):
+        F, _, info = run_nfft(gpu, q, Z, pos, eps=eps)
+        errs.append(cplx_err(F, F_ref))
+        assert errs[-1] < 3 * eps, (eps, info, errs)
+    assert errs[0] > errs[-1] * 10
+
+
+def test_nfft_three_d_volume_and_rotation(gpu):
+    Z, pos = molecule(500, 3e-9, seed=3)
+    q = oracle.hkl_3d(16, 16, 16, 16e-4, 16e-4, 16e-4, 0.05, 0.3e-9)
+    R = wl.uniform_rotations(1, np.random.default_rng(4))[0]
+    q = oracle.rotate_hkl(q, R)
+    F_ref, I_ref = oracle.sf_nfft_f64(q, Z, pos, B=4.0, threads=oracle_threads())
+    F, I, info = run_nfft(gpu, q, Z, pos, B=4.0)
+    assert rel_l2(I, I_ref) < 1e-5, info
+    assert cplx_err(F, F_ref) < 1e-5
+
+
+def test_nfft_delta_atoms(gpu):
+    Z, pos = molecule(400, 3e-9, seed=5, z0=10)
+    q = detector_q(32)
+    F_ref, I_ref = oracle.sf_nfft_f64(q, Z, pos, delta=True, B=20.0, threads=oracle_threads())
+    F, I, _ = run_nfft(gpu, q, Z, pos, B=20.0, delta=True)
+    assert rel_l2(I, I_ref) < 1e-5
+    assert cplx_err(F, F_ref) < 1e-5
+
+
+def test_nfft_degenerate_geometries(gpu):
+    q = detector_q(24)
+    # planar molecule: every atom at z = 0 (zero-width box along z)
+    Z, pos = molecule(300, 3e-9, seed=7, z0=0)
+    pos[:, 2] = 0.0
+    F_ref, I_ref = oracle.sf_nfft_f64(q, Z, pos)
+    _, I, info = run_nfft(gpu, q, Z, pos)
+    assert rel_l2(I, I_ref) < 1e-5, info
+    # one atom: F = f(|q|) exp(-2 pi i q.r)
+    p1 = np.array([[1.5e-9, -0.5e-9, 2e-9]], np.float32)
+    F_ref, _ = oracle.sf_nfft_f64(q, [8], p1, B=3.0)
+    F, _, _ = run_nfft(gpu, q, np.array([8], np.int32), p1, B=3.0)
+    assert cplx_err(F, F_ref) < 1e-5
+    # every q the same point
+    qs = np.repeat(q[100:101], 50, axis=0)
+    F_ref, _ = oracle.sf_nfft_f64(qs, Z, pos)
+    F, _, _ = run_nfft(gpu, qs, Z, pos)
+    assert cplx_err(F, F_ref) < 1e-5
+    # q = 0 only: F = sum of f(0); the error bound is about eps sum |f| (eps = 1e-6)
+    q0 = np.zeros((8, 3), np.float32)
+    F_ref, _ = oracle.sf_nfft_f64(q0, Z, pos)
+    F, _, _ = run_nfft(gpu, q0, Z, pos)
+    assert cplx_err(F, F_ref) < 3e-6
+
+
+def test_nfft_empty_inputs(gpu):
+    import torch
+
+    q = detector_q(8)
+    F, I, _ = run_nfft(gpu, q, np.zeros(0, np.int32), np.zeros((0, 3), np.float32))
+    assert np.all(F == 0) and np.all(I == 0)
+    Z, pos = molecule(20, 1e-9, seed=8)
+    atoms = dv.DeviceAtoms(dv.pack_atoms(Z, pos, mode=_lib.PACK_NFFT), gpu)
+    hkl = torch.zeros((0, 3), dtype=torch.float32, device=gpu)
+    F, I, _ = dv.structure_factor_nfft(hkl, atoms, dv.form_factor_table(device=gpu))
+    assert F.shape == (0, 2) and I.shape == (0,)
+
+
+def test_nfft_rejects_bad_inputs(gpu):
+    import torch
+
+    Z, pos = molecule(50, 2e-9, seed=9)
+    atoms = dv.DeviceAtoms(dv.pack_atoms(Z, pos, mode=_lib.PACK_NFFT), gpu)
+    ff = dv.form_factor_table(device=gpu)
+    q = detector_q(8).copy()
+    q[5, 1] = np.nan
+    with pytest.raises(RuntimeError, match="non-finite"):
+        dv.structure_factor_nfft(torch.from_numpy(q).to(gpu), atoms, ff)
+    # a box x q-range product beyond the longest FFT line
+    big = (detector_q(8) * 1e4).astype(np.float32)
+    with pytest.raises(RuntimeError, match="too large|exceeds"):
+        dv.structure_factor_nfft(torch.from_numpy(big).to(gpu), atoms, ff)
+    with pytest.raises(ValueError):
+        dv.structure_factor_nfft(torch.from_numpy(detector_q(8)).to(gpu),
+                                 dv.DeviceAtoms(dv.pack_atoms(Z, pos, atom_b=np.ones(50)), gpu), ff)
+
+
+def test_compute_sf_takes_the_nfft_path(gpu):
+    """use_nfft_for_sf selects compute_pattern_on_list_by_nfft (src/spsim.c:119-122),
+    which keeps the Z == 0 atoms and ignores the illumination profile."""
+    opts = spsim.set_defaults()
+    d = opts.detector
+    d.nx = d.ny = 32
+    d.nz = 1
+    d.pixel_width = d.pixel_height = 1e-4
+    d.width = d.height = float(np.float32(32 * 1e-4))
+    d.distance = 0.05
+    opts.experiment.wavelength = 0.25e-9
+    opts.use_nfft_for_sf = 1
+    opts.b_factor = 6.0
+    opts.verbosity_level = 0
+    Z, pos = molecule(700, 3.5e-9, seed=10, z0=6)
+    mol = spsim.get_molecule_from_atoms(Z, pos)
+    n = C.c_int(0)
+    ptr = spsim.get_HKL_list_for_detector(opts.detector, opts.experiment, C.byref(n))
+    q = np.ctypeslib.as_array(ptr, shape=(n.value * 3,)).reshape(-1, 3).copy()
+    pat = spsim.compute_sf(mol, ptr, n.value, opts)
+    assert pat
+    I = spsim.float_array(pat.ints, n.value)
+    _, I_ref = oracle.sf_nfft_f64(q, Z, pos, B=6.0)
+    assert rel_l2(I, I_ref) < 1e-5
+    np.testing.assert_array_equal(np.ctypeslib.as_array(pat.HKL_list, shape=(n.value * 3,)), q.ravel())
+    spsim.free_diffraction_pattern(pat)
+    # the direct path skips the Z == 0 atoms, so the two differ
+    _, I_plain = oracle.sf_f64(q, Z, pos, B=6.0)
+    assert rel_l2(I_plain, I_ref) > 1e-3
