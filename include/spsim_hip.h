@@ -178,7 +178,7 @@ int spsim_hip_sf_spectrum(const float *d_hkl, int npix, const spsim_hip_atom *d_
    window, the grids go through a pruned 3-D FFT, and each q interpolates from
    them. eps sets the window width W = ceil(-log10 eps) + 1 in [4, 10] (eps <= 0:
    1e-6); the error of F is about eps times sum_j |w_j| f_s. No per-atom B.
-   The grids scale with (atom box x q range) per axis; lines longer than 4096
+   The grids scale with (atom box x q range) per axis; lines longer than 3840
    points or grids above 64 GiB are rejected. info (host, may be NULL) receives
    {W, Mx, My, Mz, nfx, nfy, nfz, Lx, Ly, Lz}: spread grid, FFT and kept-output
    sizes per axis. Uses the library workspace; synchronises `stream` once (to
@@ -191,7 +191,9 @@ int spsim_hip_sf_nfft(const float *d_hkl, int npix, const spsim_hip_atom *d_atom
    complex input view [outer][M][inner] is placed at points m = m_lo + i
    (i < M) of a zero line of nf points (slot m mod nf), transformed forward
    (exp(-2 pi i l m / nf)), and the values at l = l_lo + i (i < L) are written
-   to the view [outer][L][inner]. nf must be 2,3,5-smooth, in [2, 4096], and >= M. */
+   to the view [outer][L][inner]. nf must be 2,3,5-smooth, in [2, 3840], and >= M;
+   m_lo and l_lo lie in [-nf, nf], m_lo + M and l_lo + L are at most 2 nf. Uses
+   the library workspace (the twiddle table). */
 int spsim_hip_fft_pruned(const float *d_in, long long outer, long long inner, int M, int m_lo, int nf, int L,
                          int l_lo, float *d_out, void *stream);
 

@@ -41,9 +41,12 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kMinW = 4, kMaxW = 10;
 constexpr int kMaxQuad = 2 * kMaxW + 6;
-constexpr int kMaxNf = 4096;      // longest FFT line (LDS: two buffers of kFftLds values)
-constexpr int kFftLds = 4096;     // complex values per LDS buffer per workgroup
+// LDS per FFT workgroup: two buffers of kFftLds complex values (60 KiB) and the
+// line offsets (4 KiB), within 64 KiB.
+constexpr int kMaxNf = 3840;      // longest FFT line
+constexpr int kFftLds = 3840;     // complex values per LDS buffer per workgroup
 constexpr int kMaxStages = 16;
+constexpr int kMaxT = 256;          // lines per FFT workgroup
 constexpr double kSigma = 2.0;     // oversampling of both windows
 constexpr size_t kMaxGridBytes = (size_t)64 << 30;
 
@@ -62,13 +65,26 @@ struct NufftGeom {
   float qc[kMaxQuad];  // 2 * weight * phi(node): phi_hat(xi) = sum qc cos(2 pi xi qz)
 };
 
+// Exact n / d for n, d < 2^16: q = umulhi(n, m) with m = floor(2^32 / d) + 1
+// (the error n (1 - frac(2^32/d)) / 2^32 stays below 1/d); d == 1 is m == 0.
+struct FastDiv {
+  unsigned d, m;
+};
+
+struct FftStage {
+  int R;              // radix
+  int stride;         // nf / (span R): twiddle table step
+  FastDiv nb, span;   // butterflies per line (nf / R), product of the earlier radices
+};
+
 struct FftPass {
   long long nlines;  // lines = outer * inner
-  int inner;         // stride between consecutive points of a line (1: lines are contiguous)
+  long long inner;   // stride between consecutive points of a line (1: lines are contiguous)
   int M, nf, L, m_lo, l_lo;
-  int T;             // lines per workgroup
+  int T;             // lines per workgroup (<= kMaxT)
+  FastDiv dM, dL, dT;
   int nstages;
-  int radix[kMaxStages];
+  FftStage st[kMaxStages];
 };
 
 __device__ __forceinline__ float es_window(float z, float beta) {
@@ -174,13 +190,19 @@ __global__ __launch_bounds__(kThreads) void fill_zero(float4 *__restrict__ p, si
     p[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
-// One thread per packed atom: adds the atom's windowed, phased unit strength
-// to W^3 points of its species' grid [ch][Mz][My][Mx].
-template <int W>
-__global__ __launch_bounds__(kThreads) void nufft_spread(const spsim_hip_atom *__restrict__ atoms, int natoms,
-                                                        const spsim_hip_segment *__restrict__ segs, int nsegs,
-                                                        NufftGeom g, const float *__restrict__ dec,
-                                                        float2 *__restrict__ grid) {
+// Per-atom spreading record: species grid, first grid point per axis, the
+// window offset (first point - u, in grid units) per axis and the phased strength.
+struct __align__(16) SpreadAtom {
+  int ch, bx, by, bz;
+  float fx, fy, fz, pad;
+  float cr, ci, pad2, pad3;
+};
+
+// One thread per packed atom: the record above (ch = -1 marks an atom outside
+// the grid, which the bounds exclude).
+__global__ __launch_bounds__(kThreads) void spread_prep(const spsim_hip_atom *__restrict__ atoms, int natoms,
+                                                       const spsim_hip_segment *__restrict__ segs, int nsegs,
+                                                       NufftGeom g, int W, SpreadAtom *__restrict__ rec) {
   const int j = blockIdx.x * kThreads + threadIdx.x;
   if (j >= natoms) return;
   int ch = 0;
@@ -192,37 +214,56 @@ __global__ __launch_bounds__(kThreads) void nufft_spread(const spsim_hip_atom *_
   ph -= rint(ph);
   float sn, cs;
   sincospif((float)(2.0 * ph), &sn, &cs);
-  const float cr = a.w * cs, ci = a.w * sn;
-
-  float wt[3][W];
-  int base[3];
-  int off = 0;
+  SpreadAtom r;
+  int b[3];
+  float f[3];
   for (int d = 0; d < 3; ++d) {
     const NufftDim &D = g.d[d];
     const double u = xc[d] / D.h;
     const double m0 = ceil(u - 0.5 * W);
     const double i0 = m0 - D.m_lo;
-    if (!(i0 >= 0.0 && i0 + W <= (double)D.M)) return;  // outside the grid: excluded by the bounds
-    base[d] = (int)i0;
-    const float fb = (float)(m0 - u);
-#pragma unroll
-    for (int i = 0; i < W; ++i) wt[d][i] = es_window((fb + (float)i) * (2.0f / W), g.beta) * dec[off + base[d] + i];
-    off += D.M;
+    if (!(i0 >= 0.0 && i0 + W <= (double)D.M)) ch = -1;
+    b[d] = ch < 0 ? 0 : (int)i0;
+    f[d] = (float)(m0 - u);
   }
+  r.ch = ch;
+  r.bx = b[0];
+  r.by = b[1];
+  r.bz = b[2];
+  r.fx = f[0];
+  r.fy = f[1];
+  r.fz = f[2];
+  r.pad = 0.0f;
+  r.cr = a.w * cs;
+  r.ci = a.w * sn;
+  r.pad2 = r.pad3 = 0.0f;
+  rec[j] = r;
+}
+
+// One thread per (atom, grid point of its W^3 window), x fastest, so that the
+// lanes of a wave add into a few contiguous grid rows: the atom's windowed,
+// phased strength goes to its species' grid [ch][Mz][My][Mx].
+template <int W>
+__global__ __launch_bounds__(kThreads) void spread_scatter(const SpreadAtom *__restrict__ rec, long long nwork,
+                                                          NufftGeom g, const float *__restrict__ dec,
+                                                          float2 *__restrict__ grid) {
+  constexpr int W2 = W * W, W3 = W * W * W;
+  const long long t = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (t >= nwork) return;
+  const long long j = t / W3;
+  const int r = (int)(t - j * W3);
+  const int iz = r / W2, iy = (r / W) % W, ix = r % W;
+  const SpreadAtom a = rec[j];
+  if (a.ch < 0) return;
   const int Mx = g.d[0].M, My = g.d[1].M, Mz = g.d[2].M;
-  float2 *gc = grid + (size_t)ch * Mz * My * Mx;
-  for (int iz = 0; iz < W; ++iz) {
-    for (int iy = 0; iy < W; ++iy) {
-      const float wzy = wt[2][iz] * wt[1][iy];
-      const float vr = cr * wzy, vi = ci * wzy;
-      float *row = reinterpret_cast<float *>(gc + ((size_t)(base[2] + iz) * My + base[1] + iy) * Mx + base[0]);
-#pragma unroll
-      for (int ix = 0; ix < W; ++ix) {
-        unsafeAtomicAdd(row + 2 * ix, vr * wt[0][ix]);
-        unsafeAtomicAdd(row + 2 * ix + 1, vi * wt[0][ix]);
-      }
-    }
-  }
+  const float k2 = 2.0f / W;
+  const float w = es_window((a.fx + (float)ix) * k2, g.beta) * dec[a.bx + ix] *
+                  es_window((a.fy + (float)iy) * k2, g.beta) * dec[Mx + a.by + iy] *
+                  es_window((a.fz + (float)iz) * k2, g.beta) * dec[Mx + My + a.bz + iz];
+  float *cell = reinterpret_cast<float *>(
+      grid + (((size_t)a.ch * Mz + a.bz + iz) * My + a.by + iy) * Mx + a.bx + ix);
+  unsafeAtomicAdd(cell, a.cr * w);
+  unsafeAtomicAdd(cell + 1, a.ci * w);
 }
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
@@ -233,44 +274,66 @@ __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(
 // -i * a
 __device__ __forceinline__ float2 mul_mi(float2 a) { return make_float2(a.y, -a.x); }
 
+__device__ __forceinline__ unsigned fdiv(unsigned n, FastDiv f) { return f.m ? __umulhi(n, f.m) : n; }
+
+__device__ __forceinline__ void dft4(float2 &u0, float2 &u1, float2 &u2, float2 &u3) {
+  const float2 a = cadd(u0, u2), b = csub(u0, u2);
+  const float2 c = cadd(u1, u3), d = mul_mi(csub(u1, u3));
+  u0 = cadd(a, c);
+  u2 = csub(a, c);
+  u1 = cadd(b, d);
+  u3 = csub(b, d);
+}
+
 // One Stockham stage of radix R on a line: butterfly i (of nb = n/R) reads
-// x[i + t nb], twiddles by exp(-2 pi i t k / (span R)) with k = i mod span,
-// applies the R-point DFT and writes y[(i - k) R + k + s span].
+// x[i + t nb], twiddles element t by exp(-2 pi i t k / (span R)) =
+// tw[t k stride] with k = i mod span, applies the R-point DFT and writes
+// y[(i - k) R + k + s span].
 template <int R>
-__device__ __forceinline__ void stockham_bfly(const float2 *__restrict__ x, float2 *__restrict__ y, int i, int nb,
-                                              int span) {
-  const int k = i % span;
+__device__ __forceinline__ void stockham_bfly(const float2 *__restrict__ x, float2 *__restrict__ y, int i,
+                                              const FftStage &st, const float2 *__restrict__ tw) {
+  const int nb = (int)st.nb.d, span = (int)st.span.d;
+  const int k = i - (int)fdiv((unsigned)i, st.span) * span;
   float2 u[R];
 #pragma unroll
   for (int t = 0; t < R; ++t) u[t] = x[i + t * nb];
   if (k) {
-    const float a = -2.0f * (float)k / (float)(span * R);
+    const int step = k * st.stride;
 #pragma unroll
-    for (int t = 1; t < R; ++t) {
-      float s, c;
-      sincospif(a * (float)t, &s, &c);
-      u[t] = cmul(u[t], make_float2(c, s));
-    }
+    for (int t = 1; t < R; ++t) u[t] = cmul(u[t], tw[t * step]);
   }
-  float2 v[R];
   if constexpr (R == 2) {
-    v[0] = cadd(u[0], u[1]);
-    v[1] = csub(u[0], u[1]);
+    const float2 a = u[0];
+    u[0] = cadd(a, u[1]);
+    u[1] = csub(a, u[1]);
   } else if constexpr (R == 4) {
-    const float2 a = cadd(u[0], u[2]), b = csub(u[0], u[2]);
-    const float2 c = cadd(u[1], u[3]), d = mul_mi(csub(u[1], u[3]));
-    v[0] = cadd(a, c);
-    v[2] = csub(a, c);
-    v[1] = cadd(b, d);
-    v[3] = csub(b, d);
+    dft4(u[0], u[1], u[2], u[3]);
+  } else if constexpr (R == 8) {
+    constexpr float h = 0.70710678118654752440f;  // cos(pi / 4)
+    float2 e0 = u[0], e1 = u[2], e2 = u[4], e3 = u[6];
+    float2 o0 = u[1], o1 = u[3], o2 = u[5], o3 = u[7];
+    dft4(e0, e1, e2, e3);
+    dft4(o0, o1, o2, o3);
+    // odd half times exp(-2 pi i s / 8), s = 0..3
+    o1 = make_float2(h * (o1.x + o1.y), h * (o1.y - o1.x));
+    o2 = mul_mi(o2);
+    o3 = make_float2(h * (o3.y - o3.x), -h * (o3.x + o3.y));
+    u[0] = cadd(e0, o0);
+    u[4] = csub(e0, o0);
+    u[1] = cadd(e1, o1);
+    u[5] = csub(e1, o1);
+    u[2] = cadd(e2, o2);
+    u[6] = csub(e2, o2);
+    u[3] = cadd(e3, o3);
+    u[7] = csub(e3, o3);
   } else if constexpr (R == 3) {
     constexpr float s3 = 0.86602540378443864676f;  // sin(2 pi / 3)
     const float2 t1 = cadd(u[1], u[2]), t2 = csub(u[1], u[2]);
     const float2 m = make_float2(u[0].x - 0.5f * t1.x, u[0].y - 0.5f * t1.y);
     const float2 r = mul_mi(make_float2(s3 * t2.x, s3 * t2.y));
-    v[0] = cadd(u[0], t1);
-    v[1] = cadd(m, r);
-    v[2] = csub(m, r);
+    u[0] = cadd(u[0], t1);
+    u[1] = cadd(m, r);
+    u[2] = csub(m, r);
   } else {
     static_assert(R == 5, "radix");
     constexpr float c1 = 0.30901699437494742410f;   // cos(2 pi / 5)
@@ -283,85 +346,104 @@ __device__ __forceinline__ void stockham_bfly(const float2 *__restrict__ x, floa
     const float2 m2 = make_float2(u[0].x + c2 * a1.x + c1 * a2.x, u[0].y + c2 * a1.y + c1 * a2.y);
     const float2 r1 = mul_mi(make_float2(s1 * b1.x + s2 * b2.x, s1 * b1.y + s2 * b2.y));
     const float2 r2 = mul_mi(make_float2(s2 * b1.x - s1 * b2.x, s2 * b1.y - s1 * b2.y));
-    v[0] = make_float2(u[0].x + a1.x + a2.x, u[0].y + a1.y + a2.y);
-    v[1] = cadd(m1, r1);
-    v[4] = csub(m1, r1);
-    v[2] = cadd(m2, r2);
-    v[3] = csub(m2, r2);
+    u[0] = make_float2(u[0].x + a1.x + a2.x, u[0].y + a1.y + a2.y);
+    u[1] = cadd(m1, r1);
+    u[4] = csub(m1, r1);
+    u[2] = cadd(m2, r2);
+    u[3] = csub(m2, r2);
   }
   const int j = (i - k) * R + k;
 #pragma unroll
-  for (int s = 0; s < R; ++s) y[j + s * span] = v[s];
+  for (int s2 = 0; s2 < R; ++s2) y[j + s2 * span] = u[s2];
+}
+
+// tw[j] = exp(-2 pi i j / nf), j < nf.
+__global__ __launch_bounds__(kThreads) void twiddle_table(int nf, float2 *__restrict__ tw) {
+  const int j = blockIdx.x * kThreads + threadIdx.x;
+  if (j >= nf) return;
+  double sn, cs;
+  sincospi(-2.0 * (double)j / (double)nf, &sn, &cs);
+  tw[j] = make_float2((float)cs, (float)sn);
 }
 
 // Pruned forward FFT of T lines per workgroup. A line of the input view
 // [outer][M][inner] holds points m = m_lo + i at FFT slot m mod nf (the rest
 // zero); the output view [outer][L][inner] receives FFT values at l = l_lo + i
-// (slot l mod nf).
+// (slot l mod nf). The host keeps m_lo, l_lo in [-nf, nf] and the last
+// point and output below 2 nf, so one conditional add or subtract wraps a slot.
 __global__ __launch_bounds__(kThreads) void fft_lines(FftPass p, const float2 *__restrict__ in,
-                                                     float2 *__restrict__ out) {
+                                                     float2 *__restrict__ out, const float2 *__restrict__ tw) {
   extern __shared__ float2 lds[];
+  __shared__ long long base_in[kMaxT], base_out[kMaxT];
   const int nf = p.nf, T = p.T;
   float2 *x = lds;
   float2 *y = lds + T * nf;
   const long long line0 = (long long)blockIdx.x * T;
+  for (int t = threadIdx.x; t < T; t += kThreads) {
+    const long long line = line0 + t;
+    if (line < p.nlines) {
+      const long long o = line / p.inner;
+      const long long c = line - o * p.inner;
+      base_in[t] = o * p.M * p.inner + c;
+      base_out[t] = o * p.L * p.inner + c;
+    } else {
+      base_in[t] = base_out[t] = -1;
+    }
+  }
   for (int e = threadIdx.x; e < T * nf; e += kThreads) x[e] = make_float2(0.0f, 0.0f);
   __syncthreads();
   for (int e = threadIdx.x; e < T * p.M; e += kThreads) {
     int t, i;
     if (p.inner == 1) {
-      t = e / p.M;
+      t = (int)fdiv((unsigned)e, p.dM);
       i = e - t * p.M;
     } else {
-      i = e / T;
+      i = (int)fdiv((unsigned)e, p.dT);
       t = e - i * T;
     }
-    const long long line = line0 + t;
-    if (line >= p.nlines) continue;
-    const long long o = line / p.inner;
-    const long long c = line - o * p.inner;
-    int slot = (p.m_lo + i) % nf;
-    if (slot < 0) slot += nf;
-    x[t * nf + slot] = in[(o * p.M + i) * p.inner + c];
+    const long long b = base_in[t];
+    if (b < 0) continue;
+    int slot = p.m_lo + i;
+    slot += slot < 0 ? nf : 0;
+    slot -= slot >= nf ? nf : 0;
+    x[t * nf + slot] = in[b + (long long)i * p.inner];
   }
   __syncthreads();
-  int span = 1;
   for (int s = 0; s < p.nstages; ++s) {
-    const int R = p.radix[s];
-    const int nb = nf / R;
+    const FftStage &st = p.st[s];
+    const int nb = (int)st.nb.d;
     for (int b = threadIdx.x; b < T * nb; b += kThreads) {
-      const int t = b / nb, i = b - t * nb;
+      const int t = (int)fdiv((unsigned)b, st.nb), i = b - t * nb;
       const float2 *xs = x + t * nf;
       float2 *ys = y + t * nf;
-      switch (R) {
-        case 4: stockham_bfly<4>(xs, ys, i, nb, span); break;
-        case 2: stockham_bfly<2>(xs, ys, i, nb, span); break;
-        case 3: stockham_bfly<3>(xs, ys, i, nb, span); break;
-        default: stockham_bfly<5>(xs, ys, i, nb, span); break;
+      switch (st.R) {
+        case 8: stockham_bfly<8>(xs, ys, i, st, tw); break;
+        case 4: stockham_bfly<4>(xs, ys, i, st, tw); break;
+        case 2: stockham_bfly<2>(xs, ys, i, st, tw); break;
+        case 3: stockham_bfly<3>(xs, ys, i, st, tw); break;
+        default: stockham_bfly<5>(xs, ys, i, st, tw); break;
       }
     }
     __syncthreads();
     float2 *tmp = x;
     x = y;
     y = tmp;
-    span *= R;
   }
   for (int e = threadIdx.x; e < T * p.L; e += kThreads) {
     int t, i;
     if (p.inner == 1) {
-      t = e / p.L;
+      t = (int)fdiv((unsigned)e, p.dL);
       i = e - t * p.L;
     } else {
-      i = e / T;
+      i = (int)fdiv((unsigned)e, p.dT);
       t = e - i * T;
     }
-    const long long line = line0 + t;
-    if (line >= p.nlines) continue;
-    const long long o = line / p.inner;
-    const long long c = line - o * p.inner;
-    int slot = (p.l_lo + i) % nf;
-    if (slot < 0) slot += nf;
-    out[(o * p.L + i) * p.inner + c] = x[t * nf + slot];
+    const long long b = base_out[t];
+    if (b < 0) continue;
+    int slot = p.l_lo + i;
+    slot += slot < 0 ? nf : 0;
+    slot -= slot >= nf ? nf : 0;
+    out[b + (long long)i * p.inner] = x[t * nf + slot];
   }
 }
 
@@ -484,11 +566,19 @@ int fft_length(int at_least) {
 
 int radices(int n, int *r) {
   int k = 0;
+  while (n % 8 == 0) r[k++] = 8, n /= 8;
   while (n % 4 == 0) r[k++] = 4, n /= 4;
   while (n % 2 == 0) r[k++] = 2, n /= 2;
   while (n % 3 == 0) r[k++] = 3, n /= 3;
   while (n % 5 == 0) r[k++] = 5, n /= 5;
   return k;
+}
+
+FastDiv fast_div(unsigned d) {
+  FastDiv f;
+  f.d = d;
+  f.m = d <= 1 ? 0u : (unsigned)((1ull << 32) / d + 1);
+  return f;
 }
 
 int width_for(double eps) {
@@ -520,7 +610,7 @@ int plan_axis(double xmin, double xmax, double smin, double smax, int W, NufftDi
 FftPass make_pass(const NufftDim &D, long long outer, long long inner) {
   FftPass p{};
   p.nlines = outer * inner;
-  p.inner = (int)inner;
+  p.inner = inner;
   p.M = D.M;
   p.nf = D.nf;
   p.L = D.L;
@@ -528,28 +618,55 @@ FftPass make_pass(const NufftDim &D, long long outer, long long inner) {
   p.l_lo = D.l_lo;
   p.T = kFftLds / D.nf;
   if (p.T < 1) p.T = 1;
-  p.nstages = radices(D.nf, p.radix);
+  if (p.T > kMaxT) p.T = kMaxT;
+  p.dM = fast_div((unsigned)p.M);
+  p.dL = fast_div((unsigned)p.L);
+  p.dT = fast_div((unsigned)p.T);
+  int r[kMaxStages];
+  p.nstages = radices(D.nf, r);
+  int span = 1;
+  for (int s = 0; s < p.nstages; ++s) {
+    p.st[s].R = r[s];
+    p.st[s].stride = D.nf / (span * r[s]);
+    p.st[s].nb = fast_div((unsigned)(D.nf / r[s]));
+    p.st[s].span = fast_div((unsigned)span);
+    span *= r[s];
+  }
   return p;
 }
 
-int launch_fft(const FftPass &p, const float2 *in, float2 *out, hipStream_t stream) {
+// Fills the twiddle table of an nf-point line.
+int launch_twiddles(int nf, float2 *tw, hipStream_t stream) {
+  hipLaunchKernelGGL(twiddle_table, dim3((nf + kThreads - 1) / kThreads), dim3(kThreads), 0, stream, nf, tw);
+  SPSIM_HIP_TRY(hipGetLastError());
+  return SPSIM_HIP_OK;
+}
+
+int launch_fft(const FftPass &p, const float2 *in, float2 *out, const float2 *tw, hipStream_t stream) {
+  // slot wrapping in fft_lines assumes these ranges; every index stays below 2^16
+  if (p.m_lo < -p.nf || p.m_lo + p.M > 2 * p.nf || p.l_lo < -p.nf || p.l_lo + p.L > 2 * p.nf || p.M > p.nf ||
+      p.L >= 65536 || p.nf > kMaxNf)
+    return fail(SPSIM_HIP_ERR_INVALID, "FFT line layout out of range (M %d at %d, L %d at %d, nf %d)", p.M, p.m_lo,
+                p.L, p.l_lo, p.nf);
   const long long blocks = (p.nlines + p.T - 1) / p.T;
   if (blocks <= 0) return SPSIM_HIP_OK;
   if (blocks > 0x7fffffffLL) return fail(SPSIM_HIP_ERR_INVALID, "too many FFT lines");
   const size_t lds = 2 * (size_t)p.T * p.nf * sizeof(float2);
-  hipLaunchKernelGGL(fft_lines, dim3((unsigned)blocks), dim3(kThreads), lds, stream, p, in, out);
+  hipLaunchKernelGGL(fft_lines, dim3((unsigned)blocks), dim3(kThreads), lds, stream, p, in, out, tw);
   SPSIM_HIP_TRY(hipGetLastError());
   return SPSIM_HIP_OK;
 }
 
 template <int W>
-int launch_spread_interp(bool spread, const NufftGeom &g, const spsim_hip_atom *atoms, int natoms,
-                         const spsim_hip_segment *segs, int nsegs, const float *dec, float2 *grid, const float *hkl,
-                         int npix, const float2 *H, const float *ff, float b_global, float2 *F, float *I,
-                         hipStream_t stream) {
+int launch_spread_interp(bool spread, const NufftGeom &g, const SpreadAtom *rec, int natoms, int nsegs,
+                         const spsim_hip_segment *segs, const float *dec, float2 *grid, const float *hkl, int npix,
+                         const float2 *H, const float *ff, float b_global, float2 *F, float *I, hipStream_t stream) {
   if (spread) {
-    hipLaunchKernelGGL(nufft_spread<W>, dim3((natoms + kThreads - 1) / kThreads), dim3(kThreads), 0, stream, atoms,
-                       natoms, segs, nsegs, g, dec, grid);
+    const long long nwork = (long long)natoms * W * W * W;
+    const long long blocks = (nwork + kThreads - 1) / kThreads;
+    if (blocks > 0x7fffffffLL) return fail(SPSIM_HIP_ERR_INVALID, "too many atoms for one spread launch");
+    hipLaunchKernelGGL(spread_scatter<W>, dim3((unsigned)blocks), dim3(kThreads), 0, stream, rec, nwork, g, dec,
+                       grid);
   } else {
     hipLaunchKernelGGL(nufft_interp<W>, dim3((npix + kThreads - 1) / kThreads), dim3(kThreads), 0, stream, hkl, npix,
                        g, H, nsegs, segs, ff, b_global, F, I);
@@ -558,11 +675,11 @@ int launch_spread_interp(bool spread, const NufftGeom &g, const spsim_hip_atom *
   return SPSIM_HIP_OK;
 }
 
-int dispatch_w(int W, bool spread, const NufftGeom &g, const spsim_hip_atom *atoms, int natoms,
+int dispatch_w(int W, bool spread, const NufftGeom &g, const SpreadAtom *rec, int natoms,
                const spsim_hip_segment *segs, int nsegs, const float *dec, float2 *grid, const float *hkl, int npix,
                const float2 *H, const float *ff, float b_global, float2 *F, float *I, hipStream_t s) {
 #define SPSIM_NUFFT_W(w) \
-  case w: return launch_spread_interp<w>(spread, g, atoms, natoms, segs, nsegs, dec, grid, hkl, npix, H, ff, b_global, F, I, s)
+  case w: return launch_spread_interp<w>(spread, g, rec, natoms, nsegs, segs, dec, grid, hkl, npix, H, ff, b_global, F, I, s)
   switch (W) {
     SPSIM_NUFFT_W(4);
     SPSIM_NUFFT_W(5);
@@ -645,12 +762,16 @@ extern "C" int spsim_hip_sf_nfft(const float *d_hkl, int npix, const spsim_hip_a
   const size_t nB = nch * Z.M * Y.L * X.L, nH = nch * Z.L * Y.L * X.L;
   const size_t n1 = nG > nB ? nG : nB, n2 = nA > nH ? nA : nH;
   const size_t b_dec = align256(sizeof(float) * (X.M + Y.M + Z.M));
+  const size_t b_rec = align256(sizeof(SpreadAtom) * (size_t)natoms);
+  const size_t b_tw = align256(sizeof(float2) * (size_t)(X.nf + Y.nf + Z.nf));
   const size_t b1 = align256(sizeof(float2) * n1), b2 = align256(sizeof(float2) * n2);
   if (b1 + b2 > kMaxGridBytes) return fail(SPSIM_HIP_ERR_INVALID, "NUFFT grids need %zu bytes", b1 + b2);
-  SPSIM_TRY(workspace(b_dec + b1 + b2, &ws));
+  SPSIM_TRY(workspace(b_dec + b_rec + b_tw + b1 + b2, &ws));
   float *dec = (float *)ws;
-  float2 *buf1 = (float2 *)((char *)ws + b_dec);
-  float2 *buf2 = (float2 *)((char *)ws + b_dec + b1);
+  SpreadAtom *rec = (SpreadAtom *)((char *)ws + b_dec);
+  float2 *tw = (float2 *)((char *)ws + b_dec + b_rec);
+  float2 *buf1 = (float2 *)((char *)ws + b_dec + b_rec + b_tw);
+  float2 *buf2 = (float2 *)((char *)ws + b_dec + b_rec + b_tw + b1);
 
   // 3. Deconvolution table, zeroed grids, spread.
   const int Mmax = X.M > Y.M ? (X.M > Z.M ? X.M : Z.M) : (Y.M > Z.M ? Y.M : Z.M);
@@ -660,13 +781,20 @@ extern "C" int spsim_hip_sf_nfft(const float *d_hkl, int npix, const spsim_hip_a
   hipLaunchKernelGGL(fill_zero, dim3((unsigned)((n4 + kThreads - 1) / kThreads < 4096 ? (n4 + kThreads - 1) / kThreads : 4096)),
                      dim3(kThreads), 0, s, (float4 *)buf1, n4);
   SPSIM_HIP_TRY(hipGetLastError());
-  SPSIM_TRY(dispatch_w(W, true, g, d_atoms, natoms, d_segs, nsegs, dec, buf1, nullptr, 0, nullptr, d_ff, b_global,
+  hipLaunchKernelGGL(spread_prep, dim3((natoms + kThreads - 1) / kThreads), dim3(kThreads), 0, s, d_atoms, natoms,
+                     d_segs, nsegs, g, W, rec);
+  SPSIM_HIP_TRY(hipGetLastError());
+  SPSIM_TRY(dispatch_w(W, true, g, rec, natoms, d_segs, nsegs, dec, buf1, nullptr, 0, nullptr, d_ff, b_global,
                        nullptr, nullptr, s));
 
   // 4. Pruned FFT along x, y, z: G (buf1) -> A (buf2) -> B (buf1) -> H (buf2).
-  SPSIM_TRY(launch_fft(make_pass(X, (long long)nch * Z.M * Y.M, 1), buf1, buf2, s));
-  SPSIM_TRY(launch_fft(make_pass(Y, (long long)nch * Z.M, X.L), buf2, buf1, s));
-  SPSIM_TRY(launch_fft(make_pass(Z, (long long)nch, (long long)Y.L * X.L), buf1, buf2, s));
+  float2 *twx = tw, *twy = tw + X.nf, *twz = tw + X.nf + Y.nf;
+  SPSIM_TRY(launch_twiddles(X.nf, twx, s));
+  SPSIM_TRY(launch_twiddles(Y.nf, twy, s));
+  SPSIM_TRY(launch_twiddles(Z.nf, twz, s));
+  SPSIM_TRY(launch_fft(make_pass(X, (long long)nch * Z.M * Y.M, 1), buf1, buf2, twx, s));
+  SPSIM_TRY(launch_fft(make_pass(Y, (long long)nch * Z.M, X.L), buf2, buf1, twy, s));
+  SPSIM_TRY(launch_fft(make_pass(Z, (long long)nch, (long long)Y.L * X.L), buf1, buf2, twz, s));
 
   // 5. Interpolate at every q.
   SPSIM_TRY(dispatch_w(W, false, g, nullptr, 0, d_segs, nsegs, nullptr, nullptr, d_hkl, npix, buf2, d_ff, b_global,
@@ -681,7 +809,8 @@ extern "C" int spsim_hip_fft_pruned(const float *d_in, long long outer, long lon
     return fail(SPSIM_HIP_ERR_INVALID, "bad FFT shape or NULL pointer");
   if (nf < 2 || nf > kMaxNf || !smooth_length(nf) || M > nf)
     return fail(SPSIM_HIP_ERR_INVALID, "FFT length %d must be a 2,3,5-smooth number in [2, %d] and >= M", nf, kMaxNf);
-  if (inner > 0x7fffffffLL) return fail(SPSIM_HIP_ERR_INVALID, "inner stride too large");
+  if (m_lo < -nf || m_lo + M > 2 * nf || l_lo < -nf || l_lo + L > 2 * nf)
+    return fail(SPSIM_HIP_ERR_INVALID, "first point or output outside [-nf, nf], or last one beyond 2 nf");
   SPSIM_TRY(ensure_device(nullptr));
   NufftDim D{};
   D.M = M;
@@ -689,5 +818,9 @@ extern "C" int spsim_hip_fft_pruned(const float *d_in, long long outer, long lon
   D.nf = nf;
   D.L = L;
   D.l_lo = l_lo;
-  return launch_fft(make_pass(D, outer, inner), (const float2 *)d_in, (float2 *)d_out, (hipStream_t)stream);
+  void *ws = nullptr;
+  SPSIM_TRY(workspace(sizeof(float2) * (size_t)nf, &ws));
+  SPSIM_TRY(launch_twiddles(nf, (float2 *)ws, (hipStream_t)stream));
+  return launch_fft(make_pass(D, outer, inner), (const float2 *)d_in, (float2 *)d_out, (const float2 *)ws,
+                    (hipStream_t)stream);
 }

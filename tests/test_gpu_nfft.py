@@ -30,7 +30,8 @@ def fft_reference(x, m_lo, nf, L, l_lo):
     (6, 3, -1, 7, -3, 2, 1), (12, 6, -3, 8, -4, 5, 3), (30, 15, -7, 20, -10, 4, 1), (60, 30, -15, 35, -17, 3, 17),
     (96, 48, -24, 60, -30, 6, 1), (100, 50, -25, 60, -30, 2, 9), (125, 60, -30, 70, -35, 3, 1),
     (480, 240, -120, 250, -125, 4, 1), (480, 240, -120, 250, -125, 2, 33), (1000, 500, -250, 520, -260, 2, 1),
-    (2048, 1000, -500, 1030, -515, 2, 3), (4096, 2048, -1024, 2060, -1030, 1, 2)])
+    (2048, 1000, -500, 1030, -515, 2, 3), (3840, 1900, -950, 1930, -965, 1, 2),
+    (64, 10, 60, 12, 110, 3, 2)])
 def test_fft_pruned_matches_numpy(gpu, nf, M, m_lo, L, l_lo, outer, inner):
     import torch
 
@@ -45,11 +46,16 @@ def test_fft_pruned_rejects_bad_lengths(gpu):
     import torch
 
     x = torch.zeros((1, 4, 1), dtype=torch.complex64, device=gpu)
-    for nf in (7, 4, 8192):  # not smooth / shorter than M... / too long
+    for nf in (7, 4, 4096):  # not smooth / shorter than M / longer than 3840
         M = 4 if nf != 4 else 5
         xx = x if M == 4 else torch.zeros((1, 5, 1), dtype=torch.complex64, device=gpu)
         with pytest.raises(RuntimeError):
             dv.fft_pruned(xx, M, 0, nf, 4, 0)
+    # points or outputs beyond the wrapped range [-nf, 2 nf)
+    with pytest.raises(RuntimeError, match="outside"):
+        dv.fft_pruned(x, 4, -100, 64, 4, 0)
+    with pytest.raises(RuntimeError, match="outside"):
+        dv.fft_pruned(x, 4, 0, 64, 4, 126)
 
 
 def molecule(n, radius, seed, z0=3):
