@@ -41,10 +41,10 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kMinW = 4, kMaxW = 10;
 constexpr int kMaxQuad = 2 * kMaxW + 6;
-// LDS per FFT workgroup: two buffers of kFftLds complex values (60 KiB) and the
-// line offsets (4 KiB), within 64 KiB.
+// LDS per FFT workgroup: two buffers of T lines (T nf <= kFftLds complex values,
+// 30 KiB, or one line of up to kMaxNf points, 60 KiB) and the line offsets (4 KiB).
 constexpr int kMaxNf = 3840;      // longest FFT line
-constexpr int kFftLds = 3840;     // complex values per LDS buffer per workgroup
+constexpr int kFftLds = 1920;     // complex values per LDS buffer for several lines
 constexpr int kMaxStages = 16;
 constexpr int kMaxT = 256;          // lines per FFT workgroup
 constexpr double kSigma = 2.0;     // oversampling of both windows
@@ -240,30 +240,181 @@ __global__ __launch_bounds__(kThreads) void spread_prep(const spsim_hip_atom *__
   rec[j] = r;
 }
 
-// One thread per (atom, grid point of its W^3 window), x fastest, so that the
-// lanes of a wave add into a few contiguous grid rows: the atom's windowed,
-// phased strength goes to its species' grid [ch][Mz][My][Mx].
+// Atoms are binned by the kSpreadBlock^3 grid block that holds the first point
+// of their window (their home block). One 64-lane workgroup per output block
+// then gathers the atoms of the home blocks whose windows can reach it; each
+// lane owns one (x, y) column of the block and keeps its kSpreadBlock cells
+// in registers, so the species grids need no zeroing and no atomics.
+constexpr int kSpreadBlock = 8;
+static_assert(kSpreadBlock * kSpreadBlock == 64, "one lane per (x, y) column of a spread block");
+
+struct SpreadBins {
+  int nbx, nby, nbz;  // spread blocks per axis
+  long long nbins;    // nch * nbz * nby * nbx
+};
+
+__device__ __forceinline__ long long home_bin(const SpreadAtom &r, const SpreadBins &b) {
+  return (((long long)r.ch * b.nbz + r.bz / kSpreadBlock) * b.nby + r.by / kSpreadBlock) * b.nbx + r.bx / kSpreadBlock;
+}
+
+// counts[home bin] += 1 for every atom inside the grid.
+__global__ __launch_bounds__(kThreads) void bin_count(const SpreadAtom *__restrict__ rec, int natoms, SpreadBins b,
+                                                     int *__restrict__ counts) {
+  const int j = blockIdx.x * kThreads + threadIdx.x;
+  if (j >= natoms) return;
+  const SpreadAtom r = rec[j];
+  if (r.ch < 0) return;
+  atomicAdd(counts + home_bin(r, b), 1);
+}
+
+// One workgroup: out[i] = sum_{k < i} in[k] over n entries, in chunks of
+// 4 * kThreads with a running carry.
+__global__ __launch_bounds__(kThreads) void scan_exclusive(const int *__restrict__ in, long long n,
+                                                          int *__restrict__ out) {
+  __shared__ int part[kThreads];
+  __shared__ int carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (long long base = 0; base < n; base += 4 * kThreads) {
+    int v[4], sum = 0;
+    for (int k = 0; k < 4; ++k) {
+      const long long i = base + 4 * threadIdx.x + k;
+      v[k] = i < n ? in[i] : 0;
+      sum += v[k];
+    }
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    for (int off = 1; off < kThreads; off <<= 1) {
+      const int t = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0;
+      __syncthreads();
+      part[threadIdx.x] += t;
+      __syncthreads();
+    }
+    int run = carry + (threadIdx.x ? part[threadIdx.x - 1] : 0);
+    for (int k = 0; k < 4; ++k) {
+      const long long i = base + 4 * threadIdx.x + k;
+      if (i < n) out[i] = run;
+      run += v[k];
+    }
+    __syncthreads();
+    if (threadIdx.x == kThreads - 1) carry += part[kThreads - 1];
+    __syncthreads();
+  }
+}
+
+// sorted[start[bin] + k] = the bin's atoms (k from a per-bin cursor).
+__global__ __launch_bounds__(kThreads) void bin_scatter(const SpreadAtom *__restrict__ rec, int natoms, SpreadBins b,
+                                                       const int *__restrict__ start, int *__restrict__ cursor,
+                                                       SpreadAtom *__restrict__ sorted) {
+  const int j = blockIdx.x * kThreads + threadIdx.x;
+  if (j >= natoms) return;
+  const SpreadAtom r = rec[j];
+  if (r.ch < 0) return;
+  const long long h = home_bin(r, b);
+  sorted[start[h] + atomicAdd(cursor + h, 1)] = r;
+}
+
+// A staged candidate: its window weights (times the deconvolution factor) at
+// the block's kSpreadBlock points per axis (zero outside the window) and its
+// phased strength.
+struct __align__(16) SpreadCand {
+  float wx[kSpreadBlock], wy[kSpreadBlock], wz[kSpreadBlock];
+  float cr, ci, pad0, pad1;
+};
+
+// One 64-lane workgroup per output block (channel, kz, ky, kx). The candidate
+// atoms (home blocks up to NB = ceil((W - 1) / kSpreadBlock) behind per axis)
+// are staged 64 at a time: each lane loads one, keeps it if its window reaches
+// into the block and writes its window weights there. Then every lane adds
+// every kept atom into its column (x0 + lane % 8, y0 + lane / 8) for the
+// block's 8 z-planes, in registers.
 template <int W>
-__global__ __launch_bounds__(kThreads) void spread_scatter(const SpreadAtom *__restrict__ rec, long long nwork,
-                                                          NufftGeom g, const float *__restrict__ dec,
-                                                          float2 *__restrict__ grid) {
-  constexpr int W2 = W * W, W3 = W * W * W;
-  const long long t = (long long)blockIdx.x * kThreads + threadIdx.x;
-  if (t >= nwork) return;
-  const long long j = t / W3;
-  const int r = (int)(t - j * W3);
-  const int iz = r / W2, iy = (r / W) % W, ix = r % W;
-  const SpreadAtom a = rec[j];
-  if (a.ch < 0) return;
+__global__ __launch_bounds__(64) void spread_gather(const SpreadAtom *__restrict__ sorted,
+                                                   const int *__restrict__ start, const int *__restrict__ counts,
+                                                   SpreadBins b, NufftGeom g, const float *__restrict__ dec,
+                                                   float2 *__restrict__ grid) {
+  constexpr int kB = kSpreadBlock;
+  constexpr int NB = (W + kB - 2) / kB;
+  constexpr int NH = (NB + 1) * (NB + 1) * (NB + 1);
+  __shared__ SpreadCand cand[64];
+  __shared__ int hs[NH], hp[NH + 1];
+  __shared__ int ncand;
+  long long blk = blockIdx.x;
+  const int kx = (int)(blk % b.nbx);
+  blk /= b.nbx;
+  const int ky = (int)(blk % b.nby);
+  blk /= b.nby;
+  const int kz = (int)(blk % b.nbz);
+  const int ch = (int)(blk / b.nbz);
   const int Mx = g.d[0].M, My = g.d[1].M, Mz = g.d[2].M;
+  const int x0 = kx * kB, y0 = ky * kB, z0 = kz * kB;
+  const int lane = threadIdx.x, tx = lane % kB, ty = lane / kB;
   const float k2 = 2.0f / W;
-  const float w = es_window((a.fx + (float)ix) * k2, g.beta) * dec[a.bx + ix] *
-                  es_window((a.fy + (float)iy) * k2, g.beta) * dec[Mx + a.by + iy] *
-                  es_window((a.fz + (float)iz) * k2, g.beta) * dec[Mx + My + a.bz + iz];
-  float *cell = reinterpret_cast<float *>(
-      grid + (((size_t)a.ch * Mz + a.bz + iz) * My + a.by + iy) * Mx + a.bx + ix);
-  unsafeAtomicAdd(cell, a.cr * w);
-  unsafeAtomicAdd(cell + 1, a.ci * w);
+  for (int i = lane; i < NH; i += 64) {
+    const int dx = i % (NB + 1), dy = (i / (NB + 1)) % (NB + 1), dz = i / ((NB + 1) * (NB + 1));
+    int st = 0, n = 0;
+    if (kx >= dx && ky >= dy && kz >= dz) {
+      const long long h = (((long long)ch * b.nbz + kz - dz) * b.nby + ky - dy) * b.nbx + kx - dx;
+      st = start[h];
+      n = counts[h];
+    }
+    hs[i] = st;
+    hp[i + 1] = n;
+  }
+  __syncthreads();
+  if (lane == 0) {
+    hp[0] = 0;
+    for (int i = 0; i < NH; ++i) hp[i + 1] += hp[i];
+  }
+  __syncthreads();
+  const int total = hp[NH];
+  float ar[kB], ai[kB];
+#pragma unroll
+  for (int z = 0; z < kB; ++z) ar[z] = ai[z] = 0.0f;
+  for (int base = 0; base < total; base += 64) {
+    if (lane == 0) ncand = 0;
+    __syncthreads();
+    const int v = base + lane;
+    if (v < total) {
+      int bi = 0;
+      while (v >= hp[bi + 1]) ++bi;
+      const SpreadAtom r = sorted[hs[bi] + v - hp[bi]];
+      if (r.bx + W > x0 && r.bx < x0 + kB && r.by + W > y0 && r.by < y0 + kB && r.bz + W > z0 && r.bz < z0 + kB) {
+        SpreadCand &c = cand[atomicAdd(&ncand, 1)];
+#pragma unroll
+        for (int i = 0; i < kB; ++i) {
+          const int ix = x0 + i - r.bx, iy = y0 + i - r.by, iz = z0 + i - r.bz;
+          // table indices clamped into the grid: points past its end have zero weight
+          const float dx = dec[min(x0 + i, Mx - 1)], dy = dec[Mx + min(y0 + i, My - 1)];
+          const float dz = dec[Mx + My + min(z0 + i, Mz - 1)];
+          c.wx[i] = (ix >= 0 && ix < W) ? es_window((r.fx + (float)ix) * k2, g.beta) * dx : 0.0f;
+          c.wy[i] = (iy >= 0 && iy < W) ? es_window((r.fy + (float)iy) * k2, g.beta) * dy : 0.0f;
+          c.wz[i] = (iz >= 0 && iz < W) ? es_window((r.fz + (float)iz) * k2, g.beta) * dz : 0.0f;
+        }
+        c.cr = r.cr;
+        c.ci = r.ci;
+      }
+    }
+    __syncthreads();
+    const int nc = ncand;
+    for (int a = 0; a < nc; ++a) {
+      const float w = cand[a].wx[tx] * cand[a].wy[ty];
+      const float vr = cand[a].cr * w, vi = cand[a].ci * w;
+#pragma unroll
+      for (int z = 0; z < kB; ++z) {
+        const float wz = cand[a].wz[z];
+        ar[z] += vr * wz;
+        ai[z] += vi * wz;
+      }
+    }
+    __syncthreads();
+  }
+  const int gx = x0 + tx, gy = y0 + ty;
+  if (gx < Mx && gy < My) {
+#pragma unroll
+    for (int z = 0; z < kB; ++z)
+      if (z0 + z < Mz) grid[(((size_t)ch * Mz + z0 + z) * My + gy) * Mx + gx] = make_float2(ar[z], ai[z]);
+  }
 }
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
@@ -658,15 +809,14 @@ int launch_fft(const FftPass &p, const float2 *in, float2 *out, const float2 *tw
 }
 
 template <int W>
-int launch_spread_interp(bool spread, const NufftGeom &g, const SpreadAtom *rec, int natoms, int nsegs,
-                         const spsim_hip_segment *segs, const float *dec, float2 *grid, const float *hkl, int npix,
-                         const float2 *H, const float *ff, float b_global, float2 *F, float *I, hipStream_t stream) {
+int launch_spread_interp(bool spread, const NufftGeom &g, const SpreadAtom *rec, const SpreadBins &bins,
+                         const int *start, const int *counts, int nsegs, const spsim_hip_segment *segs,
+                         const float *dec, float2 *grid, const float *hkl, int npix, const float2 *H, const float *ff,
+                         float b_global, float2 *F, float *I, hipStream_t stream) {
   if (spread) {
-    const long long nwork = (long long)natoms * W * W * W;
-    const long long blocks = (nwork + kThreads - 1) / kThreads;
-    if (blocks > 0x7fffffffLL) return fail(SPSIM_HIP_ERR_INVALID, "too many atoms for one spread launch");
-    hipLaunchKernelGGL(spread_scatter<W>, dim3((unsigned)blocks), dim3(kThreads), 0, stream, rec, nwork, g, dec,
-                       grid);
+    if (bins.nbins > 0x7fffffffLL) return fail(SPSIM_HIP_ERR_INVALID, "too many spread blocks");
+    hipLaunchKernelGGL(spread_gather<W>, dim3((unsigned)bins.nbins), dim3(64), 0, stream, rec, start, counts, bins, g,
+                       dec, grid);
   } else {
     hipLaunchKernelGGL(nufft_interp<W>, dim3((npix + kThreads - 1) / kThreads), dim3(kThreads), 0, stream, hkl, npix,
                        g, H, nsegs, segs, ff, b_global, F, I);
@@ -675,11 +825,14 @@ int launch_spread_interp(bool spread, const NufftGeom &g, const SpreadAtom *rec,
   return SPSIM_HIP_OK;
 }
 
-int dispatch_w(int W, bool spread, const NufftGeom &g, const SpreadAtom *rec, int natoms,
-               const spsim_hip_segment *segs, int nsegs, const float *dec, float2 *grid, const float *hkl, int npix,
-               const float2 *H, const float *ff, float b_global, float2 *F, float *I, hipStream_t s) {
+int dispatch_w(int W, bool spread, const NufftGeom &g, const SpreadAtom *rec, const SpreadBins &bins,
+               const int *start, const int *counts, const spsim_hip_segment *segs, int nsegs, const float *dec,
+               float2 *grid, const float *hkl, int npix, const float2 *H, const float *ff, float b_global, float2 *F,
+               float *I, hipStream_t s) {
 #define SPSIM_NUFFT_W(w) \
-  case w: return launch_spread_interp<w>(spread, g, rec, natoms, nsegs, segs, dec, grid, hkl, npix, H, ff, b_global, F, I, s)
+  case w:                                                                                                          \
+    return launch_spread_interp<w>(spread, g, rec, bins, start, counts, nsegs, segs, dec, grid, hkl, npix, H, ff, \
+                                   b_global, F, I, s)
   switch (W) {
     SPSIM_NUFFT_W(4);
     SPSIM_NUFFT_W(5);
@@ -762,30 +915,52 @@ extern "C" int spsim_hip_sf_nfft(const float *d_hkl, int npix, const spsim_hip_a
   const size_t nB = nch * Z.M * Y.L * X.L, nH = nch * Z.L * Y.L * X.L;
   const size_t n1 = nG > nB ? nG : nB, n2 = nA > nH ? nA : nH;
   const size_t b_dec = align256(sizeof(float) * (X.M + Y.M + Z.M));
+  SpreadBins bins;
+  bins.nbx = (X.M + kSpreadBlock - 1) / kSpreadBlock;
+  bins.nby = (Y.M + kSpreadBlock - 1) / kSpreadBlock;
+  bins.nbz = (Z.M + kSpreadBlock - 1) / kSpreadBlock;
+  bins.nbins = (long long)nch * bins.nbz * bins.nby * bins.nbx;
   const size_t b_rec = align256(sizeof(SpreadAtom) * (size_t)natoms);
+  const size_t b_bin = align256(sizeof(int) * (size_t)bins.nbins);
   const size_t b_tw = align256(sizeof(float2) * (size_t)(X.nf + Y.nf + Z.nf));
   const size_t b1 = align256(sizeof(float2) * n1), b2 = align256(sizeof(float2) * n2);
   if (b1 + b2 > kMaxGridBytes) return fail(SPSIM_HIP_ERR_INVALID, "NUFFT grids need %zu bytes", b1 + b2);
-  SPSIM_TRY(workspace(b_dec + b_rec + b_tw + b1 + b2, &ws));
-  float *dec = (float *)ws;
-  SpreadAtom *rec = (SpreadAtom *)((char *)ws + b_dec);
-  float2 *tw = (float2 *)((char *)ws + b_dec + b_rec);
-  float2 *buf1 = (float2 *)((char *)ws + b_dec + b_rec + b_tw);
-  float2 *buf2 = (float2 *)((char *)ws + b_dec + b_rec + b_tw + b1);
+  SPSIM_TRY(workspace(b_dec + 2 * b_rec + 3 * b_bin + b_tw + b1 + b2, &ws));
+  char *wp = (char *)ws;
+  float *dec = (float *)wp;
+  wp += b_dec;
+  SpreadAtom *rec = (SpreadAtom *)wp;
+  wp += b_rec;
+  SpreadAtom *sorted = (SpreadAtom *)wp;
+  wp += b_rec;
+  int *counts = (int *)wp;  // counts and cursor are adjacent, cleared together
+  int *cursor = (int *)(wp + b_bin);
+  int *start = (int *)(wp + 2 * b_bin);
+  wp += 3 * b_bin;
+  float2 *tw = (float2 *)wp;
+  wp += b_tw;
+  float2 *buf1 = (float2 *)wp;
+  float2 *buf2 = (float2 *)(wp + b1);
 
-  // 3. Deconvolution table, zeroed grids, spread.
+  // 3. Deconvolution table, atom records binned by spread block, spread.
   const int Mmax = X.M > Y.M ? (X.M > Z.M ? X.M : Z.M) : (Y.M > Z.M ? Y.M : Z.M);
   hipLaunchKernelGGL(deconv_table, dim3((Mmax + kThreads - 1) / kThreads), dim3(kThreads), 0, s, g, W, dec);
   SPSIM_HIP_TRY(hipGetLastError());
-  const size_t n4 = (sizeof(float2) * nG + 15) / 16;
+  const size_t n4 = 2 * b_bin / 16;
   hipLaunchKernelGGL(fill_zero, dim3((unsigned)((n4 + kThreads - 1) / kThreads < 4096 ? (n4 + kThreads - 1) / kThreads : 4096)),
-                     dim3(kThreads), 0, s, (float4 *)buf1, n4);
+                     dim3(kThreads), 0, s, (float4 *)counts, n4);
   SPSIM_HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(spread_prep, dim3((natoms + kThreads - 1) / kThreads), dim3(kThreads), 0, s, d_atoms, natoms,
-                     d_segs, nsegs, g, W, rec);
+  const unsigned ablocks = (unsigned)((natoms + kThreads - 1) / kThreads);
+  hipLaunchKernelGGL(spread_prep, dim3(ablocks), dim3(kThreads), 0, s, d_atoms, natoms, d_segs, nsegs, g, W, rec);
   SPSIM_HIP_TRY(hipGetLastError());
-  SPSIM_TRY(dispatch_w(W, true, g, rec, natoms, d_segs, nsegs, dec, buf1, nullptr, 0, nullptr, d_ff, b_global,
-                       nullptr, nullptr, s));
+  hipLaunchKernelGGL(bin_count, dim3(ablocks), dim3(kThreads), 0, s, rec, natoms, bins, counts);
+  SPSIM_HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(scan_exclusive, dim3(1), dim3(kThreads), 0, s, counts, bins.nbins, start);
+  SPSIM_HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(bin_scatter, dim3(ablocks), dim3(kThreads), 0, s, rec, natoms, bins, start, cursor, sorted);
+  SPSIM_HIP_TRY(hipGetLastError());
+  SPSIM_TRY(dispatch_w(W, true, g, sorted, bins, start, counts, d_segs, nsegs, dec, buf1, nullptr, 0, nullptr, d_ff,
+                       b_global, nullptr, nullptr, s));
 
   // 4. Pruned FFT along x, y, z: G (buf1) -> A (buf2) -> B (buf1) -> H (buf2).
   float2 *twx = tw, *twy = tw + X.nf, *twz = tw + X.nf + Y.nf;
@@ -797,8 +972,8 @@ extern "C" int spsim_hip_sf_nfft(const float *d_hkl, int npix, const spsim_hip_a
   SPSIM_TRY(launch_fft(make_pass(Z, (long long)nch, (long long)Y.L * X.L), buf1, buf2, twz, s));
 
   // 5. Interpolate at every q.
-  SPSIM_TRY(dispatch_w(W, false, g, nullptr, 0, d_segs, nsegs, nullptr, nullptr, d_hkl, npix, buf2, d_ff, b_global,
-                       (float2 *)d_F, d_I, s));
+  SPSIM_TRY(dispatch_w(W, false, g, nullptr, bins, nullptr, nullptr, d_segs, nsegs, nullptr, nullptr, d_hkl, npix,
+                       buf2, d_ff, b_global, (float2 *)d_F, d_I, s));
   return SPSIM_HIP_OK;
 }
 
