@@ -54,6 +54,8 @@ def parse():
     p.add_argument("--no-adjoint", action="store_true", help="skip the differentiation measurement")
     p.add_argument("--no-spectrum", action="store_true", help="skip the wavelength-spectrum measurement")
     p.add_argument("--no-nfft", action="store_true", help="skip the non-uniform FFT measurement")
+    p.add_argument("--nfft-only", action="store_true",
+                   help="only the non-uniform FFT section on the large-single-pattern geometry (for profiling)")
     p.add_argument("--profile", action="store_true", help="headline workload only (for rocprofv3)")
     p.add_argument("--kernel", choices=("auto", "pattern", "grid"), default="auto",
                    help="restrict the headline sweep to one structure-factor kernel")
@@ -525,6 +527,13 @@ def main():
     from spsim_amd import workloads as wl
 
     _lib.load()
+    if args.nfft_only:
+        det, Z, pos, _ = wl.config2()
+        dev = torch.device("cuda", 0)
+        q = dv.hkl_detector(det.n, det.n, det.width, det.width, det.distance, det.wavelength, device=dev)
+        log("nfft only: non-uniform FFT vs direct sum")
+        print(json.dumps({"nfft": nfft(args, torch, dv, wl, _lib, {"Z": Z, "pos": pos, "hkl": q, "npix": det.npix})}))
+        return
     if args.profile:
         # A plain copy of known size, for scaling the HBM byte counters
         # (FETCH_SIZE / WRITE_SIZE) read in the same profiled run.

@@ -42,7 +42,8 @@ constexpr int kThreads = 256;
 constexpr int kMinW = 4, kMaxW = 10;
 constexpr int kMaxQuad = 2 * kMaxW + 6;
 // LDS per FFT workgroup: two buffers of T lines (T nf <= kFftLds complex values,
-// 30 KiB, or one line of up to kMaxNf points, 60 KiB) and the line offsets (4 KiB).
+// 30 KiB) and the twiddle table (up to 15 KiB), or one line of up to kMaxNf
+// points (60 KiB, twiddles read from global memory); plus the line offsets (4 KiB).
 constexpr int kMaxNf = 3840;      // longest FFT line
 constexpr int kFftLds = 1920;     // complex values per LDS buffer for several lines
 constexpr int kMaxStages = 16;
@@ -522,13 +523,20 @@ __global__ __launch_bounds__(kThreads) void twiddle_table(int nf, float2 *__rest
 // zero); the output view [outer][L][inner] receives FFT values at l = l_lo + i
 // (slot l mod nf). The host keeps m_lo, l_lo in [-nf, nf] and the last
 // point and output below 2 nf, so one conditional add or subtract wraps a slot.
+// TWL: the twiddle table is copied into LDS after the two line buffers
+// (lines up to kFftLds points); otherwise it is read from global memory.
+template <bool TWL>
 __global__ __launch_bounds__(kThreads) void fft_lines(FftPass p, const float2 *__restrict__ in,
-                                                     float2 *__restrict__ out, const float2 *__restrict__ tw) {
+                                                     float2 *__restrict__ out, const float2 *__restrict__ tw_g) {
   extern __shared__ float2 lds[];
   __shared__ long long base_in[kMaxT], base_out[kMaxT];
   const int nf = p.nf, T = p.T;
   float2 *x = lds;
   float2 *y = lds + T * nf;
+  float2 *tw_l = lds + 2 * T * nf;
+  if constexpr (TWL)
+    for (int e = threadIdx.x; e < nf; e += kThreads) tw_l[e] = tw_g[e];
+  const float2 *tw = TWL ? (const float2 *)tw_l : tw_g;
   const long long line0 = (long long)blockIdx.x * T;
   for (int t = threadIdx.x; t < T; t += kThreads) {
     const long long line = line0 + t;
@@ -802,8 +810,13 @@ int launch_fft(const FftPass &p, const float2 *in, float2 *out, const float2 *tw
   const long long blocks = (p.nlines + p.T - 1) / p.T;
   if (blocks <= 0) return SPSIM_HIP_OK;
   if (blocks > 0x7fffffffLL) return fail(SPSIM_HIP_ERR_INVALID, "too many FFT lines");
-  const size_t lds = 2 * (size_t)p.T * p.nf * sizeof(float2);
-  hipLaunchKernelGGL(fft_lines, dim3((unsigned)blocks), dim3(kThreads), lds, stream, p, in, out, tw);
+  if (p.nf <= kFftLds) {
+    const size_t lds = (2 * (size_t)p.T + 1) * p.nf * sizeof(float2);
+    hipLaunchKernelGGL(fft_lines<true>, dim3((unsigned)blocks), dim3(kThreads), lds, stream, p, in, out, tw);
+  } else {
+    const size_t lds = 2 * (size_t)p.T * p.nf * sizeof(float2);
+    hipLaunchKernelGGL(fft_lines<false>, dim3((unsigned)blocks), dim3(kThreads), lds, stream, p, in, out, tw);
+  }
   SPSIM_HIP_TRY(hipGetLastError());
   return SPSIM_HIP_OK;
 }
