@@ -314,9 +314,10 @@ int spsim_hip_batch_terms(const spsim_hip_batch *b, double *terms) {
 }
 
 // Times each supported tile of the plan's kernel on a representative slice of
-// the work (patterns spread over the cost range, the first tiles of the
-// detector) and keeps the fastest in pixel-atom terms per second. Leaves the
-// plan on the best tile.
+// the work (up to 24 patterns spread over the cost range, all of their tiles:
+// tens of milliseconds per candidate at large plans, so that coarse tiles are
+// not judged on a launch dominated by its tail) and keeps the fastest in
+// pixel-atom terms per second. Leaves the plan on the best tile.
 int spsim_hip_batch_autotune(spsim_hip_batch *b, void *stream, spsim_hip_tile *best_out, double *best_rate) {
   using namespace spsim;
   if (!b) return fail(SPSIM_HIP_ERR_INVALID, "NULL plan");
@@ -327,7 +328,7 @@ int spsim_hip_batch_autotune(spsim_hip_batch *b, void *stream, spsim_hip_tile *b
   auto natoms = [&](int p) { const MolInfo &m = b->h_mols[b->h_pattern_mol[p]]; return m.atom_end - m.atom_begin; };
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return natoms(x) > natoms(y); });
   std::vector<int> sample;
-  const int want = std::min(b->npatterns, 6);
+  const int want = std::min(b->npatterns, 24);
   for (int i = 0; i < want; ++i) sample.push_back(order[(size_t)i * b->npatterns / want]);
   // quads per lane {1, 2} (+ octants on a square list) or pixels per lane {1, 2, 4, 8}
   const int ks_grid[3] = {1, 2, kOctantLane}, ks_pix[4] = {1, 2, 4, 8};
@@ -355,8 +356,7 @@ int spsim_hip_batch_autotune(spsim_hip_batch *b, void *stream, spsim_hip_tile *b
       if ((rc = batch_blocks_per_cu(t, b->has_b, b->grid, &per_cu)) != SPSIM_HIP_OK) break;
       const int blocks = b->cus * per_cu;
       const int units = b->grid ? grid_units(t, b->nx, b->ny) : b->npix;
-      // enough tiles per sampled pattern for ~4 items per workgroup
-      const int tiles = std::max(1, std::min((units + tu - 1) / tu, (4 * blocks + want - 1) / std::max(want, 1)));
+      const int tiles = (units + tu - 1) / tu;
       std::vector<int2> items = build_items(*b, t, &sample, tiles);
       double terms = 0.0;
       for (const int2 &it : items) {

@@ -269,18 +269,21 @@ __global__ __launch_bounds__(kThreads) void bin_count(const SpreadAtom *__restri
 }
 
 // One workgroup: out[i] = sum_{k < i} in[k] over n entries, in chunks of
-// 4 * kThreads with a running carry.
+// kScanPer * kThreads (each thread a contiguous run) with a running carry.
+constexpr int kScanPer = 16;
+
 __global__ __launch_bounds__(kThreads) void scan_exclusive(const int *__restrict__ in, long long n,
                                                           int *__restrict__ out) {
   __shared__ int part[kThreads];
   __shared__ int carry;
   if (threadIdx.x == 0) carry = 0;
   __syncthreads();
-  for (long long base = 0; base < n; base += 4 * kThreads) {
-    int v[4], sum = 0;
-    for (int k = 0; k < 4; ++k) {
-      const long long i = base + 4 * threadIdx.x + k;
-      v[k] = i < n ? in[i] : 0;
+  for (long long base = 0; base < n; base += (long long)kScanPer * kThreads) {
+    int v[kScanPer], sum = 0;
+    const long long first = base + (long long)kScanPer * threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < kScanPer; ++k) {
+      v[k] = first + k < n ? in[first + k] : 0;
       sum += v[k];
     }
     part[threadIdx.x] = sum;
@@ -292,9 +295,9 @@ __global__ __launch_bounds__(kThreads) void scan_exclusive(const int *__restrict
       __syncthreads();
     }
     int run = carry + (threadIdx.x ? part[threadIdx.x - 1] : 0);
-    for (int k = 0; k < 4; ++k) {
-      const long long i = base + 4 * threadIdx.x + k;
-      if (i < n) out[i] = run;
+#pragma unroll
+    for (int k = 0; k < kScanPer; ++k) {
+      if (first + k < n) out[first + k] = run;
       run += v[k];
     }
     __syncthreads();
