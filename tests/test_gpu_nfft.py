@@ -213,3 +213,32 @@ def test_compute_sf_takes_the_nfft_path(gpu):
     # the direct path skips the Z == 0 atoms, so the two differ
     _, I_plain = oracle.sf_f64(q, Z, pos, B=6.0)
     assert rel_l2(I_plain, I_ref) > 1e-3
+
+
+def test_nfft_long_lines_many_species_and_widest_window(gpu):
+    """A 200 nm needle along x gives FFT lines of more than 1920 points (the
+    twiddles then come from global memory); 12 elements give 12 species grids;
+    eps = 1e-9 selects the widest window (W = 10), where FP32 sets the floor."""
+    import torch
+
+    rng = np.random.default_rng(11)
+    n = 1500
+    pos = np.zeros((n, 3), np.float32)
+    pos[:, 0] = rng.uniform(-100e-9, 100e-9, n)
+    pos[:, 1:] = rng.uniform(-0.4e-9, 0.4e-9, (n, 2))
+    Z = rng.choice([1, 6, 7, 8, 11, 12, 15, 16, 17, 19, 20, 26], n).astype(np.int32)
+    q = detector_q(40, lam=0.1e-9, pix=4e-4)  # |q| up to 1.6e9 per m: x lines of 2500 points
+    F_ref, I_ref = oracle.sf_nfft_f64(q, Z, pos, B=2.0, threads=oracle_threads())
+    F, I, info = run_nfft(gpu, q, Z, pos, B=2.0)
+    assert info["nf"][0] > 1920, info
+    assert rel_l2(I, I_ref) < 1e-5, info
+    assert cplx_err(F, F_ref) < 1e-5
+    F, I, info = run_nfft(gpu, q, Z, pos, B=2.0, eps=1e-9)
+    assert info["W"] == 10
+    assert cplx_err(F, F_ref) < 3e-6, info
+    # intensities only
+    atoms = dv.DeviceAtoms(dv.pack_atoms(Z, pos, mode=_lib.PACK_NFFT), gpu)
+    F0, I0, _ = dv.structure_factor_nfft(torch.from_numpy(q).to(gpu), atoms, dv.form_factor_table(device=gpu),
+                                         B=2.0, want_F=False)
+    assert F0 is None
+    assert rel_l2(I0.cpu().numpy(), I_ref) < 1e-5
