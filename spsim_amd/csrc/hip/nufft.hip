@@ -58,12 +58,18 @@ struct NufftDim {
   int l_lo, L;       // kept FFT outputs: l in [l_lo, l_lo + L)
 };
 
+constexpr int kCorrDeg = 8;  // degree of the correction polynomial in u
+
 struct NufftGeom {
   NufftDim d[3];
   float beta;
   int nq;
   float qz[kMaxQuad];  // Gauss-Legendre nodes on [0, 1]
   float qc[kMaxQuad];  // 2 * weight * phi(node): phi_hat(xi) = sum qc cos(2 pi xi qz)
+  // 2 / (W phi_hat(xi)) for |xi| <= corr_xmax as a polynomial in
+  // u = 2 (xi / corr_xmax)^2 - 1 (monomial coefficients, constant first)
+  float corr_xmax;
+  float corr[kCorrDeg + 1];
 };
 
 // Exact n / d for n, d < 2^16: q = umulhi(n, m) with m = floor(2^32 / d) + 1
@@ -635,7 +641,12 @@ __global__ __launch_bounds__(kThreads) void nufft_interp(const float *__restrict
     const float fb = (float)(l0 - v);
 #pragma unroll
     for (int i = 0; i < W; ++i) wt[d][i] = es_window((fb + (float)i) * (2.0f / W), g.beta);
-    corr *= 2.0f / ((float)W * es_hat(g, (float)t * (0.5f * W)));
+    const float xr = (float)t * (0.5f * W) / g.corr_xmax;
+    const float u = 2.0f * xr * xr - 1.0f;
+    float c = g.corr[kCorrDeg];
+#pragma unroll
+    for (int j = kCorrDeg - 1; j >= 0; --j) c = __builtin_fmaf(c, u, g.corr[j]);
+    corr *= c;
   }
   float Fr, Fi;
   if (!ok) {
@@ -741,6 +752,48 @@ FastDiv fast_div(unsigned d) {
   f.d = d;
   f.m = d <= 1 ? 0u : (unsigned)((1ull << 32) / d + 1);
   return f;
+}
+
+// phi_hat(xi) = 2 int_0^1 phi(z) cos(2 pi xi z) dz by an n-node Gauss-Legendre rule, in FP64.
+double es_hat_host(double xi, double beta, int n, const double *z, const double *w) {
+  double sum = 0.0;
+  for (int i = 0; i < n; ++i) sum += w[i] * exp(beta * (sqrt(1.0 - z[i] * z[i]) - 1.0)) * cos(2.0 * M_PI * xi * z[i]);
+  return 2.0 * sum;
+}
+
+// Chebyshev interpolant of degree kCorrDeg of 2 / (W phi_hat(xi)) over u = 2 (xi / xmax)^2 - 1
+// in [-1, 1] (32 Chebyshev nodes, FP64), returned as monomial coefficients in u.
+void fit_correction(int W, double beta, double xmax, float *out) {
+  constexpr int kNodes = 32, kQuad = 64;
+  double z[kQuad], w[kQuad], g[kNodes], c[kCorrDeg + 1];
+  gauss_legendre01(kQuad, z, w);
+  for (int j = 0; j < kNodes; ++j) {
+    const double u = cos(M_PI * (j + 0.5) / kNodes);
+    const double xi = xmax * sqrt(0.5 * (u + 1.0));
+    g[j] = 2.0 / (W * es_hat_host(xi, beta, kQuad, z, w));
+  }
+  for (int k = 0; k <= kCorrDeg; ++k) {
+    double sum = 0.0;
+    for (int j = 0; j < kNodes; ++j) sum += g[j] * cos(M_PI * k * (j + 0.5) / kNodes);
+    c[k] = (k == 0 ? 1.0 : 2.0) * sum / kNodes;
+  }
+  // sum_k c_k T_k(u) -> sum_k m_k u^k, with T_{k+1} = 2 u T_k - T_{k-1}
+  double t_prev[kCorrDeg + 1] = {1.0}, t_cur[kCorrDeg + 1] = {0.0, 1.0}, m[kCorrDeg + 1] = {0.0};
+  m[0] += c[0];
+  for (int i = 0; i <= kCorrDeg; ++i) m[i] += c[1] * t_cur[i];
+  for (int k = 2; k <= kCorrDeg; ++k) {
+    double t_next[kCorrDeg + 1] = {0.0};
+    for (int i = 0; i <= kCorrDeg; ++i) {
+      if (i > 0) t_next[i] += 2.0 * t_cur[i - 1];
+      t_next[i] -= t_prev[i];
+    }
+    for (int i = 0; i <= kCorrDeg; ++i) {
+      m[i] += c[k] * t_next[i];
+      t_prev[i] = t_cur[i];
+      t_cur[i] = t_next[i];
+    }
+  }
+  for (int i = 0; i <= kCorrDeg; ++i) out[i] = (float)m[i];
 }
 
 int width_for(double eps) {
@@ -921,6 +974,9 @@ extern "C" int spsim_hip_sf_nfft(const float *d_hkl, int npix, const spsim_hip_a
       g.qc[i] = (float)(2.0 * w[i] * exp(g.beta * (sqrt(1.0 - z[i] * z[i]) - 1.0)));
     }
   }
+  // targets have |t| <= 1/(2 sigma) per axis, so |xi| = |t| W / 2 <= W / 8 (2 % margin)
+  g.corr_xmax = (float)(1.02 * W / (4.0 * kSigma));
+  fit_correction(W, g.beta, g.corr_xmax, g.corr);
   const NufftDim &X = g.d[0], &Y = g.d[1], &Z = g.d[2];
   if (info) {
     const int v[9] = {X.M, Y.M, Z.M, X.nf, Y.nf, Z.nf, X.L, Y.L, Z.L};
