@@ -193,3 +193,32 @@ def test_simulate_shot_vs_staged_and_oracle(gpu):
                                  d.maximum_value)
     assert np.array_equal(shot["noiseless_output"], nl)
     spsim.free_stuff(pat, opts)
+
+
+def test_simulate_shot_through_the_nfft_path(gpu):
+    """simulate_shot with use_nfft_for_sf: the pattern comes from
+    compute_pattern_on_list_by_nfft (Z == 0 atoms included) and the detector
+    stages run on its intensities."""
+    opts = detector_opts()
+    opts.use_nfft_for_sf = 1
+    Z, pos, mol = molecule(600, seed=4)
+    _, _, q = hkl_of(opts)
+    spsim.sp_srand(5)
+    pat = spsim.simulate_shot(mol, opts)
+    assert pat is not None
+    d, e = opts.detector, opts.experiment
+    n, nb = d.nx * d.ny, (d.nx // 4) * (d.ny // 4)
+    I = spsim.float_array(pat.ints, n)
+    _, I_ref = oracle.sf_nfft_f64(q, Z, pos, B=opts.b_factor)
+    assert rel_l2(I, I_ref) < 1e-5
+    _, I_plain = oracle.sf_f64(q, Z, pos, B=opts.b_factor)
+    assert rel_l2(I_plain, I_ref) > 1e-4  # the three Z == 0 atoms count on this path
+    th = ccd_array(d.thomson_correction, n)
+    om = ccd_array(d.solid_angle, n)
+    ph = oracle.photons(th, om, I, e.beam_intensity)
+    assert np.array_equal(ccd_array(d.photons_per_pixel, n), ph)
+    epp_nl = oracle.electrons_per_photon(e.wavelength, d.electron_hole_production_energy, noiseless=True)
+    nl = oracle.noiseless_output(ph, d.nx, d.ny, 4, 4, d.quantum_efficiency, epp_nl, d.linear_full_well,
+                                 d.maximum_value)
+    assert np.array_equal(ccd_array(d.noiseless_output, nb), nl)
+    spsim.free_stuff(pat, opts)
