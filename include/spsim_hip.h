@@ -25,7 +25,10 @@
  * passed as void* (NULL = the default stream). Every function returns
  * SPSIM_HIP_OK or a negative error code and never exits the process;
  * spsim_hip_last_error() describes the last failure on the calling thread.
- * Calls are stream-ordered and asynchronous unless stated otherwise.
+ * Calls are stream-ordered and asynchronous unless stated otherwise. Entry
+ * points that use "the library workspace" share one device buffer per GPU:
+ * such calls on different streams must not overlap (issue them on one stream,
+ * or synchronise between them); growing the buffer synchronises the device.
  */
 #ifndef SPSIM_HIP_H
 #define SPSIM_HIP_H 1
