@@ -176,7 +176,9 @@ int spsim_hip_sf_spectrum(const float *d_hkl, int npix, const spsim_hip_atom *d_
 
 /* F[i] = sum_s f_s(|q_i|) sum_{j in s} w_j exp(-2 pi i q_i.r_j), I = |F|^2, over the
    species segments of the packed atoms (f_s with the global B, 1 for the unit
-   species), computed by a type-3 non-uniform FFT instead of the direct sum:
+   species) and q_i rotated as q' = R q (rot9: host row-major 3x3, may be NULL;
+   applied exactly in FP64 as R^T to the atoms, like spsim_hip_sf_pattern),
+   computed by a type-3 non-uniform FFT instead of the direct sum:
    atoms are spread onto per-species grids with an exponential-of-semicircle
    window, the grids go through a pruned 3-D FFT, and each q interpolates from
    them. eps sets the window width W = ceil(-log10 eps) + 1 in [4, 10] (eps <= 0:
@@ -187,8 +189,8 @@ int spsim_hip_sf_spectrum(const float *d_hkl, int npix, const spsim_hip_atom *d_
    sizes per axis. Uses the library workspace; synchronises `stream` once (to
    read the bounds that size the grids), the rest is stream-ordered. */
 int spsim_hip_sf_nfft(const float *d_hkl, int npix, const spsim_hip_atom *d_atoms, const spsim_hip_segment *d_segs,
-                      int nsegs, int natoms, const float *d_ff, float b_global, double eps, float *d_F, float *d_I,
-                      int *info, void *stream);
+                      int nsegs, int natoms, const float *d_ff, float b_global, const double *rot9, double eps,
+                      float *d_F, float *d_I, int *info, void *stream);
 
 /* The pruned FFT stage of spsim_hip_sf_nfft on its own: every line of the
    complex input view [outer][M][inner] is placed at points m = m_lo + i

@@ -145,7 +145,7 @@ def sf_f64(hkl, Z, pos, amp=None, atom_b=None, delta=False, vector_mode=False, B
     return F[0::2] + 1j * F[1::2], I
 
 
-def sf_nfft_f64(hkl, Z, pos, delta=False, B=0.0, ff=None, threads=0):
+def sf_nfft_f64(hkl, Z, pos, delta=False, B=0.0, ff=None, threads=0, R=None):
     """FP64 value of the sums compute_pattern_on_list_by_nfft approximates
     (src/diffraction.c:528-675): over every element present, Z == 0 included
     (:556-563, :599-606), f_Z(|q|) with the global B
@@ -154,16 +154,17 @@ def sf_nfft_f64(hkl, Z, pos, delta=False, B=0.0, ff=None, threads=0):
     strengths (:619-620). The NNFFT's own approximation error is not modelled.
     Built from sf_f64: the Z > 0 atoms exactly as compute_pattern_on_list sums
     them (amplitude 1), plus the Z == 0 atoms as unit scatterers times row 0 of
-    the form-factor table. Returns (F complex128, I float64)."""
+    the form-factor table. R (3x3) rotates q exactly first (q' = R q, |q| kept).
+    Returns (F complex128, I float64)."""
     Z = np.ascontiguousarray(Z, np.int32)
     pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
     if delta:
         # delta_atoms ignores the element, so any non-zero row stands for Z == 0
-        return sf_f64(hkl, np.where(Z == 0, 1, Z).astype(np.int32), pos, delta=True, threads=threads)
-    F, _ = sf_f64(hkl, Z, pos, B=B, ff=ff, threads=threads)
+        return sf_f64(hkl, np.where(Z == 0, 1, Z).astype(np.int32), pos, delta=True, threads=threads, R=R)
+    F, _ = sf_f64(hkl, Z, pos, B=B, ff=ff, threads=threads, R=R)
     z0 = Z == 0
     if z0.any():
-        Fu, _ = sf_f64(hkl, np.ones(int(z0.sum()), np.int32), pos[z0], delta=True, threads=threads)
+        Fu, _ = sf_f64(hkl, np.ones(int(z0.sum()), np.int32), pos[z0], delta=True, threads=threads, R=R)
         q = np.ascontiguousarray(hkl, np.float32).reshape(-1, 3).astype(np.float64)
         d = np.sqrt((q * q).sum(1))
         F = F + np.array([form_factor(x, 0, B, ff) for x in d]) * Fu

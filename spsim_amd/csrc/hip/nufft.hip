@@ -51,6 +51,25 @@ constexpr int kMaxT = 256;          // lines per FFT workgroup
 constexpr double kSigma = 2.0;     // oversampling of both windows
 constexpr size_t kMaxGridBytes = (size_t)64 << 30;
 
+// Row-major R with q' = R q; the atoms are rotated instead (R^T r, FP64), which
+// gives the same phase (R q).r = q.(R^T r) and keeps the q list untouched.
+struct AtomRot {
+  int on;
+  double r[9];
+};
+
+__device__ __forceinline__ void rotate_atom(const AtomRot &R, double x, double y, double z, double p[3]) {
+  if (R.on) {
+    p[0] = __builtin_fma(R.r[6], z, __builtin_fma(R.r[3], y, R.r[0] * x));
+    p[1] = __builtin_fma(R.r[7], z, __builtin_fma(R.r[4], y, R.r[1] * x));
+    p[2] = __builtin_fma(R.r[8], z, __builtin_fma(R.r[5], y, R.r[2] * x));
+  } else {
+    p[0] = x;
+    p[1] = y;
+    p[2] = z;
+  }
+}
+
 struct NufftDim {
   double cx, cs, h;  // atom centre, q centre (1/m), grid spacing (m)
   int m_lo, M;       // spread grid: m in [m_lo, m_lo + M)
@@ -112,7 +131,7 @@ __device__ __forceinline__ double nan_max(double a, double b) { return (a != a |
 // Per-block minima and maxima of the atom positions (0..2) and q (3..5);
 // part[blk * 12 + v] for v < 6 holds minima, 6..11 maxima. NaN propagates.
 __global__ __launch_bounds__(kThreads) void bounds_partial(const spsim_hip_atom *__restrict__ atoms, int natoms,
-                                                          const float *__restrict__ hkl, int npix,
+                                                          const float *__restrict__ hkl, int npix, AtomRot R,
                                                           double *__restrict__ part) {
   double lo[6], hi[6];
   for (int v = 0; v < 6; ++v) {
@@ -122,7 +141,8 @@ __global__ __launch_bounds__(kThreads) void bounds_partial(const spsim_hip_atom 
   const int stride = gridDim.x * kThreads;
   for (int i = blockIdx.x * kThreads + threadIdx.x; i < natoms; i += stride) {
     const spsim_hip_atom a = atoms[i];
-    const double p[3] = {a.x, a.y, a.z};
+    double p[3];
+    rotate_atom(R, a.x, a.y, a.z, p);
     for (int v = 0; v < 3; ++v) {
       lo[v] = nan_min(lo[v], p[v]);
       hi[v] = nan_max(hi[v], p[v]);
@@ -209,14 +229,16 @@ struct __align__(16) SpreadAtom {
 // the grid, which the bounds exclude).
 __global__ __launch_bounds__(kThreads) void spread_prep(const spsim_hip_atom *__restrict__ atoms, int natoms,
                                                        const spsim_hip_segment *__restrict__ segs, int nsegs,
-                                                       NufftGeom g, int W, SpreadAtom *__restrict__ rec) {
+                                                       AtomRot R, NufftGeom g, int W, SpreadAtom *__restrict__ rec) {
   const int j = blockIdx.x * kThreads + threadIdx.x;
   if (j >= natoms) return;
   int ch = 0;
   for (int s = 1; s < nsegs; ++s)
     if (segs[s].start <= j) ch = s;
   const spsim_hip_atom a = atoms[j];
-  const double xc[3] = {(double)a.x - g.d[0].cx, (double)a.y - g.d[1].cx, (double)a.z - g.d[2].cx};
+  double pr[3];
+  rotate_atom(R, a.x, a.y, a.z, pr);
+  const double xc[3] = {pr[0] - g.d[0].cx, pr[1] - g.d[1].cx, pr[2] - g.d[2].cx};
   double ph = -(xc[0] * g.d[0].cs + xc[1] * g.d[1].cs + xc[2] * g.d[2].cs);
   ph -= rint(ph);
   float sn, cs;
@@ -338,11 +360,14 @@ struct __align__(16) SpreadCand {
 // into the block and writes its window weights there. Then every lane adds
 // every kept atom into its column (x0 + lane % 8, y0 + lane / 8) for the
 // block's 8 z-planes, in registers.
+// With few output blocks (small grids) the candidates of a block are split
+// round-robin by 64-atom batch over gridDim.y workgroups, each writing its own
+// partial grid (grid + blockIdx.y * grid_stride), which sum_partials adds up.
 template <int W>
 __global__ __launch_bounds__(64) void spread_gather(const SpreadAtom *__restrict__ sorted,
                                                    const int *__restrict__ start, const int *__restrict__ counts,
                                                    SpreadBins b, NufftGeom g, const float *__restrict__ dec,
-                                                   float2 *__restrict__ grid) {
+                                                   float2 *__restrict__ grid, size_t grid_stride) {
   constexpr int kB = kSpreadBlock;
   constexpr int NB = (W + kB - 2) / kB;
   constexpr int NH = (NB + 1) * (NB + 1) * (NB + 1);
@@ -381,7 +406,7 @@ __global__ __launch_bounds__(64) void spread_gather(const SpreadAtom *__restrict
   float ar[kB], ai[kB];
 #pragma unroll
   for (int z = 0; z < kB; ++z) ar[z] = ai[z] = 0.0f;
-  for (int base = 0; base < total; base += 64) {
+  for (int base = 64 * (int)blockIdx.y; base < total; base += 64 * (int)gridDim.y) {
     if (lane == 0) ncand = 0;
     __syncthreads();
     const int v = base + lane;
@@ -420,10 +445,25 @@ __global__ __launch_bounds__(64) void spread_gather(const SpreadAtom *__restrict
     __syncthreads();
   }
   const int gx = x0 + tx, gy = y0 + ty;
+  float2 *out = grid + blockIdx.y * grid_stride;
   if (gx < Mx && gy < My) {
 #pragma unroll
     for (int z = 0; z < kB; ++z)
-      if (z0 + z < Mz) grid[(((size_t)ch * Mz + z0 + z) * My + gy) * Mx + gx] = make_float2(ar[z], ai[z]);
+      if (z0 + z < Mz) out[(((size_t)ch * Mz + z0 + z) * My + gy) * Mx + gx] = make_float2(ar[z], ai[z]);
+  }
+}
+
+// grid[i] = sum over the nsplit partial grids (fixed order), i < n.
+__global__ __launch_bounds__(kThreads) void sum_partials(const float2 *__restrict__ part, int nsplit, size_t n,
+                                                        float2 *__restrict__ grid) {
+  for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (size_t)gridDim.x * kThreads) {
+    float2 acc = part[i];
+    for (int sidx = 1; sidx < nsplit; ++sidx) {
+      const float2 v = part[(size_t)sidx * n + i];
+      acc.x += v.x;
+      acc.y += v.y;
+    }
+    grid[i] = acc;
   }
 }
 
@@ -879,13 +919,13 @@ int launch_fft(const FftPass &p, const float2 *in, float2 *out, const float2 *tw
 
 template <int W>
 int launch_spread_interp(bool spread, const NufftGeom &g, const SpreadAtom *rec, const SpreadBins &bins,
-                         const int *start, const int *counts, int nsegs, const spsim_hip_segment *segs,
-                         const float *dec, float2 *grid, const float *hkl, int npix, const float2 *H, const float *ff,
-                         float b_global, float2 *F, float *I, hipStream_t stream) {
+                         const int *start, const int *counts, int nsplit, size_t grid_stride, int nsegs,
+                         const spsim_hip_segment *segs, const float *dec, float2 *grid, const float *hkl, int npix,
+                         const float2 *H, const float *ff, float b_global, float2 *F, float *I, hipStream_t stream) {
   if (spread) {
     if (bins.nbins > 0x7fffffffLL) return fail(SPSIM_HIP_ERR_INVALID, "too many spread blocks");
-    hipLaunchKernelGGL(spread_gather<W>, dim3((unsigned)bins.nbins), dim3(64), 0, stream, rec, start, counts, bins, g,
-                       dec, grid);
+    hipLaunchKernelGGL(spread_gather<W>, dim3((unsigned)bins.nbins, (unsigned)nsplit), dim3(64), 0, stream, rec,
+                       start, counts, bins, g, dec, grid, grid_stride);
   } else {
     hipLaunchKernelGGL(nufft_interp<W>, dim3((npix + kThreads - 1) / kThreads), dim3(kThreads), 0, stream, hkl, npix,
                        g, H, nsegs, segs, ff, b_global, F, I);
@@ -895,13 +935,13 @@ int launch_spread_interp(bool spread, const NufftGeom &g, const SpreadAtom *rec,
 }
 
 int dispatch_w(int W, bool spread, const NufftGeom &g, const SpreadAtom *rec, const SpreadBins &bins,
-               const int *start, const int *counts, const spsim_hip_segment *segs, int nsegs, const float *dec,
-               float2 *grid, const float *hkl, int npix, const float2 *H, const float *ff, float b_global, float2 *F,
-               float *I, hipStream_t s) {
-#define SPSIM_NUFFT_W(w) \
+               const int *start, const int *counts, int nsplit, size_t grid_stride, const spsim_hip_segment *segs,
+               int nsegs, const float *dec, float2 *grid, const float *hkl, int npix, const float2 *H,
+               const float *ff, float b_global, float2 *F, float *I, hipStream_t s) {
+#define SPSIM_NUFFT_W(w)                                                                                           \
   case w:                                                                                                          \
-    return launch_spread_interp<w>(spread, g, rec, bins, start, counts, nsegs, segs, dec, grid, hkl, npix, H, ff, \
-                                   b_global, F, I, s)
+    return launch_spread_interp<w>(spread, g, rec, bins, start, counts, nsplit, grid_stride, nsegs, segs, dec,    \
+                                   grid, hkl, npix, H, ff, b_global, F, I, s)
   switch (W) {
     SPSIM_NUFFT_W(4);
     SPSIM_NUFFT_W(5);
@@ -922,7 +962,8 @@ size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 extern "C" int spsim_hip_sf_nfft(const float *d_hkl, int npix, const spsim_hip_atom *d_atoms,
                                  const spsim_hip_segment *d_segs, int nsegs, int natoms, const float *d_ff,
-                                 float b_global, double eps, float *d_F, float *d_I, int *info, void *stream) {
+                                 float b_global, const double *rot9, double eps, float *d_F, float *d_I, int *info,
+                                 void *stream) {
   using namespace spsim;
   if (npix < 0 || natoms < 0 || nsegs < 0 || nsegs > SPSIM_HIP_NSPECIES)
     return fail(SPSIM_HIP_ERR_INVALID, "negative size or too many segments");
@@ -931,6 +972,14 @@ extern "C" int spsim_hip_sf_nfft(const float *d_hkl, int npix, const spsim_hip_a
   SPSIM_TRY(ensure_device(nullptr));
   hipStream_t s = (hipStream_t)stream;
   const int W = width_for(eps);
+  AtomRot R{};
+  if (rot9) {
+    R.on = 1;
+    for (int i = 0; i < 9; ++i) {
+      if (!std::isfinite(rot9[i])) return fail(SPSIM_HIP_ERR_INVALID, "non-finite rotation");
+      R.r[i] = rot9[i];
+    }
+  }
   if (info)
     for (int i = 0; i < 10; ++i) info[i] = 0;
   if (info) info[0] = W;
@@ -951,7 +1000,7 @@ extern "C" int spsim_hip_sf_nfft(const float *d_hkl, int npix, const spsim_hip_a
   SPSIM_TRY(workspace(align256(sizeof(double) * 12 * (nblk + 1)), &ws));
   double *part = (double *)ws;
   double *dbounds = part + (size_t)12 * nblk;
-  hipLaunchKernelGGL(bounds_partial, dim3(nblk), dim3(kThreads), 0, s, d_atoms, natoms, d_hkl, npix, part);
+  hipLaunchKernelGGL(bounds_partial, dim3(nblk), dim3(kThreads), 0, s, d_atoms, natoms, d_hkl, npix, R, part);
   SPSIM_HIP_TRY(hipGetLastError());
   hipLaunchKernelGGL(bounds_final, dim3(1), dim3(kThreads), 0, s, part, nblk, dbounds);
   SPSIM_HIP_TRY(hipGetLastError());
@@ -997,7 +1046,19 @@ extern "C" int spsim_hip_sf_nfft(const float *d_hkl, int npix, const spsim_hip_a
   const size_t b_tw = align256(sizeof(float2) * (size_t)(X.nf + Y.nf + Z.nf));
   const size_t b1 = align256(sizeof(float2) * n1), b2 = align256(sizeof(float2) * n2);
   if (b1 + b2 > kMaxGridBytes) return fail(SPSIM_HIP_ERR_INVALID, "NUFFT grids need %zu bytes", b1 + b2);
-  SPSIM_TRY(workspace(b_dec + 2 * b_rec + 3 * b_bin + b_tw + b1 + b2, &ws));
+  // Few output blocks (small grids): split each block's candidates over several
+  // workgroups (about 2048 in all, at least 256 atoms per split) and sum their
+  // partial grids.
+  int nsplit = 1;
+  if (bins.nbins < 2048) {
+    nsplit = (int)(2048 / bins.nbins);
+    const int by_atoms = (natoms + 255) / 256;
+    if (nsplit > by_atoms) nsplit = by_atoms;
+    if (nsplit > 64) nsplit = 64;
+    if (nsplit < 1) nsplit = 1;
+  }
+  const size_t b_part = nsplit > 1 ? align256(sizeof(float2) * nG * nsplit) : 0;
+  SPSIM_TRY(workspace(b_dec + 2 * b_rec + 3 * b_bin + b_tw + b1 + b2 + b_part, &ws));
   char *wp = (char *)ws;
   float *dec = (float *)wp;
   wp += b_dec;
@@ -1013,6 +1074,7 @@ extern "C" int spsim_hip_sf_nfft(const float *d_hkl, int npix, const spsim_hip_a
   wp += b_tw;
   float2 *buf1 = (float2 *)wp;
   float2 *buf2 = (float2 *)(wp + b1);
+  float2 *gpart = nsplit > 1 ? (float2 *)(wp + b1 + b2) : nullptr;
 
   // 3. Deconvolution table, atom records binned by spread block, spread.
   const int Mmax = X.M > Y.M ? (X.M > Z.M ? X.M : Z.M) : (Y.M > Z.M ? Y.M : Z.M);
@@ -1023,7 +1085,8 @@ extern "C" int spsim_hip_sf_nfft(const float *d_hkl, int npix, const spsim_hip_a
                      dim3(kThreads), 0, s, (float4 *)counts, n4);
   SPSIM_HIP_TRY(hipGetLastError());
   const unsigned ablocks = (unsigned)((natoms + kThreads - 1) / kThreads);
-  hipLaunchKernelGGL(spread_prep, dim3(ablocks), dim3(kThreads), 0, s, d_atoms, natoms, d_segs, nsegs, g, W, rec);
+  hipLaunchKernelGGL(spread_prep, dim3(ablocks), dim3(kThreads), 0, s, d_atoms, natoms, d_segs, nsegs, R, g, W,
+                     rec);
   SPSIM_HIP_TRY(hipGetLastError());
   hipLaunchKernelGGL(bin_count, dim3(ablocks), dim3(kThreads), 0, s, rec, natoms, bins, counts);
   SPSIM_HIP_TRY(hipGetLastError());
@@ -1031,8 +1094,14 @@ extern "C" int spsim_hip_sf_nfft(const float *d_hkl, int npix, const spsim_hip_a
   SPSIM_HIP_TRY(hipGetLastError());
   hipLaunchKernelGGL(bin_scatter, dim3(ablocks), dim3(kThreads), 0, s, rec, natoms, bins, start, cursor, sorted);
   SPSIM_HIP_TRY(hipGetLastError());
-  SPSIM_TRY(dispatch_w(W, true, g, sorted, bins, start, counts, d_segs, nsegs, dec, buf1, nullptr, 0, nullptr, d_ff,
-                       b_global, nullptr, nullptr, s));
+  SPSIM_TRY(dispatch_w(W, true, g, sorted, bins, start, counts, nsplit, nG, d_segs, nsegs, dec,
+                       nsplit > 1 ? gpart : buf1, nullptr, 0, nullptr, d_ff, b_global, nullptr, nullptr, s));
+  if (nsplit > 1) {
+    const size_t rb = (nG + kThreads - 1) / kThreads;
+    hipLaunchKernelGGL(sum_partials, dim3((unsigned)(rb < 4096 ? rb : 4096)), dim3(kThreads), 0, s, gpart, nsplit,
+                       nG, buf1);
+    SPSIM_HIP_TRY(hipGetLastError());
+  }
 
   // 4. Pruned FFT along x, y, z: G (buf1) -> A (buf2) -> B (buf1) -> H (buf2).
   float2 *twx = tw, *twy = tw + X.nf, *twz = tw + X.nf + Y.nf;
@@ -1044,8 +1113,8 @@ extern "C" int spsim_hip_sf_nfft(const float *d_hkl, int npix, const spsim_hip_a
   SPSIM_TRY(launch_fft(make_pass(Z, (long long)nch, (long long)Y.L * X.L), buf1, buf2, twz, s));
 
   // 5. Interpolate at every q.
-  SPSIM_TRY(dispatch_w(W, false, g, nullptr, bins, nullptr, nullptr, d_segs, nsegs, nullptr, nullptr, d_hkl, npix,
-                       buf2, d_ff, b_global, (float2 *)d_F, d_I, s));
+  SPSIM_TRY(dispatch_w(W, false, g, nullptr, bins, nullptr, nullptr, 1, 0, d_segs, nsegs, nullptr, nullptr, d_hkl,
+                       npix, buf2, d_ff, b_global, (float2 *)d_F, d_I, s));
   return SPSIM_HIP_OK;
 }
 

@@ -181,10 +181,11 @@ def structure_factor_spectrum(hkl: torch.Tensor, atoms: DeviceAtoms, ff: torch.T
 
 def structure_factor_nfft(hkl: torch.Tensor, atoms: DeviceAtoms, ff: torch.Tensor, B: float = 0.0,
                           eps: float = 1e-6, F: torch.Tensor | None = None, I: torch.Tensor | None = None,
-                          stream=None, want_F: bool = True):
-    """One pattern by the type-3 non-uniform FFT (spsim_hip_sf_nfft); atoms
-    packed with PACK_NFFT give the reference's NFFT semantics. Returns
-    (F or None, I, info) with info = {W, M, nf, L} (per-axis sizes as tuples)."""
+                          stream=None, want_F: bool = True, R=None):
+    """One pattern by the type-3 non-uniform FFT (spsim_hip_sf_nfft), with q
+    rotated exactly by R (3x3) when given; atoms packed with PACK_NFFT give the
+    reference's NFFT semantics. Returns (F or None, I, info) with
+    info = {W, M, nf, L} (per-axis sizes as tuples)."""
     if atoms.b is not None:
         raise ValueError("the NUFFT path takes no per-atom B factors")
     npix = int(hkl.shape[0])
@@ -193,9 +194,11 @@ def structure_factor_nfft(hkl: torch.Tensor, atoms: DeviceAtoms, ff: torch.Tenso
     if I is None:
         I = torch.empty((npix,), dtype=torch.float32, device=hkl.device)
     info = (C.c_int * 10)()
+    Ra, Rp = _rot(R)
     rc = _lib.load().spsim_hip_sf_nfft(_ptr(hkl), npix, _ptr(atoms.atoms), _ptr(atoms.segs), atoms.nsegs,
-                                       atoms.natoms, _ptr(ff), float(B), float(eps), _ptr(F if want_F else None),
-                                       _ptr(I), C.cast(info, C.c_void_p), _stream(stream))
+                                       atoms.natoms, _ptr(ff), float(B), Rp, float(eps),
+                                       _ptr(F if want_F else None), _ptr(I), C.cast(info, C.c_void_p),
+                                       _stream(stream))
     check(rc, "spsim_hip_sf_nfft")
     v = list(info)
     return (F if want_F else None), I, {"W": v[0], "M": tuple(v[1:4]), "nf": tuple(v[4:7]), "L": tuple(v[7:10])}

@@ -69,12 +69,12 @@ def detector_q(n=40, lam=0.3e-9, pix=1e-4, dist=0.05, cx=0.0, cy=0.0):
     return oracle.hkl_detector(n, n, n * pix, n * pix, dist, cx, cy, 0, lam)
 
 
-def run_nfft(gpu, q, Z, pos, B=0.0, eps=1e-6, delta=False):
+def run_nfft(gpu, q, Z, pos, B=0.0, eps=1e-6, delta=False, R=None):
     import torch
 
     atoms = dv.DeviceAtoms(dv.pack_atoms(Z, pos, delta_atoms=delta, mode=_lib.PACK_NFFT), gpu)
     F, I, info = dv.structure_factor_nfft(torch.from_numpy(np.ascontiguousarray(q, np.float32)).to(gpu), atoms,
-                                          dv.form_factor_table(device=gpu), B=B, eps=eps)
+                                          dv.form_factor_table(device=gpu), B=B, eps=eps, R=R)
     torch.cuda.synchronize()
     Fh = F.cpu().numpy()
     return Fh[:, 0] + 1j * Fh[:, 1], I.cpu().numpy(), info
@@ -116,6 +116,25 @@ def test_nfft_three_d_volume_and_rotation(gpu):
     F, I, info = run_nfft(gpu, q, Z, pos, B=4.0)
     assert rel_l2(I, I_ref) < 1e-5, info
     assert cplx_err(F, F_ref) < 1e-5
+
+
+def test_nfft_exact_rotation(gpu):
+    """rot9 rotates q exactly (applied as R^T to the atoms in FP64), as in the
+    direct entry points; the result matches the oracle's unrounded R q."""
+    Z, pos = molecule(700, 3.5e-9, seed=12)
+    q = detector_q(40, lam=0.2e-9)
+    rng = np.random.default_rng(13)
+    for R in wl.uniform_rotations(3, rng):
+        F_ref, I_ref = oracle.sf_nfft_f64(q, Z, pos, B=5.0, R=R, threads=oracle_threads())
+        F, I, info = run_nfft(gpu, q, Z, pos, B=5.0, R=R)
+        assert rel_l2(I, I_ref) < 1e-5, info
+        assert cplx_err(F, F_ref) < 1e-5
+    # the identity rotation is the unrotated result
+    F0, _, _ = run_nfft(gpu, q, Z, pos, B=5.0)
+    F1, _, _ = run_nfft(gpu, q, Z, pos, B=5.0, R=np.eye(3))
+    assert cplx_err(F1, F0) < 1e-6
+    with pytest.raises(RuntimeError, match="rotation"):
+        run_nfft(gpu, q, Z, pos, R=np.full((3, 3), np.nan))
 
 
 def test_nfft_delta_atoms(gpu):
