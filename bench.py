@@ -56,6 +56,8 @@ def parse():
     p.add_argument("--no-nfft", action="store_true", help="skip the non-uniform FFT measurement")
     p.add_argument("--nfft-only", action="store_true",
                    help="only the non-uniform FFT section on the large-single-pattern geometry (for profiling)")
+    p.add_argument("--pipeline-nfft-only", action="store_true",
+                   help="only the pipeline workload through the non-uniform FFT (for profiling)")
     p.add_argument("--profile", action="store_true", help="headline workload only (for rocprofv3)")
     p.add_argument("--kernel", choices=("auto", "pattern", "grid"), default="auto",
                    help="restrict the headline sweep to one structure-factor kernel")
@@ -604,6 +606,10 @@ def main():
     from spsim_amd import workloads as wl
 
     _lib.load()
+    if args.pipeline_nfft_only:
+        log("pipeline_nfft only: the pipeline workload through the NUFFT path")
+        print(json.dumps({"pipeline_nfft": pipeline_nfft(args, torch, dv, wl, _lib)}))
+        return
     if args.nfft_only:
         det, Z, pos, _ = wl.config2()
         dev = torch.device("cuda", 0)

@@ -296,6 +296,55 @@ __global__ __launch_bounds__(kThreads) void bin_count(const SpreadAtom *__restri
   atomicAdd(counts + home_bin(r, b), 1);
 }
 
+// Few bins (small grids): each workgroup counts a run of kBinRun atoms in an
+// LDS histogram and adds it to the global counts with one atomic per bin,
+// instead of one contended global atomic per atom.
+constexpr int kLocalBins = 4096;
+constexpr int kBinRun = 4096;
+
+__global__ __launch_bounds__(kThreads) void bin_count_local(const SpreadAtom *__restrict__ rec, int natoms,
+                                                           SpreadBins b, int *__restrict__ counts) {
+  __shared__ int hist[kLocalBins];
+  const int nb = (int)b.nbins;
+  for (int i = threadIdx.x; i < nb; i += kThreads) hist[i] = 0;
+  __syncthreads();
+  const int j0 = blockIdx.x * kBinRun, j1 = min(natoms, j0 + kBinRun);
+  for (int j = j0 + threadIdx.x; j < j1; j += kThreads) {
+    const SpreadAtom r = rec[j];
+    if (r.ch >= 0) atomicAdd(hist + home_bin(r, b), 1);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < nb; i += kThreads)
+    if (hist[i]) atomicAdd(counts + i, hist[i]);
+}
+
+// The matching scatter: the workgroup reserves a range of each bin for its run
+// (one global atomic per bin) and places its atoms there by local rank.
+__global__ __launch_bounds__(kThreads) void bin_scatter_local(const SpreadAtom *__restrict__ rec, int natoms,
+                                                             SpreadBins b, const int *__restrict__ start,
+                                                             int *__restrict__ cursor,
+                                                             SpreadAtom *__restrict__ sorted) {
+  __shared__ int base[kLocalBins], rank[kLocalBins];
+  const int nb = (int)b.nbins;
+  for (int i = threadIdx.x; i < nb; i += kThreads) base[i] = rank[i] = 0;
+  __syncthreads();
+  const int j0 = blockIdx.x * kBinRun, j1 = min(natoms, j0 + kBinRun);
+  for (int j = j0 + threadIdx.x; j < j1; j += kThreads) {
+    const SpreadAtom r = rec[j];
+    if (r.ch >= 0) atomicAdd(base + home_bin(r, b), 1);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < nb; i += kThreads)
+    if (base[i]) base[i] = start[i] + atomicAdd(cursor + i, base[i]);
+  __syncthreads();
+  for (int j = j0 + threadIdx.x; j < j1; j += kThreads) {
+    const SpreadAtom r = rec[j];
+    if (r.ch < 0) continue;
+    const int h = (int)home_bin(r, b);
+    sorted[base[h] + atomicAdd(rank + h, 1)] = r;
+  }
+}
+
 // One workgroup: out[i] = sum_{k < i} in[k] over n entries, in chunks of
 // kScanPer * kThreads (each thread a contiguous run) with a running carry.
 constexpr int kScanPer = 16;
@@ -1088,11 +1137,19 @@ extern "C" int spsim_hip_sf_nfft(const float *d_hkl, int npix, const spsim_hip_a
   hipLaunchKernelGGL(spread_prep, dim3(ablocks), dim3(kThreads), 0, s, d_atoms, natoms, d_segs, nsegs, R, g, W,
                      rec);
   SPSIM_HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(bin_count, dim3(ablocks), dim3(kThreads), 0, s, rec, natoms, bins, counts);
+  const bool local_bins = bins.nbins <= kLocalBins;
+  const unsigned runs = (unsigned)((natoms + kBinRun - 1) / kBinRun);
+  if (local_bins)
+    hipLaunchKernelGGL(bin_count_local, dim3(runs), dim3(kThreads), 0, s, rec, natoms, bins, counts);
+  else
+    hipLaunchKernelGGL(bin_count, dim3(ablocks), dim3(kThreads), 0, s, rec, natoms, bins, counts);
   SPSIM_HIP_TRY(hipGetLastError());
   hipLaunchKernelGGL(scan_exclusive, dim3(1), dim3(kThreads), 0, s, counts, bins.nbins, start);
   SPSIM_HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(bin_scatter, dim3(ablocks), dim3(kThreads), 0, s, rec, natoms, bins, start, cursor, sorted);
+  if (local_bins)
+    hipLaunchKernelGGL(bin_scatter_local, dim3(runs), dim3(kThreads), 0, s, rec, natoms, bins, start, cursor, sorted);
+  else
+    hipLaunchKernelGGL(bin_scatter, dim3(ablocks), dim3(kThreads), 0, s, rec, natoms, bins, start, cursor, sorted);
   SPSIM_HIP_TRY(hipGetLastError());
   SPSIM_TRY(dispatch_w(W, true, g, sorted, bins, start, counts, nsplit, nG, d_segs, nsegs, dec,
                        nsplit > 1 ? gpart : buf1, nullptr, 0, nullptr, d_ff, b_global, nullptr, nullptr, s));
