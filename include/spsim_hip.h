@@ -192,6 +192,20 @@ int spsim_hip_sf_nfft(const float *d_hkl, int npix, const spsim_hip_atom *d_atom
                       int nsegs, int natoms, const float *d_ff, float b_global, const double *rot9, double eps,
                       float *d_F, float *d_I, int *info, void *stream);
 
+/* Many orientations of one molecule through the NUFFT: pattern p is
+   spsim_hip_sf_nfft's result for q rotated as q' = R_p q (rot9s: host
+   [npatterns][9], row-major, applied exactly in FP64), written to
+   d_F + p*npix*2 and d_I + p*npix. One set of species grids is built for the
+   unrotated atoms over the ball |q| <= max |q| (which holds every R q) and
+   every pattern only interpolates from it, so the spread and FFT run once.
+   The grids are larger than a single pattern's when q does not fill that ball
+   (a flat detector's Ewald cap). Returns after the last pattern is written
+   (it synchronises `stream`). Other arguments as spsim_hip_sf_nfft. */
+int spsim_hip_sf_nfft_batch(const float *d_hkl, int npix, const spsim_hip_atom *d_atoms,
+                            const spsim_hip_segment *d_segs, int nsegs, int natoms, const float *d_ff, float b_global,
+                            int npatterns, const double *rot9s, double eps, float *d_F, float *d_I, int *info,
+                            void *stream);
+
 /* The pruned FFT stage of spsim_hip_sf_nfft on its own: every line of the
    complex input view [outer][M][inner] is placed at points m = m_lo + i
    (i < M) of a zero line of nf points (slot m mod nf), transformed forward

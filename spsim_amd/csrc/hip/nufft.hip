@@ -128,73 +128,67 @@ __device__ __forceinline__ float es_hat(const NufftGeom &g, float xi) {
 __device__ __forceinline__ double nan_min(double a, double b) { return (a != a || b != b) ? a + b : fmin(a, b); }
 __device__ __forceinline__ double nan_max(double a, double b) { return (a != a || b != b) ? a + b : fmax(a, b); }
 
-// Per-block minima and maxima of the atom positions (0..2) and q (3..5);
-// part[blk * 12 + v] for v < 6 holds minima, 6..11 maxima. NaN propagates.
+// Per-block minima and maxima of the atom positions (0..2) and q (3..5), and
+// the largest |q|^2 (12): part[blk * kNb + v], v < 6 minima, 6..11 maxima.
+// NaN propagates.
+constexpr int kNb = 14;
+
+__device__ __forceinline__ double bound_combine(int v, double a, double b) {
+  return v < 6 ? nan_min(a, b) : nan_max(a, b);
+}
+
 __global__ __launch_bounds__(kThreads) void bounds_partial(const spsim_hip_atom *__restrict__ atoms, int natoms,
                                                           const float *__restrict__ hkl, int npix, AtomRot R,
                                                           double *__restrict__ part) {
-  double lo[6], hi[6];
-  for (int v = 0; v < 6; ++v) {
-    lo[v] = INFINITY;
-    hi[v] = -INFINITY;
-  }
+  double acc[kNb];
+  for (int v = 0; v < kNb; ++v) acc[v] = v < 6 ? INFINITY : -INFINITY;
   const int stride = gridDim.x * kThreads;
   for (int i = blockIdx.x * kThreads + threadIdx.x; i < natoms; i += stride) {
     const spsim_hip_atom a = atoms[i];
     double p[3];
     rotate_atom(R, a.x, a.y, a.z, p);
     for (int v = 0; v < 3; ++v) {
-      lo[v] = nan_min(lo[v], p[v]);
-      hi[v] = nan_max(hi[v], p[v]);
+      acc[v] = nan_min(acc[v], p[v]);
+      acc[6 + v] = nan_max(acc[6 + v], p[v]);
     }
   }
   for (int i = blockIdx.x * kThreads + threadIdx.x; i < npix; i += stride) {
+    double n2 = 0.0;
     for (int v = 0; v < 3; ++v) {
       const double q = hkl[3 * (size_t)i + v];
-      lo[3 + v] = nan_min(lo[3 + v], q);
-      hi[3 + v] = nan_max(hi[3 + v], q);
+      acc[3 + v] = nan_min(acc[3 + v], q);
+      acc[9 + v] = nan_max(acc[9 + v], q);
+      n2 += q * q;
     }
+    acc[12] = nan_max(acc[12], n2);
   }
-  __shared__ double red[12][kThreads];
-  for (int v = 0; v < 6; ++v) {
-    red[v][threadIdx.x] = lo[v];
-    red[6 + v][threadIdx.x] = hi[v];
-  }
+  __shared__ double red[kNb][kThreads];
+  for (int v = 0; v < kNb; ++v) red[v][threadIdx.x] = acc[v];
   __syncthreads();
   for (int w = kThreads / 2; w > 0; w >>= 1) {
-    if (threadIdx.x < w) {
-      for (int v = 0; v < 6; ++v) {
-        red[v][threadIdx.x] = nan_min(red[v][threadIdx.x], red[v][threadIdx.x + w]);
-        red[6 + v][threadIdx.x] = nan_max(red[6 + v][threadIdx.x], red[6 + v][threadIdx.x + w]);
-      }
-    }
+    if (threadIdx.x < w)
+      for (int v = 0; v < kNb; ++v) red[v][threadIdx.x] = bound_combine(v, red[v][threadIdx.x], red[v][threadIdx.x + w]);
     __syncthreads();
   }
-  if (threadIdx.x < 12) part[(size_t)blockIdx.x * 12 + threadIdx.x] = red[threadIdx.x][0];
+  if (threadIdx.x < kNb) part[(size_t)blockIdx.x * kNb + threadIdx.x] = red[threadIdx.x][0];
 }
 
-// One workgroup folds nblk partial records into out[12].
+// One workgroup folds nblk partial records into out[kNb].
 __global__ __launch_bounds__(kThreads) void bounds_final(const double *__restrict__ part, int nblk,
                                                         double *__restrict__ out) {
-  __shared__ double red[12][kThreads];
-  for (int v = 0; v < 12; ++v) {
+  __shared__ double red[kNb][kThreads];
+  for (int v = 0; v < kNb; ++v) {
     double acc = v < 6 ? INFINITY : -INFINITY;
-    for (int b = threadIdx.x; b < nblk; b += kThreads) {
-      const double x = part[(size_t)b * 12 + v];
-      acc = v < 6 ? nan_min(acc, x) : nan_max(acc, x);
-    }
+    for (int b = threadIdx.x; b < nblk; b += kThreads) acc = bound_combine(v, acc, part[(size_t)b * kNb + v]);
     red[v][threadIdx.x] = acc;
   }
   __syncthreads();
   for (int w = kThreads / 2; w > 0; w >>= 1) {
-    if (threadIdx.x < w) {
-      for (int v = 0; v < 12; ++v)
-        red[v][threadIdx.x] = v < 6 ? nan_min(red[v][threadIdx.x], red[v][threadIdx.x + w])
-                                    : nan_max(red[v][threadIdx.x], red[v][threadIdx.x + w]);
-    }
+    if (threadIdx.x < w)
+      for (int v = 0; v < kNb; ++v) red[v][threadIdx.x] = bound_combine(v, red[v][threadIdx.x], red[v][threadIdx.x + w]);
     __syncthreads();
   }
-  if (threadIdx.x < 12) out[threadIdx.x] = red[threadIdx.x][0];
+  if (threadIdx.x < kNb) out[threadIdx.x] = red[threadIdx.x][0];
 }
 
 // D_m = 2 / (W phi_hat(m W / (2 nf))) for the spread grid of every axis,
@@ -706,15 +700,26 @@ __global__ __launch_bounds__(kThreads) void fft_lines(FftPass p, const float2 *_
 
 // One thread per q: W^3 gathers per species grid [ch][Lz][Ly][Lx], window
 // correction, form factor, species sum and the q phase.
+// Patterns of a batch (one species grid set, many orientations) are
+// blockIdx.y: pattern p interpolates at q' = R_p q (rots: row-major [p][9],
+// FP64) and writes F + p npix, I + p npix; rots == nullptr means q' = q.
 template <int W>
 __global__ __launch_bounds__(kThreads) void nufft_interp(const float *__restrict__ hkl, int npix, NufftGeom g,
                                                         const float2 *__restrict__ H, int nch,
                                                         const spsim_hip_segment *__restrict__ segs,
                                                         const float *__restrict__ ff, float b_global,
+                                                        const double *__restrict__ rots, int p0,
                                                         float2 *__restrict__ F, float *__restrict__ I) {
   const int k = blockIdx.x * kThreads + threadIdx.x;
   if (k >= npix) return;
-  const double q[3] = {hkl[3 * (size_t)k], hkl[3 * (size_t)k + 1], hkl[3 * (size_t)k + 2]};
+  const int pat = p0 + (int)blockIdx.y;
+  const double q0[3] = {hkl[3 * (size_t)k], hkl[3 * (size_t)k + 1], hkl[3 * (size_t)k + 2]};
+  double q[3] = {q0[0], q0[1], q0[2]};
+  if (rots) {
+    const double *R = rots + 9 * (size_t)pat;
+    for (int i = 0; i < 3; ++i) q[i] = __builtin_fma(R[3 * i + 2], q0[2], __builtin_fma(R[3 * i + 1], q0[1], R[3 * i] * q0[0]));
+  }
+  const size_t out = (size_t)pat * npix + k;
   float wt[3][W];
   int base[3];
   float corr = 1.0f;
@@ -742,7 +747,7 @@ __global__ __launch_bounds__(kThreads) void nufft_interp(const float *__restrict
     // Unreachable for q inside the bounds the grids were sized for.
     Fr = Fi = __builtin_nanf("");
   } else {
-    const float s2 = (float)((q[0] * q[0] + q[1] * q[1] + q[2] * q[2]) * 2.5e-21);
+    const float s2 = (float)((q0[0] * q0[0] + q0[1] * q0[1] + q0[2] * q0[2]) * 2.5e-21);
     const int Lx = g.d[0].L, Ly = g.d[1].L, Lz = g.d[2].L;
     const size_t chs = (size_t)Lz * Ly * Lx;
     float fr = 0.0f, fi = 0.0f;
@@ -778,15 +783,15 @@ __global__ __launch_bounds__(kThreads) void nufft_interp(const float *__restrict
     Fr = fr * cs - fi * sn;
     Fi = fr * sn + fi * cs;
   }
-  if (F) F[k] = make_float2(Fr, Fi);
-  if (I) I[k] = Fr * Fr + Fi * Fi;
+  if (F) F[out] = make_float2(Fr, Fi);
+  if (I) I[out] = Fr * Fr + Fi * Fi;
 }
 
-__global__ __launch_bounds__(kThreads) void zero_pattern(int npix, float2 *__restrict__ F, float *__restrict__ I) {
-  const int k = blockIdx.x * kThreads + threadIdx.x;
-  if (k >= npix) return;
-  if (F) F[k] = make_float2(0.0f, 0.0f);
-  if (I) I[k] = 0.0f;
+__global__ __launch_bounds__(kThreads) void zero_pattern(size_t n, float2 *__restrict__ F, float *__restrict__ I) {
+  for (size_t k = (size_t)blockIdx.x * kThreads + threadIdx.x; k < n; k += (size_t)gridDim.x * kThreads) {
+    if (F) F[k] = make_float2(0.0f, 0.0f);
+    if (I) I[k] = 0.0f;
+  }
 }
 
 // ---- host side ----
@@ -970,14 +975,19 @@ template <int W>
 int launch_spread_interp(bool spread, const NufftGeom &g, const SpreadAtom *rec, const SpreadBins &bins,
                          const int *start, const int *counts, int nsplit, size_t grid_stride, int nsegs,
                          const spsim_hip_segment *segs, const float *dec, float2 *grid, const float *hkl, int npix,
-                         const float2 *H, const float *ff, float b_global, float2 *F, float *I, hipStream_t stream) {
+                         const float2 *H, const float *ff, float b_global, const double *rots, int npat, float2 *F,
+                         float *I, hipStream_t stream) {
   if (spread) {
     if (bins.nbins > 0x7fffffffLL) return fail(SPSIM_HIP_ERR_INVALID, "too many spread blocks");
     hipLaunchKernelGGL(spread_gather<W>, dim3((unsigned)bins.nbins, (unsigned)nsplit), dim3(64), 0, stream, rec,
                        start, counts, bins, g, dec, grid, grid_stride);
   } else {
-    hipLaunchKernelGGL(nufft_interp<W>, dim3((npix + kThreads - 1) / kThreads), dim3(kThreads), 0, stream, hkl, npix,
-                       g, H, nsegs, segs, ff, b_global, F, I);
+    for (int p0 = 0; p0 < npat; p0 += 65535) {
+      const int n = npat - p0 < 65535 ? npat - p0 : 65535;
+      hipLaunchKernelGGL(nufft_interp<W>, dim3((npix + kThreads - 1) / kThreads, n), dim3(kThreads), 0, stream, hkl,
+                         npix, g, H, nsegs, segs, ff, b_global, rots, p0, F, I);
+      SPSIM_HIP_TRY(hipGetLastError());
+    }
   }
   SPSIM_HIP_TRY(hipGetLastError());
   return SPSIM_HIP_OK;
@@ -986,11 +996,11 @@ int launch_spread_interp(bool spread, const NufftGeom &g, const SpreadAtom *rec,
 int dispatch_w(int W, bool spread, const NufftGeom &g, const SpreadAtom *rec, const SpreadBins &bins,
                const int *start, const int *counts, int nsplit, size_t grid_stride, const spsim_hip_segment *segs,
                int nsegs, const float *dec, float2 *grid, const float *hkl, int npix, const float2 *H,
-               const float *ff, float b_global, float2 *F, float *I, hipStream_t s) {
+               const float *ff, float b_global, const double *rots, int npat, float2 *F, float *I, hipStream_t s) {
 #define SPSIM_NUFFT_W(w)                                                                                           \
   case w:                                                                                                          \
     return launch_spread_interp<w>(spread, g, rec, bins, start, counts, nsplit, grid_stride, nsegs, segs, dec,    \
-                                   grid, hkl, npix, H, ff, b_global, F, I, s)
+                                   grid, hkl, npix, H, ff, b_global, rots, npat, F, I, s)
   switch (W) {
     SPSIM_NUFFT_W(4);
     SPSIM_NUFFT_W(5);
@@ -1009,17 +1019,23 @@ size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 }  // namespace
 }  // namespace spsim
 
-extern "C" int spsim_hip_sf_nfft(const float *d_hkl, int npix, const spsim_hip_atom *d_atoms,
-                                 const spsim_hip_segment *d_segs, int nsegs, int natoms, const float *d_ff,
-                                 float b_global, const double *rot9, double eps, float *d_F, float *d_I, int *info,
-                                 void *stream) {
-  using namespace spsim;
-  if (npix < 0 || natoms < 0 || nsegs < 0 || nsegs > SPSIM_HIP_NSPECIES)
+namespace spsim {
+namespace {
+
+// One NUFFT evaluation. Single pattern (npat == 0): q as given, atoms rotated
+// by rot9 (R^T r), grids sized to the atom box and the q box. Batch (npat >= 1):
+// one set of species grids for the unrotated atoms over the ball
+// |q| <= max |q| (the box of every R q), then each pattern p interpolates at
+// q' = R_p q (rots: host [npat][9]) into F/I + p npix.
+int nufft_impl(const float *d_hkl, int npix, const spsim_hip_atom *d_atoms, const spsim_hip_segment *d_segs,
+               int nsegs, int natoms, const float *d_ff, float b_global, const double *rot9, int npat,
+               const double *rots, double eps, float *d_F, float *d_I, int *info, hipStream_t s) {
+  if (npix < 0 || natoms < 0 || nsegs < 0 || nsegs > SPSIM_HIP_NSPECIES || npat < 0)
     return fail(SPSIM_HIP_ERR_INVALID, "negative size or too many segments");
   if ((npix > 0 && !d_hkl) || (natoms > 0 && (!d_atoms || !d_segs || nsegs < 1)) || !d_ff)
     return fail(SPSIM_HIP_ERR_INVALID, "NULL input pointer");
+  if (npat > 0 && !rots) return fail(SPSIM_HIP_ERR_INVALID, "NULL rotations");
   SPSIM_TRY(ensure_device(nullptr));
-  hipStream_t s = (hipStream_t)stream;
   const int W = width_for(eps);
   AtomRot R{};
   if (rot9) {
@@ -1029,14 +1045,18 @@ extern "C" int spsim_hip_sf_nfft(const float *d_hkl, int npix, const spsim_hip_a
       R.r[i] = rot9[i];
     }
   }
+  for (size_t i = 0; i < (size_t)npat * 9; ++i)
+    if (!std::isfinite(rots[i])) return fail(SPSIM_HIP_ERR_INVALID, "non-finite rotation");
   if (info)
     for (int i = 0; i < 10; ++i) info[i] = 0;
   if (info) info[0] = W;
+  const int nout = npat > 0 ? npat : 1;
   if (npix == 0) return SPSIM_HIP_OK;
   if (!d_F && !d_I) return fail(SPSIM_HIP_ERR_INVALID, "both outputs are NULL");
-  const unsigned pblocks = (unsigned)((npix + kThreads - 1) / kThreads);
   if (natoms == 0) {
-    hipLaunchKernelGGL(zero_pattern, dim3(pblocks), dim3(kThreads), 0, s, npix, (float2 *)d_F, d_I);
+    const size_t n = (size_t)nout * npix, zb = (n + kThreads - 1) / kThreads;
+    hipLaunchKernelGGL(zero_pattern, dim3((unsigned)(zb < 4096 ? zb : 4096)), dim3(kThreads), 0, s, n, (float2 *)d_F,
+                       d_I);
     SPSIM_HIP_TRY(hipGetLastError());
     return SPSIM_HIP_OK;
   }
@@ -1046,18 +1066,26 @@ extern "C" int spsim_hip_sf_nfft(const float *d_hkl, int npix, const spsim_hip_a
   int nblk = (most + kThreads - 1) / kThreads;
   if (nblk > 1024) nblk = 1024;
   void *ws = nullptr;
-  SPSIM_TRY(workspace(align256(sizeof(double) * 12 * (nblk + 1)), &ws));
+  SPSIM_TRY(workspace(align256(sizeof(double) * kNb * (nblk + 1)), &ws));
   double *part = (double *)ws;
-  double *dbounds = part + (size_t)12 * nblk;
+  double *dbounds = part + (size_t)kNb * nblk;
   hipLaunchKernelGGL(bounds_partial, dim3(nblk), dim3(kThreads), 0, s, d_atoms, natoms, d_hkl, npix, R, part);
   SPSIM_HIP_TRY(hipGetLastError());
   hipLaunchKernelGGL(bounds_final, dim3(1), dim3(kThreads), 0, s, part, nblk, dbounds);
   SPSIM_HIP_TRY(hipGetLastError());
-  double hb[12];
+  double hb[kNb];
   SPSIM_HIP_TRY(hipMemcpyAsync(hb, dbounds, sizeof hb, hipMemcpyDeviceToHost, s));
   SPSIM_HIP_TRY(hipStreamSynchronize(s));
-  for (double v : hb)
-    if (!std::isfinite(v)) return fail(SPSIM_HIP_ERR_INVALID, "non-finite atom position or q");
+  for (int v = 0; v <= 12; ++v)
+    if (!std::isfinite(hb[v])) return fail(SPSIM_HIP_ERR_INVALID, "non-finite atom position or q");
+  if (npat > 0) {
+    // every R q lies in the ball of radius max |q|
+    const double r = sqrt(hb[12]);
+    for (int d = 0; d < 3; ++d) {
+      hb[3 + d] = -r;
+      hb[9 + d] = r;
+    }
+  }
 
   // 2. Plan the grids.
   NufftGeom g{};
@@ -1093,6 +1121,7 @@ extern "C" int spsim_hip_sf_nfft(const float *d_hkl, int npix, const spsim_hip_a
   const size_t b_rec = align256(sizeof(SpreadAtom) * (size_t)natoms);
   const size_t b_bin = align256(sizeof(int) * (size_t)bins.nbins);
   const size_t b_tw = align256(sizeof(float2) * (size_t)(X.nf + Y.nf + Z.nf));
+  const size_t b_rot = align256(sizeof(double) * 9 * (size_t)npat);
   const size_t b1 = align256(sizeof(float2) * n1), b2 = align256(sizeof(float2) * n2);
   if (b1 + b2 > kMaxGridBytes) return fail(SPSIM_HIP_ERR_INVALID, "NUFFT grids need %zu bytes", b1 + b2);
   // Few output blocks (small grids): split each block's candidates over several
@@ -1107,7 +1136,7 @@ extern "C" int spsim_hip_sf_nfft(const float *d_hkl, int npix, const spsim_hip_a
     if (nsplit < 1) nsplit = 1;
   }
   const size_t b_part = nsplit > 1 ? align256(sizeof(float2) * nG * nsplit) : 0;
-  SPSIM_TRY(workspace(b_dec + 2 * b_rec + 3 * b_bin + b_tw + b1 + b2 + b_part, &ws));
+  SPSIM_TRY(workspace(b_dec + 2 * b_rec + 3 * b_bin + b_tw + b_rot + b1 + b2 + b_part, &ws));
   char *wp = (char *)ws;
   float *dec = (float *)wp;
   wp += b_dec;
@@ -1121,9 +1150,15 @@ extern "C" int spsim_hip_sf_nfft(const float *d_hkl, int npix, const spsim_hip_a
   wp += 3 * b_bin;
   float2 *tw = (float2 *)wp;
   wp += b_tw;
+  double *d_rots = npat > 0 ? (double *)wp : nullptr;
+  wp += b_rot;
   float2 *buf1 = (float2 *)wp;
   float2 *buf2 = (float2 *)(wp + b1);
   float2 *gpart = nsplit > 1 ? (float2 *)(wp + b1 + b2) : nullptr;
+  if (npat > 0) {
+    // pageable source: the copy has completed when the call returns
+    SPSIM_HIP_TRY(hipMemcpyAsync(d_rots, rots, sizeof(double) * 9 * (size_t)npat, hipMemcpyHostToDevice, s));
+  }
 
   // 3. Deconvolution table, atom records binned by spread block, spread.
   const int Mmax = X.M > Y.M ? (X.M > Z.M ? X.M : Z.M) : (Y.M > Z.M ? Y.M : Z.M);
@@ -1152,7 +1187,8 @@ extern "C" int spsim_hip_sf_nfft(const float *d_hkl, int npix, const spsim_hip_a
     hipLaunchKernelGGL(bin_scatter, dim3(ablocks), dim3(kThreads), 0, s, rec, natoms, bins, start, cursor, sorted);
   SPSIM_HIP_TRY(hipGetLastError());
   SPSIM_TRY(dispatch_w(W, true, g, sorted, bins, start, counts, nsplit, nG, d_segs, nsegs, dec,
-                       nsplit > 1 ? gpart : buf1, nullptr, 0, nullptr, d_ff, b_global, nullptr, nullptr, s));
+                       nsplit > 1 ? gpart : buf1, nullptr, 0, nullptr, d_ff, b_global, nullptr, 0, nullptr, nullptr,
+                       s));
   if (nsplit > 1) {
     const size_t rb = (nG + kThreads - 1) / kThreads;
     hipLaunchKernelGGL(sum_partials, dim3((unsigned)(rb < 4096 ? rb : 4096)), dim3(kThreads), 0, s, gpart, nsplit,
@@ -1169,10 +1205,31 @@ extern "C" int spsim_hip_sf_nfft(const float *d_hkl, int npix, const spsim_hip_a
   SPSIM_TRY(launch_fft(make_pass(Y, (long long)nch * Z.M, X.L), buf2, buf1, twy, s));
   SPSIM_TRY(launch_fft(make_pass(Z, (long long)nch, (long long)Y.L * X.L), buf1, buf2, twz, s));
 
-  // 5. Interpolate at every q.
+  // 5. Interpolate at every q (of every pattern).
   SPSIM_TRY(dispatch_w(W, false, g, nullptr, bins, nullptr, nullptr, 1, 0, d_segs, nsegs, nullptr, nullptr, d_hkl,
-                       npix, buf2, d_ff, b_global, (float2 *)d_F, d_I, s));
+                       npix, buf2, d_ff, b_global, d_rots, nout, (float2 *)d_F, d_I, s));
+  if (npat > 0) SPSIM_HIP_TRY(hipStreamSynchronize(s));  // the rotations' host buffer may go after return
   return SPSIM_HIP_OK;
+}
+
+}  // namespace
+}  // namespace spsim
+
+extern "C" int spsim_hip_sf_nfft(const float *d_hkl, int npix, const spsim_hip_atom *d_atoms,
+                                 const spsim_hip_segment *d_segs, int nsegs, int natoms, const float *d_ff,
+                                 float b_global, const double *rot9, double eps, float *d_F, float *d_I, int *info,
+                                 void *stream) {
+  return spsim::nufft_impl(d_hkl, npix, d_atoms, d_segs, nsegs, natoms, d_ff, b_global, rot9, 0, nullptr, eps, d_F,
+                           d_I, info, (hipStream_t)stream);
+}
+
+extern "C" int spsim_hip_sf_nfft_batch(const float *d_hkl, int npix, const spsim_hip_atom *d_atoms,
+                                       const spsim_hip_segment *d_segs, int nsegs, int natoms, const float *d_ff,
+                                       float b_global, int npatterns, const double *rot9s, double eps, float *d_F,
+                                       float *d_I, int *info, void *stream) {
+  if (npatterns < 1) return spsim::fail(SPSIM_HIP_ERR_INVALID, "npatterns=%d", npatterns);
+  return spsim::nufft_impl(d_hkl, npix, d_atoms, d_segs, nsegs, natoms, d_ff, b_global, nullptr, npatterns, rot9s,
+                           eps, d_F, d_I, info, (hipStream_t)stream);
 }
 
 extern "C" int spsim_hip_fft_pruned(const float *d_in, long long outer, long long inner, int M, int m_lo, int nf,

@@ -204,6 +204,30 @@ def structure_factor_nfft(hkl: torch.Tensor, atoms: DeviceAtoms, ff: torch.Tenso
     return (F if want_F else None), I, {"W": v[0], "M": tuple(v[1:4]), "nf": tuple(v[4:7]), "L": tuple(v[7:10])}
 
 
+def structure_factor_nfft_batch(hkl: torch.Tensor, atoms: DeviceAtoms, ff: torch.Tensor, rots, B: float = 0.0,
+                                eps: float = 1e-6, F: torch.Tensor | None = None, I: torch.Tensor | None = None,
+                                stream=None, want_F: bool = True):
+    """Many orientations of one molecule through the NUFFT
+    (spsim_hip_sf_nfft_batch): pattern p at q rotated exactly by rots[p].
+    Returns (F [npat, npix, 2] or None, I [npat, npix], info)."""
+    if atoms.b is not None:
+        raise ValueError("the NUFFT path takes no per-atom B factors")
+    R = np.ascontiguousarray(rots, np.float64).reshape(-1, 9)
+    npat, npix = R.shape[0], int(hkl.shape[0])
+    if want_F and F is None:
+        F = torch.empty((npat, npix, 2), dtype=torch.float32, device=hkl.device)
+    if I is None:
+        I = torch.empty((npat, npix), dtype=torch.float32, device=hkl.device)
+    info = (C.c_int * 10)()
+    rc = _lib.load().spsim_hip_sf_nfft_batch(_ptr(hkl), npix, _ptr(atoms.atoms), _ptr(atoms.segs), atoms.nsegs,
+                                             atoms.natoms, _ptr(ff), float(B), npat, R.ctypes.data_as(C.c_void_p),
+                                             float(eps), _ptr(F if want_F else None), _ptr(I),
+                                             C.cast(info, C.c_void_p), _stream(stream))
+    check(rc, "spsim_hip_sf_nfft_batch")
+    v = list(info)
+    return (F if want_F else None), I, {"W": v[0], "M": tuple(v[1:4]), "nf": tuple(v[4:7]), "L": tuple(v[7:10])}
+
+
 def fft_pruned(x: torch.Tensor, M: int, m_lo: int, nf: int, L: int, l_lo: int, stream=None) -> torch.Tensor:
     """Pruned forward FFT along axis 1 of a complex64 device tensor [outer, M, inner]
     (spsim_hip_fft_pruned); returns [outer, L, inner]."""

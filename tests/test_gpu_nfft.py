@@ -261,3 +261,35 @@ def test_nfft_long_lines_many_species_and_widest_window(gpu):
                                          B=2.0, want_F=False)
     assert F0 is None
     assert rel_l2(I0.cpu().numpy(), I_ref) < 1e-5
+
+
+def test_nfft_batch_of_orientations(gpu):
+    """spsim_hip_sf_nfft_batch: one species grid set over the ball |q| <= max|q|,
+    interpolated at R_p q for every pattern; each pattern matches the FP64
+    oracle with that rotation and the single-pattern call with rot9."""
+    import torch
+
+    Z, pos = molecule(800, 3.5e-9, seed=14)
+    q = detector_q(36, lam=0.25e-9)
+    rots = wl.uniform_rotations(4, np.random.default_rng(15))
+    atoms = dv.DeviceAtoms(dv.pack_atoms(Z, pos, mode=_lib.PACK_NFFT), gpu)
+    ff = dv.form_factor_table(device=gpu)
+    hkl = torch.from_numpy(q).to(gpu)
+    F, I, info = dv.structure_factor_nfft_batch(hkl, atoms, ff, rots, B=3.0)
+    torch.cuda.synchronize()
+    Fh, Ih = F.cpu().numpy(), I.cpu().numpy()
+    assert Fh.shape == (4, q.shape[0], 2) and info["W"] == 7
+    for p, R in enumerate(rots):
+        F_ref, I_ref = oracle.sf_nfft_f64(q, Z, pos, B=3.0, R=R, threads=oracle_threads())
+        Fp = Fh[p, :, 0] + 1j * Fh[p, :, 1]
+        assert rel_l2(Ih[p], I_ref) < 1e-5, (p, info)
+        assert cplx_err(Fp, F_ref) < 1e-5
+        F1, _, _ = run_nfft(gpu, q, Z, pos, B=3.0, R=R)
+        assert cplx_err(Fp, F1) < 3e-6
+    # intensities only, and bad arguments
+    _, I2, _ = dv.structure_factor_nfft_batch(hkl, atoms, ff, rots[:2], B=3.0, want_F=False)
+    assert rel_l2(I2.cpu().numpy(), Ih[:2]) < 1e-6
+    with pytest.raises(RuntimeError, match="rotation"):
+        dv.structure_factor_nfft_batch(hkl, atoms, ff, np.full((1, 3, 3), np.inf))
+    with pytest.raises(RuntimeError, match="npatterns"):
+        dv.structure_factor_nfft_batch(hkl, atoms, ff, np.zeros((0, 9)))
