@@ -184,10 +184,14 @@ int spsim_hip_sf_spectrum(const float *d_hkl, int npix, const spsim_hip_atom *d_
    them. eps sets the window width W = ceil(-log10 eps) + 1 in [4, 10] (eps <= 0:
    1e-6); the error of F is about eps times sum_j |w_j| f_s. No per-atom B.
    The grids scale with (atom box x q range) per axis; lines longer than 3840
-   points or grids above 64 GiB are rejected. info (host, may be NULL) receives
-   {W, Mx, My, Mz, nfx, nfy, nfz, Lx, Ly, Lz}: spread grid, FFT and kept-output
-   sizes per axis. Uses the library workspace; synchronises `stream` once (to
-   read the bounds that size the grids), the rest is stream-ordered. */
+   points or grids above 64 GiB are rejected. Each q interpolates with a second
+   window: the spread window itself (FFT lines >= 2 M), or, for W >= 7 when that
+   costs less (many q per grid), a window of width W - 1 on lines >= 4 M, which
+   keeps the error. info (host, may be NULL) receives 11 ints
+   {W, Mx, My, Mz, nfx, nfy, nfz, Lx, Ly, Lz, W2}: spread window width, spread
+   grid, FFT and kept-output sizes per axis, interpolation window width. Uses
+   the library workspace; synchronises `stream` once (to read the bounds that
+   size the grids), the rest is stream-ordered. */
 int spsim_hip_sf_nfft(const float *d_hkl, int npix, const spsim_hip_atom *d_atoms, const spsim_hip_segment *d_segs,
                       int nsegs, int natoms, const float *d_ff, float b_global, const double *rot9, double eps,
                       float *d_F, float *d_I, int *info, void *stream);

@@ -15,16 +15,22 @@
 //     boxes; half-widths X, S). exp(-2 pi i q.x) = e^{-2 pi i cs.x'}
 //     e^{-2 pi i s'.x'} e^{-2 pi i q.cx}: a phase per atom, the centred sum,
 //     a phase per q.
-//  2. Spread: grid spacing h = 1/(2 sigma S) (sigma = 2); every atom adds
-//     c_j phi((m - x'_j/h) 2/W) D_m to W^3 grid points m of its species'
-//     grid, where phi(z) = exp(beta (sqrt(1 - z^2) - 1)) and D_m deconvolves
-//     the second window in advance.
+//  2. Spread: grid spacing h = 1/(2 sigma1 S) (sigma1 = 2); every atom adds
+//     c_j phi1((m - x'_j/h) 2/W) D_m to W^3 grid points m of its species'
+//     grid, where phi(z) = exp(beta (sqrt(1 - z^2) - 1)) (beta1 = 2.30 W) and
+//     D_m deconvolves the second window in advance.
 //  3. Pruned FFT: along x, y, then z, each grid line (M points) is zero-padded
-//     to nf >= 2M points, transformed in LDS (Stockham, radices 4/2/3/5) and
-//     only the L outputs the targets can reach are kept.
-//  4. Interpolate: each q gathers W^3 FFT values per species with the same
-//     window, divides by the first window's transform at s', multiplies by
-//     f_Z(|q|), sums the species and applies the q phase.
+//     to nf >= sigma2 M points, transformed in LDS (Stockham, radices
+//     4/2/3/5) and only the L outputs the targets can reach are kept.
+//  4. Interpolate: each q gathers W2^3 FFT values per species with the
+//     second window phi2 (width W2, beta2), divides by the first window's
+//     transform at s', multiplies by f_Z(|q|), sums the species and applies
+//     the q phase.
+// The second window is either the first one (sigma2 = 2, W2 = W) or, when
+// the interpolation dominates the cost (many q or many orientations per
+// grid), a narrower one on a finer frequency grid: sigma2 = 4, W2 = W - 1,
+// beta2 = 2.65 W2, which keeps the error of W = 7..10 (the first window's
+// aliasing dominates) at 0.63 of the gathers.
 // The error of the centred sum is about 10^{1-W} relative to the number of
 // atoms (W = 7 by default); the grids and FFT run in FP32, the phases in FP64.
 // Atom contributions reach the grid through FP32 atomics, so the last bits
@@ -48,7 +54,8 @@ constexpr int kMaxNf = 3840;      // longest FFT line
 constexpr int kFftLds = 1920;     // complex values per LDS buffer for several lines
 constexpr int kMaxStages = 16;
 constexpr int kMaxT = 256;          // lines per FFT workgroup
-constexpr double kSigma = 2.0;     // oversampling of both windows
+constexpr double kSigma = 2.0;     // oversampling of the spread window (and of the default second window)
+constexpr double kSigmaFine = 4.0;  // oversampling of the narrow second window
 constexpr size_t kMaxGridBytes = (size_t)64 << 30;
 
 // Row-major R with q' = R q; the atoms are rotated instead (R^T r, FP64), which
@@ -81,13 +88,14 @@ constexpr int kCorrDeg = 8;  // degree of the correction polynomial in u
 
 struct NufftGeom {
   NufftDim d[3];
-  float beta;
+  float beta1, beta2;  // spread window (width W), interpolation window (width W2)
   int nq;
   float qz[kMaxQuad];  // Gauss-Legendre nodes on [0, 1]
-  float qc[kMaxQuad];  // 2 * weight * phi(node): phi_hat(xi) = sum qc cos(2 pi xi qz)
-  // 2 / (W phi_hat(xi)) for |xi| <= corr_xmax as a polynomial in
-  // u = 2 (xi / corr_xmax)^2 - 1 (monomial coefficients, constant first)
-  float corr_xmax;
+  float qc[kMaxQuad];  // 2 * weight * phi2(node): phi2_hat(xi) = sum qc cos(2 pi xi qz)
+  // 2 / (W phi1_hat(xi)) for |xi| <= corr_xmax as a polynomial in
+  // u = 2 (xi / corr_xmax)^2 - 1 (monomial coefficients, constant first);
+  // xi = t W / 2 = t * corr_scale * corr_xmax for a target at t = s' h
+  float corr_xmax, corr_scale;
   float corr[kCorrDeg + 1];
 };
 
@@ -118,7 +126,7 @@ __device__ __forceinline__ float es_window(float z, float beta) {
   return r > 0.0f ? expf(beta * (sqrtf(r) - 1.0f)) : 0.0f;
 }
 
-// Fourier transform of the window at xi (cycles per half-width), by quadrature.
+// Fourier transform of the interpolation window at xi (cycles per half-width), by quadrature.
 __device__ __forceinline__ float es_hat(const NufftGeom &g, float xi) {
   float s = 0.0f;
   for (int i = 0; i < g.nq; ++i) s += g.qc[i] * cospif(2.0f * xi * g.qz[i]);
@@ -191,7 +199,8 @@ __global__ __launch_bounds__(kThreads) void bounds_final(const double *__restric
   if (threadIdx.x < kNb) out[threadIdx.x] = red[threadIdx.x][0];
 }
 
-// D_m = 2 / (W phi_hat(m W / (2 nf))) for the spread grid of every axis,
+// D_m = 2 / (W phi2_hat(m W / (2 nf))) (W: the interpolation window's width)
+// for the spread grid of every axis,
 // stored back to back: dec[0 .. Mx) | dec[Mx .. Mx+My) | dec[.. + Mz).
 __global__ __launch_bounds__(kThreads) void deconv_table(NufftGeom g, int W, float *__restrict__ dec) {
   const int i = blockIdx.x * kThreads + threadIdx.x;
@@ -465,9 +474,9 @@ __global__ __launch_bounds__(64) void spread_gather(const SpreadAtom *__restrict
           // table indices clamped into the grid: points past its end have zero weight
           const float dx = dec[min(x0 + i, Mx - 1)], dy = dec[Mx + min(y0 + i, My - 1)];
           const float dz = dec[Mx + My + min(z0 + i, Mz - 1)];
-          c.wx[i] = (ix >= 0 && ix < W) ? es_window((r.fx + (float)ix) * k2, g.beta) * dx : 0.0f;
-          c.wy[i] = (iy >= 0 && iy < W) ? es_window((r.fy + (float)iy) * k2, g.beta) * dy : 0.0f;
-          c.wz[i] = (iz >= 0 && iz < W) ? es_window((r.fz + (float)iz) * k2, g.beta) * dz : 0.0f;
+          c.wx[i] = (ix >= 0 && ix < W) ? es_window((r.fx + (float)ix) * k2, g.beta1) * dx : 0.0f;
+          c.wy[i] = (iy >= 0 && iy < W) ? es_window((r.fy + (float)iy) * k2, g.beta1) * dy : 0.0f;
+          c.wz[i] = (iz >= 0 && iz < W) ? es_window((r.fz + (float)iz) * k2, g.beta1) * dz : 0.0f;
         }
         c.cr = r.cr;
         c.ci = r.ci;
@@ -698,7 +707,8 @@ __global__ __launch_bounds__(kThreads) void fft_lines(FftPass p, const float2 *_
   }
 }
 
-// One thread per q: W^3 gathers per species grid [ch][Lz][Ly][Lx], window
+// One thread per q: W^3 gathers (W: the interpolation window's width) per
+// species grid [ch][Lz][Ly][Lx], window
 // correction, form factor, species sum and the q phase.
 // Patterns of a batch (one species grid set, many orientations) are
 // blockIdx.y: pattern p interpolates at q' = R_p q (rots: row-major [p][9],
@@ -734,8 +744,8 @@ __global__ __launch_bounds__(kThreads) void nufft_interp(const float *__restrict
     base[d] = ok ? (int)i0 : 0;
     const float fb = (float)(l0 - v);
 #pragma unroll
-    for (int i = 0; i < W; ++i) wt[d][i] = es_window((fb + (float)i) * (2.0f / W), g.beta);
-    const float xr = (float)t * (0.5f * W) / g.corr_xmax;
+    for (int i = 0; i < W; ++i) wt[d][i] = es_window((fb + (float)i) * (2.0f / W), g.beta2);
+    const float xr = (float)t * g.corr_scale;
     const float u = 2.0f * xr * xr - 1.0f;
     float c = g.corr[kCorrDeg];
 #pragma unroll
@@ -896,24 +906,38 @@ int width_for(double eps) {
   return W < kMinW ? kMinW : (W > kMaxW ? kMaxW : W);
 }
 
-// Grid of one axis for atoms in [xmin, xmax] and q in [smin, smax].
-int plan_axis(double xmin, double xmax, double smin, double smax, int W, NufftDim *D) {
+// Grid of one axis for atoms in [xmin, xmax] and q in [smin, smax]: spread
+// window width W, FFT length >= sigma2 M, interpolation window width W2.
+// quiet: report a grid that does not fit by the return code only.
+int plan_axis(double xmin, double xmax, double smin, double smax, int W, double sigma2, int W2, NufftDim *D,
+              bool quiet = false) {
   D->cx = 0.5 * (xmin + xmax);
   D->cs = 0.5 * (smin + smax);
   const double X = 0.5 * (xmax - xmin), S = 0.5 * (smax - smin);
   D->h = S > 0.0 ? 1.0 / (2.0 * kSigma * S) : (X > 0.0 ? 2.0 * X : 1e-9);
   const double umax = X / D->h;
-  if (!(umax < 0.25 * kMaxNf)) return fail(SPSIM_HIP_ERR_INVALID, "NUFFT grid too large (box x q-range %g)", umax);
+  if (!(umax < 0.25 * kMaxNf))
+    return quiet ? SPSIM_HIP_ERR_INVALID : fail(SPSIM_HIP_ERR_INVALID, "NUFFT grid too large (box x q-range %g)", umax);
   D->m_lo = (int)floor(-umax - 0.5 * W) - 1;
   const int m_hi = (int)ceil(umax + 0.5 * W) + 1;
   D->M = m_hi - D->m_lo + 1;
-  D->nf = fft_length((int)ceil(kSigma * D->M));
-  if (D->nf > kMaxNf) return fail(SPSIM_HIP_ERR_INVALID, "NUFFT line of %d points exceeds %d", D->nf, kMaxNf);
+  D->nf = fft_length((int)ceil(sigma2 * D->M));
+  if (D->nf > kMaxNf)
+    return quiet ? SPSIM_HIP_ERR_INVALID : fail(SPSIM_HIP_ERR_INVALID, "NUFFT line of %d points exceeds %d", D->nf, kMaxNf);
   const double vmax = S * D->h * D->nf;
-  D->l_lo = (int)floor(-vmax - 0.5 * W) - 1;
-  const int l_hi = (int)ceil(vmax + 0.5 * W) + 1;
+  D->l_lo = (int)floor(-vmax - 0.5 * W2) - 1;
+  const int l_hi = (int)ceil(vmax + 0.5 * W2) + 1;
   D->L = l_hi - D->l_lo + 1;
   return SPSIM_HIP_OK;
+}
+
+// Cost of a plan in units of one window gather: the three pruned FFT passes
+// (about 18 gathers per line point per pass, measured on MI355X) and the
+// interpolation (W2^3 gathers per q and species).
+double plan_cost(const NufftDim d[3], int W2, double nq) {
+  const double fft = (double)d[0].nf * d[1].M * d[2].M + (double)d[1].nf * d[0].L * d[2].M +
+                     (double)d[2].nf * d[0].L * d[1].L;
+  return 18.0 * fft + nq * W2 * W2 * W2;
 }
 
 FftPass make_pass(const NufftDim &D, long long outer, long long inner) {
@@ -1016,6 +1040,18 @@ int dispatch_w(int W, bool spread, const NufftGeom &g, const SpreadAtom *rec, co
 
 size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
+// Bytes of the two ping-pong buffers of the spread grid and the FFT passes:
+// G (spread) and B (after y) share the first, A (after x) and H (after z) the second.
+size_t grid_bytes(const NufftDim d[3], size_t nch, size_t *b1 = nullptr, size_t *b2 = nullptr) {
+  const NufftDim &X = d[0], &Y = d[1], &Z = d[2];
+  const size_t nG = nch * Z.M * Y.M * X.M, nA = nch * Z.M * Y.M * X.L;
+  const size_t nB = nch * Z.M * Y.L * X.L, nH = nch * Z.L * Y.L * X.L;
+  const size_t s1 = align256(sizeof(float2) * (nG > nB ? nG : nB)), s2 = align256(sizeof(float2) * (nA > nH ? nA : nH));
+  if (b1) *b1 = s1;
+  if (b2) *b2 = s2;
+  return s1 + s2;
+}
+
 }  // namespace
 }  // namespace spsim
 
@@ -1048,7 +1084,7 @@ int nufft_impl(const float *d_hkl, int npix, const spsim_hip_atom *d_atoms, cons
   for (size_t i = 0; i < (size_t)npat * 9; ++i)
     if (!std::isfinite(rots[i])) return fail(SPSIM_HIP_ERR_INVALID, "non-finite rotation");
   if (info)
-    for (int i = 0; i < 10; ++i) info[i] = 0;
+    for (int i = 0; i < 11; ++i) info[i] = 0;
   if (info) info[0] = W;
   const int nout = npat > 0 ? npat : 1;
   if (npix == 0) return SPSIM_HIP_OK;
@@ -1087,31 +1123,44 @@ int nufft_impl(const float *d_hkl, int npix, const spsim_hip_atom *d_atoms, cons
     }
   }
 
-  // 2. Plan the grids.
+  // 2. Plan the grids: the second window equal to the first, or for W >= 7
+  // the narrow one (W - 1, finer frequency grid) when it costs less and fits.
   NufftGeom g{};
-  for (int d = 0; d < 3; ++d) SPSIM_TRY(plan_axis(hb[d], hb[6 + d], hb[3 + d], hb[9 + d], W, &g.d[d]));
-  g.beta = (float)(2.30 * W);
+  const size_t nch = (size_t)nsegs;
+  for (int d = 0; d < 3; ++d) SPSIM_TRY(plan_axis(hb[d], hb[6 + d], hb[3 + d], hb[9 + d], W, kSigma, W, &g.d[d]));
+  int W2 = W;
+  if (W >= 7) {
+    NufftDim fine[3];
+    bool fits = true;
+    for (int d = 0; d < 3 && fits; ++d)
+      fits = plan_axis(hb[d], hb[6 + d], hb[3 + d], hb[9 + d], W, kSigmaFine, W - 1, &fine[d], true) == SPSIM_HIP_OK;
+    const double nq = (double)nout * npix;
+    if (fits && grid_bytes(fine, nch) <= kMaxGridBytes && plan_cost(fine, W - 1, nq) < plan_cost(g.d, W, nq)) {
+      for (int d = 0; d < 3; ++d) g.d[d] = fine[d];
+      W2 = W - 1;
+    }
+  }
+  g.beta1 = (float)(2.30 * W);
+  g.beta2 = W2 == W ? g.beta1 : (float)(2.65 * W2);
   g.nq = 2 * W + 6;
   {
     double z[kMaxQuad], w[kMaxQuad];
     gauss_legendre01(g.nq, z, w);
     for (int i = 0; i < g.nq; ++i) {
       g.qz[i] = (float)z[i];
-      g.qc[i] = (float)(2.0 * w[i] * exp(g.beta * (sqrt(1.0 - z[i] * z[i]) - 1.0)));
+      g.qc[i] = (float)(2.0 * w[i] * exp((double)g.beta2 * (sqrt(1.0 - z[i] * z[i]) - 1.0)));
     }
   }
   // targets have |t| <= 1/(2 sigma) per axis, so |xi| = |t| W / 2 <= W / 8 (2 % margin)
   g.corr_xmax = (float)(1.02 * W / (4.0 * kSigma));
-  fit_correction(W, g.beta, g.corr_xmax, g.corr);
+  g.corr_scale = (float)(0.5 * W / g.corr_xmax);
+  fit_correction(W, g.beta1, g.corr_xmax, g.corr);
   const NufftDim &X = g.d[0], &Y = g.d[1], &Z = g.d[2];
   if (info) {
-    const int v[9] = {X.M, Y.M, Z.M, X.nf, Y.nf, Z.nf, X.L, Y.L, Z.L};
-    for (int i = 0; i < 9; ++i) info[1 + i] = v[i];
+    const int v[10] = {X.M, Y.M, Z.M, X.nf, Y.nf, Z.nf, X.L, Y.L, Z.L, W2};
+    for (int i = 0; i < 10; ++i) info[1 + i] = v[i];
   }
-  const size_t nch = (size_t)nsegs;
-  const size_t nG = nch * Z.M * Y.M * X.M, nA = nch * Z.M * Y.M * X.L;
-  const size_t nB = nch * Z.M * Y.L * X.L, nH = nch * Z.L * Y.L * X.L;
-  const size_t n1 = nG > nB ? nG : nB, n2 = nA > nH ? nA : nH;
+  const size_t nG = nch * Z.M * Y.M * X.M;
   const size_t b_dec = align256(sizeof(float) * (X.M + Y.M + Z.M));
   SpreadBins bins;
   bins.nbx = (X.M + kSpreadBlock - 1) / kSpreadBlock;
@@ -1122,8 +1171,9 @@ int nufft_impl(const float *d_hkl, int npix, const spsim_hip_atom *d_atoms, cons
   const size_t b_bin = align256(sizeof(int) * (size_t)bins.nbins);
   const size_t b_tw = align256(sizeof(float2) * (size_t)(X.nf + Y.nf + Z.nf));
   const size_t b_rot = align256(sizeof(double) * 9 * (size_t)npat);
-  const size_t b1 = align256(sizeof(float2) * n1), b2 = align256(sizeof(float2) * n2);
-  if (b1 + b2 > kMaxGridBytes) return fail(SPSIM_HIP_ERR_INVALID, "NUFFT grids need %zu bytes", b1 + b2);
+  size_t b1, b2;
+  const size_t b12 = grid_bytes(g.d, nch, &b1, &b2);
+  if (b12 > kMaxGridBytes) return fail(SPSIM_HIP_ERR_INVALID, "NUFFT grids need %zu bytes", b12);
   // Few output blocks (small grids): split each block's candidates over several
   // workgroups (about 2048 in all, at least 256 atoms per split) and sum their
   // partial grids.
@@ -1162,7 +1212,7 @@ int nufft_impl(const float *d_hkl, int npix, const spsim_hip_atom *d_atoms, cons
 
   // 3. Deconvolution table, atom records binned by spread block, spread.
   const int Mmax = X.M > Y.M ? (X.M > Z.M ? X.M : Z.M) : (Y.M > Z.M ? Y.M : Z.M);
-  hipLaunchKernelGGL(deconv_table, dim3((Mmax + kThreads - 1) / kThreads), dim3(kThreads), 0, s, g, W, dec);
+  hipLaunchKernelGGL(deconv_table, dim3((Mmax + kThreads - 1) / kThreads), dim3(kThreads), 0, s, g, W2, dec);
   SPSIM_HIP_TRY(hipGetLastError());
   const size_t n4 = 2 * b_bin / 16;
   hipLaunchKernelGGL(fill_zero, dim3((unsigned)((n4 + kThreads - 1) / kThreads < 4096 ? (n4 + kThreads - 1) / kThreads : 4096)),
@@ -1206,7 +1256,7 @@ int nufft_impl(const float *d_hkl, int npix, const spsim_hip_atom *d_atoms, cons
   SPSIM_TRY(launch_fft(make_pass(Z, (long long)nch, (long long)Y.L * X.L), buf1, buf2, twz, s));
 
   // 5. Interpolate at every q (of every pattern).
-  SPSIM_TRY(dispatch_w(W, false, g, nullptr, bins, nullptr, nullptr, 1, 0, d_segs, nsegs, nullptr, nullptr, d_hkl,
+  SPSIM_TRY(dispatch_w(W2, false, g, nullptr, bins, nullptr, nullptr, 1, 0, d_segs, nsegs, nullptr, nullptr, d_hkl,
                        npix, buf2, d_ff, b_global, d_rots, nout, (float2 *)d_F, d_I, s));
   if (npat > 0) SPSIM_HIP_TRY(hipStreamSynchronize(s));  // the rotations' host buffer may go after return
   return SPSIM_HIP_OK;

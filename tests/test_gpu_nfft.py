@@ -90,7 +90,8 @@ def test_nfft_vs_exact_detector(gpu, B):
     q = detector_q(48, cx=1e-4, cy=-2e-4)
     F_ref, I_ref = oracle.sf_nfft_f64(q, Z, pos, B=B, threads=oracle_threads())
     F, I, info = run_nfft(gpu, q, Z, pos, B=B)
-    assert info["W"] == 7
+    # one pattern of few q: the FFT dominates, so q interpolates with the spread window
+    assert info["W"] == 7 and info["W2"] == 7 and all(n >= 2 * m for n, m in zip(info["nf"], info["M"]))
     assert rel_l2(I, I_ref) < 1e-5, info
     assert cplx_err(F, F_ref) < 1e-5
 
@@ -293,3 +294,33 @@ def test_nfft_batch_of_orientations(gpu):
         dv.structure_factor_nfft_batch(hkl, atoms, ff, np.full((1, 3, 3), np.inf))
     with pytest.raises(RuntimeError, match="npatterns"):
         dv.structure_factor_nfft_batch(hkl, atoms, ff, np.zeros((0, 9)))
+
+
+def test_nfft_batch_narrow_interpolation_window(gpu):
+    """Many orientations over one grid: the interpolation dominates the cost, so
+    the plan takes the narrow second window (width W - 1 on FFT lines of at least
+    4 M points), with the error of the default one. At W = 6 (eps 1e-5) the
+    window stays."""
+    import torch
+
+    Z, pos = molecule(600, 3e-9, seed=21)
+    q = detector_q(48, lam=0.25e-9)
+    rots = wl.uniform_rotations(24, np.random.default_rng(22))
+    atoms = dv.DeviceAtoms(dv.pack_atoms(Z, pos, mode=_lib.PACK_NFFT), gpu)
+    ff = dv.form_factor_table(device=gpu)
+    hkl = torch.from_numpy(q).to(gpu)
+    F, I, info = dv.structure_factor_nfft_batch(hkl, atoms, ff, rots, B=2.0)
+    torch.cuda.synchronize()
+    assert info["W"] == 7 and info["W2"] == 6, info
+    assert all(n >= 4 * m for n, m in zip(info["nf"], info["M"])), info
+    Fh, Ih = F.cpu().numpy(), I.cpu().numpy()
+    for p in (0, 11, 23):
+        F_ref, I_ref = oracle.sf_nfft_f64(q, Z, pos, B=2.0, R=rots[p], threads=oracle_threads())
+        assert rel_l2(Ih[p], I_ref) < 1e-5, (p, info)
+        assert cplx_err(Fh[p, :, 0] + 1j * Fh[p, :, 1], F_ref) < 1e-5, (p, info)
+    F6, _, info6 = dv.structure_factor_nfft_batch(hkl, atoms, ff, rots[:3], B=2.0, eps=1e-5)
+    torch.cuda.synchronize()
+    assert info6["W"] == 6 and info6["W2"] == 6, info6
+    F6h = F6.cpu().numpy()
+    F_ref, _ = oracle.sf_nfft_f64(q, Z, pos, B=2.0, R=rots[2], threads=oracle_threads())
+    assert cplx_err(F6h[2, :, 0] + 1j * F6h[2, :, 1], F_ref) < 3e-5
