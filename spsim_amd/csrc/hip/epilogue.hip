@@ -6,9 +6,10 @@
 // order (FP64 geometry rounded to FP32, FP32 products left to right, FP32
 // bin sums in (y, x) order) with FMA contraction off, so the deterministic
 // outputs agree with a host evaluation of the same expressions. The stage
-// kernels back the reference-shaped host API; detector_chain_kernel fuses all
-// stages into one pass over the intensities (one lane per binned pixel) and
-// is what the batched pipeline uses. Random draws are addressed by
+// kernels back the reference-shaped host API; chain_binned_runs_kernel (bins
+// of up to 32 raw pixels) and chain_binned_kernel fuse all stages into one
+// pass over the intensities (one lane per binned pixel) and are what the
+// batched pipeline uses. Random draws are addressed by
 // (seed, pixel, pattern), so fused and staged runs produce identical noise.
 #include "epilogue.hpp"
 
@@ -66,20 +67,26 @@ struct RawPixel {
   float th, om, ph, cnt, el;
 };
 
-// Stages of one raw pixel: photons (src/diffraction.c:1202), Poisson count
-// (src/noise.c:42-46), electrons (src/amplification.c:66-68).
-// idx is the raw pixel's flat index (z * ny + y) * nx + x, the Poisson draw's counter.
-__device__ __forceinline__ RawPixel raw_pixel(const DetConst &d, int x, int y, uint32_t idx, float I, uint64_t seed,
-                                              uint32_t pattern) {
+// Stages of one raw pixel from its Thomson factor and solid angle: photons
+// (src/diffraction.c:1202), Poisson count (src/noise.c:42-46), electrons
+// (src/amplification.c:66-68). idx is the raw pixel's flat index
+// (z * ny + y) * nx + x, the Poisson draw's counter.
+__device__ __forceinline__ RawPixel raw_from_geometry(const DetConst &d, float th, float om, uint32_t idx, float I,
+                                                      uint64_t seed, uint32_t pattern) {
 #pragma clang fp contract(off)
   RawPixel r;
-  r.th = thomson_at(d, x, y);
-  r.om = solid_angle_at(d, x, y);
+  r.th = th;
+  r.om = om;
   r.ph = r.th * r.om * I * d.beam_intensity;
   const float mean = r.ph * d.qe;
   r.cnt = (float)poisson_sample((double)mean, seed, idx, pattern);
   r.el = r.cnt * d.e_per_photon;
   return r;
+}
+
+__device__ __forceinline__ RawPixel raw_pixel(const DetConst &d, int x, int y, uint32_t idx, float I, uint64_t seed,
+                                              uint32_t pattern) {
+  return raw_from_geometry(d, thomson_at(d, x, y), solid_angle_at(d, x, y), idx, I, seed, pattern);
 }
 
 __device__ __forceinline__ float readout_noise(const DetConst &d, uint64_t seed, uint32_t bin, uint32_t pattern) {
@@ -138,6 +145,69 @@ __global__ __launch_bounds__(kT) void chain_binned_kernel(DetConst d, const floa
   if (o.noiseless) o.noiseless[ob] = accn;
 }
 
+// The same chain for a run of patterns per lane: the Thomson factor and solid
+// angle of a bin's raw pixels do not depend on the pattern (FP64 geometry,
+// most of a pixel's arithmetic), so each lane forms them once into its own LDS
+// slots, geo[t][lane] and geo[nraw + t][lane], and then takes its bin through
+// patterns [blockIdx.y * run, + run). Same arithmetic and draws as
+// chain_binned_kernel, so the outputs are bitwise those of one pattern at a time.
+constexpr int kRunMaxRaw = 32;  // raw pixels per bin held in LDS (64 KiB per workgroup)
+
+__global__ __launch_bounds__(kT) void chain_binned_runs_kernel(DetConst d, const float *__restrict__ I, int npatterns,
+                                                               int run, uint64_t seed, uint64_t pattern_base,
+                                                               ChainOut o) {
+#pragma clang fp contract(off)
+  extern __shared__ float geo[];
+  const int nbx = d.nx / d.bin_x, nby = d.ny / d.bin_y, nbz = d.nz / d.bin_z;
+  const int nraw = d.bin_x * d.bin_y * d.bin_z;
+  const long long nb = (long long)nbx * nby * nbz;
+  const long long gid = (long long)blockIdx.x * kT + threadIdx.x;
+  if (gid >= nb) return;  // lanes use only their own LDS slots: no barrier
+  const int bin = (int)gid;
+  const int bx = bin % nbx, by = (bin / nbx) % nby, bz = bin / (nbx * nby);
+  {
+    int t = 0;
+    for (int zi = 0; zi < d.bin_z; ++zi)
+      for (int yi = 0; yi < d.bin_y; ++yi)
+        for (int xi = 0; xi < d.bin_x; ++xi, ++t) {
+          const int x = bx * d.bin_x + xi, y = by * d.bin_y + yi;
+          geo[t * kT + threadIdx.x] = thomson_at(d, x, y);
+          geo[(nraw + t) * kT + threadIdx.x] = solid_angle_at(d, x, y);
+        }
+  }
+  const size_t npix = (size_t)d.nx * d.ny * d.nz;
+  const float dark = d.dark_current * d.exposure_time;
+  const int p0 = (int)blockIdx.y * run, p1 = min(npatterns, p0 + run);
+  for (int p = p0; p < p1; ++p) {
+    const uint32_t pat = (uint32_t)(pattern_base + p);
+    const float *Ip = I + (size_t)p * npix;
+    float acc = 0.0f, accn = 0.0f;
+    int t = 0;
+    // bin sums in (z, y, x) order (src/amplification.c:94-102, 139-147)
+    for (int zi = 0; zi < d.bin_z; ++zi) {
+      for (int yi = 0; yi < d.bin_y; ++yi) {
+        for (int xi = 0; xi < d.bin_x; ++xi, ++t) {
+          const int x = bx * d.bin_x + xi, y = by * d.bin_y + yi, z = bz * d.bin_z + zi;
+          const size_t i = ((size_t)z * d.ny + y) * d.nx + x;
+          const RawPixel r = raw_from_geometry(d, geo[t * kT + threadIdx.x], geo[(nraw + t) * kT + threadIdx.x],
+                                               (uint32_t)i, Ip[i], seed, pat);
+          acc += (r.el + dark);
+          accn += r.ph * d.qe * d.e_per_photon_noiseless * d.adc;
+          const size_t oi = (size_t)p * npix + i;
+          if (o.th) o.th[oi] = r.th;
+          if (o.om) o.om[oi] = r.om;
+          if (o.ph) o.ph[oi] = r.ph;
+          if (o.cnt) o.cnt[oi] = r.cnt;
+          if (o.el) o.el[oi] = r.el;
+        }
+      }
+    }
+    const size_t ob = (size_t)p * nb + bin;
+    if (o.real) o.real[ob] = finish_real(d, acc, seed, (uint32_t)bin, pat);
+    if (o.noiseless) o.noiseless[ob] = accn;
+  }
+}
+
 // Raw outputs for pixels outside complete bins (only when nx, ny or nz is not
 // a multiple of the binning and raw outputs were requested).
 __global__ __launch_bounds__(kT) void chain_leftover_kernel(DetConst d, const float *__restrict__ I, int npatterns,
@@ -149,7 +219,7 @@ __global__ __launch_bounds__(kT) void chain_leftover_kernel(DetConst d, const fl
   const size_t i = (size_t)(gid % (long long)npix);
   const int x = (int)(i % d.nx), y = (int)((i / d.nx) % d.ny), z = (int)(i / ((size_t)d.nx * d.ny));
   const int nbx = d.nx / d.bin_x, nby = d.ny / d.bin_y, nbz = d.nz / d.bin_z;
-  if (x < nbx * d.bin_x && y < nby * d.bin_y && z < nbz * d.bin_z) return;  // covered by chain_binned_kernel
+  if (x < nbx * d.bin_x && y < nby * d.bin_y && z < nbz * d.bin_z) return;  // covered by the binned kernels
   const RawPixel r = raw_pixel(d, x, y, (uint32_t)i, I[(size_t)p * npix + i], seed, (uint32_t)(pattern_base + p));
   const size_t oi = (size_t)p * npix + i;
   if (o.th) o.th[oi] = r.th;
@@ -312,7 +382,20 @@ int launch_chain(const DetConst &d, const float *I, int npatterns, uint64_t seed
   if (npatterns <= 0) return SPSIM_HIP_OK;
   ChainOut o{th, om, ph, cnt, el, real, noiseless};
   const long long nb = (long long)(d.nx / d.bin_x) * (d.ny / d.bin_y) * (d.nz / d.bin_z);
-  if (nb > 0) {
+  const int nraw = d.bin_x * d.bin_y * d.bin_z;
+  if (nb > 0 && nraw <= kRunMaxRaw) {
+    // runs of patterns per lane: about 2048 workgroups in all
+    const long long gx = (nb + kT - 1) / kT;
+    long long groups = (2048 + gx - 1) / gx;
+    if (groups > npatterns) groups = npatterns;
+    const int run = (int)((npatterns + groups - 1) / groups);
+    groups = (npatterns + run - 1) / run;
+    if (gx > 0x7fffffffLL) return fail(SPSIM_HIP_ERR_INVALID, "too many bins");
+    const size_t lds = 2 * (size_t)nraw * kT * sizeof(float);
+    hipLaunchKernelGGL(chain_binned_runs_kernel, dim3((unsigned)gx, (unsigned)groups), dim3(kT), lds, s, d, I,
+                       npatterns, run, seed, pattern_base, o);
+    SPSIM_HIP_TRY(hipGetLastError());
+  } else if (nb > 0) {
     hipLaunchKernelGGL(chain_binned_kernel, dim3(blocks_for(nb * npatterns)), dim3(kT), 0, s, d, I, npatterns, seed,
                        pattern_base, o);
     SPSIM_HIP_TRY(hipGetLastError());

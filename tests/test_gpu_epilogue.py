@@ -50,19 +50,43 @@ def test_geometry_stages(gpu, pol):
     assert np.mean(o["solid_angle"][0] == om) > 0.95
 
 
-def test_deterministic_stages_exact(gpu):
-    p = params()
+@pytest.mark.parametrize("bins", [(4, 4), (1, 1), (8, 8)])
+def test_deterministic_stages_exact(gpu, bins):
+    """Bins of up to 32 raw pixels run the kernel that keeps the geometry in LDS,
+    larger ones (8 x 8) the one-pattern-per-lane kernel."""
+    bx, by = bins
+    p = params(bx=bx, by=by)
     rng = np.random.default_rng(0)
     I = rng.gamma(0.5, 2e6, p["nx"] * p["ny"]).astype(np.float32)
     o = run_chain(gpu, p, I)
+    th = oracle.thomson(p["nx"], p["ny"], p["width"], p["height"], p["distance"], p["center_x"], p["center_y"], 0)
+    assert np.array_equal(o["thomson"][0], th)
     ph = oracle.photons(o["thomson"][0], o["solid_angle"][0], I, p["beam_intensity"])
     assert np.array_equal(o["photons"][0], ph)
     epp = oracle.electrons_per_photon(p["wavelength"], p["electron_hole_production_energy"])
     np.testing.assert_array_equal(o["electrons"][0], o["count"][0] * np.float32(epp))
     epp_nl = oracle.electrons_per_photon(p["wavelength"], p["electron_hole_production_energy"], noiseless=True)
-    nl = oracle.noiseless_output(ph, p["nx"], p["ny"], 4, 4, p["quantum_efficiency"], epp_nl, p["linear_full_well"],
-                                 p["maximum_value"])
+    nl = oracle.noiseless_output(ph, p["nx"], p["ny"], bx, by, p["quantum_efficiency"], epp_nl,
+                                 p["linear_full_well"], p["maximum_value"])
     assert np.array_equal(o["noiseless_output"][0], nl)
+
+
+@pytest.mark.parametrize("bins", [(2, 2), (8, 8)])
+def test_many_patterns_match_one_at_a_time(gpu, bins):
+    """One call over 37 patterns gives bitwise the outputs of 37 one-pattern calls
+    (pattern_base + p). With 2 x 2 bins on 256 x 256 pixels each lane takes its bin
+    through runs of 2 patterns, the last run holding one."""
+    bx, by = bins
+    n, npat = 256, 37
+    p = params(nx=n, ny=n, bx=bx, by=by, beam_intensity=1e30)
+    rng = np.random.default_rng(17)
+    I = (10.0 ** rng.uniform(3.0, 7.5, (npat, n * n))).astype(np.float32)
+    seed, base = 11, 5
+    o = run_chain(gpu, p, I.ravel(), seed=seed, base=base, npatterns=npat)
+    for q in (0, 1, 2, 18, 35, 36):
+        o1 = run_chain(gpu, p, I[q], seed=seed, base=base + q)
+        for k in o1:
+            assert np.array_equal(o[k][q], o1[k][0]), (k, q)
 
 
 def test_random_draws_match_oracle_counters(gpu):
