@@ -187,9 +187,11 @@ int spsim_hip_sf_spectrum(const float *d_hkl, int npix, const spsim_hip_atom *d_
    points or grids above 64 GiB are rejected. Each q interpolates with a second
    window: the spread window itself (FFT lines >= 2 M), or, for W >= 7 when that
    costs less (many q per grid), a window of width W - 1 on lines >= 4 M, which
-   keeps the error. info (host, may be NULL) receives 11 ints
-   {W, Mx, My, Mz, nfx, nfy, nfz, Lx, Ly, Lz, W2}: spread window width, spread
-   grid, FFT and kept-output sizes per axis, interpolation window width. Uses
+   keeps the error. info (host, may be NULL) receives 12 ints
+   {W, Mx, My, Mz, nfx, nfy, nfz, Lx, Ly, Lz, W2, staged}: spread window width,
+   spread grid, FFT and kept-output sizes per axis, interpolation window width,
+   and 1 when the interpolation staged each wave's grid box in LDS (lists whose
+   consecutive q are close, such as detector rows). Uses
    the library workspace; synchronises `stream` once (to read the bounds that
    size the grids), the rest is stream-ordered. */
 int spsim_hip_sf_nfft(const float *d_hkl, int npix, const spsim_hip_atom *d_atoms, const spsim_hip_segment *d_segs,

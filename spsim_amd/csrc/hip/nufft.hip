@@ -96,6 +96,7 @@ struct NufftGeom {
   // u = 2 (xi / corr_xmax)^2 - 1 (monomial coefficients, constant first);
   // xi = t W / 2 = t * corr_scale * corr_xmax for a target at t = s' h
   float corr_xmax, corr_scale;
+  int lds_interp;  // host: interpolate with nufft_interp_lds (else nufft_interp)
   float corr[kCorrDeg + 1];
 };
 
@@ -707,12 +708,105 @@ __global__ __launch_bounds__(kThreads) void fft_lines(FftPass p, const float2 *_
   }
 }
 
+// Per-q state of the interpolation: the (rotated) q in FP64, the window
+// weights and first grid point per axis, the first window's correction.
+template <int W>
+struct InterpQ {
+  double q0[3], q[3];
+  float wt[3][W];
+  int base[3];
+  float corr;
+  bool ok;
+};
+
+// Pattern pat interpolates at q' = R_pat q (rots: row-major [p][9], FP64);
+// rots == nullptr means q' = q. ok is false for a q outside the grids
+// (unreachable for q inside the bounds the grids were sized for).
+template <int W>
+__device__ __forceinline__ void interp_prepare(const float *__restrict__ hkl, int k, const NufftGeom &g,
+                                               const double *__restrict__ rots, int pat, InterpQ<W> &s) {
+  for (int i = 0; i < 3; ++i) s.q0[i] = s.q[i] = hkl[3 * (size_t)k + i];
+  if (rots) {
+    const double *R = rots + 9 * (size_t)pat;
+    for (int i = 0; i < 3; ++i)
+      s.q[i] = __builtin_fma(R[3 * i + 2], s.q0[2], __builtin_fma(R[3 * i + 1], s.q0[1], R[3 * i] * s.q0[0]));
+  }
+  s.corr = 1.0f;
+  s.ok = true;
+  for (int d = 0; d < 3; ++d) {
+    const NufftDim &D = g.d[d];
+    const double t = (s.q[d] - D.cs) * D.h;
+    const double v = t * D.nf;
+    const double l0 = ceil(v - 0.5 * W);
+    const double i0 = l0 - D.l_lo;
+    s.ok = s.ok && (i0 >= 0.0 && i0 + W <= (double)D.L);
+    s.base[d] = s.ok ? (int)i0 : 0;
+    const float fb = (float)(l0 - v);
+#pragma unroll
+    for (int i = 0; i < W; ++i) s.wt[d][i] = es_window((fb + (float)i) * (2.0f / W), g.beta2);
+    const float xr = (float)t * g.corr_scale;
+    const float u = 2.0f * xr * xr - 1.0f;
+    float c = g.corr[kCorrDeg];
+#pragma unroll
+    for (int j = kCorrDeg - 1; j >= 0; --j) c = __builtin_fmaf(c, u, g.corr[j]);
+    s.corr *= c;
+  }
+}
+
+// The separable window sum over one species' W^3 values; row(iz, iy) points at
+// the W values of an x-run. kRollZ keeps the z loop rolled (fewer loads in
+// flight, for LDS rows whose latency is short).
+template <int W, bool kRollZ = false, typename Row>
+__device__ __forceinline__ void window_sum(const InterpQ<W> &s, Row row, float &ar, float &ai) {
+  ar = ai = 0.0f;
+#pragma unroll(kRollZ ? 1 : W)
+  for (int iz = 0; iz < W; ++iz) {
+    for (int iy = 0; iy < W; ++iy) {
+      const float2 *r = row(iz, iy);
+      float rr = 0.0f, ri = 0.0f;
+#pragma unroll
+      for (int ix = 0; ix < W; ++ix) {
+        const float2 h = r[ix];
+        rr += h.x * s.wt[0][ix];
+        ri += h.y * s.wt[0][ix];
+      }
+      const float wzy = s.wt[2][iz] * s.wt[1][iy];
+      ar += wzy * rr;
+      ai += wzy * ri;
+    }
+  }
+}
+
+// Correction, q phase and the stores of F and I for one q.
+template <int W>
+__device__ __forceinline__ void interp_store(const NufftGeom &g, const InterpQ<W> &s, float fr, float fi, size_t out,
+                                             float2 *__restrict__ F, float *__restrict__ I) {
+  float Fr, Fi;
+  if (!s.ok) {
+    Fr = Fi = __builtin_nanf("");
+  } else {
+    fr *= s.corr;
+    fi *= s.corr;
+    double ph = -(s.q[0] * g.d[0].cx + s.q[1] * g.d[1].cx + s.q[2] * g.d[2].cx);
+    ph -= rint(ph);
+    float sn, cs;
+    sincospif((float)(2.0 * ph), &sn, &cs);
+    Fr = fr * cs - fi * sn;
+    Fi = fr * sn + fi * cs;
+  }
+  if (F) F[out] = make_float2(Fr, Fi);
+  if (I) I[out] = Fr * Fr + Fi * Fi;
+}
+
+__device__ __forceinline__ float q_s2(const double q0[3]) {
+  return (float)((q0[0] * q0[0] + q0[1] * q0[1] + q0[2] * q0[2]) * 2.5e-21);
+}
+
 // One thread per q: W^3 gathers (W: the interpolation window's width) per
-// species grid [ch][Lz][Ly][Lx], window
-// correction, form factor, species sum and the q phase.
-// Patterns of a batch (one species grid set, many orientations) are
-// blockIdx.y: pattern p interpolates at q' = R_p q (rots: row-major [p][9],
-// FP64) and writes F + p npix, I + p npix; rots == nullptr means q' = q.
+// species grid [ch][Lz][Ly][Lx] from global memory, window correction, form
+// factor, species sum and the q phase. Patterns of a batch (one species grid
+// set, many orientations) are blockIdx.y: pattern p interpolates at R_p q and
+// writes F + p npix, I + p npix.
 template <int W>
 __global__ __launch_bounds__(kThreads) void nufft_interp(const float *__restrict__ hkl, int npix, NufftGeom g,
                                                         const float2 *__restrict__ H, int nch,
@@ -723,78 +817,102 @@ __global__ __launch_bounds__(kThreads) void nufft_interp(const float *__restrict
   const int k = blockIdx.x * kThreads + threadIdx.x;
   if (k >= npix) return;
   const int pat = p0 + (int)blockIdx.y;
-  const double q0[3] = {hkl[3 * (size_t)k], hkl[3 * (size_t)k + 1], hkl[3 * (size_t)k + 2]};
-  double q[3] = {q0[0], q0[1], q0[2]};
-  if (rots) {
-    const double *R = rots + 9 * (size_t)pat;
-    for (int i = 0; i < 3; ++i) q[i] = __builtin_fma(R[3 * i + 2], q0[2], __builtin_fma(R[3 * i + 1], q0[1], R[3 * i] * q0[0]));
-  }
-  const size_t out = (size_t)pat * npix + k;
-  float wt[3][W];
-  int base[3];
-  float corr = 1.0f;
-  bool ok = true;
-  for (int d = 0; d < 3; ++d) {
-    const NufftDim &D = g.d[d];
-    const double t = (q[d] - D.cs) * D.h;
-    const double v = t * D.nf;
-    const double l0 = ceil(v - 0.5 * W);
-    const double i0 = l0 - D.l_lo;
-    ok = ok && (i0 >= 0.0 && i0 + W <= (double)D.L);
-    base[d] = ok ? (int)i0 : 0;
-    const float fb = (float)(l0 - v);
-#pragma unroll
-    for (int i = 0; i < W; ++i) wt[d][i] = es_window((fb + (float)i) * (2.0f / W), g.beta2);
-    const float xr = (float)t * g.corr_scale;
-    const float u = 2.0f * xr * xr - 1.0f;
-    float c = g.corr[kCorrDeg];
-#pragma unroll
-    for (int j = kCorrDeg - 1; j >= 0; --j) c = __builtin_fmaf(c, u, g.corr[j]);
-    corr *= c;
-  }
-  float Fr, Fi;
-  if (!ok) {
-    // Unreachable for q inside the bounds the grids were sized for.
-    Fr = Fi = __builtin_nanf("");
-  } else {
-    const float s2 = (float)((q0[0] * q0[0] + q0[1] * q0[1] + q0[2] * q0[2]) * 2.5e-21);
+  InterpQ<W> s;
+  interp_prepare<W>(hkl, k, g, rots, pat, s);
+  float fr = 0.0f, fi = 0.0f;
+  if (s.ok) {
+    const float s2 = q_s2(s.q0);
     const int Lx = g.d[0].L, Ly = g.d[1].L, Lz = g.d[2].L;
     const size_t chs = (size_t)Lz * Ly * Lx;
-    float fr = 0.0f, fi = 0.0f;
     for (int ch = 0; ch < nch; ++ch) {
       const int sp = segs[ch].species;
       const float f = form_factor(ff + sp * SPSIM_HIP_FF_STRIDE, s2, sp == SPSIM_HIP_UNIT_SPECIES ? 0.0f : b_global);
       const float2 *Hc = H + ch * chs;
-      float ar = 0.0f, ai = 0.0f;
-      for (int iz = 0; iz < W; ++iz) {
-        for (int iy = 0; iy < W; ++iy) {
-          const float2 *row = Hc + ((size_t)(base[2] + iz) * Ly + base[1] + iy) * Lx + base[0];
-          float rr = 0.0f, ri = 0.0f;
-#pragma unroll
-          for (int ix = 0; ix < W; ++ix) {
-            const float2 h = row[ix];
-            rr += h.x * wt[0][ix];
-            ri += h.y * wt[0][ix];
-          }
-          const float wzy = wt[2][iz] * wt[1][iy];
-          ar += wzy * rr;
-          ai += wzy * ri;
-        }
-      }
+      float ar, ai;
+      window_sum<W>(s, [&](int iz, int iy) { return Hc + ((size_t)(s.base[2] + iz) * Ly + s.base[1] + iy) * Lx + s.base[0]; },
+                    ar, ai);
       fr += f * ar;
       fi += f * ai;
     }
-    fr *= corr;
-    fi *= corr;
-    double ph = -(q[0] * g.d[0].cx + q[1] * g.d[1].cx + q[2] * g.d[2].cx);
-    ph -= rint(ph);
-    float sn, cs;
-    sincospif((float)(2.0 * ph), &sn, &cs);
-    Fr = fr * cs - fi * sn;
-    Fi = fr * sn + fi * cs;
   }
-  if (F) F[out] = make_float2(Fr, Fi);
-  if (I) I[out] = Fr * Fr + Fi * Fi;
+  interp_store<W>(g, s, fr, fi, (size_t)pat * npix + k, F, I);
+}
+
+// The same interpolation with each wave's footprint staged in LDS, one
+// species at a time: neighbouring q (consecutive pixels) share most of their
+// windows, so a wave's W^3 x 64 gathers touch a small box of grid points.
+// The wave's lanes copy that box (the span of their first grid points plus W
+// per axis, at most kBoxCap points) from global memory into their own LDS
+// region and then gather from LDS, which serves twice the bytes per clock of
+// the vector L1. A wave whose box is larger gathers from global memory.
+constexpr int kBoxCap = 1024;  // grid points per wave (8 KiB of LDS per species)
+
+template <int W>
+__global__ __launch_bounds__(kThreads) void nufft_interp_lds(const float *__restrict__ hkl, int npix, NufftGeom g,
+                                                            const float2 *__restrict__ H, int nch,
+                                                            const spsim_hip_segment *__restrict__ segs,
+                                                            const float *__restrict__ ff, float b_global,
+                                                            const double *__restrict__ rots, int p0,
+                                                            float2 *__restrict__ F, float *__restrict__ I) {
+  __shared__ float2 box[kThreads / 64][kBoxCap];
+  const int k = blockIdx.x * kThreads + threadIdx.x;
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int pat = p0 + (int)blockIdx.y;
+  InterpQ<W> s;
+  if (k < npix) {
+    interp_prepare<W>(hkl, k, g, rots, pat, s);
+  } else {
+    s.ok = false;
+  }
+  // the wave's box: [lo, hi + W) per axis over its lanes with ok
+  int lo[3], hi[3];
+  for (int d = 0; d < 3; ++d) {
+    lo[d] = s.ok ? s.base[d] : 0x7fffffff;
+    hi[d] = s.ok ? s.base[d] : -1;
+  }
+  for (int m = 32; m >= 1; m >>= 1)
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = min(lo[d], __shfl_xor(lo[d], m, 64));
+      hi[d] = max(hi[d], __shfl_xor(hi[d], m, 64));
+    }
+  const int ex = hi[0] - lo[0] + W, ey = hi[1] - lo[1] + W, ez = hi[2] - lo[2] + W;
+  // wave-uniform: some lane has ok (hi >= 0) and the box fits
+  const bool staged = hi[0] >= 0 && (long long)ex * ey * ez <= kBoxCap;
+  const int vol = staged ? ex * ey * ez : 0;
+  const int lb = staged && s.ok ? ((s.base[2] - lo[2]) * ey + (s.base[1] - lo[1])) * ex + (s.base[0] - lo[0]) : 0;
+  const float rex = 1.0f / (float)ex, rexy = 1.0f / (float)(ex * ey);
+  const float s2 = k < npix ? q_s2(s.q0) : 0.0f;
+  const int Lx = g.d[0].L, Ly = g.d[1].L, Lz = g.d[2].L;
+  const size_t chs = (size_t)Lz * Ly * Lx;
+  float2 *B = box[wave];
+  float fr = 0.0f, fi = 0.0f;
+  for (int ch = 0; ch < nch; ++ch) {
+    const float2 *Hc = H + ch * chs;
+    // i = (iz ey + iy) ex + ix < vol <= 1024: the float quotients are exact
+    for (int i = lane; i < vol; i += 64) {
+      const int iz = (int)(((float)i + 0.5f) * rexy);
+      const int r = i - iz * ex * ey;
+      const int iy = (int)(((float)r + 0.5f) * rex);
+      const int ix = r - iy * ex;
+      B[i] = Hc[((size_t)(lo[2] + iz) * Ly + lo[1] + iy) * Lx + lo[0] + ix];
+    }
+    __syncthreads();
+    if (s.ok) {
+      const int sp = segs[ch].species;
+      const float f = form_factor(ff + sp * SPSIM_HIP_FF_STRIDE, s2, sp == SPSIM_HIP_UNIT_SPECIES ? 0.0f : b_global);
+      float ar, ai;
+      if (staged)
+        window_sum<W, true>(s, [&](int iz, int iy) { return B + lb + (iz * ey + iy) * ex; }, ar, ai);
+      else
+        window_sum<W, true>(
+            s, [&](int iz, int iy) { return Hc + ((size_t)(s.base[2] + iz) * Ly + s.base[1] + iy) * Lx + s.base[0]; },
+            ar, ai);
+      fr += f * ar;
+      fi += f * ai;
+    }
+    __syncthreads();
+  }
+  if (k < npix) interp_store<W>(g, s, fr, fi, (size_t)pat * npix + k, F, I);
 }
 
 __global__ __launch_bounds__(kThreads) void zero_pattern(size_t n, float2 *__restrict__ F, float *__restrict__ I) {
@@ -1008,8 +1126,12 @@ int launch_spread_interp(bool spread, const NufftGeom &g, const SpreadAtom *rec,
   } else {
     for (int p0 = 0; p0 < npat; p0 += 65535) {
       const int n = npat - p0 < 65535 ? npat - p0 : 65535;
-      hipLaunchKernelGGL(nufft_interp<W>, dim3((npix + kThreads - 1) / kThreads, n), dim3(kThreads), 0, stream, hkl,
-                         npix, g, H, nsegs, segs, ff, b_global, rots, p0, F, I);
+      if (g.lds_interp)
+        hipLaunchKernelGGL(nufft_interp_lds<W>, dim3((npix + kThreads - 1) / kThreads, n), dim3(kThreads), 0, stream,
+                           hkl, npix, g, H, nsegs, segs, ff, b_global, rots, p0, F, I);
+      else
+        hipLaunchKernelGGL(nufft_interp<W>, dim3((npix + kThreads - 1) / kThreads, n), dim3(kThreads), 0, stream, hkl,
+                           npix, g, H, nsegs, segs, ff, b_global, rots, p0, F, I);
       SPSIM_HIP_TRY(hipGetLastError());
     }
   }
@@ -1084,7 +1206,7 @@ int nufft_impl(const float *d_hkl, int npix, const spsim_hip_atom *d_atoms, cons
   for (size_t i = 0; i < (size_t)npat * 9; ++i)
     if (!std::isfinite(rots[i])) return fail(SPSIM_HIP_ERR_INVALID, "non-finite rotation");
   if (info)
-    for (int i = 0; i < 11; ++i) info[i] = 0;
+    for (int i = 0; i < 12; ++i) info[i] = 0;
   if (info) info[0] = W;
   const int nout = npat > 0 ? npat : 1;
   if (npix == 0) return SPSIM_HIP_OK;
@@ -1156,9 +1278,19 @@ int nufft_impl(const float *d_hkl, int npix, const spsim_hip_atom *d_atoms, cons
   g.corr_scale = (float)(0.5 * W / g.corr_xmax);
   fit_correction(W, g.beta1, g.corr_xmax, g.corr);
   const NufftDim &X = g.d[0], &Y = g.d[1], &Z = g.d[2];
+  // Interpolation with staged boxes when a wave's 64 consecutive q are
+  // expected to fit one: for a detector-shaped list of rows of about sqrt(npix)
+  // pixels that span the kept grid, 64 pixels span 64 (L - W2) / sqrt(npix)
+  // cells, a box of about (span + W2) (W2 + 1)^2 points.
+  {
+    int Lmax = X.L > Y.L ? X.L : Y.L;
+    if (Z.L > Lmax) Lmax = Z.L;
+    const double span = 64.0 * (Lmax - W2) / sqrt((double)npix);
+    g.lds_interp = (span + W2) * (W2 + 1) * (W2 + 1) <= kBoxCap ? 1 : 0;
+  }
   if (info) {
-    const int v[10] = {X.M, Y.M, Z.M, X.nf, Y.nf, Z.nf, X.L, Y.L, Z.L, W2};
-    for (int i = 0; i < 10; ++i) info[1 + i] = v[i];
+    const int v[11] = {X.M, Y.M, Z.M, X.nf, Y.nf, Z.nf, X.L, Y.L, Z.L, W2, g.lds_interp};
+    for (int i = 0; i < 11; ++i) info[1 + i] = v[i];
   }
   const size_t nG = nch * Z.M * Y.M * X.M;
   const size_t b_dec = align256(sizeof(float) * (X.M + Y.M + Z.M));

@@ -324,3 +324,34 @@ def test_nfft_batch_narrow_interpolation_window(gpu):
     F6h = F6.cpu().numpy()
     F_ref, _ = oracle.sf_nfft_f64(q, Z, pos, B=2.0, R=rots[2], threads=oracle_threads())
     assert cplx_err(F6h[2, :, 0] + 1j * F6h[2, :, 1], F_ref) < 3e-5
+
+
+def test_nfft_staged_interpolation_on_a_large_detector(gpu):
+    """A 512 x 512 detector over a small molecule: 64 consecutive pixels of a row
+    share most of their windows, so the interpolation stages each wave's grid box
+    in LDS. One pattern and a batch of orientations match the FP64 oracle on a
+    pixel sample, and the batch agrees with one call per orientation."""
+    import torch
+
+    Z, pos = molecule(500, 3e-9, seed=31)
+    q = detector_q(512, lam=0.25e-9, pix=2e-5)
+    rng = np.random.default_rng(32)
+    sample = np.sort(rng.choice(q.shape[0], 1536, replace=False))
+    F, I, info = run_nfft(gpu, q, Z, pos, B=2.0)
+    assert info["staged"] == 1, info
+    F_ref, I_ref = oracle.sf_nfft_f64(q[sample], Z, pos, B=2.0, threads=oracle_threads())
+    assert rel_l2(I[sample], I_ref) < 1e-5, info
+    assert cplx_err(F[sample], F_ref) < 1e-5, info
+    rots = wl.uniform_rotations(3, np.random.default_rng(33))
+    atoms = dv.DeviceAtoms(dv.pack_atoms(Z, pos, mode=_lib.PACK_NFFT), gpu)
+    ff = dv.form_factor_table(device=gpu)
+    Fb, _, infob = dv.structure_factor_nfft_batch(torch.from_numpy(q).to(gpu), atoms, ff, rots, B=2.0)
+    torch.cuda.synchronize()
+    assert infob["staged"] == 1, infob
+    Fbh = Fb.cpu().numpy()
+    for p, R in enumerate(rots):
+        Fp = Fbh[p, :, 0] + 1j * Fbh[p, :, 1]
+        F_ref, _ = oracle.sf_nfft_f64(q[sample], Z, pos, B=2.0, R=R, threads=oracle_threads())
+        assert cplx_err(Fp[sample], F_ref) < 1e-5, (p, infob)
+        F1, _, _ = run_nfft(gpu, q, Z, pos, B=2.0, R=R)
+        assert cplx_err(Fp, F1) < 3e-6
