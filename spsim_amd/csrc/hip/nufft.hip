@@ -1306,12 +1306,12 @@ int nufft_impl(const float *d_hkl, int npix, const spsim_hip_atom *d_atoms, cons
   size_t b1, b2;
   const size_t b12 = grid_bytes(g.d, nch, &b1, &b2);
   if (b12 > kMaxGridBytes) return fail(SPSIM_HIP_ERR_INVALID, "NUFFT grids need %zu bytes", b12);
-  // Few output blocks (small grids): split each block's candidates over several
-  // workgroups (about 2048 in all, at least 256 atoms per split) and sum their
-  // partial grids.
+  // Few output blocks for the atoms (small or thin grids): split each block's
+  // candidates over several one-wave workgroups (about 8192 in all, 8 waves
+  // per SIMD; at most one split per 256 atoms) and sum their partial grids.
   int nsplit = 1;
-  if (bins.nbins < 2048) {
-    nsplit = (int)(2048 / bins.nbins);
+  if (bins.nbins < 8192) {
+    nsplit = (int)(8192 / bins.nbins);
     const int by_atoms = (natoms + 255) / 256;
     if (nsplit > by_atoms) nsplit = by_atoms;
     if (nsplit > 64) nsplit = 64;
