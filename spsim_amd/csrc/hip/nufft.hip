@@ -36,6 +36,7 @@
 // Atom contributions reach the grid through FP32 atomics, so the last bits
 // of F can vary from run to run.
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "common.hpp"
@@ -1018,6 +1019,46 @@ void fit_correction(int W, double beta, double xmax, float *out) {
   for (int i = 0; i <= kCorrDeg; ++i) out[i] = (float)m[i];
 }
 
+// Window constants that depend only on the widths: the quadrature of the
+// second window's transform and the first window's correction polynomial.
+// Formed once per (W, W2) on first use (FP64 quadrature and fit).
+struct WindowTables {
+  float beta1, beta2;
+  int nq;
+  float qz[kMaxQuad], qc[kMaxQuad];
+  float corr_xmax, corr_scale;
+  float corr[kCorrDeg + 1];
+};
+
+void fill_window_tables(int W, int W2, WindowTables *t) {
+  t->beta1 = (float)(2.30 * W);
+  t->beta2 = W2 == W ? t->beta1 : (float)(2.65 * W2);
+  t->nq = 2 * W + 6;
+  double z[kMaxQuad], w[kMaxQuad];
+  gauss_legendre01(t->nq, z, w);
+  for (int i = 0; i < t->nq; ++i) {
+    t->qz[i] = (float)z[i];
+    t->qc[i] = (float)(2.0 * w[i] * exp((double)t->beta2 * (sqrt(1.0 - z[i] * z[i]) - 1.0)));
+  }
+  // targets have |t| <= 1/(2 sigma) per axis, so |xi| = |t| W / 2 <= W / 8 (2 % margin)
+  t->corr_xmax = (float)(1.02 * W / (4.0 * kSigma));
+  t->corr_scale = (float)(0.5 * W / t->corr_xmax);
+  fit_correction(W, t->beta1, t->corr_xmax, t->corr);
+}
+
+// W in [kMinW, kMaxW]; W2 is W, or W - 1 for W >= 7.
+const WindowTables &window_tables(int W, int W2) {
+  static WindowTables tab[kMaxW + 1][2];
+  static std::once_flag once;
+  std::call_once(once, [] {
+    for (int w = kMinW; w <= kMaxW; ++w) {
+      fill_window_tables(w, w, &tab[w][0]);
+      if (w >= 7) fill_window_tables(w, w - 1, &tab[w][1]);
+    }
+  });
+  return tab[W][W2 == W ? 0 : 1];
+}
+
 int width_for(double eps) {
   if (!(eps > 0.0)) eps = 1e-6;
   int W = (int)ceil(-log10(eps)) + 1;
@@ -1262,21 +1303,19 @@ int nufft_impl(const float *d_hkl, int npix, const spsim_hip_atom *d_atoms, cons
       W2 = W - 1;
     }
   }
-  g.beta1 = (float)(2.30 * W);
-  g.beta2 = W2 == W ? g.beta1 : (float)(2.65 * W2);
-  g.nq = 2 * W + 6;
   {
-    double z[kMaxQuad], w[kMaxQuad];
-    gauss_legendre01(g.nq, z, w);
-    for (int i = 0; i < g.nq; ++i) {
-      g.qz[i] = (float)z[i];
-      g.qc[i] = (float)(2.0 * w[i] * exp((double)g.beta2 * (sqrt(1.0 - z[i] * z[i]) - 1.0)));
+    const WindowTables &t = window_tables(W, W2);
+    g.beta1 = t.beta1;
+    g.beta2 = t.beta2;
+    g.nq = t.nq;
+    for (int i = 0; i < t.nq; ++i) {
+      g.qz[i] = t.qz[i];
+      g.qc[i] = t.qc[i];
     }
+    g.corr_xmax = t.corr_xmax;
+    g.corr_scale = t.corr_scale;
+    for (int i = 0; i <= kCorrDeg; ++i) g.corr[i] = t.corr[i];
   }
-  // targets have |t| <= 1/(2 sigma) per axis, so |xi| = |t| W / 2 <= W / 8 (2 % margin)
-  g.corr_xmax = (float)(1.02 * W / (4.0 * kSigma));
-  g.corr_scale = (float)(0.5 * W / g.corr_xmax);
-  fit_correction(W, g.beta1, g.corr_xmax, g.corr);
   const NufftDim &X = g.d[0], &Y = g.d[1], &Z = g.d[2];
   // Interpolation with staged boxes when a wave's 64 consecutive q are
   // expected to fit one: for a detector-shaped list of rows of about sqrt(npix)
