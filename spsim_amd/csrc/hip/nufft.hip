@@ -142,6 +142,7 @@ __device__ __forceinline__ double nan_max(double a, double b) { return (a != a |
 // the largest |q|^2 (12): part[blk * kNb + v], v < 6 minima, 6..11 maxima.
 // NaN propagates.
 constexpr int kNb = 14;
+constexpr int kBoundBlocks = 256;  // workgroups of bounds_partial (records folded on the host)
 
 __device__ __forceinline__ double bound_combine(int v, double a, double b) {
   return v < 6 ? nan_min(a, b) : nan_max(a, b);
@@ -183,43 +184,33 @@ __global__ __launch_bounds__(kThreads) void bounds_partial(const spsim_hip_atom 
   if (threadIdx.x < kNb) part[(size_t)blockIdx.x * kNb + threadIdx.x] = red[threadIdx.x][0];
 }
 
-// One workgroup folds nblk partial records into out[kNb].
-__global__ __launch_bounds__(kThreads) void bounds_final(const double *__restrict__ part, int nblk,
-                                                        double *__restrict__ out) {
-  __shared__ double red[kNb][kThreads];
-  for (int v = 0; v < kNb; ++v) {
-    double acc = v < 6 ? INFINITY : -INFINITY;
-    for (int b = threadIdx.x; b < nblk; b += kThreads) acc = bound_combine(v, acc, part[(size_t)b * kNb + v]);
-    red[v][threadIdx.x] = acc;
-  }
-  __syncthreads();
-  for (int w = kThreads / 2; w > 0; w >>= 1) {
-    if (threadIdx.x < w)
-      for (int v = 0; v < kNb; ++v) red[v][threadIdx.x] = bound_combine(v, red[v][threadIdx.x], red[v][threadIdx.x + w]);
-    __syncthreads();
-  }
-  if (threadIdx.x < kNb) out[threadIdx.x] = red[threadIdx.x][0];
-}
-
-// D_m = 2 / (W phi2_hat(m W / (2 nf))) (W: the interpolation window's width)
-// for the spread grid of every axis,
-// stored back to back: dec[0 .. Mx) | dec[Mx .. Mx+My) | dec[.. + Mz).
-__global__ __launch_bounds__(kThreads) void deconv_table(NufftGeom g, int W, float *__restrict__ dec) {
+// The per-call tables in one launch, thread i filling entry i of each:
+//  - the deconvolution table D_m = 2 / (W phi2_hat(m W / (2 nf))) (W: the
+//    interpolation window's width) for the spread grid of every axis, stored
+//    back to back: dec[0 .. Mx) | dec[Mx .. Mx+My) | dec[.. + Mz);
+//  - the twiddles tw[j] = exp(-2 pi i j / nf) of the three FFT lengths, back to
+//    back (nfx | nfy | nfz);
+//  - n4 zeroed 16-byte words at zero (the bin counters), grid-strided.
+__global__ __launch_bounds__(kThreads) void setup_tables(NufftGeom g, int W, float *__restrict__ dec,
+                                                        float2 *__restrict__ tw, float4 *__restrict__ zero,
+                                                        size_t n4) {
   const int i = blockIdx.x * kThreads + threadIdx.x;
-  int off = 0;
+  int off = 0, toff = 0;
   for (int d = 0; d < 3; ++d) {
     const NufftDim &D = g.d[d];
     if (i < D.M) {
       const float xi = (float)((double)(D.m_lo + i) * W / (2.0 * D.nf));
       dec[off + i] = 2.0f / ((float)W * es_hat(g, xi));
     }
+    if (i < D.nf) {
+      double sn, cs;
+      sincospi(-2.0 * (double)i / (double)D.nf, &sn, &cs);
+      tw[toff + i] = make_float2((float)cs, (float)sn);
+    }
     off += D.M;
+    toff += D.nf;
   }
-}
-
-__global__ __launch_bounds__(kThreads) void fill_zero(float4 *__restrict__ p, size_t n4) {
-  for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n4; i += (size_t)gridDim.x * kThreads)
-    p[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  for (size_t k = i; k < n4; k += (size_t)gridDim.x * kThreads) zero[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
 // Per-atom spreading record: species grid, first grid point per axis, the
@@ -305,7 +296,7 @@ __global__ __launch_bounds__(kThreads) void bin_count(const SpreadAtom *__restri
 // LDS histogram and adds it to the global counts with one atomic per bin,
 // instead of one contended global atomic per atom.
 constexpr int kLocalBins = 4096;
-constexpr int kBinRun = 4096;
+constexpr int kBinRun = 1024;
 
 __global__ __launch_bounds__(kThreads) void bin_count_local(const SpreadAtom *__restrict__ rec, int natoms,
                                                            SpreadBins b, int *__restrict__ counts) {
@@ -1260,21 +1251,27 @@ int nufft_impl(const float *d_hkl, int npix, const spsim_hip_atom *d_atoms, cons
     return SPSIM_HIP_OK;
   }
 
-  // 1. Bounds of the atoms and of q (one host synchronisation).
+  // 1. Bounds of the atoms and of q: per-workgroup records, folded on the host
+  // after the call's one synchronisation.
   const int most = natoms > npix ? natoms : npix;
   int nblk = (most + kThreads - 1) / kThreads;
-  if (nblk > 1024) nblk = 1024;
+  if (nblk > kBoundBlocks) nblk = kBoundBlocks;
   void *ws = nullptr;
-  SPSIM_TRY(workspace(align256(sizeof(double) * kNb * (nblk + 1)), &ws));
+  SPSIM_TRY(workspace(align256(sizeof(double) * kNb * nblk), &ws));
   double *part = (double *)ws;
-  double *dbounds = part + (size_t)kNb * nblk;
   hipLaunchKernelGGL(bounds_partial, dim3(nblk), dim3(kThreads), 0, s, d_atoms, natoms, d_hkl, npix, R, part);
   SPSIM_HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(bounds_final, dim3(1), dim3(kThreads), 0, s, part, nblk, dbounds);
-  SPSIM_HIP_TRY(hipGetLastError());
-  double hb[kNb];
-  SPSIM_HIP_TRY(hipMemcpyAsync(hb, dbounds, sizeof hb, hipMemcpyDeviceToHost, s));
+  double hpart[kNb * kBoundBlocks], hb[kNb];
+  SPSIM_HIP_TRY(hipMemcpyAsync(hpart, part, sizeof(double) * kNb * nblk, hipMemcpyDeviceToHost, s));
   SPSIM_HIP_TRY(hipStreamSynchronize(s));
+  for (int v = 0; v < kNb; ++v) {
+    double acc = v < 6 ? INFINITY : -INFINITY;
+    for (int b = 0; b < nblk; ++b) {
+      const double x = hpart[(size_t)b * kNb + v];
+      acc = (acc != acc || x != x) ? acc + x : (v < 6 ? fmin(acc, x) : fmax(acc, x));
+    }
+    hb[v] = acc;
+  }
   for (int v = 0; v <= 12; ++v)
     if (!std::isfinite(hb[v])) return fail(SPSIM_HIP_ERR_INVALID, "non-finite atom position or q");
   if (npat > 0) {
@@ -1381,14 +1378,22 @@ int nufft_impl(const float *d_hkl, int npix, const spsim_hip_atom *d_atoms, cons
     SPSIM_HIP_TRY(hipMemcpyAsync(d_rots, rots, sizeof(double) * 9 * (size_t)npat, hipMemcpyHostToDevice, s));
   }
 
-  // 3. Deconvolution table, atom records binned by spread block, spread.
-  const int Mmax = X.M > Y.M ? (X.M > Z.M ? X.M : Z.M) : (Y.M > Z.M ? Y.M : Z.M);
-  hipLaunchKernelGGL(deconv_table, dim3((Mmax + kThreads - 1) / kThreads), dim3(kThreads), 0, s, g, W2, dec);
-  SPSIM_HIP_TRY(hipGetLastError());
-  const size_t n4 = 2 * b_bin / 16;
-  hipLaunchKernelGGL(fill_zero, dim3((unsigned)((n4 + kThreads - 1) / kThreads < 4096 ? (n4 + kThreads - 1) / kThreads : 4096)),
-                     dim3(kThreads), 0, s, (float4 *)counts, n4);
-  SPSIM_HIP_TRY(hipGetLastError());
+  // 3. Tables (deconvolution, twiddles, cleared bin counters), atom records
+  // binned by spread block, spread.
+  float2 *twx = tw, *twy = tw + X.nf, *twz = tw + X.nf + Y.nf;
+  {
+    int most_t = X.M > X.nf ? X.M : X.nf;
+    for (const NufftDim *D : {&Y, &Z}) {
+      if (D->M > most_t) most_t = D->M;
+      if (D->nf > most_t) most_t = D->nf;
+    }
+    const size_t n4 = 2 * b_bin / 16;
+    size_t tb = (size_t)(most_t + kThreads - 1) / kThreads;
+    const size_t zb = (n4 + kThreads - 1) / kThreads;
+    if (zb > tb) tb = zb < 4096 ? zb : 4096;
+    hipLaunchKernelGGL(setup_tables, dim3((unsigned)tb), dim3(kThreads), 0, s, g, W2, dec, tw, (float4 *)counts, n4);
+    SPSIM_HIP_TRY(hipGetLastError());
+  }
   const unsigned ablocks = (unsigned)((natoms + kThreads - 1) / kThreads);
   hipLaunchKernelGGL(spread_prep, dim3(ablocks), dim3(kThreads), 0, s, d_atoms, natoms, d_segs, nsegs, R, g, W,
                      rec);
@@ -1418,10 +1423,6 @@ int nufft_impl(const float *d_hkl, int npix, const spsim_hip_atom *d_atoms, cons
   }
 
   // 4. Pruned FFT along x, y, z: G (buf1) -> A (buf2) -> B (buf1) -> H (buf2).
-  float2 *twx = tw, *twy = tw + X.nf, *twz = tw + X.nf + Y.nf;
-  SPSIM_TRY(launch_twiddles(X.nf, twx, s));
-  SPSIM_TRY(launch_twiddles(Y.nf, twy, s));
-  SPSIM_TRY(launch_twiddles(Z.nf, twz, s));
   SPSIM_TRY(launch_fft(make_pass(X, (long long)nch * Z.M * Y.M, 1), buf1, buf2, twx, s));
   SPSIM_TRY(launch_fft(make_pass(Y, (long long)nch * Z.M, X.L), buf2, buf1, twy, s));
   SPSIM_TRY(launch_fft(make_pass(Z, (long long)nch, (long long)Y.L * X.L), buf1, buf2, twz, s));
