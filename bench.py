@@ -591,7 +591,10 @@ def nfft(args, torch, dv, wl, _lib, h):
     geometry with that path's semantics (global B only, unit strengths):
     the same 100,000-atom molecule and 1024 x 1024 q list. One NUFFT call
     synchronises once to size its grids from the data bounds, so it is timed
-    end to end (HIP events and host wall clock around whole calls)."""
+    end to end (HIP events and host wall clock around whole calls).
+    With the headline's per-atom B factors (h["B"]) it also times the
+    headline workload itself through the NUFFT (Gaussian spreading carries each
+    atom's Debye-Waller factor), against the headline's direct kernel time."""
     dev = torch.device("cuda", 0)
     Z, pos, q = h["Z"], h["pos"], h["hkl"]
     n = int(round(np.sqrt(h["npix"])))
@@ -626,6 +629,23 @@ def nfft(args, torch, dv, wl, _lib, h):
 
     ms_n, wall_n = timed(run_n)
     ms_d, wall_d = timed(run_d)
+    with_b = None
+    if h.get("B") is not None:
+        atoms_b = dv.DeviceAtoms(dv.pack_atoms(Z, pos, atom_b=h["B"]), dev)
+        I_b = torch.empty(h["npix"], dtype=torch.float32, device=dev)
+        info_b = {}
+
+        def run_b():
+            info_b.update(dv.structure_factor_nfft(q, atoms_b, ff, eps=eps, I=I_b, want_F=False)[2])
+
+        def run_db():
+            dv.structure_factor_grid(q, n, n, atoms_b, ff, I=I_d, want_F=False, tile=dv.make_tile(8, 256, 4))
+
+        ms_b, wall_b = timed(run_b)
+        ms_db, _ = timed(run_db)
+        with_b = {"workload": "the headline workload: per-atom B U[10,40] A^2, plain packing", "grid": info_b,
+                  "ms": round(ms_b, 3), "wall_ms": round(wall_b, 3), "ms_direct": round(ms_db, 3),
+                  "speedup_vs_direct": round(ms_db / ms_b, 2)}
     terms = float(h["npix"]) * atoms_n.natoms
     err_n = err_d = None
     try:
@@ -633,12 +653,17 @@ def nfft(args, torch, dv, wl, _lib, h):
 
         rng = np.random.default_rng(5)
         sub = np.sort(rng.choice(h["npix"], 256, replace=False))
-        _, I_ref = oracle.sf_nfft_f64(q.cpu().numpy()[sub], Z, pos, threads=min(16, os.cpu_count() or 1))
+        hs = q.cpu().numpy()[sub]
+        nt = min(16, os.cpu_count() or 1)
+        _, I_ref = oracle.sf_nfft_f64(hs, Z, pos, threads=nt)
         err_n = rel_l2(I_n.cpu().numpy()[sub], I_ref)
         err_d = rel_l2(I_d.cpu().numpy()[sub], I_ref)
+        if with_b is not None:
+            _, I_ref_b = oracle.sf_f64(hs, Z, pos, atom_b=h["B"], threads=nt)
+            with_b["rel_err_vs_fp64"] = rel_l2(I_b.cpu().numpy()[sub], I_ref_b)
     except Exception as e:  # the oracle is a checker only
         print(f"nfft accuracy check skipped: {e}", file=sys.stderr)
-    return {
+    out = {
         "workload": "100000-atom C/N/O/S molecule (radius 12 nm), 1024x1024 detector 75 um at 0.1 m, lambda 0.15 nm, "
                     "global B = 0, unit strengths (use_nfft_for_sf semantics)",
         "eps": eps, "grid": info,
@@ -648,6 +673,9 @@ def nfft(args, torch, dv, wl, _lib, h):
         "effective_gterms_per_s": round(terms / (ms_n * 1e-3) / 1e9, 1),
         "rel_err_vs_fp64": err_n, "rel_err_direct_vs_fp64": err_d,
     }
+    if with_b is not None:
+        out["per_atom_b"] = with_b
+    return out
 
 
 def cpu_baseline(h):
@@ -698,11 +726,12 @@ def main():
         print(json.dumps({"pipeline_nfft": pipeline_nfft(args, torch, dv, wl, _lib)}))
         return
     if args.nfft_only:
-        det, Z, pos, _ = wl.config2()
+        det, Z, pos, B = wl.config2()
         dev = torch.device("cuda", 0)
         q = dv.hkl_detector(det.n, det.n, det.width, det.width, det.distance, det.wavelength, device=dev)
         log("nfft only: non-uniform FFT vs direct sum")
-        print(json.dumps({"nfft": nfft(args, torch, dv, wl, _lib, {"Z": Z, "pos": pos, "hkl": q, "npix": det.npix})}))
+        print(json.dumps({"nfft": nfft(args, torch, dv, wl, _lib,
+                                        {"Z": Z, "pos": pos, "B": B, "hkl": q, "npix": det.npix})}))
         return
     if args.profile:
         # A plain copy of known size, for scaling the HBM byte counters

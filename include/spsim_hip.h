@@ -182,21 +182,27 @@ int spsim_hip_sf_spectrum(const float *d_hkl, int npix, const spsim_hip_atom *d_
    atoms are spread onto per-species grids with an exponential-of-semicircle
    window, the grids go through a pruned 3-D FFT, and each q interpolates from
    them. eps sets the window width W = ceil(-log10 eps) + 1 in [4, 10] (eps <= 0:
-   1e-6); the error of F is about eps times sum_j |w_j| f_s. No per-atom B.
+   1e-6); the error of F is about eps times sum_j |w_j| f_s.
+   d_atom_b (packed order, may be NULL) gives every atom its own B factor
+   (f_s exp(-B_j s^2) with s = |q| / 2 in 1/A, as in spsim_hip_sf_pattern):
+   each atom is then spread as a Gaussian whose width carries its B (17, 25
+   or 33 points, chosen from the largest B; larger needs are rejected), and
+   the grids are not centred in frequency.
    The grids scale with (atom box x q range) per axis; lines longer than 3840
    points or grids above 64 GiB are rejected. Each q interpolates with a second
    window: the spread window itself (FFT lines >= 2 M), or, for W >= 7 when that
    costs less (many q per grid), a window of width W - 1 on lines >= 4 M, which
-   keeps the error. info (host, may be NULL) receives 12 ints
-   {W, Mx, My, Mz, nfx, nfy, nfz, Lx, Ly, Lz, W2, staged}: spread window width,
+   keeps the error. info (host, may be NULL) receives 13 ints
+   {W, Mx, My, Mz, nfx, nfy, nfz, Lx, Ly, Lz, W2, staged, Wg}: window width,
    spread grid, FFT and kept-output sizes per axis, interpolation window width,
-   and 1 when the interpolation staged each wave's grid box in LDS (lists whose
-   consecutive q are close, such as detector rows). Uses
+   1 when the interpolation staged each wave's grid box in LDS (lists whose
+   consecutive q are close, such as detector rows), and the Gaussian spread
+   window's width (0 without per-atom B). Uses
    the library workspace; synchronises `stream` once (to read the bounds that
    size the grids), the rest is stream-ordered. */
-int spsim_hip_sf_nfft(const float *d_hkl, int npix, const spsim_hip_atom *d_atoms, const spsim_hip_segment *d_segs,
-                      int nsegs, int natoms, const float *d_ff, float b_global, const double *rot9, double eps,
-                      float *d_F, float *d_I, int *info, void *stream);
+int spsim_hip_sf_nfft(const float *d_hkl, int npix, const spsim_hip_atom *d_atoms, const float *d_atom_b,
+                      const spsim_hip_segment *d_segs, int nsegs, int natoms, const float *d_ff, float b_global,
+                      const double *rot9, double eps, float *d_F, float *d_I, int *info, void *stream);
 
 /* Many orientations of one molecule through the NUFFT: pattern p is
    spsim_hip_sf_nfft's result for q rotated as q' = R_p q (rot9s: host
@@ -207,7 +213,7 @@ int spsim_hip_sf_nfft(const float *d_hkl, int npix, const spsim_hip_atom *d_atom
    The grids are larger than a single pattern's when q does not fill that ball
    (a flat detector's Ewald cap). Returns after the last pattern is written
    (it synchronises `stream`). Other arguments as spsim_hip_sf_nfft. */
-int spsim_hip_sf_nfft_batch(const float *d_hkl, int npix, const spsim_hip_atom *d_atoms,
+int spsim_hip_sf_nfft_batch(const float *d_hkl, int npix, const spsim_hip_atom *d_atoms, const float *d_atom_b,
                             const spsim_hip_segment *d_segs, int nsegs, int natoms, const float *d_ff, float b_global,
                             int npatterns, const double *rot9s, double eps, float *d_F, float *d_I, int *info,
                             void *stream);

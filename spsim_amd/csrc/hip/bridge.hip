@@ -209,7 +209,7 @@ int spsim_bridge_structure_factor_nfft(const float *hkl, int npix, const spsim_h
   SPSIM_TRY(b.upload(&dff, ff, (size_t)SPSIM_HIP_NSPECIES * SPSIM_HIP_FF_STRIDE, st.s));
   if (F) SPSIM_TRY(b.alloc(&dF, (size_t)npix * 2));
   SPSIM_TRY(b.alloc(&dI, (size_t)npix));
-  SPSIM_TRY(spsim_hip_sf_nfft(dq, npix, da, ds, nsegs, natoms, dff, b_global, nullptr, eps, dF, dI, nullptr, st.s));
+  SPSIM_TRY(spsim_hip_sf_nfft(dq, npix, da, nullptr, ds, nsegs, natoms, dff, b_global, nullptr, eps, dF, dI, nullptr, st.s));
   if (F) SPSIM_TRY(download((float *)F, (const float *)dF, (size_t)npix * 2, st.s));
   SPSIM_TRY(download(I, (const float *)dI, (size_t)npix, st.s));
   SPSIM_HIP_TRY(hipStreamSynchronize(st.s));

@@ -97,6 +97,8 @@ struct NufftGeom {
   // u = 2 (xi / corr_xmax)^2 - 1 (monomial coefficients, constant first);
   // xi = t W / 2 = t * corr_scale * corr_xmax for a target at t = s' h
   float corr_xmax, corr_scale;
+  // > 0: Gaussian spreading; the correction is exp(2 pi^2 gauss_v0 t^2) per axis
+  float gauss_v0;
   int lds_interp;  // host: interpolate with nufft_interp_lds (else nufft_interp)
   float corr[kCorrDeg + 1];
 };
@@ -138,9 +140,9 @@ __device__ __forceinline__ float es_hat(const NufftGeom &g, float xi) {
 __device__ __forceinline__ double nan_min(double a, double b) { return (a != a || b != b) ? a + b : fmin(a, b); }
 __device__ __forceinline__ double nan_max(double a, double b) { return (a != a || b != b) ? a + b : fmax(a, b); }
 
-// Per-block minima and maxima of the atom positions (0..2) and q (3..5), and
-// the largest |q|^2 (12): part[blk * kNb + v], v < 6 minima, 6..11 maxima.
-// NaN propagates.
+// Per-block minima and maxima of the atom positions (0..2) and q (3..5), the
+// largest |q|^2 (12) and the largest per-atom B (13, when atom_b is given):
+// part[blk * kNb + v], v < 6 minima, 6..13 maxima. NaN propagates.
 constexpr int kNb = 14;
 constexpr int kBoundBlocks = 256;  // workgroups of bounds_partial (records folded on the host)
 
@@ -148,7 +150,8 @@ __device__ __forceinline__ double bound_combine(int v, double a, double b) {
   return v < 6 ? nan_min(a, b) : nan_max(a, b);
 }
 
-__global__ __launch_bounds__(kThreads) void bounds_partial(const spsim_hip_atom *__restrict__ atoms, int natoms,
+__global__ __launch_bounds__(kThreads) void bounds_partial(const spsim_hip_atom *__restrict__ atoms,
+                                                          const float *__restrict__ atom_b, int natoms,
                                                           const float *__restrict__ hkl, int npix, AtomRot R,
                                                           double *__restrict__ part) {
   double acc[kNb];
@@ -162,6 +165,7 @@ __global__ __launch_bounds__(kThreads) void bounds_partial(const spsim_hip_atom 
       acc[v] = nan_min(acc[v], p[v]);
       acc[6 + v] = nan_max(acc[6 + v], p[v]);
     }
+    if (atom_b) acc[13] = nan_max(acc[13], (double)atom_b[i]);
   }
   for (int i = blockIdx.x * kThreads + threadIdx.x; i < npix; i += stride) {
     double n2 = 0.0;
@@ -214,18 +218,39 @@ __global__ __launch_bounds__(kThreads) void setup_tables(NufftGeom g, int W, flo
 }
 
 // Per-atom spreading record: species grid, first grid point per axis, the
-// window offset (first point - u, in grid units) per axis and the phased strength.
+// window offset (first point - u, in grid units) per axis, the window's width
+// in points, the phased strength and, for Gaussian windows, 1 / (2 v) per axis.
 struct __align__(16) SpreadAtom {
   int ch, bx, by, bz;
-  float fx, fy, fz, pad;
-  float cr, ci, pad2, pad3;
+  float fx, fy, fz;
+  int w;
+  float cr, ci, ax, ay;
+  float az, pad0, pad1, pad2;
 };
+
+// Gaussian spreading (atoms with their own B factors): the atom is spread as
+// the normalised Gaussian of variance v_d = v0 + sigma^2 / h_d^2 grid units^2
+// per axis, whose transform exp(-2 pi^2 v_d t^2) at t = q_d h_d is
+// exp(-2 pi^2 v0 t^2) (divided out after the interpolation) times the atom's
+// Debye-Waller factor exp(-2 pi^2 sigma^2 q_d^2), with sigma^2 =
+// B 2.5e-21 / (2 pi^2) m^2 for B in A^2 (exp(-B s^2), s = |q| / 2 in 1/A).
+// The window keeps the points within sqrt(2 v tcut) of the atom
+// (truncation exp(-tcut)); v0 = tcut / pi^2 balances that against the aliasing
+// of the sigma = 2 grid, exp(-pi^2 v0).
+struct GaussSpread {
+  float v0, tcut;
+};
+constexpr double kBToSigma2 = 2.5e-21 / (2.0 * M_PI * M_PI);  // m^2 per A^2 of B
 
 // One thread per packed atom: the record above (ch = -1 marks an atom outside
 // the grid, which the bounds exclude).
+// atom_b (packed order, may be NULL): Gaussian windows of per-atom width
+// (at most W points); else the exponential-of-semicircle window of width W.
 __global__ __launch_bounds__(kThreads) void spread_prep(const spsim_hip_atom *__restrict__ atoms, int natoms,
                                                        const spsim_hip_segment *__restrict__ segs, int nsegs,
-                                                       AtomRot R, NufftGeom g, int W, SpreadAtom *__restrict__ rec) {
+                                                       AtomRot R, NufftGeom g, int W,
+                                                       const float *__restrict__ atom_b, GaussSpread gs,
+                                                       SpreadAtom *__restrict__ rec) {
   const int j = blockIdx.x * kThreads + threadIdx.x;
   if (j >= natoms) return;
   int ch = 0;
@@ -241,13 +266,32 @@ __global__ __launch_bounds__(kThreads) void spread_prep(const spsim_hip_atom *__
   sincospif((float)(2.0 * ph), &sn, &cs);
   SpreadAtom r;
   int b[3];
-  float f[3];
+  float f[3], ia[3] = {0.0f, 0.0f, 0.0f};
+  // window: W points centred on u (exponential of semicircle), or for a
+  // Gaussian the points within sqrt(2 v tcut) of u, the widest axis's count
+  // on every axis (so the record holds one width)
+  int w = W;
+  double half = 0.5 * W, norm = 1.0;
+  if (atom_b) {
+    const double sig2 = (double)atom_b[j] * kBToSigma2;
+    int hmax = 0;
+    for (int d = 0; d < 3; ++d) {
+      const double v = (double)gs.v0 + sig2 / (g.d[d].h * g.d[d].h);
+      const int hw = (int)ceil(sqrt(2.0 * v * (double)gs.tcut));
+      hmax = hw > hmax ? hw : hmax;
+      ia[d] = (float)(0.5 / v);
+      norm /= sqrt(2.0 * M_PI * v);
+    }
+    w = 2 * hmax + 1;
+    half = (double)hmax;
+    if (w > W) ch = -1;  // excluded by the host's choice of W from the largest B
+  }
   for (int d = 0; d < 3; ++d) {
     const NufftDim &D = g.d[d];
     const double u = xc[d] / D.h;
-    const double m0 = ceil(u - 0.5 * W);
+    const double m0 = ceil(u - half);
     const double i0 = m0 - D.m_lo;
-    if (!(i0 >= 0.0 && i0 + W <= (double)D.M)) ch = -1;
+    if (!(i0 >= 0.0 && i0 + w <= (double)D.M)) ch = -1;
     b[d] = ch < 0 ? 0 : (int)i0;
     f[d] = (float)(m0 - u);
   }
@@ -258,10 +302,13 @@ __global__ __launch_bounds__(kThreads) void spread_prep(const spsim_hip_atom *__
   r.fx = f[0];
   r.fy = f[1];
   r.fz = f[2];
-  r.pad = 0.0f;
-  r.cr = a.w * cs;
-  r.ci = a.w * sn;
-  r.pad2 = r.pad3 = 0.0f;
+  r.w = w;
+  r.cr = (float)(a.w * norm) * cs;
+  r.ci = (float)(a.w * norm) * sn;
+  r.ax = ia[0];
+  r.ay = ia[1];
+  r.az = ia[2];
+  r.pad0 = r.pad1 = r.pad2 = 0.0f;
   rec[j] = r;
 }
 
@@ -408,7 +455,9 @@ struct __align__(16) SpreadCand {
 // With few output blocks (small grids) the candidates of a block are split
 // round-robin by 64-atom batch over gridDim.y workgroups, each writing its own
 // partial grid (grid + blockIdx.y * grid_stride), which sum_partials adds up.
-template <int W>
+// GAUSS: Gaussian windows of per-atom width r.w <= W (else every window is the
+// exponential of semicircle of width W).
+template <int W, bool GAUSS>
 __global__ __launch_bounds__(64) void spread_gather(const SpreadAtom *__restrict__ sorted,
                                                    const int *__restrict__ start, const int *__restrict__ counts,
                                                    SpreadBins b, NufftGeom g, const float *__restrict__ dec,
@@ -459,7 +508,9 @@ __global__ __launch_bounds__(64) void spread_gather(const SpreadAtom *__restrict
       int bi = 0;
       while (v >= hp[bi + 1]) ++bi;
       const SpreadAtom r = sorted[hs[bi] + v - hp[bi]];
-      if (r.bx + W > x0 && r.bx < x0 + kB && r.by + W > y0 && r.by < y0 + kB && r.bz + W > z0 && r.bz < z0 + kB) {
+      const int rw = GAUSS ? r.w : W;
+      if (r.bx + rw > x0 && r.bx < x0 + kB && r.by + rw > y0 && r.by < y0 + kB && r.bz + rw > z0 &&
+          r.bz < z0 + kB) {
         SpreadCand &c = cand[atomicAdd(&ncand, 1)];
 #pragma unroll
         for (int i = 0; i < kB; ++i) {
@@ -467,9 +518,16 @@ __global__ __launch_bounds__(64) void spread_gather(const SpreadAtom *__restrict
           // table indices clamped into the grid: points past its end have zero weight
           const float dx = dec[min(x0 + i, Mx - 1)], dy = dec[Mx + min(y0 + i, My - 1)];
           const float dz = dec[Mx + My + min(z0 + i, Mz - 1)];
-          c.wx[i] = (ix >= 0 && ix < W) ? es_window((r.fx + (float)ix) * k2, g.beta1) * dx : 0.0f;
-          c.wy[i] = (iy >= 0 && iy < W) ? es_window((r.fy + (float)iy) * k2, g.beta1) * dy : 0.0f;
-          c.wz[i] = (iz >= 0 && iz < W) ? es_window((r.fz + (float)iz) * k2, g.beta1) * dz : 0.0f;
+          if constexpr (GAUSS) {
+            const float ex = r.fx + (float)ix, ey = r.fy + (float)iy, ez = r.fz + (float)iz;
+            c.wx[i] = (ix >= 0 && ix < rw) ? expf(-ex * ex * r.ax) * dx : 0.0f;
+            c.wy[i] = (iy >= 0 && iy < rw) ? expf(-ey * ey * r.ay) * dy : 0.0f;
+            c.wz[i] = (iz >= 0 && iz < rw) ? expf(-ez * ez * r.az) * dz : 0.0f;
+          } else {
+            c.wx[i] = (ix >= 0 && ix < W) ? es_window((r.fx + (float)ix) * k2, g.beta1) * dx : 0.0f;
+            c.wy[i] = (iy >= 0 && iy < W) ? es_window((r.fy + (float)iy) * k2, g.beta1) * dy : 0.0f;
+            c.wz[i] = (iz >= 0 && iz < W) ? es_window((r.fz + (float)iz) * k2, g.beta1) * dz : 0.0f;
+          }
         }
         c.cr = r.cr;
         c.ci = r.ci;
@@ -736,11 +794,17 @@ __device__ __forceinline__ void interp_prepare(const float *__restrict__ hkl, in
     const float fb = (float)(l0 - v);
 #pragma unroll
     for (int i = 0; i < W; ++i) s.wt[d][i] = es_window((fb + (float)i) * (2.0f / W), g.beta2);
-    const float xr = (float)t * g.corr_scale;
-    const float u = 2.0f * xr * xr - 1.0f;
-    float c = g.corr[kCorrDeg];
+    float c;
+    if (g.gauss_v0 > 0.0f) {
+      const float tf = (float)t;
+      c = expf(19.739208802178716f * g.gauss_v0 * tf * tf);  // 2 pi^2 v0 t^2
+    } else {
+      const float xr = (float)t * g.corr_scale;
+      const float u = 2.0f * xr * xr - 1.0f;
+      c = g.corr[kCorrDeg];
 #pragma unroll
-    for (int j = kCorrDeg - 1; j >= 0; --j) c = __builtin_fmaf(c, u, g.corr[j]);
+      for (int j = kCorrDeg - 1; j >= 0; --j) c = __builtin_fmaf(c, u, g.corr[j]);
+    }
     s.corr *= c;
   }
 }
@@ -1145,51 +1209,82 @@ int launch_fft(const FftPass &p, const float2 *in, float2 *out, const float2 *tw
   return SPSIM_HIP_OK;
 }
 
-template <int W>
-int launch_spread_interp(bool spread, const NufftGeom &g, const SpreadAtom *rec, const SpreadBins &bins,
-                         const int *start, const int *counts, int nsplit, size_t grid_stride, int nsegs,
-                         const spsim_hip_segment *segs, const float *dec, float2 *grid, const float *hkl, int npix,
-                         const float2 *H, const float *ff, float b_global, const double *rots, int npat, float2 *F,
-                         float *I, hipStream_t stream) {
-  if (spread) {
-    if (bins.nbins > 0x7fffffffLL) return fail(SPSIM_HIP_ERR_INVALID, "too many spread blocks");
-    hipLaunchKernelGGL(spread_gather<W>, dim3((unsigned)bins.nbins, (unsigned)nsplit), dim3(64), 0, stream, rec,
-                       start, counts, bins, g, dec, grid, grid_stride);
-  } else {
-    for (int p0 = 0; p0 < npat; p0 += 65535) {
-      const int n = npat - p0 < 65535 ? npat - p0 : 65535;
-      if (g.lds_interp)
-        hipLaunchKernelGGL(nufft_interp_lds<W>, dim3((npix + kThreads - 1) / kThreads, n), dim3(kThreads), 0, stream,
-                           hkl, npix, g, H, nsegs, segs, ff, b_global, rots, p0, F, I);
-      else
-        hipLaunchKernelGGL(nufft_interp<W>, dim3((npix + kThreads - 1) / kThreads, n), dim3(kThreads), 0, stream, hkl,
-                           npix, g, H, nsegs, segs, ff, b_global, rots, p0, F, I);
-      SPSIM_HIP_TRY(hipGetLastError());
-    }
-  }
+template <int W, bool GAUSS>
+int launch_spread(const NufftGeom &g, const SpreadAtom *rec, const SpreadBins &bins, const int *start,
+                  const int *counts, int nsplit, size_t grid_stride, const float *dec, float2 *grid,
+                  hipStream_t stream) {
+  if (bins.nbins > 0x7fffffffLL) return fail(SPSIM_HIP_ERR_INVALID, "too many spread blocks");
+  hipLaunchKernelGGL((spread_gather<W, GAUSS>), dim3((unsigned)bins.nbins, (unsigned)nsplit), dim3(64), 0, stream,
+                     rec, start, counts, bins, g, dec, grid, grid_stride);
   SPSIM_HIP_TRY(hipGetLastError());
   return SPSIM_HIP_OK;
 }
 
-int dispatch_w(int W, bool spread, const NufftGeom &g, const SpreadAtom *rec, const SpreadBins &bins,
-               const int *start, const int *counts, int nsplit, size_t grid_stride, const spsim_hip_segment *segs,
-               int nsegs, const float *dec, float2 *grid, const float *hkl, int npix, const float2 *H,
-               const float *ff, float b_global, const double *rots, int npat, float2 *F, float *I, hipStream_t s) {
-#define SPSIM_NUFFT_W(w)                                                                                           \
-  case w:                                                                                                          \
-    return launch_spread_interp<w>(spread, g, rec, bins, start, counts, nsplit, grid_stride, nsegs, segs, dec,    \
-                                   grid, hkl, npix, H, ff, b_global, rots, npat, F, I, s)
+template <int W>
+int launch_interp(const NufftGeom &g, int nsegs, const spsim_hip_segment *segs, const float *hkl, int npix,
+                  const float2 *H, const float *ff, float b_global, const double *rots, int npat, float2 *F, float *I,
+                  hipStream_t stream) {
+  for (int p0 = 0; p0 < npat; p0 += 65535) {
+    const int n = npat - p0 < 65535 ? npat - p0 : 65535;
+    if (g.lds_interp)
+      hipLaunchKernelGGL(nufft_interp_lds<W>, dim3((npix + kThreads - 1) / kThreads, n), dim3(kThreads), 0, stream,
+                         hkl, npix, g, H, nsegs, segs, ff, b_global, rots, p0, F, I);
+    else
+      hipLaunchKernelGGL(nufft_interp<W>, dim3((npix + kThreads - 1) / kThreads, n), dim3(kThreads), 0, stream, hkl,
+                         npix, g, H, nsegs, segs, ff, b_global, rots, p0, F, I);
+    SPSIM_HIP_TRY(hipGetLastError());
+  }
+  return SPSIM_HIP_OK;
+}
+
+// Widths of the Gaussian spread windows (the smallest one that holds the
+// widest atom's window is used).
+constexpr int kGaussWidths[] = {17, 25, 33};
+
+int dispatch_spread(int W, bool gauss, const NufftGeom &g, const SpreadAtom *rec, const SpreadBins &bins,
+                    const int *start, const int *counts, int nsplit, size_t grid_stride, const float *dec,
+                    float2 *grid, hipStream_t s) {
+#define SPSIM_SPREAD_W(w, ga) \
+  case w:                     \
+    return launch_spread<w, ga>(g, rec, bins, start, counts, nsplit, grid_stride, dec, grid, s)
+  if (gauss) {
+    switch (W) {
+      SPSIM_SPREAD_W(17, true);
+      SPSIM_SPREAD_W(25, true);
+      SPSIM_SPREAD_W(33, true);
+      default: return fail(SPSIM_HIP_ERR_INVALID, "Gaussian window width %d", W);
+    }
+  }
   switch (W) {
-    SPSIM_NUFFT_W(4);
-    SPSIM_NUFFT_W(5);
-    SPSIM_NUFFT_W(6);
-    SPSIM_NUFFT_W(7);
-    SPSIM_NUFFT_W(8);
-    SPSIM_NUFFT_W(9);
-    SPSIM_NUFFT_W(10);
+    SPSIM_SPREAD_W(4, false);
+    SPSIM_SPREAD_W(5, false);
+    SPSIM_SPREAD_W(6, false);
+    SPSIM_SPREAD_W(7, false);
+    SPSIM_SPREAD_W(8, false);
+    SPSIM_SPREAD_W(9, false);
+    SPSIM_SPREAD_W(10, false);
     default: return fail(SPSIM_HIP_ERR_INVALID, "window width %d", W);
   }
-#undef SPSIM_NUFFT_W
+#undef SPSIM_SPREAD_W
+}
+
+int dispatch_interp(int W, const NufftGeom &g, int nsegs, const spsim_hip_segment *segs, const float *hkl, int npix,
+                    const float2 *H, const float *ff, float b_global, const double *rots, int npat, float2 *F,
+                    float *I, hipStream_t s) {
+#define SPSIM_INTERP_W(w) \
+  case w:                 \
+    return launch_interp<w>(g, nsegs, segs, hkl, npix, H, ff, b_global, rots, npat, F, I, s)
+  switch (W) {
+    SPSIM_INTERP_W(4);
+    SPSIM_INTERP_W(5);
+    SPSIM_INTERP_W(6);
+    SPSIM_INTERP_W(7);
+    SPSIM_INTERP_W(8);
+    SPSIM_INTERP_W(9);
+    SPSIM_INTERP_W(10);
+    default: return fail(SPSIM_HIP_ERR_INVALID, "window width %d", W);
+  }
+#undef SPSIM_INTERP_W
 }
 
 size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
@@ -1217,9 +1312,10 @@ namespace {
 // one set of species grids for the unrotated atoms over the ball
 // |q| <= max |q| (the box of every R q), then each pattern p interpolates at
 // q' = R_p q (rots: host [npat][9]) into F/I + p npix.
-int nufft_impl(const float *d_hkl, int npix, const spsim_hip_atom *d_atoms, const spsim_hip_segment *d_segs,
-               int nsegs, int natoms, const float *d_ff, float b_global, const double *rot9, int npat,
-               const double *rots, double eps, float *d_F, float *d_I, int *info, hipStream_t s) {
+int nufft_impl(const float *d_hkl, int npix, const spsim_hip_atom *d_atoms, const float *d_atom_b,
+               const spsim_hip_segment *d_segs, int nsegs, int natoms, const float *d_ff, float b_global,
+               const double *rot9, int npat, const double *rots, double eps, float *d_F, float *d_I, int *info,
+               hipStream_t s) {
   if (npix < 0 || natoms < 0 || nsegs < 0 || nsegs > SPSIM_HIP_NSPECIES || npat < 0)
     return fail(SPSIM_HIP_ERR_INVALID, "negative size or too many segments");
   if ((npix > 0 && !d_hkl) || (natoms > 0 && (!d_atoms || !d_segs || nsegs < 1)) || !d_ff)
@@ -1238,7 +1334,7 @@ int nufft_impl(const float *d_hkl, int npix, const spsim_hip_atom *d_atoms, cons
   for (size_t i = 0; i < (size_t)npat * 9; ++i)
     if (!std::isfinite(rots[i])) return fail(SPSIM_HIP_ERR_INVALID, "non-finite rotation");
   if (info)
-    for (int i = 0; i < 12; ++i) info[i] = 0;
+    for (int i = 0; i < 13; ++i) info[i] = 0;
   if (info) info[0] = W;
   const int nout = npat > 0 ? npat : 1;
   if (npix == 0) return SPSIM_HIP_OK;
@@ -1259,7 +1355,8 @@ int nufft_impl(const float *d_hkl, int npix, const spsim_hip_atom *d_atoms, cons
   void *ws = nullptr;
   SPSIM_TRY(workspace(align256(sizeof(double) * kNb * nblk), &ws));
   double *part = (double *)ws;
-  hipLaunchKernelGGL(bounds_partial, dim3(nblk), dim3(kThreads), 0, s, d_atoms, natoms, d_hkl, npix, R, part);
+  hipLaunchKernelGGL(bounds_partial, dim3(nblk), dim3(kThreads), 0, s, d_atoms, d_atom_b, natoms, d_hkl, npix, R,
+                     part);
   SPSIM_HIP_TRY(hipGetLastError());
   double hpart[kNb * kBoundBlocks], hb[kNb];
   SPSIM_HIP_TRY(hipMemcpyAsync(hpart, part, sizeof(double) * kNb * nblk, hipMemcpyDeviceToHost, s));
@@ -1274,6 +1371,8 @@ int nufft_impl(const float *d_hkl, int npix, const spsim_hip_atom *d_atoms, cons
   }
   for (int v = 0; v <= 12; ++v)
     if (!std::isfinite(hb[v])) return fail(SPSIM_HIP_ERR_INVALID, "non-finite atom position or q");
+  const bool gauss = d_atom_b != nullptr;
+  if (gauss && !(hb[13] == hb[13] && hb[13] < INFINITY)) return fail(SPSIM_HIP_ERR_INVALID, "non-finite B factor");
   if (npat > 0) {
     // every R q lies in the ball of radius max |q|
     const double r = sqrt(hb[12]);
@@ -1281,19 +1380,54 @@ int nufft_impl(const float *d_hkl, int npix, const spsim_hip_atom *d_atoms, cons
       hb[3 + d] = -r;
       hb[9 + d] = r;
     }
+  } else if (gauss) {
+    // no centring in frequency (cs = 0): each atom's Gaussian must be
+    // transformed at q itself to give its Debye-Waller factor
+    for (int d = 0; d < 3; ++d) {
+      const double r = fmax(fabs(hb[3 + d]), fabs(hb[9 + d]));
+      hb[3 + d] = -r;
+      hb[9 + d] = r;
+    }
+  }
+  // Spread window: the exponential of semicircle of width W, or with per-atom
+  // B factors Gaussians, the widest (largest B on the finest axis) deciding
+  // the kernel's width Wsp. The grid spacing h matches plan_axis.
+  GaussSpread gs{0.0f, 0.0f};
+  int Wsp = W;
+  if (gauss) {
+    const double e = eps > 0.0 ? (eps < 1e-10 ? 1e-10 : eps) : 1e-6;
+    gs.tcut = (float)(-log(e));
+    gs.v0 = (float)((double)gs.tcut / (M_PI * M_PI));
+    const double bmax = hb[13] > 0.0 ? hb[13] : 0.0;
+    int hmax = 0;
+    for (int d = 0; d < 3; ++d) {
+      const double S = 0.5 * (hb[9 + d] - hb[3 + d]), X = 0.5 * (hb[6 + d] - hb[d]);
+      const double h = S > 0.0 ? 1.0 / (2.0 * kSigma * S) : (X > 0.0 ? 2.0 * X : 1e-9);
+      const double v = (double)gs.v0 + bmax * kBToSigma2 / (h * h);
+      const int hw = (int)ceil(sqrt(2.0 * v * (double)gs.tcut));
+      hmax = hw > hmax ? hw : hmax;
+    }
+    const int need = 2 * hmax + 1;
+    Wsp = 0;
+    for (int w : kGaussWidths)
+      if (!Wsp && w >= need) Wsp = w;
+    if (!Wsp)
+      return fail(SPSIM_HIP_ERR_INVALID,
+                  "B factors up to %g A^2 need a %d-point Gaussian window at this grid spacing (at most %d)", bmax,
+                  need, kGaussWidths[2]);
   }
 
   // 2. Plan the grids: the second window equal to the first, or for W >= 7
   // the narrow one (W - 1, finer frequency grid) when it costs less and fits.
   NufftGeom g{};
   const size_t nch = (size_t)nsegs;
-  for (int d = 0; d < 3; ++d) SPSIM_TRY(plan_axis(hb[d], hb[6 + d], hb[3 + d], hb[9 + d], W, kSigma, W, &g.d[d]));
+  for (int d = 0; d < 3; ++d) SPSIM_TRY(plan_axis(hb[d], hb[6 + d], hb[3 + d], hb[9 + d], Wsp, kSigma, W, &g.d[d]));
   int W2 = W;
   if (W >= 7) {
     NufftDim fine[3];
     bool fits = true;
     for (int d = 0; d < 3 && fits; ++d)
-      fits = plan_axis(hb[d], hb[6 + d], hb[3 + d], hb[9 + d], W, kSigmaFine, W - 1, &fine[d], true) == SPSIM_HIP_OK;
+      fits = plan_axis(hb[d], hb[6 + d], hb[3 + d], hb[9 + d], Wsp, kSigmaFine, W - 1, &fine[d], true) == SPSIM_HIP_OK;
     const double nq = (double)nout * npix;
     if (fits && grid_bytes(fine, nch) <= kMaxGridBytes && plan_cost(fine, W - 1, nq) < plan_cost(g.d, W, nq)) {
       for (int d = 0; d < 3; ++d) g.d[d] = fine[d];
@@ -1312,6 +1446,7 @@ int nufft_impl(const float *d_hkl, int npix, const spsim_hip_atom *d_atoms, cons
     g.corr_xmax = t.corr_xmax;
     g.corr_scale = t.corr_scale;
     for (int i = 0; i <= kCorrDeg; ++i) g.corr[i] = t.corr[i];
+    g.gauss_v0 = gauss ? gs.v0 : 0.0f;
   }
   const NufftDim &X = g.d[0], &Y = g.d[1], &Z = g.d[2];
   // Interpolation with staged boxes when a wave's 64 consecutive q are
@@ -1325,8 +1460,8 @@ int nufft_impl(const float *d_hkl, int npix, const spsim_hip_atom *d_atoms, cons
     g.lds_interp = (span + W2) * (W2 + 1) * (W2 + 1) <= kBoxCap ? 1 : 0;
   }
   if (info) {
-    const int v[11] = {X.M, Y.M, Z.M, X.nf, Y.nf, Z.nf, X.L, Y.L, Z.L, W2, g.lds_interp};
-    for (int i = 0; i < 11; ++i) info[1 + i] = v[i];
+    const int v[12] = {X.M, Y.M, Z.M, X.nf, Y.nf, Z.nf, X.L, Y.L, Z.L, W2, g.lds_interp, gauss ? Wsp : 0};
+    for (int i = 0; i < 12; ++i) info[1 + i] = v[i];
   }
   const size_t nG = nch * Z.M * Y.M * X.M;
   const size_t b_dec = align256(sizeof(float) * (X.M + Y.M + Z.M));
@@ -1395,8 +1530,8 @@ int nufft_impl(const float *d_hkl, int npix, const spsim_hip_atom *d_atoms, cons
     SPSIM_HIP_TRY(hipGetLastError());
   }
   const unsigned ablocks = (unsigned)((natoms + kThreads - 1) / kThreads);
-  hipLaunchKernelGGL(spread_prep, dim3(ablocks), dim3(kThreads), 0, s, d_atoms, natoms, d_segs, nsegs, R, g, W,
-                     rec);
+  hipLaunchKernelGGL(spread_prep, dim3(ablocks), dim3(kThreads), 0, s, d_atoms, natoms, d_segs, nsegs, R, g, Wsp,
+                     d_atom_b, gs, rec);
   SPSIM_HIP_TRY(hipGetLastError());
   const bool local_bins = bins.nbins <= kLocalBins;
   const unsigned runs = (unsigned)((natoms + kBinRun - 1) / kBinRun);
@@ -1412,9 +1547,8 @@ int nufft_impl(const float *d_hkl, int npix, const spsim_hip_atom *d_atoms, cons
   else
     hipLaunchKernelGGL(bin_scatter, dim3(ablocks), dim3(kThreads), 0, s, rec, natoms, bins, start, cursor, sorted);
   SPSIM_HIP_TRY(hipGetLastError());
-  SPSIM_TRY(dispatch_w(W, true, g, sorted, bins, start, counts, nsplit, nG, d_segs, nsegs, dec,
-                       nsplit > 1 ? gpart : buf1, nullptr, 0, nullptr, d_ff, b_global, nullptr, 0, nullptr, nullptr,
-                       s));
+  SPSIM_TRY(dispatch_spread(Wsp, gauss, g, sorted, bins, start, counts, nsplit, nG, dec, nsplit > 1 ? gpart : buf1,
+                            s));
   if (nsplit > 1) {
     const size_t rb = (nG + kThreads - 1) / kThreads;
     hipLaunchKernelGGL(sum_partials, dim3((unsigned)(rb < 4096 ? rb : 4096)), dim3(kThreads), 0, s, gpart, nsplit,
@@ -1428,8 +1562,8 @@ int nufft_impl(const float *d_hkl, int npix, const spsim_hip_atom *d_atoms, cons
   SPSIM_TRY(launch_fft(make_pass(Z, (long long)nch, (long long)Y.L * X.L), buf1, buf2, twz, s));
 
   // 5. Interpolate at every q (of every pattern).
-  SPSIM_TRY(dispatch_w(W2, false, g, nullptr, bins, nullptr, nullptr, 1, 0, d_segs, nsegs, nullptr, nullptr, d_hkl,
-                       npix, buf2, d_ff, b_global, d_rots, nout, (float2 *)d_F, d_I, s));
+  SPSIM_TRY(dispatch_interp(W2, g, nsegs, d_segs, d_hkl, npix, buf2, d_ff, b_global, d_rots, nout, (float2 *)d_F, d_I,
+                            s));
   if (npat > 0) SPSIM_HIP_TRY(hipStreamSynchronize(s));  // the rotations' host buffer may go after return
   return SPSIM_HIP_OK;
 }
@@ -1438,20 +1572,20 @@ int nufft_impl(const float *d_hkl, int npix, const spsim_hip_atom *d_atoms, cons
 }  // namespace spsim
 
 extern "C" int spsim_hip_sf_nfft(const float *d_hkl, int npix, const spsim_hip_atom *d_atoms,
-                                 const spsim_hip_segment *d_segs, int nsegs, int natoms, const float *d_ff,
-                                 float b_global, const double *rot9, double eps, float *d_F, float *d_I, int *info,
-                                 void *stream) {
-  return spsim::nufft_impl(d_hkl, npix, d_atoms, d_segs, nsegs, natoms, d_ff, b_global, rot9, 0, nullptr, eps, d_F,
-                           d_I, info, (hipStream_t)stream);
+                                 const float *d_atom_b, const spsim_hip_segment *d_segs, int nsegs, int natoms,
+                                 const float *d_ff, float b_global, const double *rot9, double eps, float *d_F,
+                                 float *d_I, int *info, void *stream) {
+  return spsim::nufft_impl(d_hkl, npix, d_atoms, d_atom_b, d_segs, nsegs, natoms, d_ff, b_global, rot9, 0, nullptr,
+                           eps, d_F, d_I, info, (hipStream_t)stream);
 }
 
 extern "C" int spsim_hip_sf_nfft_batch(const float *d_hkl, int npix, const spsim_hip_atom *d_atoms,
-                                       const spsim_hip_segment *d_segs, int nsegs, int natoms, const float *d_ff,
-                                       float b_global, int npatterns, const double *rot9s, double eps, float *d_F,
-                                       float *d_I, int *info, void *stream) {
+                                       const float *d_atom_b, const spsim_hip_segment *d_segs, int nsegs, int natoms,
+                                       const float *d_ff, float b_global, int npatterns, const double *rot9s,
+                                       double eps, float *d_F, float *d_I, int *info, void *stream) {
   if (npatterns < 1) return spsim::fail(SPSIM_HIP_ERR_INVALID, "npatterns=%d", npatterns);
-  return spsim::nufft_impl(d_hkl, npix, d_atoms, d_segs, nsegs, natoms, d_ff, b_global, nullptr, npatterns, rot9s,
-                           eps, d_F, d_I, info, (hipStream_t)stream);
+  return spsim::nufft_impl(d_hkl, npix, d_atoms, d_atom_b, d_segs, nsegs, natoms, d_ff, b_global, nullptr, npatterns,
+                           rot9s, eps, d_F, d_I, info, (hipStream_t)stream);
 }
 
 extern "C" int spsim_hip_fft_pruned(const float *d_in, long long outer, long long inner, int M, int m_lo, int nf,

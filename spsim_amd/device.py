@@ -184,26 +184,25 @@ def structure_factor_nfft(hkl: torch.Tensor, atoms: DeviceAtoms, ff: torch.Tenso
                           stream=None, want_F: bool = True, R=None):
     """One pattern by the type-3 non-uniform FFT (spsim_hip_sf_nfft), with q
     rotated exactly by R (3x3) when given; atoms packed with PACK_NFFT give the
-    reference's NFFT semantics. Returns (F or None, I, info) with
-    info = {W, M, nf, L, W2, staged} (per-axis sizes as tuples; W2 the
+    reference's NFFT semantics, atoms packed with per-atom B factors are spread
+    as Gaussians that carry them. Returns (F or None, I, info) with
+    info = {W, M, nf, L, W2, staged, Wg} (per-axis sizes as tuples; W2 the
     interpolation window's width; staged 1 when the interpolation staged grid
-    boxes in LDS)."""
-    if atoms.b is not None:
-        raise ValueError("the NUFFT path takes no per-atom B factors")
+    boxes in LDS; Wg the Gaussian spread width, 0 without per-atom B)."""
     npix = int(hkl.shape[0])
     if want_F and F is None:
         F = torch.empty((npix, 2), dtype=torch.float32, device=hkl.device)
     if I is None:
         I = torch.empty((npix,), dtype=torch.float32, device=hkl.device)
-    info = (C.c_int * 12)()
+    info = (C.c_int * 13)()
     Ra, Rp = _rot(R)
-    rc = _lib.load().spsim_hip_sf_nfft(_ptr(hkl), npix, _ptr(atoms.atoms), _ptr(atoms.segs), atoms.nsegs,
-                                       atoms.natoms, _ptr(ff), float(B), Rp, float(eps),
+    rc = _lib.load().spsim_hip_sf_nfft(_ptr(hkl), npix, _ptr(atoms.atoms), _ptr(atoms.b), _ptr(atoms.segs),
+                                       atoms.nsegs, atoms.natoms, _ptr(ff), float(B), Rp, float(eps),
                                        _ptr(F if want_F else None), _ptr(I), C.cast(info, C.c_void_p),
                                        _stream(stream))
     check(rc, "spsim_hip_sf_nfft")
     v = list(info)
-    return (F if want_F else None), I, {"W": v[0], "M": tuple(v[1:4]), "nf": tuple(v[4:7]), "L": tuple(v[7:10]), "W2": v[10], "staged": v[11]}
+    return (F if want_F else None), I, {"W": v[0], "M": tuple(v[1:4]), "nf": tuple(v[4:7]), "L": tuple(v[7:10]), "W2": v[10], "staged": v[11], "Wg": v[12]}
 
 
 def structure_factor_nfft_batch(hkl: torch.Tensor, atoms: DeviceAtoms, ff: torch.Tensor, rots, B: float = 0.0,
@@ -212,22 +211,21 @@ def structure_factor_nfft_batch(hkl: torch.Tensor, atoms: DeviceAtoms, ff: torch
     """Many orientations of one molecule through the NUFFT
     (spsim_hip_sf_nfft_batch): pattern p at q rotated exactly by rots[p].
     Returns (F [npat, npix, 2] or None, I [npat, npix], info)."""
-    if atoms.b is not None:
-        raise ValueError("the NUFFT path takes no per-atom B factors")
     R = np.ascontiguousarray(rots, np.float64).reshape(-1, 9)
     npat, npix = R.shape[0], int(hkl.shape[0])
     if want_F and F is None:
         F = torch.empty((npat, npix, 2), dtype=torch.float32, device=hkl.device)
     if I is None:
         I = torch.empty((npat, npix), dtype=torch.float32, device=hkl.device)
-    info = (C.c_int * 12)()
-    rc = _lib.load().spsim_hip_sf_nfft_batch(_ptr(hkl), npix, _ptr(atoms.atoms), _ptr(atoms.segs), atoms.nsegs,
-                                             atoms.natoms, _ptr(ff), float(B), npat, R.ctypes.data_as(C.c_void_p),
+    info = (C.c_int * 13)()
+    rc = _lib.load().spsim_hip_sf_nfft_batch(_ptr(hkl), npix, _ptr(atoms.atoms), _ptr(atoms.b), _ptr(atoms.segs),
+                                             atoms.nsegs, atoms.natoms, _ptr(ff), float(B), npat,
+                                             R.ctypes.data_as(C.c_void_p),
                                              float(eps), _ptr(F if want_F else None), _ptr(I),
                                              C.cast(info, C.c_void_p), _stream(stream))
     check(rc, "spsim_hip_sf_nfft_batch")
     v = list(info)
-    return (F if want_F else None), I, {"W": v[0], "M": tuple(v[1:4]), "nf": tuple(v[4:7]), "L": tuple(v[7:10]), "W2": v[10], "staged": v[11]}
+    return (F if want_F else None), I, {"W": v[0], "M": tuple(v[1:4]), "nf": tuple(v[4:7]), "L": tuple(v[7:10]), "W2": v[10], "staged": v[11], "Wg": v[12]}
 
 
 def fft_pruned(x: torch.Tensor, M: int, m_lo: int, nf: int, L: int, l_lo: int, stream=None) -> torch.Tensor:

@@ -199,9 +199,12 @@ def test_nfft_rejects_bad_inputs(gpu):
     big = (detector_q(8) * 1e4).astype(np.float32)
     with pytest.raises(RuntimeError, match="too large|exceeds"):
         dv.structure_factor_nfft(torch.from_numpy(big).to(gpu), atoms, ff)
-    with pytest.raises(ValueError):
+    # per-atom B factors must be finite
+    bad_b = np.ones(50, np.float32)
+    bad_b[7] = np.nan
+    with pytest.raises(RuntimeError, match="non-finite B"):
         dv.structure_factor_nfft(torch.from_numpy(detector_q(8)).to(gpu),
-                                 dv.DeviceAtoms(dv.pack_atoms(Z, pos, atom_b=np.ones(50)), gpu), ff)
+                                 dv.DeviceAtoms(dv.pack_atoms(Z, pos, atom_b=bad_b), gpu), ff)
 
 
 def test_compute_sf_takes_the_nfft_path(gpu):
@@ -355,3 +358,54 @@ def test_nfft_staged_interpolation_on_a_large_detector(gpu):
         assert cplx_err(Fp[sample], F_ref) < 1e-5, (p, infob)
         F1, _, _ = run_nfft(gpu, q, Z, pos, B=2.0, R=R)
         assert cplx_err(Fp, F1) < 3e-6
+
+
+def test_nfft_per_atom_b_factors(gpu):
+    """Per-atom B factors through the NUFFT: every atom is spread as a Gaussian
+    whose transform carries its Debye-Waller factor exp(-B_j s^2). With plain
+    packing (Z = 0 skipped, amplitudes kept) the result is the direct sum's:
+    one off-centre pattern and a batch of orientations match the FP64 oracle
+    with the same per-atom B; a looser eps stays within its error; B factors
+    too wide for the grid are rejected."""
+    import torch
+
+    Z, pos = molecule(900, 4e-9, seed=41)
+    rng = np.random.default_rng(42)
+    B = rng.uniform(10, 40, len(Z)).astype(np.float32)
+    q = detector_q(48, lam=0.15e-9, pix=2e-4, cx=1e-4, cy=-5e-5)
+    atoms = dv.DeviceAtoms(dv.pack_atoms(Z, pos, atom_b=B), gpu)
+    ff = dv.form_factor_table(device=gpu)
+    hkl = torch.from_numpy(q).to(gpu)
+    F, I, info = dv.structure_factor_nfft(hkl, atoms, ff, B=2.0)
+    torch.cuda.synchronize()
+    assert info["Wg"] in (17, 25, 33), info
+    Fh = F.cpu().numpy()
+    F_ref, I_ref = oracle.sf_f64(q, Z, pos, atom_b=B, B=2.0, threads=oracle_threads())
+    assert rel_l2(I.cpu().numpy(), I_ref) < 1e-5, info
+    assert cplx_err(Fh[:, 0] + 1j * Fh[:, 1], F_ref) < 1e-5, info
+    # a looser accuracy target: narrower Gaussians, error within it
+    F4, _, info4 = dv.structure_factor_nfft(hkl, atoms, ff, B=2.0, eps=1e-4)
+    torch.cuda.synchronize()
+    assert info4["Wg"] <= info["Wg"], (info4, info)
+    F4h = F4.cpu().numpy()
+    assert cplx_err(F4h[:, 0] + 1j * F4h[:, 1], F_ref) < 3e-4
+    # many orientations over one grid
+    rots = wl.uniform_rotations(3, np.random.default_rng(43))
+    Fb, _, infob = dv.structure_factor_nfft_batch(hkl, atoms, ff, rots, B=2.0)
+    torch.cuda.synchronize()
+    assert infob["Wg"] in (17, 25, 33), infob
+    Fbh = Fb.cpu().numpy()
+    for p, R in enumerate(rots):
+        F_ref, _ = oracle.sf_f64(q, Z, pos, atom_b=B, B=2.0, R=R, threads=oracle_threads())
+        assert cplx_err(Fbh[p, :, 0] + 1j * Fbh[p, :, 1], F_ref) < 1e-5, (p, infob)
+    # B = 0 for every atom: the exact Gaussian spread of point atoms
+    atoms0 = dv.DeviceAtoms(dv.pack_atoms(Z, pos, atom_b=np.zeros(len(Z), np.float32)), gpu)
+    F0, _, info0 = dv.structure_factor_nfft(hkl, atoms0, ff)
+    torch.cuda.synchronize()
+    F_ref0, _ = oracle.sf_f64(q, Z, pos, threads=oracle_threads())
+    F0h = F0.cpu().numpy()
+    assert info0["Wg"] == 17 and cplx_err(F0h[:, 0] + 1j * F0h[:, 1], F_ref0) < 1e-5, info0
+    # B far too wide for the grid spacing
+    big = dv.DeviceAtoms(dv.pack_atoms(Z, pos, atom_b=np.full(len(Z), 5e4, np.float32)), gpu)
+    with pytest.raises(RuntimeError, match="Gaussian window"):
+        dv.structure_factor_nfft(hkl, big, ff)
