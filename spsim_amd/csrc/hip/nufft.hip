@@ -815,7 +815,8 @@ __device__ __forceinline__ void interp_prepare(const float *__restrict__ hkl, in
 template <int W, bool kRollZ = false, typename Row>
 __device__ __forceinline__ void window_sum(const InterpQ<W> &s, Row row, float &ar, float &ai) {
   ar = ai = 0.0f;
-#pragma unroll(kRollZ ? 1 : W)
+  constexpr int kUnrollZ = kRollZ ? 1 : W;
+#pragma unroll kUnrollZ
   for (int iz = 0; iz < W; ++iz) {
     for (int iy = 0; iy < W; ++iy) {
       const float2 *r = row(iz, iy);
