@@ -141,9 +141,10 @@ __device__ __forceinline__ double nan_min(double a, double b) { return (a != a |
 __device__ __forceinline__ double nan_max(double a, double b) { return (a != a || b != b) ? a + b : fmax(a, b); }
 
 // Per-block minima and maxima of the atom positions (0..2) and q (3..5), the
-// largest |q|^2 (12) and the largest per-atom B (13, when atom_b is given):
-// part[blk * kNb + v], v < 6 minima, 6..13 maxima. NaN propagates.
-constexpr int kNb = 14;
+// largest |q|^2 (12), and, when atom_b is given, the largest per-atom B (13)
+// and the largest -B (14): part[blk * kNb + v], v < 6 minima, 6..14 maxima.
+// NaN propagates.
+constexpr int kNb = 15;
 constexpr int kBoundBlocks = 256;  // workgroups of bounds_partial (records folded on the host)
 
 __device__ __forceinline__ double bound_combine(int v, double a, double b) {
@@ -165,7 +166,10 @@ __global__ __launch_bounds__(kThreads) void bounds_partial(const spsim_hip_atom 
       acc[v] = nan_min(acc[v], p[v]);
       acc[6 + v] = nan_max(acc[6 + v], p[v]);
     }
-    if (atom_b) acc[13] = nan_max(acc[13], (double)atom_b[i]);
+    if (atom_b) {
+      acc[13] = nan_max(acc[13], (double)atom_b[i]);
+      acc[14] = nan_max(acc[14], -(double)atom_b[i]);
+    }
   }
   for (int i = blockIdx.x * kThreads + threadIdx.x; i < npix; i += stride) {
     double n2 = 0.0;
@@ -1373,7 +1377,10 @@ int nufft_impl(const float *d_hkl, int npix, const spsim_hip_atom *d_atoms, cons
   for (int v = 0; v <= 12; ++v)
     if (!std::isfinite(hb[v])) return fail(SPSIM_HIP_ERR_INVALID, "non-finite atom position or q");
   const bool gauss = d_atom_b != nullptr;
-  if (gauss && !(hb[13] == hb[13] && hb[13] < INFINITY)) return fail(SPSIM_HIP_ERR_INVALID, "non-finite B factor");
+  if (gauss && !(hb[13] == hb[13] && hb[13] < INFINITY && hb[14] < INFINITY))
+    return fail(SPSIM_HIP_ERR_INVALID, "non-finite B factor");
+  // a Gaussian of negative width does not exist; the direct kernels take any B
+  if (gauss && hb[14] > 0.0) return fail(SPSIM_HIP_ERR_INVALID, "negative B factor (%g) in the NUFFT", -hb[14]);
   if (npat > 0) {
     // every R q lies in the ball of radius max |q|
     const double r = sqrt(hb[12]);

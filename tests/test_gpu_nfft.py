@@ -409,3 +409,7 @@ def test_nfft_per_atom_b_factors(gpu):
     big = dv.DeviceAtoms(dv.pack_atoms(Z, pos, atom_b=np.full(len(Z), 5e4, np.float32)), gpu)
     with pytest.raises(RuntimeError, match="Gaussian window"):
         dv.structure_factor_nfft(hkl, big, ff)
+    neg = B.copy()
+    neg[3] = -1.0
+    with pytest.raises(RuntimeError, match="negative B"):
+        dv.structure_factor_nfft(hkl, dv.DeviceAtoms(dv.pack_atoms(Z, pos, atom_b=neg), gpu), ff)
