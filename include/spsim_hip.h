@@ -187,7 +187,8 @@ int spsim_hip_sf_spectrum(const float *d_hkl, int npix, const spsim_hip_atom *d_
    (f_s exp(-B_j s^2) with s = |q| / 2 in 1/A, as in spsim_hip_sf_pattern):
    each atom is then spread as a Gaussian whose width carries its B (17, 25
    or 33 points, chosen from the largest B; larger needs and negative B are
-   rejected), and the grids are not centred in frequency.
+   rejected); the x and y grids are then not centred in frequency, and z
+   keeps its centring through complex z weights.
    The grids scale with (atom box x q range) per axis; lines longer than 3840
    points or grids above 64 GiB are rejected. Each q interpolates with a second
    window: the spread window itself (FFT lines >= 2 M), or, for W >= 7 when that

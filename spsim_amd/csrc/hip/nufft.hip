@@ -229,7 +229,7 @@ struct __align__(16) SpreadAtom {
   float fx, fy, fz;
   int w;
   float cr, ci, ax, ay;
-  float az, pad0, pad1, pad2;
+  float az, kz, pad1, pad2;
 };
 
 // Gaussian spreading (atoms with their own B factors): the atom is spread as
@@ -238,6 +238,12 @@ struct __align__(16) SpreadAtom {
 // exp(-2 pi^2 v0 t^2) (divided out after the interpolation) times the atom's
 // Debye-Waller factor exp(-2 pi^2 sigma^2 q_d^2), with sigma^2 =
 // B 2.5e-21 / (2 pi^2) m^2 for B in A^2 (exp(-B s^2), s = |q| / 2 in 1/A).
+// On an axis centred in frequency (t = (q - cs) h) the factor
+// exp(-2 pi^2 sigma^2 (cs + t / h)^2) comes from the same Gaussian moved by the
+// imaginary offset -2 pi i sigma^2 cs / h: its weights gain the phase
+// exp(-2 pi i k (m - u)), k = sigma^2 cs / (h v), and the atom the factor
+// exp(-2 pi^2 sigma^2 cs^2 (1 - sigma^2 / (h^2 v))). Only z is centred in
+// this mode (the grids of x and y are symmetric about q = 0).
 // The window keeps the points within sqrt(2 v tcut) of the atom
 // (truncation exp(-tcut)); v0 = tcut / pi^2 balances that against the aliasing
 // of the sigma = 2 grid, exp(-pi^2 v0).
@@ -275,16 +281,22 @@ __global__ __launch_bounds__(kThreads) void spread_prep(const spsim_hip_atom *__
   // Gaussian the points within sqrt(2 v tcut) of u, the widest axis's count
   // on every axis (so the record holds one width)
   int w = W;
-  double half = 0.5 * W, norm = 1.0;
+  double half = 0.5 * W, norm = 1.0, kz = 0.0;
   if (atom_b) {
     const double sig2 = (double)atom_b[j] * kBToSigma2;
     int hmax = 0;
     for (int d = 0; d < 3; ++d) {
-      const double v = (double)gs.v0 + sig2 / (g.d[d].h * g.d[d].h);
+      const double h = g.d[d].h;
+      const double v = (double)gs.v0 + sig2 / (h * h);
       const int hw = (int)ceil(sqrt(2.0 * v * (double)gs.tcut));
       hmax = hw > hmax ? hw : hmax;
       ia[d] = (float)(0.5 / v);
       norm /= sqrt(2.0 * M_PI * v);
+      if (d == 2) {
+        const double cs = g.d[2].cs;
+        kz = sig2 * cs / (h * v);
+        norm *= exp(-2.0 * M_PI * M_PI * sig2 * cs * cs * (1.0 - sig2 / (h * h * v)));
+      }
     }
     w = 2 * hmax + 1;
     half = (double)hmax;
@@ -312,7 +324,8 @@ __global__ __launch_bounds__(kThreads) void spread_prep(const spsim_hip_atom *__
   r.ax = ia[0];
   r.ay = ia[1];
   r.az = ia[2];
-  r.pad0 = r.pad1 = r.pad2 = 0.0f;
+  r.kz = (float)kz;
+  r.pad1 = r.pad2 = 0.0f;
   rec[j] = r;
 }
 
@@ -446,7 +459,7 @@ __global__ __launch_bounds__(kThreads) void bin_scatter(const SpreadAtom *__rest
 // the block's kSpreadBlock points per axis (zero outside the window) and its
 // phased strength.
 struct __align__(16) SpreadCand {
-  float wx[kSpreadBlock], wy[kSpreadBlock], wz[kSpreadBlock];
+  float wx[kSpreadBlock], wy[kSpreadBlock], wz[kSpreadBlock], wzi[kSpreadBlock];
   float cr, ci, pad0, pad1;
 };
 
@@ -460,8 +473,9 @@ struct __align__(16) SpreadCand {
 // round-robin by 64-atom batch over gridDim.y workgroups, each writing its own
 // partial grid (grid + blockIdx.y * grid_stride), which sum_partials adds up.
 // GAUSS: Gaussian windows of per-atom width r.w <= W (else every window is the
-// exponential of semicircle of width W).
-template <int W, bool GAUSS>
+// exponential of semicircle of width W). CZ: complex z weights (the phase
+// exp(-2 pi i r.kz (m - u)) of a Gaussian on a z axis centred in frequency).
+template <int W, bool GAUSS, bool CZ>
 __global__ __launch_bounds__(64) void spread_gather(const SpreadAtom *__restrict__ sorted,
                                                    const int *__restrict__ start, const int *__restrict__ counts,
                                                    SpreadBins b, NufftGeom g, const float *__restrict__ dec,
@@ -526,7 +540,15 @@ __global__ __launch_bounds__(64) void spread_gather(const SpreadAtom *__restrict
             const float ex = r.fx + (float)ix, ey = r.fy + (float)iy, ez = r.fz + (float)iz;
             c.wx[i] = (ix >= 0 && ix < rw) ? expf(-ex * ex * r.ax) * dx : 0.0f;
             c.wy[i] = (iy >= 0 && iy < rw) ? expf(-ey * ey * r.ay) * dy : 0.0f;
-            c.wz[i] = (iz >= 0 && iz < rw) ? expf(-ez * ez * r.az) * dz : 0.0f;
+            const float gz = (iz >= 0 && iz < rw) ? expf(-ez * ez * r.az) * dz : 0.0f;
+            if constexpr (CZ) {
+              float sn, cs;
+              sincospif(-2.0f * r.kz * ez, &sn, &cs);
+              c.wz[i] = gz * cs;
+              c.wzi[i] = gz * sn;
+            } else {
+              c.wz[i] = gz;
+            }
           } else {
             c.wx[i] = (ix >= 0 && ix < W) ? es_window((r.fx + (float)ix) * k2, g.beta1) * dx : 0.0f;
             c.wy[i] = (iy >= 0 && iy < W) ? es_window((r.fy + (float)iy) * k2, g.beta1) * dy : 0.0f;
@@ -545,8 +567,14 @@ __global__ __launch_bounds__(64) void spread_gather(const SpreadAtom *__restrict
 #pragma unroll
       for (int z = 0; z < kB; ++z) {
         const float wz = cand[a].wz[z];
-        ar[z] += vr * wz;
-        ai[z] += vi * wz;
+        if constexpr (CZ) {
+          const float wzi = cand[a].wzi[z];
+          ar[z] += vr * wz - vi * wzi;
+          ai[z] += vr * wzi + vi * wz;
+        } else {
+          ar[z] += vr * wz;
+          ai[z] += vi * wz;
+        }
       }
     }
     __syncthreads();
@@ -1214,12 +1242,12 @@ int launch_fft(const FftPass &p, const float2 *in, float2 *out, const float2 *tw
   return SPSIM_HIP_OK;
 }
 
-template <int W, bool GAUSS>
+template <int W, bool GAUSS, bool CZ>
 int launch_spread(const NufftGeom &g, const SpreadAtom *rec, const SpreadBins &bins, const int *start,
                   const int *counts, int nsplit, size_t grid_stride, const float *dec, float2 *grid,
                   hipStream_t stream) {
   if (bins.nbins > 0x7fffffffLL) return fail(SPSIM_HIP_ERR_INVALID, "too many spread blocks");
-  hipLaunchKernelGGL((spread_gather<W, GAUSS>), dim3((unsigned)bins.nbins, (unsigned)nsplit), dim3(64), 0, stream,
+  hipLaunchKernelGGL((spread_gather<W, GAUSS, CZ>), dim3((unsigned)bins.nbins, (unsigned)nsplit), dim3(64), 0, stream,
                      rec, start, counts, bins, g, dec, grid, grid_stride);
   SPSIM_HIP_TRY(hipGetLastError());
   return SPSIM_HIP_OK;
@@ -1246,28 +1274,37 @@ int launch_interp(const NufftGeom &g, int nsegs, const spsim_hip_segment *segs, 
 // widest atom's window is used).
 constexpr int kGaussWidths[] = {17, 25, 33};
 
-int dispatch_spread(int W, bool gauss, const NufftGeom &g, const SpreadAtom *rec, const SpreadBins &bins,
+// gauss: Gaussian windows; cz: with complex z weights (z centred in frequency).
+int dispatch_spread(int W, bool gauss, bool cz, const NufftGeom &g, const SpreadAtom *rec, const SpreadBins &bins,
                     const int *start, const int *counts, int nsplit, size_t grid_stride, const float *dec,
                     float2 *grid, hipStream_t s) {
-#define SPSIM_SPREAD_W(w, ga) \
-  case w:                     \
-    return launch_spread<w, ga>(g, rec, bins, start, counts, nsplit, grid_stride, dec, grid, s)
+#define SPSIM_SPREAD_W(w, ga, c) \
+  case w:                        \
+    return launch_spread<w, ga, c>(g, rec, bins, start, counts, nsplit, grid_stride, dec, grid, s)
+  if (gauss && cz) {
+    switch (W) {
+      SPSIM_SPREAD_W(17, true, true);
+      SPSIM_SPREAD_W(25, true, true);
+      SPSIM_SPREAD_W(33, true, true);
+      default: return fail(SPSIM_HIP_ERR_INVALID, "Gaussian window width %d", W);
+    }
+  }
   if (gauss) {
     switch (W) {
-      SPSIM_SPREAD_W(17, true);
-      SPSIM_SPREAD_W(25, true);
-      SPSIM_SPREAD_W(33, true);
+      SPSIM_SPREAD_W(17, true, false);
+      SPSIM_SPREAD_W(25, true, false);
+      SPSIM_SPREAD_W(33, true, false);
       default: return fail(SPSIM_HIP_ERR_INVALID, "Gaussian window width %d", W);
     }
   }
   switch (W) {
-    SPSIM_SPREAD_W(4, false);
-    SPSIM_SPREAD_W(5, false);
-    SPSIM_SPREAD_W(6, false);
-    SPSIM_SPREAD_W(7, false);
-    SPSIM_SPREAD_W(8, false);
-    SPSIM_SPREAD_W(9, false);
-    SPSIM_SPREAD_W(10, false);
+    SPSIM_SPREAD_W(4, false, false);
+    SPSIM_SPREAD_W(5, false, false);
+    SPSIM_SPREAD_W(6, false, false);
+    SPSIM_SPREAD_W(7, false, false);
+    SPSIM_SPREAD_W(8, false, false);
+    SPSIM_SPREAD_W(9, false, false);
+    SPSIM_SPREAD_W(10, false, false);
     default: return fail(SPSIM_HIP_ERR_INVALID, "window width %d", W);
   }
 #undef SPSIM_SPREAD_W
@@ -1389,9 +1426,10 @@ int nufft_impl(const float *d_hkl, int npix, const spsim_hip_atom *d_atoms, cons
       hb[9 + d] = r;
     }
   } else if (gauss) {
-    // no centring in frequency (cs = 0): each atom's Gaussian must be
-    // transformed at q itself to give its Debye-Waller factor
-    for (int d = 0; d < 3; ++d) {
+    // x and y not centred in frequency (cs = 0): the Gaussians' transforms at
+    // q itself give the Debye-Waller factors; z stays centred (complex z
+    // weights) since a detector's q_z range lies on one side of 0
+    for (int d = 0; d < 2; ++d) {
       const double r = fmax(fabs(hb[3 + d]), fabs(hb[9 + d]));
       hb[3 + d] = -r;
       hb[9 + d] = r;
@@ -1555,8 +1593,8 @@ int nufft_impl(const float *d_hkl, int npix, const spsim_hip_atom *d_atoms, cons
   else
     hipLaunchKernelGGL(bin_scatter, dim3(ablocks), dim3(kThreads), 0, s, rec, natoms, bins, start, cursor, sorted);
   SPSIM_HIP_TRY(hipGetLastError());
-  SPSIM_TRY(dispatch_spread(Wsp, gauss, g, sorted, bins, start, counts, nsplit, nG, dec, nsplit > 1 ? gpart : buf1,
-                            s));
+  SPSIM_TRY(dispatch_spread(Wsp, gauss, gauss && g.d[2].cs != 0.0, g, sorted, bins, start, counts, nsplit, nG, dec,
+                            nsplit > 1 ? gpart : buf1, s));
   if (nsplit > 1) {
     const size_t rb = (nG + kThreads - 1) / kThreads;
     hipLaunchKernelGGL(sum_partials, dim3((unsigned)(rb < 4096 ? rb : 4096)), dim3(kThreads), 0, s, gpart, nsplit,
