@@ -21,7 +21,7 @@
 //     D_m deconvolves the second window in advance.
 //  3. Pruned FFT: along x, y, then z, each grid line (M points) is zero-padded
 //     to nf >= sigma2 M points, transformed in LDS (Stockham, radices
-//     4/2/3/5) and only the L outputs the targets can reach are kept.
+//     8/4/2/3/5) and only the L outputs the targets can reach are kept.
 //  4. Interpolate: each q gathers W2^3 FFT values per species with the
 //     second window phi2 (width W2, beta2), divides by the first window's
 //     transform at s', multiplies by f_Z(|q|), sums the species and applies
@@ -31,9 +31,12 @@
 // grid), a narrower one on a finer frequency grid: sigma2 = 4, W2 = W - 1,
 // beta2 = 2.65 W2, which keeps the error of W = 7..10 (the first window's
 // aliasing dominates) at 0.63 of the gathers.
+// With per-atom B factors the spread window is instead a Gaussian per atom
+// whose transform carries the atom's Debye-Waller factor (see GaussSpread).
 // The error of the centred sum is about 10^{1-W} relative to the number of
 // atoms (W = 7 by default); the grids and FFT run in FP32, the phases in FP64.
-// Atom contributions reach the grid through FP32 atomics, so the last bits
+// The spread sums each grid cell's atoms in registers without FP atomics, but
+// in an order that integer atomics of the binning decide, so the last bits
 // of F can vary from run to run.
 #include <cmath>
 #include <mutex>
