@@ -446,6 +446,54 @@ __global__ __launch_bounds__(kThreads) void scan_exclusive(const int *__restrict
   }
 }
 
+// Many bins: the same scan over chunks of kScanChunk entries in parallel.
+// chunk_sums adds up each chunk (one workgroup per chunk); scan_exclusive then
+// scans those sums into chunk offsets, and scan_chunks scans every chunk from
+// its offset (one workgroup per chunk).
+constexpr int kScanChunk = kScanPer * kThreads;
+
+__global__ __launch_bounds__(kThreads) void chunk_sums(const int *__restrict__ in, long long n,
+                                                      int *__restrict__ sums) {
+  __shared__ int part[kThreads];
+  const long long first = (long long)blockIdx.x * kScanChunk + (long long)kScanPer * threadIdx.x;
+  int sum = 0;
+#pragma unroll
+  for (int k = 0; k < kScanPer; ++k) sum += first + k < n ? in[first + k] : 0;
+  part[threadIdx.x] = sum;
+  __syncthreads();
+  for (int w = kThreads / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < (unsigned)w) part[threadIdx.x] += part[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) sums[blockIdx.x] = part[0];
+}
+
+__global__ __launch_bounds__(kThreads) void scan_chunks(const int *__restrict__ in, long long n,
+                                                       const int *__restrict__ offsets, int *__restrict__ out) {
+  __shared__ int part[kThreads];
+  const long long first = (long long)blockIdx.x * kScanChunk + (long long)kScanPer * threadIdx.x;
+  int v[kScanPer], sum = 0;
+#pragma unroll
+  for (int k = 0; k < kScanPer; ++k) {
+    v[k] = first + k < n ? in[first + k] : 0;
+    sum += v[k];
+  }
+  part[threadIdx.x] = sum;
+  __syncthreads();
+  for (int off = 1; off < kThreads; off <<= 1) {
+    const int t = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0;
+    __syncthreads();
+    part[threadIdx.x] += t;
+    __syncthreads();
+  }
+  int run = offsets[blockIdx.x] + (threadIdx.x ? part[threadIdx.x - 1] : 0);
+#pragma unroll
+  for (int k = 0; k < kScanPer; ++k) {
+    if (first + k < n) out[first + k] = run;
+    run += v[k];
+  }
+}
+
 // sorted[start[bin] + k] = the bin's atoms (k from a per-bin cursor).
 __global__ __launch_bounds__(kThreads) void bin_scatter(const SpreadAtom *__restrict__ rec, int natoms, SpreadBins b,
                                                        const int *__restrict__ start, int *__restrict__ cursor,
@@ -1538,7 +1586,9 @@ int nufft_impl(const float *d_hkl, int npix, const spsim_hip_atom *d_atoms, cons
     if (nsplit < 1) nsplit = 1;
   }
   const size_t b_part = nsplit > 1 ? align256(sizeof(float2) * nG * nsplit) : 0;
-  SPSIM_TRY(workspace(b_dec + 2 * b_rec + 3 * b_bin + b_tw + b_rot + b1 + b2 + b_part, &ws));
+  const long long nchunks = (bins.nbins + kScanChunk - 1) / kScanChunk;
+  const size_t b_scan = align256(sizeof(int) * (size_t)nchunks);
+  SPSIM_TRY(workspace(b_dec + 2 * b_rec + 3 * b_bin + b_tw + b_rot + b1 + b2 + b_part + 2 * b_scan, &ws));
   char *wp = (char *)ws;
   float *dec = (float *)wp;
   wp += b_dec;
@@ -1557,6 +1607,8 @@ int nufft_impl(const float *d_hkl, int npix, const spsim_hip_atom *d_atoms, cons
   float2 *buf1 = (float2 *)wp;
   float2 *buf2 = (float2 *)(wp + b1);
   float2 *gpart = nsplit > 1 ? (float2 *)(wp + b1 + b2) : nullptr;
+  int *csum = (int *)(wp + b1 + b2 + b_part);
+  int *coff = (int *)(wp + b1 + b2 + b_part + b_scan);
   if (npat > 0) {
     // pageable source: the copy has completed when the call returns
     SPSIM_HIP_TRY(hipMemcpyAsync(d_rots, rots, sizeof(double) * 9 * (size_t)npat, hipMemcpyHostToDevice, s));
@@ -1589,7 +1641,16 @@ int nufft_impl(const float *d_hkl, int npix, const spsim_hip_atom *d_atoms, cons
   else
     hipLaunchKernelGGL(bin_count, dim3(ablocks), dim3(kThreads), 0, s, rec, natoms, bins, counts);
   SPSIM_HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(scan_exclusive, dim3(1), dim3(kThreads), 0, s, counts, bins.nbins, start);
+  if (nchunks <= 1) {
+    hipLaunchKernelGGL(scan_exclusive, dim3(1), dim3(kThreads), 0, s, counts, bins.nbins, start);
+  } else {
+    if (nchunks > 0x7fffffffLL) return fail(SPSIM_HIP_ERR_INVALID, "too many spread blocks");
+    hipLaunchKernelGGL(chunk_sums, dim3((unsigned)nchunks), dim3(kThreads), 0, s, counts, bins.nbins, csum);
+    SPSIM_HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(scan_exclusive, dim3(1), dim3(kThreads), 0, s, csum, nchunks, coff);
+    SPSIM_HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(scan_chunks, dim3((unsigned)nchunks), dim3(kThreads), 0, s, counts, bins.nbins, coff, start);
+  }
   SPSIM_HIP_TRY(hipGetLastError());
   if (local_bins)
     hipLaunchKernelGGL(bin_scatter_local, dim3(runs), dim3(kThreads), 0, s, rec, natoms, bins, start, cursor, sorted);
