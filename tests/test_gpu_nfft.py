@@ -413,3 +413,42 @@ def test_nfft_per_atom_b_factors(gpu):
     neg[3] = -1.0
     with pytest.raises(RuntimeError, match="negative B"):
         dv.structure_factor_nfft(hkl, dv.DeviceAtoms(dv.pack_atoms(Z, pos, atom_b=neg), gpu), ff)
+
+
+def test_nfft_per_atom_b_rotation_volume_and_high_b(gpu):
+    """Per-atom B with an exact rotation (rot9), on a rotated 3-D q volume (every
+    axis off-centre in frequency), and with large B factors that need the
+    25-point Gaussian window: each against the FP64 direct oracle."""
+    import torch
+
+    ff = dv.form_factor_table(device=gpu)
+    Z, pos = molecule(700, 3.5e-9, seed=51)
+    rng = np.random.default_rng(52)
+    B = rng.uniform(0, 60, len(Z)).astype(np.float32)
+    atoms = dv.DeviceAtoms(dv.pack_atoms(Z, pos, atom_b=B), gpu)
+    # exact rotation of a detector's q list
+    q = detector_q(40, lam=0.2e-9, pix=2e-4)
+    R = wl.uniform_rotations(1, np.random.default_rng(53))[0]
+    F, _, info = dv.structure_factor_nfft(torch.from_numpy(q).to(gpu), atoms, ff, B=1.0, R=R)
+    torch.cuda.synchronize()
+    F_ref, _ = oracle.sf_f64(q, Z, pos, atom_b=B, B=1.0, R=R, threads=oracle_threads())
+    Fh = F.cpu().numpy()
+    assert cplx_err(Fh[:, 0] + 1j * Fh[:, 1], F_ref) < 1e-5, info
+    # a rotated 3-D volume of q
+    qv = oracle.rotate_hkl(oracle.hkl_3d(16, 16, 16, 16e-4, 16e-4, 16e-4, 0.05, 0.3e-9),
+                           wl.uniform_rotations(1, np.random.default_rng(54))[0])
+    F, _, info = dv.structure_factor_nfft(torch.from_numpy(np.ascontiguousarray(qv, np.float32)).to(gpu), atoms, ff)
+    torch.cuda.synchronize()
+    F_ref, _ = oracle.sf_f64(qv, Z, pos, atom_b=B, threads=oracle_threads())
+    Fh = F.cpu().numpy()
+    assert cplx_err(Fh[:, 0] + 1j * Fh[:, 1], F_ref) < 1e-5, info
+    # B factors wide enough for the 25-point window at this resolution
+    q2 = detector_q(48, lam=0.1e-9, pix=4e-4)
+    Bh = rng.uniform(100, 400, len(Z)).astype(np.float32)
+    atoms_h = dv.DeviceAtoms(dv.pack_atoms(Z, pos, atom_b=Bh), gpu)
+    F, _, info = dv.structure_factor_nfft(torch.from_numpy(q2).to(gpu), atoms_h, ff)
+    torch.cuda.synchronize()
+    assert info["Wg"] == 25, info
+    F_ref, _ = oracle.sf_f64(q2, Z, pos, atom_b=Bh, threads=oracle_threads())
+    Fh = F.cpu().numpy()
+    assert cplx_err(Fh[:, 0] + 1j * Fh[:, 1], F_ref) < 1e-5, info
