@@ -1,8 +1,11 @@
 """PyTorch autograd front end of the structure factor and its adjoint.
 
-`intensity(pos, atom_b, ...)` returns I = |F|^2 on a q list; its backward is
-the hand-written adjoint kernel (spsim_hip_sf_adjoint), which recomputes
-phases per pixel chunk instead of storing atom x pixel terms.
+`intensity(pos, atom_b, ...)` returns I = |F|^2 on a q list, by the direct
+sum (spsim_hip_sf_pattern) or, with method="nufft", by the non-uniform FFT
+(spsim_hip_sf_nfft, per-atom B by Gaussian spreading) to a stated accuracy.
+Its backward is the hand-written adjoint kernel (spsim_hip_sf_adjoint), which
+recomputes phases per pixel chunk instead of storing atom x pixel terms and
+takes F from the forward.
 `poisson_nll(I, y, K)` is the Poisson negative log-likelihood
 sum(K I - y log(K I)) whose forward and gradient come from
 spsim_hip_poisson_nll. All tensors live on the GPU; only pointers and the
@@ -46,17 +49,25 @@ def _p(t):
 
 class _Intensity(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, pos, atom_b, hkl, layout, ff, b_global, tile):
+    def forward(ctx, pos, atom_b, hkl, layout, ff, b_global, tile, method, eps):
         packed_pos = pos.detach().float()[layout.perm].contiguous()
         atoms = torch.cat([packed_pos, layout.amp[:, None]], dim=1).contiguous()
         pb = None if atom_b is None else atom_b.detach().float()[layout.perm].contiguous()
         npix = int(hkl.shape[0])
         F = torch.empty((npix, 2), dtype=torch.float32, device=hkl.device)
         I = torch.empty((npix,), dtype=torch.float32, device=hkl.device)
-        check(_lib.load().spsim_hip_sf_pattern(_p(hkl), npix, _p(atoms), _p(pb), _p(layout.segs), layout.nsegs,
-                                               layout.natoms, _p(ff), float(b_global), None, _p(F), _p(I),
-                                               C.byref(tile) if tile is not None else None, _sp()),
-              "spsim_hip_sf_pattern")
+        if method == "nufft":
+            check(_lib.load().spsim_hip_sf_nfft(_p(hkl), npix, _p(atoms), _p(pb), _p(layout.segs), layout.nsegs,
+                                                layout.natoms, _p(ff), float(b_global), None, float(eps), _p(F),
+                                                _p(I), None, _sp()),
+                  "spsim_hip_sf_nfft")
+        elif method == "direct":
+            check(_lib.load().spsim_hip_sf_pattern(_p(hkl), npix, _p(atoms), _p(pb), _p(layout.segs), layout.nsegs,
+                                                   layout.natoms, _p(ff), float(b_global), None, _p(F), _p(I),
+                                                   C.byref(tile) if tile is not None else None, _sp()),
+                  "spsim_hip_sf_pattern")
+        else:
+            raise ValueError(f"method must be 'direct' or 'nufft', not {method!r}")
         ctx.save_for_backward(atoms, pb if pb is not None else torch.empty(0, device=hkl.device), F)
         ctx.has_b = pb is not None
         ctx.want_b = atom_b is not None and atom_b.requires_grad
@@ -86,13 +97,17 @@ class _Intensity(torch.autograd.Function):
             gb = torch.zeros((ctx.n_orig,), dtype=torch.float32, device=atoms.device)
             if n:
                 gb[layout.perm] = gb_p[:n]
-        return gpos, gb, None, None, None, None, None
+        return gpos, gb, None, None, None, None, None, None, None
 
 
 def intensity(pos: torch.Tensor, hkl: torch.Tensor, layout: SpeciesLayout, ff: torch.Tensor,
-              atom_b: torch.Tensor | None = None, b_global: float = 0.0, tile=None) -> torch.Tensor:
-    """I = |F|^2 on hkl for atoms at pos [N, 3] (metres), differentiable in pos and atom_b."""
-    return _Intensity.apply(pos, atom_b, hkl, layout, ff, b_global, tile)
+              atom_b: torch.Tensor | None = None, b_global: float = 0.0, tile=None, method: str = "direct",
+              eps: float = 1e-6) -> torch.Tensor:
+    """I = |F|^2 on hkl for atoms at pos [N, 3] (metres), differentiable in pos and
+    atom_b. method "direct" sums every term; "nufft" evaluates F by the
+    non-uniform FFT at accuracy eps (B factors must then be >= 0), and the
+    backward uses that F."""
+    return _Intensity.apply(pos, atom_b, hkl, layout, ff, b_global, tile, method, eps)
 
 
 class _PoissonNLL(torch.autograd.Function):
