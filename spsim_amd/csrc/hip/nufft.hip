@@ -103,6 +103,9 @@ struct NufftGeom {
   // > 0: Gaussian spreading; the correction is exp(2 pi^2 gauss_v0 t^2) per axis
   float gauss_v0;
   int lds_interp;  // host: interpolate with nufft_interp_lds (else nufft_interp)
+  // > 0: nufft_interp_lds reads the q list as a tile_n x tile_n image, one
+  // 16 x 16 pixel tile per workgroup (16 x 4 per wave)
+  int tile_n;
   float corr[kCorrDeg + 1];
 };
 
@@ -995,8 +998,15 @@ __global__ __launch_bounds__(kThreads) void nufft_interp_lds(const float *__rest
                                                             const double *__restrict__ rots, int p0,
                                                             float2 *__restrict__ F, float *__restrict__ I) {
   __shared__ float2 box[kThreads / 64][kBoxCap];
-  const int k = blockIdx.x * kThreads + threadIdx.x;
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  int k;  // this lane's q (npix: none)
+  if (g.tile_n > 0) {
+    const int n = g.tile_n, ntx = (n + 15) / 16;
+    const int x = (int)(blockIdx.x % ntx) * 16 + lane % 16, y = (int)(blockIdx.x / ntx) * 16 + wave * 4 + lane / 16;
+    k = (x < n && y < n) ? y * n + x : npix;
+  } else {
+    k = blockIdx.x * kThreads + threadIdx.x;
+  }
   const int pat = p0 + (int)blockIdx.y;
   InterpQ<W> s;
   if (k < npix) {
@@ -1310,12 +1320,15 @@ int launch_interp(const NufftGeom &g, int nsegs, const spsim_hip_segment *segs, 
                   hipStream_t stream) {
   for (int p0 = 0; p0 < npat; p0 += 65535) {
     const int n = npat - p0 < 65535 ? npat - p0 : 65535;
-    if (g.lds_interp)
-      hipLaunchKernelGGL(nufft_interp_lds<W>, dim3((npix + kThreads - 1) / kThreads, n), dim3(kThreads), 0, stream,
-                         hkl, npix, g, H, nsegs, segs, ff, b_global, rots, p0, F, I);
-    else
+    if (g.lds_interp) {
+      const int nt = (g.tile_n + 15) / 16;
+      const unsigned gx = g.tile_n > 0 ? (unsigned)(nt * nt) : (unsigned)((npix + kThreads - 1) / kThreads);
+      hipLaunchKernelGGL(nufft_interp_lds<W>, dim3(gx, n), dim3(kThreads), 0, stream, hkl, npix, g, H, nsegs, segs,
+                         ff, b_global, rots, p0, F, I);
+    } else {
       hipLaunchKernelGGL(nufft_interp<W>, dim3((npix + kThreads - 1) / kThreads, n), dim3(kThreads), 0, stream, hkl,
                          npix, g, H, nsegs, segs, ff, b_global, rots, p0, F, I);
+    }
     SPSIM_HIP_TRY(hipGetLastError());
   }
   return SPSIM_HIP_OK;
@@ -1546,15 +1559,21 @@ int nufft_impl(const float *d_hkl, int npix, const spsim_hip_atom *d_atoms, cons
     g.gauss_v0 = gauss ? gs.v0 : 0.0f;
   }
   const NufftDim &X = g.d[0], &Y = g.d[1], &Z = g.d[2];
-  // Interpolation with staged boxes when a wave's 64 consecutive q are
-  // expected to fit one: for a detector-shaped list of rows of about sqrt(npix)
-  // pixels that span the kept grid, 64 pixels span 64 (L - W2) / sqrt(npix)
-  // cells, a box of about (span + W2) (W2 + 1)^2 points.
+  // Interpolation with staged boxes when a wave's q are expected to fit one.
+  // For a detector-shaped list of rows of about sqrt(npix) pixels that span the
+  // kept grid, one pixel moves d = (L - W2) / sqrt(npix) cells. A square list
+  // (n x n) is taken in 16 x 4 pixel patches per wave: a box of about
+  // (16 d + W2 + 1)(4 d + W2 + 1)(W2 + 2) points; otherwise 64 consecutive q: a
+  // box of about (64 d + W2)(W2 + 1)^2.
   {
     int Lmax = X.L > Y.L ? X.L : Y.L;
     if (Z.L > Lmax) Lmax = Z.L;
-    const double span = 64.0 * (Lmax - W2) / sqrt((double)npix);
-    g.lds_interp = (span + W2) * (W2 + 1) * (W2 + 1) <= kBoxCap ? 1 : 0;
+    const double d = (Lmax - W2) / sqrt((double)npix);
+    const int n = (int)llround(sqrt((double)npix));
+    g.tile_n = (n >= 16 && (long long)n * n == npix) ? n : 0;
+    const double box = g.tile_n > 0 ? (16.0 * d + W2 + 1) * (4.0 * d + W2 + 1) * (W2 + 2)
+                                    : (64.0 * d + W2) * (W2 + 1) * (W2 + 1);
+    g.lds_interp = box <= kBoxCap ? 1 : 0;
   }
   if (info) {
     const int v[12] = {X.M, Y.M, Z.M, X.nf, Y.nf, Z.nf, X.L, Y.L, Z.L, W2, g.lds_interp, gauss ? Wsp : 0};
