@@ -103,7 +103,7 @@ struct NufftGeom {
   // > 0: Gaussian spreading; the correction is exp(2 pi^2 gauss_v0 t^2) per axis
   float gauss_v0;
   int lds_interp;  // host: interpolate with nufft_interp_lds (else nufft_interp)
-  // > 0: nufft_interp_lds reads the q list as a tile_n x tile_n image, one
+  // > 0: the interpolation reads the q list as a tile_n x tile_n image, one
   // 16 x 16 pixel tile per workgroup (16 x 4 per wave)
   int tile_n;
   float corr[kCorrDeg + 1];
@@ -945,6 +945,20 @@ __device__ __forceinline__ float q_s2(const double q0[3]) {
   return (float)((q0[0] * q0[0] + q0[1] * q0[1] + q0[2] * q0[2]) * 2.5e-21);
 }
 
+// The q of a lane: with g.tile_n > 0 the list is a tile_n x tile_n image read
+// in 16 x 16 pixel tiles per workgroup (16 x 4 per wave, so neighbouring lanes
+// have neighbouring q in both directions); otherwise consecutive q. Returns
+// npix for a lane past the image.
+__device__ __forceinline__ int interp_q_index(const NufftGeom &g, int npix) {
+  if (g.tile_n > 0) {
+    const int n = g.tile_n, ntx = (n + 15) / 16, lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+    const int x = (int)(blockIdx.x % ntx) * 16 + lane % 16, y = (int)(blockIdx.x / ntx) * 16 + wave * 4 + lane / 16;
+    return (x < n && y < n) ? y * n + x : npix;
+  }
+  const int k = blockIdx.x * kThreads + threadIdx.x;
+  return k < npix ? k : npix;
+}
+
 // One thread per q: W^3 gathers (W: the interpolation window's width) per
 // species grid [ch][Lz][Ly][Lx] from global memory, window correction, form
 // factor, species sum and the q phase. Patterns of a batch (one species grid
@@ -957,7 +971,7 @@ __global__ __launch_bounds__(kThreads) void nufft_interp(const float *__restrict
                                                         const float *__restrict__ ff, float b_global,
                                                         const double *__restrict__ rots, int p0,
                                                         float2 *__restrict__ F, float *__restrict__ I) {
-  const int k = blockIdx.x * kThreads + threadIdx.x;
+  const int k = interp_q_index(g, npix);
   if (k >= npix) return;
   const int pat = p0 + (int)blockIdx.y;
   InterpQ<W> s;
@@ -999,14 +1013,7 @@ __global__ __launch_bounds__(kThreads) void nufft_interp_lds(const float *__rest
                                                             float2 *__restrict__ F, float *__restrict__ I) {
   __shared__ float2 box[kThreads / 64][kBoxCap];
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-  int k;  // this lane's q (npix: none)
-  if (g.tile_n > 0) {
-    const int n = g.tile_n, ntx = (n + 15) / 16;
-    const int x = (int)(blockIdx.x % ntx) * 16 + lane % 16, y = (int)(blockIdx.x / ntx) * 16 + wave * 4 + lane / 16;
-    k = (x < n && y < n) ? y * n + x : npix;
-  } else {
-    k = blockIdx.x * kThreads + threadIdx.x;
-  }
+  const int k = interp_q_index(g, npix);  // npix: no q
   const int pat = p0 + (int)blockIdx.y;
   InterpQ<W> s;
   if (k < npix) {
@@ -1320,15 +1327,14 @@ int launch_interp(const NufftGeom &g, int nsegs, const spsim_hip_segment *segs, 
                   hipStream_t stream) {
   for (int p0 = 0; p0 < npat; p0 += 65535) {
     const int n = npat - p0 < 65535 ? npat - p0 : 65535;
-    if (g.lds_interp) {
-      const int nt = (g.tile_n + 15) / 16;
-      const unsigned gx = g.tile_n > 0 ? (unsigned)(nt * nt) : (unsigned)((npix + kThreads - 1) / kThreads);
+    const int nt = (g.tile_n + 15) / 16;
+    const unsigned gx = g.tile_n > 0 ? (unsigned)(nt * nt) : (unsigned)((npix + kThreads - 1) / kThreads);
+    if (g.lds_interp)
       hipLaunchKernelGGL(nufft_interp_lds<W>, dim3(gx, n), dim3(kThreads), 0, stream, hkl, npix, g, H, nsegs, segs,
                          ff, b_global, rots, p0, F, I);
-    } else {
-      hipLaunchKernelGGL(nufft_interp<W>, dim3((npix + kThreads - 1) / kThreads, n), dim3(kThreads), 0, stream, hkl,
-                         npix, g, H, nsegs, segs, ff, b_global, rots, p0, F, I);
-    }
+    else
+      hipLaunchKernelGGL(nufft_interp<W>, dim3(gx, n), dim3(kThreads), 0, stream, hkl, npix, g, H, nsegs, segs, ff,
+                         b_global, rots, p0, F, I);
     SPSIM_HIP_TRY(hipGetLastError());
   }
   return SPSIM_HIP_OK;
