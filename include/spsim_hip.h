@@ -219,6 +219,22 @@ int spsim_hip_sf_nfft_batch(const float *d_hkl, int npix, const spsim_hip_atom *
                             int npatterns, const double *rot9s, double eps, float *d_F, float *d_I, int *info,
                             void *stream);
 
+/* Many molecules through the NUFFT, one pattern each: pattern m is
+   spsim_hip_sf_nfft's result for molecule m at q rotated by rot9s[m] (host
+   [nmol][9], row-major; NULL: no rotation), written to d_F + m*npix*2 and
+   d_I + m*npix. Molecule m owns the packed atoms (and B factors, when
+   d_atom_b is given) [atom_offsets[m], atom_offsets[m+1]) and the species
+   segments [seg_offsets[m], seg_offsets[m+1]), whose starts count from the
+   molecule's first atom (host offset arrays of nmol + 1 entries, from 0). The
+   bounds of every molecule are read with one synchronisation; the molecules'
+   stages are then enqueued back to back on `stream`, so the host does not wait
+   between molecules. Same grids, accuracy and workspace use as
+   spsim_hip_sf_nfft. */
+int spsim_hip_sf_nfft_molecules(const float *d_hkl, int npix, int nmol, const spsim_hip_atom *d_atoms,
+                                const float *d_atom_b, const int *atom_offsets, const spsim_hip_segment *d_segs,
+                                const int *seg_offsets, const float *d_ff, float b_global, const double *rot9s,
+                                double eps, float *d_F, float *d_I, void *stream);
+
 /* The pruned FFT stage of spsim_hip_sf_nfft on its own: every line of the
    complex input view [outer][M][inner] is placed at points m = m_lo + i
    (i < M) of a zero line of nf points (slot m mod nf), transformed forward

@@ -1424,6 +1424,24 @@ namespace {
 // one set of species grids for the unrotated atoms over the ball
 // |q| <= max |q| (the box of every R q), then each pattern p interpolates at
 // q' = R_p q (rots: host [npat][9]) into F/I + p npix.
+// hb[v] = the minimum (v < 6) or maximum of nblk workgroup records of
+// bounds_partial; NaN propagates.
+void fold_bounds(const double *hpart, int nblk, double *hb) {
+  for (int v = 0; v < kNb; ++v) {
+    double acc = v < 6 ? INFINITY : -INFINITY;
+    for (int b = 0; b < nblk; ++b) {
+      const double x = hpart[(size_t)b * kNb + v];
+      acc = (acc != acc || x != x) ? acc + x : (v < 6 ? fmin(acc, x) : fmax(acc, x));
+    }
+    hb[v] = acc;
+  }
+}
+
+int nufft_run(double *hb, const float *d_hkl, int npix, const spsim_hip_atom *d_atoms, const float *d_atom_b,
+              const spsim_hip_segment *d_segs, int nsegs, int natoms, const float *d_ff, float b_global,
+              const AtomRot &R, int npat, const double *rots, int W, double eps, float *d_F, float *d_I, int *info,
+              hipStream_t s);
+
 int nufft_impl(const float *d_hkl, int npix, const spsim_hip_atom *d_atoms, const float *d_atom_b,
                const spsim_hip_segment *d_segs, int nsegs, int natoms, const float *d_ff, float b_global,
                const double *rot9, int npat, const double *rots, double eps, float *d_F, float *d_I, int *info,
@@ -1473,14 +1491,19 @@ int nufft_impl(const float *d_hkl, int npix, const spsim_hip_atom *d_atoms, cons
   double hpart[kNb * kBoundBlocks], hb[kNb];
   SPSIM_HIP_TRY(hipMemcpyAsync(hpart, part, sizeof(double) * kNb * nblk, hipMemcpyDeviceToHost, s));
   SPSIM_HIP_TRY(hipStreamSynchronize(s));
-  for (int v = 0; v < kNb; ++v) {
-    double acc = v < 6 ? INFINITY : -INFINITY;
-    for (int b = 0; b < nblk; ++b) {
-      const double x = hpart[(size_t)b * kNb + v];
-      acc = (acc != acc || x != x) ? acc + x : (v < 6 ? fmin(acc, x) : fmax(acc, x));
-    }
-    hb[v] = acc;
-  }
+  fold_bounds(hpart, nblk, hb);
+  return nufft_run(hb, d_hkl, npix, d_atoms, d_atom_b, d_segs, nsegs, natoms, d_ff, b_global, R, npat, rots, W, eps,
+                   d_F, d_I, info, s);
+}
+
+// Plans the grids from the folded bounds hb and enqueues every stage after the
+// bounds on s (a batch synchronises at the end for its rotations' host buffer).
+int nufft_run(double *hb, const float *d_hkl, int npix, const spsim_hip_atom *d_atoms, const float *d_atom_b,
+              const spsim_hip_segment *d_segs, int nsegs, int natoms, const float *d_ff, float b_global,
+              const AtomRot &R, int npat, const double *rots, int W, double eps, float *d_F, float *d_I, int *info,
+              hipStream_t s) {
+  const int nout = npat > 0 ? npat : 1;
+  void *ws = nullptr;
   for (int v = 0; v <= 12; ++v)
     if (!std::isfinite(hb[v])) return fail(SPSIM_HIP_ERR_INVALID, "non-finite atom position or q");
   const bool gauss = d_atom_b != nullptr;
@@ -1721,6 +1744,74 @@ extern "C" int spsim_hip_sf_nfft_batch(const float *d_hkl, int npix, const spsim
   if (npatterns < 1) return spsim::fail(SPSIM_HIP_ERR_INVALID, "npatterns=%d", npatterns);
   return spsim::nufft_impl(d_hkl, npix, d_atoms, d_atom_b, d_segs, nsegs, natoms, d_ff, b_global, nullptr, npatterns,
                            rot9s, eps, d_F, d_I, info, (hipStream_t)stream);
+}
+
+extern "C" int spsim_hip_sf_nfft_molecules(const float *d_hkl, int npix, int nmol, const spsim_hip_atom *d_atoms,
+                                           const float *d_atom_b, const int *atom_offsets,
+                                           const spsim_hip_segment *d_segs, const int *seg_offsets,
+                                           const float *d_ff, float b_global, const double *rot9s, double eps,
+                                           float *d_F, float *d_I, void *stream) {
+  using namespace spsim;
+  hipStream_t s = (hipStream_t)stream;
+  if (nmol < 0 || npix < 0) return fail(SPSIM_HIP_ERR_INVALID, "negative size");
+  if (nmol == 0 || npix == 0) return SPSIM_HIP_OK;
+  if (!atom_offsets || !seg_offsets || !d_hkl || !d_ff) return fail(SPSIM_HIP_ERR_INVALID, "NULL input pointer");
+  if (!d_F && !d_I) return fail(SPSIM_HIP_ERR_INVALID, "both outputs are NULL");
+  if (atom_offsets[0] != 0 || seg_offsets[0] != 0) return fail(SPSIM_HIP_ERR_INVALID, "offsets must start at 0");
+  for (int m = 0; m < nmol; ++m) {
+    const int na = atom_offsets[m + 1] - atom_offsets[m], ns = seg_offsets[m + 1] - seg_offsets[m];
+    if (na < 0 || ns < 0 || ns > SPSIM_HIP_NSPECIES) return fail(SPSIM_HIP_ERR_INVALID, "molecule %d: bad offsets", m);
+    if (na > 0 && (!d_atoms || !d_segs || ns < 1)) return fail(SPSIM_HIP_ERR_INVALID, "molecule %d: no atoms or segments", m);
+    if (rot9s)
+      for (int i = 0; i < 9; ++i)
+        if (!std::isfinite(rot9s[9 * (size_t)m + i])) return fail(SPSIM_HIP_ERR_INVALID, "non-finite rotation");
+  }
+  SPSIM_TRY(ensure_device(nullptr));
+  const int W = width_for(eps);
+  // 1. Bounds of every molecule (one synchronisation for all of them).
+  constexpr int kMolBlocks = 64;
+  void *ws = nullptr;
+  SPSIM_TRY(workspace(align256(sizeof(double) * kNb * kMolBlocks * (size_t)nmol), &ws));
+  double *part = (double *)ws;
+  std::vector<int> nblk(nmol, 0);
+  std::vector<AtomRot> Rs(nmol);
+  for (int m = 0; m < nmol; ++m) {
+    const int na = atom_offsets[m + 1] - atom_offsets[m];
+    Rs[m] = AtomRot{};
+    if (rot9s) {
+      Rs[m].on = 1;
+      for (int i = 0; i < 9; ++i) Rs[m].r[i] = rot9s[9 * (size_t)m + i];
+    }
+    if (na == 0) continue;
+    const int most = na > npix ? na : npix;
+    nblk[m] = (most + kThreads - 1) / kThreads < kMolBlocks ? (most + kThreads - 1) / kThreads : kMolBlocks;
+    hipLaunchKernelGGL(bounds_partial, dim3(nblk[m]), dim3(kThreads), 0, s, d_atoms + atom_offsets[m],
+                       d_atom_b ? d_atom_b + atom_offsets[m] : nullptr, na, d_hkl, npix, Rs[m],
+                       part + (size_t)m * kMolBlocks * kNb);
+    SPSIM_HIP_TRY(hipGetLastError());
+  }
+  std::vector<double> hpart((size_t)nmol * kMolBlocks * kNb);
+  SPSIM_HIP_TRY(hipMemcpyAsync(hpart.data(), part, sizeof(double) * hpart.size(), hipMemcpyDeviceToHost, s));
+  SPSIM_HIP_TRY(hipStreamSynchronize(s));
+  // 2. Every molecule's grids and stages, enqueued back to back.
+  for (int m = 0; m < nmol; ++m) {
+    const int na = atom_offsets[m + 1] - atom_offsets[m], ns = seg_offsets[m + 1] - seg_offsets[m];
+    float *F = d_F ? d_F + (size_t)m * npix * 2 : nullptr;
+    float *I = d_I ? d_I + (size_t)m * npix : nullptr;
+    if (na == 0) {
+      const size_t zb = ((size_t)npix + kThreads - 1) / kThreads;
+      hipLaunchKernelGGL(zero_pattern, dim3((unsigned)(zb < 4096 ? zb : 4096)), dim3(kThreads), 0, s, (size_t)npix,
+                         (float2 *)F, I);
+      SPSIM_HIP_TRY(hipGetLastError());
+      continue;
+    }
+    double hb[kNb];
+    fold_bounds(hpart.data() + (size_t)m * kMolBlocks * kNb, nblk[m], hb);
+    SPSIM_TRY(nufft_run(hb, d_hkl, npix, d_atoms + atom_offsets[m], d_atom_b ? d_atom_b + atom_offsets[m] : nullptr,
+                        d_segs + seg_offsets[m], ns, na, d_ff, b_global, Rs[m], 0, nullptr, W, eps, F, I, nullptr,
+                        s));
+  }
+  return SPSIM_HIP_OK;
 }
 
 extern "C" int spsim_hip_fft_pruned(const float *d_in, long long outer, long long inner, int M, int m_lo, int nf,

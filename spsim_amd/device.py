@@ -228,6 +228,53 @@ def structure_factor_nfft_batch(hkl: torch.Tensor, atoms: DeviceAtoms, ff: torch
     return (F if want_F else None), I, {"W": v[0], "M": tuple(v[1:4]), "nf": tuple(v[4:7]), "L": tuple(v[7:10]), "W2": v[10], "staged": v[11], "Wg": v[12]}
 
 
+class DeviceMolecules:
+    """Several molecules packed one after another in HBM for
+    spsim_hip_sf_nfft_molecules: atoms [n, 4] (and B factors), segments whose
+    starts count from each molecule's first atom, and the host offset arrays."""
+
+    def __init__(self, packed_list, device=None):
+        dev = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+        self.nmol = len(packed_list)
+        self.atom_offsets = np.zeros(self.nmol + 1, np.int32)
+        self.seg_offsets = np.zeros(self.nmol + 1, np.int32)
+        for m, p in enumerate(packed_list):
+            self.atom_offsets[m + 1] = self.atom_offsets[m] + p.natoms
+            self.seg_offsets[m + 1] = self.seg_offsets[m] + p.segs.shape[0]
+        atoms = [p.atoms.reshape(-1, 4)[:p.natoms] for p in packed_list]
+        segs = [p.segs for p in packed_list]
+        has_b = any(p.b is not None for p in packed_list)
+        if has_b and not all(p.b is not None for p in packed_list):
+            raise ValueError("either every molecule has per-atom B factors or none has")
+        self.atoms = torch.from_numpy(np.ascontiguousarray(np.concatenate(atoms + [np.zeros((1, 4), np.float32)]))).to(dev)
+        self.segs = torch.from_numpy(np.ascontiguousarray(np.concatenate(segs + [np.zeros((1, 4), np.int32)]))).to(dev)
+        self.b = None
+        if has_b:
+            self.b = torch.from_numpy(np.ascontiguousarray(np.concatenate([p.b[:p.natoms] for p in packed_list]
+                                                                          + [np.zeros(1, np.float32)]))).to(dev)
+
+
+def structure_factor_nfft_molecules(hkl: torch.Tensor, mols: DeviceMolecules, ff: torch.Tensor, rots=None,
+                                    B: float = 0.0, eps: float = 1e-6, F: torch.Tensor | None = None,
+                                    I: torch.Tensor | None = None, stream=None, want_F: bool = True):
+    """One pattern per molecule through the NUFFT (spsim_hip_sf_nfft_molecules),
+    molecule m at q rotated exactly by rots[m] (None: no rotation). Returns
+    (F [nmol, npix, 2] or None, I [nmol, npix])."""
+    npix = int(hkl.shape[0])
+    if want_F and F is None:
+        F = torch.empty((mols.nmol, npix, 2), dtype=torch.float32, device=hkl.device)
+    if I is None:
+        I = torch.empty((mols.nmol, npix), dtype=torch.float32, device=hkl.device)
+    R = None if rots is None else np.ascontiguousarray(rots, np.float64).reshape(mols.nmol, 9)
+    rc = _lib.load().spsim_hip_sf_nfft_molecules(
+        _ptr(hkl), npix, mols.nmol, _ptr(mols.atoms), _ptr(mols.b), mols.atom_offsets.ctypes.data_as(C.c_void_p),
+        _ptr(mols.segs), mols.seg_offsets.ctypes.data_as(C.c_void_p), _ptr(ff), float(B),
+        None if R is None else R.ctypes.data_as(C.c_void_p), float(eps), _ptr(F if want_F else None), _ptr(I),
+        _stream(stream))
+    check(rc, "spsim_hip_sf_nfft_molecules")
+    return (F if want_F else None), I
+
+
 def fft_pruned(x: torch.Tensor, M: int, m_lo: int, nf: int, L: int, l_lo: int, stream=None) -> torch.Tensor:
     """Pruned forward FFT along axis 1 of a complex64 device tensor [outer, M, inner]
     (spsim_hip_fft_pruned); returns [outer, L, inner]."""
