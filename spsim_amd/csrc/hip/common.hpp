@@ -22,6 +22,15 @@ int ensure_device(int *cus);
 int workspace(size_t bytes, void **ptr);
 void workspace_release();
 
+// Streams the library forks independent work onto, kHelperStreams per device,
+// created at the first call at the default priority and kept until the process
+// ends, with kHelperStreams + 1 events for the fork and the join (handles
+// copied out). Calls that
+// use them on one device at the same time race, as calls sharing the
+// workspace do.
+constexpr int kHelperStreams = 4;
+int helper_streams(hipStream_t streams[kHelperStreams], hipEvent_t events[kHelperStreams + 1]);
+
 constexpr double kLog2e = 1.4426950408889634073599246810019;
 
 }  // namespace spsim

@@ -1440,7 +1440,7 @@ void fold_bounds(const double *hpart, int nblk, double *hb) {
 int nufft_run(double *hb, const float *d_hkl, int npix, const spsim_hip_atom *d_atoms, const float *d_atom_b,
               const spsim_hip_segment *d_segs, int nsegs, int natoms, const float *d_ff, float b_global,
               const AtomRot &R, int npat, const double *rots, int W, double eps, float *d_F, float *d_I, int *info,
-              hipStream_t s);
+              hipStream_t s, size_t *need = nullptr, void *ws_in = nullptr);
 
 int nufft_impl(const float *d_hkl, int npix, const spsim_hip_atom *d_atoms, const float *d_atom_b,
                const spsim_hip_segment *d_segs, int nsegs, int natoms, const float *d_ff, float b_global,
@@ -1498,10 +1498,12 @@ int nufft_impl(const float *d_hkl, int npix, const spsim_hip_atom *d_atoms, cons
 
 // Plans the grids from the folded bounds hb and enqueues every stage after the
 // bounds on s (a batch synchronises at the end for its rotations' host buffer).
+// With need set it only plans and stores the workspace bytes the stages take;
+// with ws_in set the stages use that buffer instead of the library workspace.
 int nufft_run(double *hb, const float *d_hkl, int npix, const spsim_hip_atom *d_atoms, const float *d_atom_b,
               const spsim_hip_segment *d_segs, int nsegs, int natoms, const float *d_ff, float b_global,
               const AtomRot &R, int npat, const double *rots, int W, double eps, float *d_F, float *d_I, int *info,
-              hipStream_t s) {
+              hipStream_t s, size_t *need, void *ws_in) {
   const int nout = npat > 0 ? npat : 1;
   void *ws = nullptr;
   for (int v = 0; v <= 12; ++v)
@@ -1636,7 +1638,15 @@ int nufft_run(double *hb, const float *d_hkl, int npix, const spsim_hip_atom *d_
   const size_t b_part = nsplit > 1 ? align256(sizeof(float2) * nG * nsplit) : 0;
   const long long nchunks = (bins.nbins + kScanChunk - 1) / kScanChunk;
   const size_t b_scan = align256(sizeof(int) * (size_t)nchunks);
-  SPSIM_TRY(workspace(b_dec + 2 * b_rec + 3 * b_bin + b_tw + b_rot + b1 + b2 + b_part + 2 * b_scan, &ws));
+  const size_t b_all = b_dec + 2 * b_rec + 3 * b_bin + b_tw + b_rot + b1 + b2 + b_part + 2 * b_scan;
+  if (need) {
+    *need = b_all;
+    return SPSIM_HIP_OK;
+  }
+  if (ws_in)
+    ws = ws_in;
+  else
+    SPSIM_TRY(workspace(b_all, &ws));
   char *wp = (char *)ws;
   float *dec = (float *)wp;
   wp += b_dec;
@@ -1793,25 +1803,61 @@ extern "C" int spsim_hip_sf_nfft_molecules(const float *d_hkl, int npix, int nmo
   std::vector<double> hpart((size_t)nmol * kMolBlocks * kNb);
   SPSIM_HIP_TRY(hipMemcpyAsync(hpart.data(), part, sizeof(double) * hpart.size(), hipMemcpyDeviceToHost, s));
   SPSIM_HIP_TRY(hipStreamSynchronize(s));
-  // 2. Every molecule's grids and stages, enqueued back to back.
+  // 2. Every molecule's grids planned on the host (workspace bytes only), then
+  // the stages enqueued back to back on kHelperStreams streams forked from s:
+  // molecule m on stream m mod kHelperStreams, each stream with a workspace
+  // slice as large as its largest molecule's. Small molecules' kernels fill
+  // few CUs, so the streams overlap them; s waits for all of them.
+  std::vector<double> hbs((size_t)nmol * kNb);
+  std::vector<size_t> need(nmol, 0);
   for (int m = 0; m < nmol; ++m) {
     const int na = atom_offsets[m + 1] - atom_offsets[m], ns = seg_offsets[m + 1] - seg_offsets[m];
+    if (na == 0) continue;
+    double *hb = hbs.data() + (size_t)m * kNb;
+    fold_bounds(hpart.data() + (size_t)m * kMolBlocks * kNb, nblk[m], hb);
+    double hcopy[kNb];
+    for (int v = 0; v < kNb; ++v) hcopy[v] = hb[v];
+    SPSIM_TRY(nufft_run(hcopy, d_hkl, npix, d_atoms + atom_offsets[m], d_atom_b ? d_atom_b + atom_offsets[m] : nullptr,
+                        d_segs + seg_offsets[m], ns, na, d_ff, b_global, Rs[m], 0, nullptr, W, eps, nullptr, nullptr,
+                        nullptr, s, &need[m]));
+  }
+  size_t slice[kHelperStreams] = {}, off[kHelperStreams] = {}, total = 0;
+  for (int m = 0; m < nmol; ++m)
+    if (need[m] > slice[m % kHelperStreams]) slice[m % kHelperStreams] = need[m];
+  for (int k = 0; k < kHelperStreams; ++k) {
+    off[k] = total;
+    total += align256(slice[k]);
+  }
+  SPSIM_TRY(workspace(total > 0 ? total : 256, &ws));  // the bounds' records are on the host already
+  hipStream_t hs[kHelperStreams];
+  hipEvent_t ev[kHelperStreams + 1];
+  SPSIM_TRY(helper_streams(hs, ev));
+  SPSIM_HIP_TRY(hipEventRecord(ev[kHelperStreams], s));
+  for (int k = 0; k < kHelperStreams; ++k) SPSIM_HIP_TRY(hipStreamWaitEvent(hs[k], ev[kHelperStreams], 0));
+  int rc = SPSIM_HIP_OK;
+  for (int m = 0; m < nmol && rc == SPSIM_HIP_OK; ++m) {
+    const int na = atom_offsets[m + 1] - atom_offsets[m], ns = seg_offsets[m + 1] - seg_offsets[m];
+    const int k = m % kHelperStreams;
     float *F = d_F ? d_F + (size_t)m * npix * 2 : nullptr;
     float *I = d_I ? d_I + (size_t)m * npix : nullptr;
     if (na == 0) {
       const size_t zb = ((size_t)npix + kThreads - 1) / kThreads;
-      hipLaunchKernelGGL(zero_pattern, dim3((unsigned)(zb < 4096 ? zb : 4096)), dim3(kThreads), 0, s, (size_t)npix,
-                         (float2 *)F, I);
-      SPSIM_HIP_TRY(hipGetLastError());
+      hipLaunchKernelGGL(zero_pattern, dim3((unsigned)(zb < 4096 ? zb : 4096)), dim3(kThreads), 0, hs[k],
+                         (size_t)npix, (float2 *)F, I);
+      const hipError_t e = hipGetLastError();
+      if (e != hipSuccess) rc = hip_fail(e, "zero_pattern");
       continue;
     }
-    double hb[kNb];
-    fold_bounds(hpart.data() + (size_t)m * kMolBlocks * kNb, nblk[m], hb);
-    SPSIM_TRY(nufft_run(hb, d_hkl, npix, d_atoms + atom_offsets[m], d_atom_b ? d_atom_b + atom_offsets[m] : nullptr,
-                        d_segs + seg_offsets[m], ns, na, d_ff, b_global, Rs[m], 0, nullptr, W, eps, F, I, nullptr,
-                        s));
+    rc = nufft_run(hbs.data() + (size_t)m * kNb, d_hkl, npix, d_atoms + atom_offsets[m],
+                   d_atom_b ? d_atom_b + atom_offsets[m] : nullptr, d_segs + seg_offsets[m], ns, na, d_ff, b_global,
+                   Rs[m], 0, nullptr, W, eps, F, I, nullptr, hs[k], nullptr, (char *)ws + off[k]);
   }
-  return SPSIM_HIP_OK;
+  // join even after a failure, so that s orders after whatever was enqueued
+  for (int k = 0; k < kHelperStreams; ++k) {
+    SPSIM_HIP_TRY(hipEventRecord(ev[k], hs[k]));
+    SPSIM_HIP_TRY(hipStreamWaitEvent(s, ev[k], 0));
+  }
+  return rc;
 }
 
 extern "C" int spsim_hip_fft_pruned(const float *d_in, long long outer, long long inner, int M, int m_lo, int nf,

@@ -18,6 +18,9 @@ struct DeviceState {
   int cus = 0;
   void *ws = nullptr;
   size_t ws_bytes = 0;
+  int helpers = 0;  // 1 once hs and ev exist
+  hipStream_t hs[kHelperStreams] = {};
+  hipEvent_t ev[kHelperStreams + 1] = {};
 };
 std::mutex g_mu;
 std::vector<DeviceState> g_dev;
@@ -85,6 +88,22 @@ int workspace(size_t bytes, void **ptr) {
     st->ws_bytes = want;
   }
   *ptr = st->ws;
+  return SPSIM_HIP_OK;
+}
+
+int helper_streams(hipStream_t streams[kHelperStreams], hipEvent_t events[kHelperStreams + 1]) {
+  DeviceState *st = state_for_current(nullptr);
+  if (!st) return fail(SPSIM_HIP_ERR_NODEVICE, "no HIP device is available");
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!st->helpers) {
+    for (int k = 0; k < kHelperStreams; ++k)
+      if (!st->hs[k]) SPSIM_HIP_TRY(hipStreamCreateWithFlags(&st->hs[k], hipStreamNonBlocking));
+    for (int k = 0; k <= kHelperStreams; ++k)
+      if (!st->ev[k]) SPSIM_HIP_TRY(hipEventCreateWithFlags(&st->ev[k], hipEventDisableTiming));
+    st->helpers = 1;
+  }
+  for (int k = 0; k < kHelperStreams; ++k) streams[k] = st->hs[k];
+  for (int k = 0; k <= kHelperStreams; ++k) events[k] = st->ev[k];
   return SPSIM_HIP_OK;
 }
 

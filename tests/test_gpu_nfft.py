@@ -456,14 +456,17 @@ def test_nfft_per_atom_b_rotation_volume_and_high_b(gpu):
 
 def test_nfft_many_molecules_in_one_call(gpu):
     """spsim_hip_sf_nfft_molecules: molecules of uneven sizes (one empty), each at
-    its own rotation, with one synchronisation for all their bounds. Every pattern
-    matches the FP64 oracle and the one-molecule call; with per-atom B factors too."""
+    its own rotation, with one synchronisation for all their bounds. There are
+    more molecules than the call's streams, so streams run several in turn over
+    their workspace slice. Every pattern matches the FP64 oracle and the
+    one-molecule call; with per-atom B factors too."""
     import torch
 
     q = detector_q(40, lam=0.2e-9, pix=2e-4)
     hkl = torch.from_numpy(q).to(gpu)
     ff = dv.form_factor_table(device=gpu)
-    spec = [(300, 2e-9, 61), (0, 0.0, 0), (900, 4e-9, 62), (60, 1e-9, 63)]
+    spec = [(300, 2e-9, 61), (0, 0.0, 0), (900, 4e-9, 62), (60, 1e-9, 63), (1500, 5e-9, 66), (120, 1.5e-9, 67),
+            (40, 0.8e-9, 68), (700, 3e-9, 69), (200, 2.5e-9, 70)]
     mols = []
     for n, r, seed in spec:
         if n == 0:
