@@ -157,14 +157,16 @@ __device__ __forceinline__ double bound_combine(int v, double a, double b) {
   return v < 6 ? nan_min(a, b) : nan_max(a, b);
 }
 
-__global__ __launch_bounds__(kThreads) void bounds_partial(const spsim_hip_atom *__restrict__ atoms,
-                                                          const float *__restrict__ atom_b, int natoms,
-                                                          const float *__restrict__ hkl, int npix, AtomRot R,
-                                                          double *__restrict__ part) {
+// One workgroup's record over atoms (and q) blk, blk + nblk, ... in steps of
+// kThreads-wide strips, written to rec[0 .. kNb).
+__device__ __forceinline__ void bounds_block(const spsim_hip_atom *__restrict__ atoms,
+                                             const float *__restrict__ atom_b, int natoms,
+                                             const float *__restrict__ hkl, int npix, const AtomRot &R, int blk,
+                                             int nblk, double *__restrict__ rec) {
   double acc[kNb];
   for (int v = 0; v < kNb; ++v) acc[v] = v < 6 ? INFINITY : -INFINITY;
-  const int stride = gridDim.x * kThreads;
-  for (int i = blockIdx.x * kThreads + threadIdx.x; i < natoms; i += stride) {
+  const int stride = nblk * kThreads;
+  for (int i = blk * kThreads + threadIdx.x; i < natoms; i += stride) {
     const spsim_hip_atom a = atoms[i];
     double p[3];
     rotate_atom(R, a.x, a.y, a.z, p);
@@ -177,7 +179,7 @@ __global__ __launch_bounds__(kThreads) void bounds_partial(const spsim_hip_atom 
       acc[14] = nan_max(acc[14], -(double)atom_b[i]);
     }
   }
-  for (int i = blockIdx.x * kThreads + threadIdx.x; i < npix; i += stride) {
+  for (int i = blk * kThreads + threadIdx.x; i < npix; i += stride) {
     double n2 = 0.0;
     for (int v = 0; v < 3; ++v) {
       const double q = hkl[3 * (size_t)i + v];
@@ -195,7 +197,34 @@ __global__ __launch_bounds__(kThreads) void bounds_partial(const spsim_hip_atom 
       for (int v = 0; v < kNb; ++v) red[v][threadIdx.x] = bound_combine(v, red[v][threadIdx.x], red[v][threadIdx.x + w]);
     __syncthreads();
   }
-  if (threadIdx.x < kNb) part[(size_t)blockIdx.x * kNb + threadIdx.x] = red[threadIdx.x][0];
+  if (threadIdx.x < kNb) rec[threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ __launch_bounds__(kThreads) void bounds_partial(const spsim_hip_atom *__restrict__ atoms,
+                                                          const float *__restrict__ atom_b, int natoms,
+                                                          const float *__restrict__ hkl, int npix, AtomRot R,
+                                                          double *__restrict__ part) {
+  bounds_block(atoms, atom_b, natoms, hkl, npix, R, blockIdx.x, gridDim.x, part + (size_t)blockIdx.x * kNb);
+}
+
+// The atoms' records of many molecules in one launch: workgroup (x, m) covers
+// molecule m's atoms [off[m], off[m + 1]) rotated by rot[9 m ..] (rot NULL: no
+// rotation), into part[(m gridDim.x + x) kNb ..]. The q entries stay at their
+// empty values (+inf minima, -inf maxima).
+__global__ __launch_bounds__(kThreads) void bounds_molecules(const spsim_hip_atom *__restrict__ atoms,
+                                                            const float *__restrict__ atom_b,
+                                                            const int *__restrict__ off,
+                                                            const double *__restrict__ rot,
+                                                            double *__restrict__ part) {
+  const int m = blockIdx.y;
+  AtomRot R{};
+  if (rot) {
+    R.on = 1;
+    for (int i = 0; i < 9; ++i) R.r[i] = rot[9 * (size_t)m + i];
+  }
+  const int a0 = off[m];
+  bounds_block(atoms + a0, atom_b ? atom_b + a0 : nullptr, off[m + 1] - a0, nullptr, 0, R, blockIdx.x, gridDim.x,
+               part + ((size_t)m * gridDim.x + blockIdx.x) * kNb);
 }
 
 // The per-call tables in one launch, thread i filling entry i of each:
@@ -1778,31 +1807,48 @@ extern "C" int spsim_hip_sf_nfft_molecules(const float *d_hkl, int npix, int nmo
   }
   SPSIM_TRY(ensure_device(nullptr));
   const int W = width_for(eps);
-  // 1. Bounds of every molecule (one synchronisation for all of them).
+  // 1. Bounds: the q list's once (every molecule shares it) and every
+  // molecule's atoms in one launch, read with one synchronisation.
   constexpr int kMolBlocks = 64;
+  if (nmol > 65535) return fail(SPSIM_HIP_ERR_INVALID, "at most 65535 molecules per call (%d)", nmol);
+  const int qblk = (npix + kThreads - 1) / kThreads < kBoundBlocks ? (npix + kThreads - 1) / kThreads : kBoundBlocks;
+  const size_t b_q = align256(sizeof(double) * kNb * qblk);
+  const size_t b_part = align256(sizeof(double) * kNb * kMolBlocks * (size_t)nmol);
+  const size_t b_off = align256(sizeof(int) * ((size_t)nmol + 1));
+  const size_t b_rot = rot9s ? align256(sizeof(double) * 9 * (size_t)nmol) : 0;
   void *ws = nullptr;
-  SPSIM_TRY(workspace(align256(sizeof(double) * kNb * kMolBlocks * (size_t)nmol), &ws));
-  double *part = (double *)ws;
-  std::vector<int> nblk(nmol, 0);
+  SPSIM_TRY(workspace(b_q + b_part + b_off + b_rot, &ws));
+  double *qpart = (double *)ws;
+  double *part = (double *)((char *)ws + b_q);
+  int *d_off = (int *)((char *)ws + b_q + b_part);
+  double *d_rot = rot9s ? (double *)((char *)ws + b_q + b_part + b_off) : nullptr;
   std::vector<AtomRot> Rs(nmol);
   for (int m = 0; m < nmol; ++m) {
-    const int na = atom_offsets[m + 1] - atom_offsets[m];
     Rs[m] = AtomRot{};
     if (rot9s) {
       Rs[m].on = 1;
       for (int i = 0; i < 9; ++i) Rs[m].r[i] = rot9s[9 * (size_t)m + i];
     }
-    if (na == 0) continue;
-    const int most = na > npix ? na : npix;
-    nblk[m] = (most + kThreads - 1) / kThreads < kMolBlocks ? (most + kThreads - 1) / kThreads : kMolBlocks;
-    hipLaunchKernelGGL(bounds_partial, dim3(nblk[m]), dim3(kThreads), 0, s, d_atoms + atom_offsets[m],
-                       d_atom_b ? d_atom_b + atom_offsets[m] : nullptr, na, d_hkl, npix, Rs[m],
-                       part + (size_t)m * kMolBlocks * kNb);
+  }
+  hipLaunchKernelGGL(bounds_partial, dim3(qblk), dim3(kThreads), 0, s, nullptr, nullptr, 0, d_hkl, npix, AtomRot{},
+                     qpart);
+  SPSIM_HIP_TRY(hipGetLastError());
+  if (atom_offsets[nmol] > 0) {
+    // pageable sources: both copies have completed by the synchronisation below
+    SPSIM_HIP_TRY(hipMemcpyAsync(d_off, atom_offsets, sizeof(int) * ((size_t)nmol + 1), hipMemcpyHostToDevice, s));
+    if (rot9s)
+      SPSIM_HIP_TRY(hipMemcpyAsync(d_rot, rot9s, sizeof(double) * 9 * (size_t)nmol, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(bounds_molecules, dim3(kMolBlocks, (unsigned)nmol), dim3(kThreads), 0, s, d_atoms, d_atom_b,
+                       d_off, d_rot, part);
     SPSIM_HIP_TRY(hipGetLastError());
   }
-  std::vector<double> hpart((size_t)nmol * kMolBlocks * kNb);
-  SPSIM_HIP_TRY(hipMemcpyAsync(hpart.data(), part, sizeof(double) * hpart.size(), hipMemcpyDeviceToHost, s));
+  std::vector<double> hq((size_t)qblk * kNb), hpart((size_t)nmol * kMolBlocks * kNb);
+  SPSIM_HIP_TRY(hipMemcpyAsync(hq.data(), qpart, sizeof(double) * hq.size(), hipMemcpyDeviceToHost, s));
+  if (atom_offsets[nmol] > 0)
+    SPSIM_HIP_TRY(hipMemcpyAsync(hpart.data(), part, sizeof(double) * hpart.size(), hipMemcpyDeviceToHost, s));
   SPSIM_HIP_TRY(hipStreamSynchronize(s));
+  double hbq[kNb];
+  fold_bounds(hq.data(), qblk, hbq);
   // 2. Every molecule's grids planned on the host (workspace bytes only), then
   // the stages enqueued back to back on kHelperStreams streams forked from s:
   // molecule m on stream m mod kHelperStreams, each stream with a workspace
@@ -1814,7 +1860,8 @@ extern "C" int spsim_hip_sf_nfft_molecules(const float *d_hkl, int npix, int nmo
     const int na = atom_offsets[m + 1] - atom_offsets[m], ns = seg_offsets[m + 1] - seg_offsets[m];
     if (na == 0) continue;
     double *hb = hbs.data() + (size_t)m * kNb;
-    fold_bounds(hpart.data() + (size_t)m * kMolBlocks * kNb, nblk[m], hb);
+    fold_bounds(hpart.data() + (size_t)m * kMolBlocks * kNb, kMolBlocks, hb);
+    for (int v : {3, 4, 5, 9, 10, 11, 12}) hb[v] = hbq[v];
     double hcopy[kNb];
     for (int v = 0; v < kNb; ++v) hcopy[v] = hb[v];
     SPSIM_TRY(nufft_run(hcopy, d_hkl, npix, d_atoms + atom_offsets[m], d_atom_b ? d_atom_b + atom_offsets[m] : nullptr,
