@@ -228,9 +228,10 @@ int spsim_hip_sf_nfft_batch(const float *d_hkl, int npix, const spsim_hip_atom *
    molecule's first atom (host offset arrays of nmol + 1 entries, from 0). The
    bounds of every molecule are read with one synchronisation; the molecules'
    stages are then enqueued back to back, so the host does not wait between
-   molecules. They run on four library-owned streams forked from `stream` and
-   joined back into it (work enqueued on `stream` afterwards runs after every
-   pattern), each stream with its own slice of the library workspace. Calls
+   molecules. They run on `stream` and on four library-owned streams forked
+   from it and joined back into it (work enqueued on `stream` afterwards runs
+   after every pattern), each stream with its own slice of the library
+   workspace. Calls
    that overlap on one device race, as for every entry point sharing the
    workspace. Same grids and accuracy as spsim_hip_sf_nfft. */
 int spsim_hip_sf_nfft_molecules(const float *d_hkl, int npix, int nmol, const spsim_hip_atom *d_atoms,

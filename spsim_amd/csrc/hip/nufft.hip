@@ -1850,10 +1850,11 @@ extern "C" int spsim_hip_sf_nfft_molecules(const float *d_hkl, int npix, int nmo
   double hbq[kNb];
   fold_bounds(hq.data(), qblk, hbq);
   // 2. Every molecule's grids planned on the host (workspace bytes only), then
-  // the stages enqueued back to back on kHelperStreams streams forked from s:
-  // molecule m on stream m mod kHelperStreams, each stream with a workspace
-  // slice as large as its largest molecule's. Small molecules' kernels fill
-  // few CUs, so the streams overlap them; s waits for all of them.
+  // the stages enqueued back to back on kWorkers streams: the kHelperStreams
+  // streams forked from s, and s itself. Molecule m goes to worker
+  // m mod kWorkers, each worker with a workspace slice as large as its largest
+  // molecule's. Small molecules' kernels fill few CUs, so the streams overlap
+  // them; s then waits for the helpers.
   std::vector<double> hbs((size_t)nmol * kNb);
   std::vector<size_t> need(nmol, 0);
   for (int m = 0; m < nmol; ++m) {
@@ -1868,23 +1869,25 @@ extern "C" int spsim_hip_sf_nfft_molecules(const float *d_hkl, int npix, int nmo
                         d_segs + seg_offsets[m], ns, na, d_ff, b_global, Rs[m], 0, nullptr, W, eps, nullptr, nullptr,
                         nullptr, s, &need[m]));
   }
-  size_t slice[kHelperStreams] = {}, off[kHelperStreams] = {}, total = 0;
+  constexpr int kWorkers = kHelperStreams + 1;
+  size_t slice[kWorkers] = {}, off[kWorkers] = {}, total = 0;
   for (int m = 0; m < nmol; ++m)
-    if (need[m] > slice[m % kHelperStreams]) slice[m % kHelperStreams] = need[m];
-  for (int k = 0; k < kHelperStreams; ++k) {
+    if (need[m] > slice[m % kWorkers]) slice[m % kWorkers] = need[m];
+  for (int k = 0; k < kWorkers; ++k) {
     off[k] = total;
     total += align256(slice[k]);
   }
   SPSIM_TRY(workspace(total > 0 ? total : 256, &ws));  // the bounds' records are on the host already
-  hipStream_t hs[kHelperStreams];
+  hipStream_t hs[kWorkers];
   hipEvent_t ev[kHelperStreams + 1];
   SPSIM_TRY(helper_streams(hs, ev));
+  hs[kHelperStreams] = s;
   SPSIM_HIP_TRY(hipEventRecord(ev[kHelperStreams], s));
   for (int k = 0; k < kHelperStreams; ++k) SPSIM_HIP_TRY(hipStreamWaitEvent(hs[k], ev[kHelperStreams], 0));
   int rc = SPSIM_HIP_OK;
   for (int m = 0; m < nmol && rc == SPSIM_HIP_OK; ++m) {
     const int na = atom_offsets[m + 1] - atom_offsets[m], ns = seg_offsets[m + 1] - seg_offsets[m];
-    const int k = m % kHelperStreams;
+    const int k = m % kWorkers;
     float *F = d_F ? d_F + (size_t)m * npix * 2 : nullptr;
     float *I = d_I ? d_I + (size_t)m * npix : nullptr;
     if (na == 0) {
