@@ -126,6 +126,8 @@ struct FftPass {
   long long inner;   // stride between consecutive points of a line (1: lines are contiguous)
   int M, nf, L, m_lo, l_lo;
   int T;             // lines per workgroup (<= kMaxT)
+  int nsum;          // input = the sum of nsum grids sum_stride apart (fixed order; 1: one grid)
+  long long sum_stride;
   FastDiv dM, dL, dT;
   int nstages;
   FftStage st[kMaxStages];
@@ -554,7 +556,8 @@ struct __align__(16) SpreadCand {
 // block's 8 z-planes, in registers.
 // With few output blocks (small grids) the candidates of a block are split
 // round-robin by 64-atom batch over gridDim.y workgroups, each writing its own
-// partial grid (grid + blockIdx.y * grid_stride), which sum_partials adds up.
+// partial grid (grid + blockIdx.y * grid_stride), which the first FFT pass adds
+// up as it loads its lines.
 // GAUSS: Gaussian windows of per-atom width r.w <= W (else every window is the
 // exponential of semicircle of width W). CZ: complex z weights (the phase
 // exp(-2 pi i r.kz (m - u)) of a Gaussian on a z axis centred in frequency).
@@ -668,20 +671,6 @@ __global__ __launch_bounds__(64) void spread_gather(const SpreadAtom *__restrict
 #pragma unroll
     for (int z = 0; z < kB; ++z)
       if (z0 + z < Mz) out[(((size_t)ch * Mz + z0 + z) * My + gy) * Mx + gx] = make_float2(ar[z], ai[z]);
-  }
-}
-
-// grid[i] = sum over the nsplit partial grids (fixed order), i < n.
-__global__ __launch_bounds__(kThreads) void sum_partials(const float2 *__restrict__ part, int nsplit, size_t n,
-                                                        float2 *__restrict__ grid) {
-  for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (size_t)gridDim.x * kThreads) {
-    float2 acc = part[i];
-    for (int sidx = 1; sidx < nsplit; ++sidx) {
-      const float2 v = part[(size_t)sidx * n + i];
-      acc.x += v.x;
-      acc.y += v.y;
-    }
-    grid[i] = acc;
   }
 }
 
@@ -832,7 +821,14 @@ __global__ __launch_bounds__(kThreads) void fft_lines(FftPass p, const float2 *_
     int slot = p.m_lo + i;
     slot += slot < 0 ? nf : 0;
     slot -= slot >= nf ? nf : 0;
-    x[t * nf + slot] = in[b + (long long)i * p.inner];
+    const long long at = b + (long long)i * p.inner;
+    float2 v = in[at];
+    for (int k = 1; k < p.nsum; ++k) {
+      const float2 w = in[at + k * p.sum_stride];
+      v.x += w.x;
+      v.y += w.y;
+    }
+    x[t * nf + slot] = v;
   }
   __syncthreads();
   for (int s = 0; s < p.nstages; ++s) {
@@ -1294,6 +1290,8 @@ FftPass make_pass(const NufftDim &D, long long outer, long long inner) {
   p.m_lo = D.m_lo;
   p.l_lo = D.l_lo;
   p.T = kFftLds / D.nf;
+  p.nsum = 1;
+  p.sum_stride = 0;
   if (p.T < 1) p.T = 1;
   if (p.T > kMaxT) p.T = kMaxT;
   p.dM = fast_div((unsigned)p.M);
@@ -1746,15 +1744,15 @@ int nufft_run(double *hb, const float *d_hkl, int npix, const spsim_hip_atom *d_
   SPSIM_HIP_TRY(hipGetLastError());
   SPSIM_TRY(dispatch_spread(Wsp, gauss, gauss && g.d[2].cs != 0.0, g, sorted, bins, start, counts, nsplit, nG, dec,
                             nsplit > 1 ? gpart : buf1, s));
-  if (nsplit > 1) {
-    const size_t rb = (nG + kThreads - 1) / kThreads;
-    hipLaunchKernelGGL(sum_partials, dim3((unsigned)(rb < 4096 ? rb : 4096)), dim3(kThreads), 0, s, gpart, nsplit,
-                       nG, buf1);
-    SPSIM_HIP_TRY(hipGetLastError());
-  }
 
-  // 4. Pruned FFT along x, y, z: G (buf1) -> A (buf2) -> B (buf1) -> H (buf2).
-  SPSIM_TRY(launch_fft(make_pass(X, (long long)nch * Z.M * Y.M, 1), buf1, buf2, twx, s));
+  // 4. Pruned FFT along x, y, z: G (buf1, or the sum of the nsplit partial
+  // grids, added up as the x pass loads them) -> A (buf2) -> B (buf1) -> H (buf2).
+  FftPass px = make_pass(X, (long long)nch * Z.M * Y.M, 1);
+  if (nsplit > 1) {
+    px.nsum = nsplit;
+    px.sum_stride = (long long)nG;
+  }
+  SPSIM_TRY(launch_fft(px, nsplit > 1 ? gpart : buf1, buf2, twx, s));
   SPSIM_TRY(launch_fft(make_pass(Y, (long long)nch * Z.M, X.L), buf2, buf1, twy, s));
   SPSIM_TRY(launch_fft(make_pass(Z, (long long)nch, (long long)Y.L * X.L), buf1, buf2, twz, s));
 
