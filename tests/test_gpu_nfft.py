@@ -498,3 +498,37 @@ def test_nfft_many_molecules_in_one_call(gpu):
         bad = dv.DeviceMolecules([dv.pack_atoms(*mols[0])], gpu)
         bad.atom_offsets[1] = -1
         dv.structure_factor_nfft_molecules(hkl, bad, ff)
+
+
+def test_nfft_molecules_intensity_only_mixed_windows(gpu):
+    """spsim_hip_sf_nfft_molecules without rotations and with only the intensity
+    output, over molecules whose per-atom B factors need different Gaussian
+    windows (17 and 25 points at this resolution): each pattern matches the FP64
+    oracle and the one-molecule call."""
+    import torch
+
+    q = detector_q(48, lam=0.1e-9, pix=4e-4)
+    hkl = torch.from_numpy(q).to(gpu)
+    ff = dv.form_factor_table(device=gpu)
+    rng = np.random.default_rng(81)
+    spec = [(500, 3e-9, 82, (5, 30)), (700, 3.5e-9, 83, (100, 400)), (300, 2e-9, 84, (10, 40)),
+            (400, 2.5e-9, 85, (150, 300)), (200, 1.5e-9, 86, (0, 5)), (600, 3e-9, 87, (20, 60))]
+    mols, Bs = [], []
+    for n, r, seed, (b0, b1) in spec:
+        mols.append(molecule(n, r, seed, z0=0))
+        Bs.append(rng.uniform(b0, b1, n).astype(np.float32))
+    packed = [dv.pack_atoms(Z, pos, atom_b=B) for (Z, pos), B in zip(mols, Bs)]
+    dm = dv.DeviceMolecules(packed, gpu)
+    F, I = dv.structure_factor_nfft_molecules(hkl, dm, ff, None, B=0.5, want_F=False)
+    torch.cuda.synchronize()
+    assert F is None
+    Ih = I.cpu().numpy()
+    widths = set()
+    for m, ((Z, pos), B) in enumerate(zip(mols, Bs)):
+        _, I_ref = oracle.sf_f64(q, Z, pos, atom_b=B, B=0.5, threads=oracle_threads())
+        assert rel_l2(Ih[m], I_ref) < 1e-5, m
+        _, I1, info = dv.structure_factor_nfft(hkl, dv.DeviceAtoms(packed[m], gpu), ff, B=0.5)
+        torch.cuda.synchronize()
+        widths.add(info["Wg"])
+        assert rel_l2(Ih[m], I1.cpu().numpy()) < 1e-6, (m, info)
+    assert widths == {17, 25}, widths
