@@ -532,3 +532,30 @@ def test_nfft_molecules_intensity_only_mixed_windows(gpu):
         widths.add(info["Wg"])
         assert rel_l2(Ih[m], I1.cpu().numpy()) < 1e-6, (m, info)
     assert widths == {17, 25}, widths
+
+
+def test_nfft_molecules_full_batched_workload_against_direct(gpu):
+    """The full batched workload (512 molecules of 2e3-2e5 atoms at random
+    orientations, 512 x 512 detector) in one spsim_hip_sf_nfft_molecules call:
+    every pattern's intensities within 1e-5 (normwise) of the direct kernel's,
+    which the other tests pin to the FP64 oracle at 3e-7. Both W2 plans occur."""
+    import torch
+
+    det, mols, rots = wl.config3(512)
+    q = dv.hkl_detector(det.n, det.n, det.width, det.width, det.distance, det.wavelength, device=gpu)
+    packed = [dv.pack_atoms(Z, pos, mode=_lib.PACK_NFFT) for Z, pos in mols]
+    dm = dv.DeviceMolecules(packed, gpu)
+    ff = dv.form_factor_table(device=gpu)
+    _, I = dv.structure_factor_nfft_molecules(q, dm, ff, rots, eps=1e-6, want_F=False)
+    torch.cuda.synchronize()
+    worst, plans = 0.0, set()
+    for m in range(len(mols)):
+        a = dv.DeviceAtoms(packed[m], gpu)
+        _, Id = dv.structure_factor(q, a, ff, want_F=False, R=rots[m])
+        e = (torch.linalg.vector_norm(I[m].double() - Id.double()) / torch.linalg.vector_norm(Id.double())).item()
+        assert e < 1e-5, (m, a.natoms, e)
+        worst = max(worst, e)
+        _, _, info = dv.structure_factor_nfft(q, a, ff, eps=1e-6, want_F=False, R=rots[m])
+        plans.add(info["W2"])
+    assert plans == {6, 7}, plans
+    print(f"worst of 512 patterns vs the direct kernel: {worst:.3g}")
