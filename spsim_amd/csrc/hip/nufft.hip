@@ -236,10 +236,9 @@ __global__ __launch_bounds__(kThreads) void bounds_molecules(const spsim_hip_ato
 //  - the twiddles tw[j] = exp(-2 pi i j / nf) of the three FFT lengths, back to
 //    back (nfx | nfy | nfz);
 //  - n4 zeroed 16-byte words at zero (the bin counters), grid-strided.
-__global__ __launch_bounds__(kThreads) void setup_tables(NufftGeom g, int W, float *__restrict__ dec,
-                                                        float2 *__restrict__ tw, float4 *__restrict__ zero,
-                                                        size_t n4) {
-  const int i = blockIdx.x * kThreads + threadIdx.x;
+__device__ __forceinline__ void setup_tables(const NufftGeom &g, int W, float *__restrict__ dec,
+                                             float2 *__restrict__ tw, float4 *__restrict__ zero, size_t n4, int i,
+                                             int stride) {
   int off = 0, toff = 0;
   for (int d = 0; d < 3; ++d) {
     const NufftDim &D = g.d[d];
@@ -255,7 +254,7 @@ __global__ __launch_bounds__(kThreads) void setup_tables(NufftGeom g, int W, flo
     off += D.M;
     toff += D.nf;
   }
-  for (size_t k = i; k < n4; k += (size_t)gridDim.x * kThreads) zero[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  for (size_t k = i; k < n4; k += (size_t)stride) zero[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
 // Per-atom spreading record: species grid, first grid point per axis, the
@@ -293,12 +292,11 @@ constexpr double kBToSigma2 = 2.5e-21 / (2.0 * M_PI * M_PI);  // m^2 per A^2 of 
 // the grid, which the bounds exclude).
 // atom_b (packed order, may be NULL): Gaussian windows of per-atom width
 // (at most W points); else the exponential-of-semicircle window of width W.
-__global__ __launch_bounds__(kThreads) void spread_prep(const spsim_hip_atom *__restrict__ atoms, int natoms,
-                                                       const spsim_hip_segment *__restrict__ segs, int nsegs,
-                                                       AtomRot R, NufftGeom g, int W,
-                                                       const float *__restrict__ atom_b, GaussSpread gs,
-                                                       SpreadAtom *__restrict__ rec) {
-  const int j = blockIdx.x * kThreads + threadIdx.x;
+__device__ __forceinline__ void spread_prep(const spsim_hip_atom *__restrict__ atoms, int natoms,
+                                            const spsim_hip_segment *__restrict__ segs, int nsegs,
+                                            const AtomRot &R, const NufftGeom &g, int W,
+                                            const float *__restrict__ atom_b, GaussSpread gs,
+                                            SpreadAtom *__restrict__ rec, int j) {
   if (j >= natoms) return;
   int ch = 0;
   for (int s = 1; s < nsegs; ++s)
@@ -364,6 +362,23 @@ __global__ __launch_bounds__(kThreads) void spread_prep(const spsim_hip_atom *__
   r.kz = (float)kz;
   r.pad1 = r.pad2 = 0.0f;
   rec[j] = r;
+}
+
+// The tables and the atom records in one launch (they are independent): the
+// first tb workgroups fill the tables (grid-strided over tb workgroups), the
+// rest take one atom per thread.
+__global__ __launch_bounds__(kThreads) void tables_and_prep(NufftGeom g, int W2, float *__restrict__ dec,
+                                                           float2 *__restrict__ tw, float4 *__restrict__ zero,
+                                                           size_t n4, int tb,
+                                                           const spsim_hip_atom *__restrict__ atoms, int natoms,
+                                                           const spsim_hip_segment *__restrict__ segs, int nsegs,
+                                                           AtomRot R, int Wsp, const float *__restrict__ atom_b,
+                                                           GaussSpread gs, SpreadAtom *__restrict__ rec) {
+  if ((int)blockIdx.x < tb)
+    setup_tables(g, W2, dec, tw, zero, n4, blockIdx.x * kThreads + threadIdx.x, tb * kThreads);
+  else
+    spread_prep(atoms, natoms, segs, nsegs, R, g, Wsp, atom_b, gs, rec,
+                ((int)blockIdx.x - tb) * kThreads + threadIdx.x);
 }
 
 // Atoms are binned by the kSpreadBlock^3 grid block that holds the first point
@@ -1712,13 +1727,12 @@ int nufft_run(double *hb, const float *d_hkl, int npix, const spsim_hip_atom *d_
     size_t tb = (size_t)(most_t + kThreads - 1) / kThreads;
     const size_t zb = (n4 + kThreads - 1) / kThreads;
     if (zb > tb) tb = zb < 4096 ? zb : 4096;
-    hipLaunchKernelGGL(setup_tables, dim3((unsigned)tb), dim3(kThreads), 0, s, g, W2, dec, tw, (float4 *)counts, n4);
+    const unsigned pblocks = (unsigned)((natoms + kThreads - 1) / kThreads);
+    hipLaunchKernelGGL(tables_and_prep, dim3((unsigned)tb + pblocks), dim3(kThreads), 0, s, g, W2, dec, tw,
+                       (float4 *)counts, n4, (int)tb, d_atoms, natoms, d_segs, nsegs, R, Wsp, d_atom_b, gs, rec);
     SPSIM_HIP_TRY(hipGetLastError());
   }
   const unsigned ablocks = (unsigned)((natoms + kThreads - 1) / kThreads);
-  hipLaunchKernelGGL(spread_prep, dim3(ablocks), dim3(kThreads), 0, s, d_atoms, natoms, d_segs, nsegs, R, g, Wsp,
-                     d_atom_b, gs, rec);
-  SPSIM_HIP_TRY(hipGetLastError());
   const bool local_bins = bins.nbins <= kLocalBins;
   const unsigned runs = (unsigned)((natoms + kBinRun - 1) / kBinRun);
   if (local_bins)
