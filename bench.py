@@ -56,6 +56,8 @@ def parse():
     p.add_argument("--no-nfft", action="store_true", help="skip the non-uniform FFT measurement")
     p.add_argument("--nfft-only", action="store_true",
                    help="only the non-uniform FFT section on the large-single-pattern geometry (for profiling)")
+    p.add_argument("--spectrum-only", action="store_true",
+                   help="only the wavelength-spectrum section, direct and through the NUFFT (for profiling)")
     p.add_argument("--batched-nfft-only", action="store_true",
                    help="only the batched (many molecules) workload through the NUFFT (for profiling)")
     p.add_argument("--pipeline-nfft-only", action="store_true",
@@ -598,6 +600,22 @@ def spectrum(args, torch, dv, wl, _lib):
         err = rel_l2(I.cpu().numpy()[sub], I_ref)
     except Exception as e:  # the oracle is a checker only
         print(f"spectrum accuracy check skipped: {e}", file=sys.stderr)
+    # the same spectrum through the NUFFT (every sample from one set of grids),
+    # timed end to end over whole calls
+    I_n = torch.empty(det.npix, dtype=torch.float32, device=dev)
+    run_n = lambda: dv.structure_factor_spectrum_nfft(hkl, atoms, ff, sc, w, I=I_n, want_F=False)
+    run_n()
+    torch.cuda.synchronize()
+    t_n = []
+    for _ in range(5):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        run_n()
+        e1.record()
+        torch.cuda.synchronize()
+        t_n.append(e0.elapsed_time(e1))
+    ms_n = float(np.median(t_n))
+    err_n_direct = (torch.linalg.vector_norm(I_n.double() - I.double()) / torch.linalg.vector_norm(I.double())).item()
     return {
         "workload": "20000-atom C/N/O/S molecule (radius 7 nm), 512x512 detector 75 um at 0.1 m, lambda 0.2 nm, "
                     f"{len(sc)} wavelength samples over a 2 % bandwidth (amplitude average, reference GPU semantics)",
@@ -606,6 +624,8 @@ def spectrum(args, torch, dv, wl, _lib):
         "roofline_frac": round(ts * w_eq / (ms * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, 4),
         "w_eq_flops_per_term_sample": w_eq,
         "rel_err_vs_fp64": err,
+        "nufft": {"entry": "spsim_hip_sf_spectrum_nfft", "eps": 1e-6, "ms": round(ms_n, 3),
+                  "speedup_vs_direct": round(ms / ms_n, 2), "rel_err_vs_direct_whole_pattern": err_n_direct},
     }
 
 
@@ -741,6 +761,10 @@ def main():
     from spsim_amd import workloads as wl
 
     _lib.load()
+    if args.spectrum_only:
+        log("spectrum only: direct and through the NUFFT")
+        print(json.dumps({"spectrum": spectrum(args, torch, dv, wl, _lib)}))
+        return
     if args.batched_nfft_only:
         log("batched_nfft only: the batched workload through the NUFFT path")
         print(json.dumps({"batched_nfft": batched_nfft(args, torch, dv, wl, _lib)}))

@@ -209,8 +209,10 @@ int spsim_hip_sf_nfft(const float *d_hkl, int npix, const spsim_hip_atom *d_atom
    spsim_hip_sf_nfft's result for q rotated as q' = R_p q (rot9s: host
    [npatterns][9], row-major, applied exactly in FP64), written to
    d_F + p*npix*2 and d_I + p*npix. One set of species grids is built for the
-   unrotated atoms over the ball |q| <= max |q| (which holds every R q) and
-   every pattern only interpolates from it, so the spread and FFT run once.
+   unrotated atoms over the ball |q| <= max |q| max_p ||R_p||_2 (which holds
+   every R q; the norm is bounded by Gershgorin's bound on R^T R, so any R_p
+   is covered: 1 for a rotation, s for s times a rotation) and every pattern
+   only interpolates from it, so the spread and FFT run once.
    The grids are larger than a single pattern's when q does not fill that ball
    (a flat detector's Ewald cap). Returns after the last pattern is written
    (it synchronises `stream`). Other arguments as spsim_hip_sf_nfft. */
@@ -218,6 +220,24 @@ int spsim_hip_sf_nfft_batch(const float *d_hkl, int npix, const spsim_hip_atom *
                             const spsim_hip_segment *d_segs, int nsegs, int natoms, const float *d_ff, float b_global,
                             int npatterns, const double *rot9s, double eps, float *d_F, float *d_I, int *info,
                             void *stream);
+
+/* A spectrum (as spsim_hip_sf_spectrum) through the NUFFT: the sums at
+   q'_k = scales[k] R q of every sample come from one set of species grids, as
+   spsim_hip_sf_nfft_batch computes them for the matrices scales[k] R (R =
+   rot9, or the identity when NULL), into d_Fk [nsamples][npix][2] (a device
+   buffer the caller provides); then F = sum_k weights[k] F_k and I = |F|^2
+   (d_F and d_I may each be NULL). Form factors and the global B are taken at
+   the unscaled |q|, as in spsim_hip_sf_spectrum. Per-atom B factors are not
+   taken: a Gaussian-spread atom's Debye-Waller factor would be evaluated at the
+   scaled q. nsamples <= SPSIM_HIP_MAX_SPECTRUM; scales finite and > 0. No
+   counterpart in the reference (its NFFT path has no spectrum); it computes
+   the same sums as the spectrum branch of src/diffraction.cu:300-318 to the
+   NUFFT's accuracy. Synchronises `stream` once, before the weighted sum. */
+int spsim_hip_sf_spectrum_nfft(const float *d_hkl, int npix, const spsim_hip_atom *d_atoms,
+                               const spsim_hip_segment *d_segs, int nsegs, int natoms, const float *d_ff,
+                               float b_global, const double *rot9, int nsamples, const double *scales,
+                               const double *weights, double eps, float *d_Fk, float *d_F, float *d_I, int *info,
+                               void *stream);
 
 /* Many molecules through the NUFFT, one pattern each: pattern m is
    spsim_hip_sf_nfft's result for molecule m at q rotated by rot9s[m] (host

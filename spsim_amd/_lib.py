@@ -73,6 +73,8 @@ SIGNATURES = {
                                      _vp, _vp]),
     "spsim_hip_sf_nfft_molecules": (_i, [_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _f, _vp, C.c_double, _vp, _vp,
                                          _vp]),
+    "spsim_hip_sf_spectrum_nfft": (_i, [_vp, _i, _vp, _vp, _i, _i, _vp, _f, _vp, _i, _vp, _vp, C.c_double, _vp,
+                                        _vp, _vp, _vp, _vp]),
     "spsim_hip_fft_pruned": (_i, [_vp, C.c_longlong, C.c_longlong, _i, _i, _i, _i, _i, _vp, _vp]),
     "spsim_hip_release": (_i, []),
     "spsim_hip_detector_chain": (_i, [C.POINTER(DetectorParams), _vp, _i, C.c_uint64, C.c_uint64,

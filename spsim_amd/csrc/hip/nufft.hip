@@ -1112,6 +1112,26 @@ __global__ __launch_bounds__(kThreads) void nufft_interp_lds(const float *__rest
   if (k < npix) interp_store<W>(g, s, fr, fi, (size_t)pat * npix + k, F, I);
 }
 
+// The spectrum's amplitude average: F[i] = sum_k w[k] Fk[k][i] (k ascending),
+// I[i] = |F[i]|^2; F or I may be NULL.
+struct SpectrumWeights {
+  float w[SPSIM_HIP_MAX_SPECTRUM];
+};
+__global__ __launch_bounds__(kThreads) void spectrum_combine(size_t npix, int ns, SpectrumWeights wt,
+                                                            const float2 *__restrict__ Fk, float2 *__restrict__ F,
+                                                            float *__restrict__ I) {
+  for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < npix; i += (size_t)gridDim.x * kThreads) {
+    float re = 0.0f, im = 0.0f;
+    for (int k = 0; k < ns; ++k) {
+      const float2 v = Fk[(size_t)k * npix + i];
+      re = __builtin_fmaf(wt.w[k], v.x, re);
+      im = __builtin_fmaf(wt.w[k], v.y, im);
+    }
+    if (F) F[i] = make_float2(re, im);
+    if (I) I[i] = __builtin_fmaf(re, re, im * im);
+  }
+}
+
 __global__ __launch_bounds__(kThreads) void zero_pattern(size_t n, float2 *__restrict__ F, float *__restrict__ I) {
   for (size_t k = (size_t)blockIdx.x * kThreads + threadIdx.x; k < n; k += (size_t)gridDim.x * kThreads) {
     if (F) F[k] = make_float2(0.0f, 0.0f);
@@ -1556,8 +1576,20 @@ int nufft_run(double *hb, const float *d_hkl, int npix, const spsim_hip_atom *d_
   // a Gaussian of negative width does not exist; the direct kernels take any B
   if (gauss && hb[14] > 0.0) return fail(SPSIM_HIP_ERR_INVALID, "negative B factor (%g) in the NUFFT", -hb[14]);
   if (npat > 0) {
-    // every R q lies in the ball of radius max |q|
-    const double r = sqrt(hb[12]);
+    // every R q lies in the ball of radius max |q| ||R||_2; ||R||_2^2, the
+    // largest eigenvalue of R^T R, is at most its largest absolute row sum
+    // (Gershgorin): 1 for a rotation, s^2 for s times a rotation
+    double g2 = 0.0;
+    for (int p = 0; p < npat; ++p) {
+      const double *Rp = rots + 9 * (size_t)p;
+      for (int i = 0; i < 3; ++i) {
+        double row = 0.0;
+        for (int j = 0; j < 3; ++j)
+          row += fabs(Rp[i] * Rp[j] + Rp[3 + i] * Rp[3 + j] + Rp[6 + i] * Rp[6 + j]);
+        g2 = row > g2 ? row : g2;
+      }
+    }
+    const double r = sqrt(hb[12] * g2);
     for (int d = 0; d < 3; ++d) {
       hb[3 + d] = -r;
       hb[9 + d] = r;
@@ -1795,6 +1827,41 @@ extern "C" int spsim_hip_sf_nfft_batch(const float *d_hkl, int npix, const spsim
   if (npatterns < 1) return spsim::fail(SPSIM_HIP_ERR_INVALID, "npatterns=%d", npatterns);
   return spsim::nufft_impl(d_hkl, npix, d_atoms, d_atom_b, d_segs, nsegs, natoms, d_ff, b_global, nullptr, npatterns,
                            rot9s, eps, d_F, d_I, info, (hipStream_t)stream);
+}
+
+extern "C" int spsim_hip_sf_spectrum_nfft(const float *d_hkl, int npix, const spsim_hip_atom *d_atoms,
+                                          const spsim_hip_segment *d_segs, int nsegs, int natoms,
+                                          const float *d_ff, float b_global, const double *rot9, int nsamples,
+                                          const double *scales, const double *weights, double eps, float *d_Fk,
+                                          float *d_F, float *d_I, int *info, void *stream) {
+  using namespace spsim;
+  if (nsamples < 1 || nsamples > SPSIM_HIP_MAX_SPECTRUM)
+    return fail(SPSIM_HIP_ERR_INVALID, "nsamples=%d outside [1, %d]", nsamples, SPSIM_HIP_MAX_SPECTRUM);
+  if (!scales || !weights) return fail(SPSIM_HIP_ERR_INVALID, "NULL scales or weights");
+  if (npix < 0) return fail(SPSIM_HIP_ERR_INVALID, "negative size");
+  if (npix > 0 && !d_Fk) return fail(SPSIM_HIP_ERR_INVALID, "NULL per-sample buffer");
+  if (npix > 0 && !d_F && !d_I) return fail(SPSIM_HIP_ERR_INVALID, "both outputs are NULL");
+  double R[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+  if (rot9)
+    for (int i = 0; i < 9; ++i) R[i] = rot9[i];
+  std::vector<double> rots(9 * (size_t)nsamples);
+  SpectrumWeights wt{};
+  for (int k = 0; k < nsamples; ++k) {
+    if (!(std::isfinite(scales[k]) && scales[k] > 0.0 && std::isfinite(weights[k])))
+      return fail(SPSIM_HIP_ERR_INVALID, "sample %d: scale must be finite and > 0, weight finite", k);
+    for (int i = 0; i < 9; ++i) rots[9 * (size_t)k + i] = scales[k] * R[i];
+    wt.w[k] = (float)weights[k];
+  }
+  // per-sample sums over one set of grids (q' = s_k R q; form factors and the
+  // global B at the unscaled |q|), then the weighted sum
+  SPSIM_TRY(nufft_impl(d_hkl, npix, d_atoms, nullptr, d_segs, nsegs, natoms, d_ff, b_global, nullptr, nsamples,
+                       rots.data(), eps, d_Fk, nullptr, info, (hipStream_t)stream));
+  if (npix == 0) return SPSIM_HIP_OK;
+  const size_t zb = ((size_t)npix + kThreads - 1) / kThreads;
+  hipLaunchKernelGGL(spectrum_combine, dim3((unsigned)(zb < 4096 ? zb : 4096)), dim3(kThreads), 0,
+                     (hipStream_t)stream, (size_t)npix, nsamples, wt, (const float2 *)d_Fk, (float2 *)d_F, d_I);
+  SPSIM_HIP_TRY(hipGetLastError());
+  return SPSIM_HIP_OK;
 }
 
 extern "C" int spsim_hip_sf_nfft_molecules(const float *d_hkl, int npix, int nmol, const spsim_hip_atom *d_atoms,

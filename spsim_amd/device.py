@@ -205,6 +205,36 @@ def structure_factor_nfft(hkl: torch.Tensor, atoms: DeviceAtoms, ff: torch.Tenso
     return (F if want_F else None), I, {"W": v[0], "M": tuple(v[1:4]), "nf": tuple(v[4:7]), "L": tuple(v[7:10]), "W2": v[10], "staged": v[11], "Wg": v[12]}
 
 
+def structure_factor_spectrum_nfft(hkl: torch.Tensor, atoms: DeviceAtoms, ff: torch.Tensor, scales, weights,
+                                   B: float = 0.0, eps: float = 1e-6, R=None, F: torch.Tensor | None = None,
+                                   I: torch.Tensor | None = None, stream=None, want_F: bool = True):
+    """One pattern averaged over wavelength samples through the NUFFT
+    (spsim_hip_sf_spectrum_nfft; no per-atom B). Returns (F or None, I, info)."""
+    if atoms.b is not None:
+        raise ValueError("the spectrum NUFFT takes no per-atom B factors")
+    npix = int(hkl.shape[0])
+    sc = np.ascontiguousarray(scales, np.float64)
+    w = np.ascontiguousarray(weights, np.float64)
+    if sc.shape != w.shape:
+        raise ValueError("scales and weights differ in length")
+    if want_F and F is None:
+        F = torch.empty((npix, 2), dtype=torch.float32, device=hkl.device)
+    if I is None:
+        I = torch.empty((npix,), dtype=torch.float32, device=hkl.device)
+    Fk = torch.empty((int(sc.shape[0]), npix, 2), dtype=torch.float32, device=hkl.device)
+    info = (C.c_int * 13)()
+    Ra, Rp = _rot(R)  # Ra keeps the matrix alive through the call
+    rc = _lib.load().spsim_hip_sf_spectrum_nfft(_ptr(hkl), npix, _ptr(atoms.atoms), _ptr(atoms.segs), atoms.nsegs,
+                                                atoms.natoms, _ptr(ff), float(B), Rp, int(sc.shape[0]),
+                                                sc.ctypes.data_as(C.c_void_p), w.ctypes.data_as(C.c_void_p),
+                                                float(eps), _ptr(Fk), _ptr(F if want_F else None), _ptr(I),
+                                                C.cast(info, C.c_void_p), _stream(stream))
+    check(rc, "spsim_hip_sf_spectrum_nfft")
+    v = list(info)
+    return (F if want_F else None), I, {"W": v[0], "M": tuple(v[1:4]), "nf": tuple(v[4:7]), "L": tuple(v[7:10]),
+                                        "W2": v[10], "staged": v[11], "Wg": v[12]}
+
+
 def structure_factor_nfft_batch(hkl: torch.Tensor, atoms: DeviceAtoms, ff: torch.Tensor, rots, B: float = 0.0,
                                 eps: float = 1e-6, F: torch.Tensor | None = None, I: torch.Tensor | None = None,
                                 stream=None, want_F: bool = True):

@@ -120,3 +120,54 @@ def test_compute_sf_takes_the_spectrum_path(gpu):
     pat = spsim.compute_sf(mol, ptr, n.value, opts)
     assert rel_l2(spsim.float_array(pat.ints, n.value), I_plain) < TOL
     spsim.free_diffraction_pattern(pat)
+
+
+@pytest.mark.parametrize("nsamples,bw,rotate", [(5, 0.02, False), (9, 0.1, True), (1, 0.02, False)])
+def test_spectrum_nfft_vs_f64(gpu, nsamples, bw, rotate):
+    """The spectrum through the NUFFT (spsim_hip_sf_spectrum_nfft): every sample's
+    sums from one set of grids at q'_k = s_k R q, then the weighted sum. Against
+    the FP64 spectrum oracle, with the global B and skipped Z = 0 atoms."""
+    import torch
+
+    Z, pos, _, q = setup_case(seed=5, with_b=False)
+    sc, w = dv.spectrum_samples(0.5e-9, bw, nsamples)
+    R = wl.uniform_rotations(1, np.random.default_rng(9))[0] if rotate else None
+    F_ref, I_ref = oracle.sf_spectrum_f64(q, Z, pos, sc, w, B=3.0, R=R, threads=oracle_threads())
+    atoms = dv.DeviceAtoms(dv.pack_atoms(Z, pos), gpu)
+    hkl = torch.from_numpy(q).to(gpu)
+    ff = dv.form_factor_table(device=gpu)
+    F, I, info = dv.structure_factor_spectrum_nfft(hkl, atoms, ff, sc, w, B=3.0, R=R)
+    torch.cuda.synchronize()
+    Fh = F.cpu().numpy()
+    assert rel_l2(I.cpu().numpy(), I_ref) < TOL, info
+    assert rel_l2(Fh, np.c_[F_ref.real, F_ref.imag]) < TOL, info
+    _, I_only, _ = dv.structure_factor_spectrum_nfft(hkl, atoms, ff, sc, w, B=3.0, R=R, want_F=False)
+    torch.cuda.synchronize()
+    # the spread bins atoms with global atomics, so calls agree to rounding, not bitwise
+    assert rel_l2(I_only.cpu().numpy(), I.cpu().numpy()) < 1e-6
+    with pytest.raises(ValueError, match="per-atom B"):
+        dv.structure_factor_spectrum_nfft(hkl, dv.DeviceAtoms(dv.pack_atoms(Z, pos, atom_b=np.ones(len(Z), np.float32)),
+                                                              gpu), ff, sc, w)
+    with pytest.raises(RuntimeError, match="scale"):
+        dv.structure_factor_spectrum_nfft(hkl, atoms, ff, [1.0, -1.0], [0.5, 0.5])
+
+
+def test_nfft_batch_scaled_rotations(gpu):
+    """spsim_hip_sf_nfft_batch with matrices s_p R_p (s_p from 0.6 to 1.4): the
+    grids cover the ball of radius max |q| max_p s_p, so every pattern matches
+    the FP64 oracle's sum at q' = s_p R_p q (form factors at the unscaled |q|,
+    the spectrum oracle with one sample)."""
+    import torch
+
+    Z, pos, _, q = setup_case(seed=6, with_b=False)
+    atoms = dv.DeviceAtoms(dv.pack_atoms(Z, pos), gpu)
+    hkl = torch.from_numpy(q).to(gpu)
+    ff = dv.form_factor_table(device=gpu)
+    Rs = wl.uniform_rotations(3, np.random.default_rng(10))
+    scales = (1.4, 0.6, 1.0)
+    F, _, info = dv.structure_factor_nfft_batch(hkl, atoms, ff, [s * R for s, R in zip(scales, Rs)], B=1.0)
+    torch.cuda.synchronize()
+    Fh = F.cpu().numpy()
+    for p, (s, R) in enumerate(zip(scales, Rs)):
+        F_ref, _ = oracle.sf_spectrum_f64(q, Z, pos, [s], [1.0], B=1.0, R=R, threads=oracle_threads())
+        assert rel_l2(Fh[p], np.c_[F_ref.real, F_ref.imag]) < TOL, (p, s, info)
