@@ -211,7 +211,8 @@ int spsim_hip_sf_nfft(const float *d_hkl, int npix, const spsim_hip_atom *d_atom
    d_F + p*npix*2 and d_I + p*npix. One set of species grids is built for the
    unrotated atoms over the ball |q| <= max |q| max_p ||R_p||_2 (which holds
    every R q; the norm is bounded by Gershgorin's bound on R^T R, so any R_p
-   is covered: 1 for a rotation, s for s times a rotation) and every pattern
+   is covered: 1 for a rotation, s for s times a rotation), narrowed per axis
+   to the interval-arithmetic range of R_p over the q box, and every pattern
    only interpolates from it, so the spread and FFT run once.
    The grids are larger than a single pattern's when q does not fill that ball
    (a flat detector's Ewald cap). Returns after the last pattern is written

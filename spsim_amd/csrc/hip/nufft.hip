@@ -1590,11 +1590,30 @@ int nufft_run(double *hb, const float *d_hkl, int npix, const spsim_hip_atom *d_
       }
     }
     const double r = sqrt(hb[12] * g2);
-    for (int d = 0; d < 3; ++d) {
-      hb[3 + d] = -r;
-      hb[9 + d] = r;
+    // and axis d of R q spans at most the sum over j of R_dj [q_j min, q_j max]
+    // (interval arithmetic over the q box): much tighter for matrices close to
+    // s times the identity (a wavelength spectrum), where the ball would
+    // inflate a detector's thin q_z range to the whole sphere
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int p = 0; p < npat; ++p) {
+      const double *Rp = rots + 9 * (size_t)p;
+      for (int d = 0; d < 3; ++d) {
+        double a = 0.0, b = 0.0;
+        for (int j = 0; j < 3; ++j) {
+          const double u = Rp[3 * d + j] * hb[3 + j], v = Rp[3 * d + j] * hb[9 + j];
+          a += fmin(u, v);
+          b += fmax(u, v);
+        }
+        lo[d] = fmin(lo[d], a);
+        hi[d] = fmax(hi[d], b);
+      }
     }
-  } else if (gauss) {
+    for (int d = 0; d < 3; ++d) {
+      hb[3 + d] = fmax(lo[d], -r);
+      hb[9 + d] = fmin(hi[d], r);
+    }
+  }
+  if (gauss) {
     // x and y not centred in frequency (cs = 0): the Gaussians' transforms at
     // q itself give the Debye-Waller factors; z stays centred (complex z
     // weights) since a detector's q_z range lies on one side of 0
