@@ -103,6 +103,7 @@ struct NufftGeom {
   // > 0: Gaussian spreading; the correction is exp(2 pi^2 gauss_v0 t^2) per axis
   float gauss_v0;
   int lds_interp;  // host: interpolate with nufft_interp_lds (else nufft_interp)
+  int box_cap;      // grid points per wave of the staged interpolation's LDS (<= kBoxCap)
   // > 0: the interpolation reads the q list as a tile_n x tile_n image, one
   // 16 x 16 pixel tile per workgroup (16 x 4 per wave)
   int tile_n;
@@ -1051,7 +1052,7 @@ __global__ __launch_bounds__(kThreads) void nufft_interp_lds(const float *__rest
                                                             const float *__restrict__ ff, float b_global,
                                                             const double *__restrict__ rots, int p0,
                                                             float2 *__restrict__ F, float *__restrict__ I) {
-  __shared__ float2 box[kThreads / 64][kBoxCap];
+  extern __shared__ float2 box[];  // g.box_cap points per wave
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   const int k = interp_q_index(g, npix);  // npix: no q
   const int pat = p0 + (int)blockIdx.y;
@@ -1074,14 +1075,14 @@ __global__ __launch_bounds__(kThreads) void nufft_interp_lds(const float *__rest
     }
   const int ex = hi[0] - lo[0] + W, ey = hi[1] - lo[1] + W, ez = hi[2] - lo[2] + W;
   // wave-uniform: some lane has ok (hi >= 0) and the box fits
-  const bool staged = hi[0] >= 0 && (long long)ex * ey * ez <= kBoxCap;
+  const bool staged = hi[0] >= 0 && (long long)ex * ey * ez <= g.box_cap;
   const int vol = staged ? ex * ey * ez : 0;
   const int lb = staged && s.ok ? ((s.base[2] - lo[2]) * ey + (s.base[1] - lo[1])) * ex + (s.base[0] - lo[0]) : 0;
   const float rex = 1.0f / (float)ex, rexy = 1.0f / (float)(ex * ey);
   const float s2 = k < npix ? q_s2(s.q0) : 0.0f;
   const int Lx = g.d[0].L, Ly = g.d[1].L, Lz = g.d[2].L;
   const size_t chs = (size_t)Lz * Ly * Lx;
-  float2 *B = box[wave];
+  float2 *B = box + (size_t)wave * g.box_cap;
   float fr = 0.0f, fi = 0.0f;
   for (int ch = 0; ch < nch; ++ch) {
     const float2 *Hc = H + ch * chs;
@@ -1392,7 +1393,8 @@ int launch_interp(const NufftGeom &g, int nsegs, const spsim_hip_segment *segs, 
     const int nt = (g.tile_n + 15) / 16;
     const unsigned gx = g.tile_n > 0 ? (unsigned)(nt * nt) : (unsigned)((npix + kThreads - 1) / kThreads);
     if (g.lds_interp)
-      hipLaunchKernelGGL(nufft_interp_lds<W>, dim3(gx, n), dim3(kThreads), 0, stream, hkl, npix, g, H, nsegs, segs,
+      hipLaunchKernelGGL(nufft_interp_lds<W>, dim3(gx, n), dim3(kThreads),
+                         sizeof(float2) * (kThreads / 64) * (size_t)g.box_cap, stream, hkl, npix, g, H, nsegs, segs,
                          ff, b_global, rots, p0, F, I);
     else
       hipLaunchKernelGGL(nufft_interp<W>, dim3(gx, n), dim3(kThreads), 0, stream, hkl, npix, g, H, nsegs, segs, ff,
@@ -1698,6 +1700,11 @@ int nufft_run(double *hb, const float *d_hkl, int npix, const spsim_hip_atom *d_
     const double box = g.tile_n > 0 ? (16.0 * d + W2 + 1) * (4.0 * d + W2 + 1) * (W2 + 2)
                                     : (64.0 * d + W2) * (W2 + 1) * (W2 + 1);
     g.lds_interp = box <= kBoxCap ? 1 : 0;
+    // LDS per wave: half again the expected box, in 64-point steps (a wave
+    // whose box does not fit gathers directly); smaller boxes leave room for
+    // more workgroups per CU
+    int cap = ((int)ceil(fmin(1.5 * box, (double)kBoxCap)) + 63) / 64 * 64;
+    g.box_cap = cap < 256 ? 256 : (cap > kBoxCap ? kBoxCap : cap);
   }
   if (info) {
     const int v[12] = {X.M, Y.M, Z.M, X.nf, Y.nf, Z.nf, X.L, Y.L, Z.L, W2, g.lds_interp, gauss ? Wsp : 0};
