@@ -84,19 +84,25 @@ def _opt(a, dtype):
 
 # --------------------------------------------------------------- form factors
 
-def default_form_factors() -> np.ndarray:
-    """[101][9] table: rows 0..99 from the Cromer-Mann table the product ships
-    (spsim_amd/csrc/host/cromer_mann.c, parsed as text), row 100 the unit scatterer."""
-    import re
+ATOMSF_FIXTURE = HERE.parent / "tests" / "golden" / "atomsf_table.npy"
 
-    text = (HERE.parent / "spsim_amd" / "csrc" / "host" / "cromer_mann.c").read_text()
-    body = text[text.index("= {") + 3:]
-    rows = re.findall(r"\{([^{}]*)\}", body)
-    vals = np.array([[float(v.strip().rstrip("f")) for v in r.split(",")] for r in rows], dtype=np.float32)
-    assert vals.shape == (100, 9), vals.shape
+
+def reference_atomsf() -> np.ndarray:
+    """The reference's compiled-in Cromer-Mann table atomsf[100][9]
+    (include/atomsf.cdata, src/diffraction.c:58-60) as float32, read from the
+    golden fixture that tests/golden/make_atomsf_fixture.py wrote from that
+    file. It does not come from the product's own copy of the table."""
+    vals = np.load(ATOMSF_FIXTURE, allow_pickle=False)
+    assert vals.shape == (100, 9) and vals.dtype == np.float32, (vals.shape, vals.dtype)
+    return vals
+
+
+def default_form_factors() -> np.ndarray:
+    """[101][9] table: rows 0..99 the reference's atomsf table (reference_atomsf),
+    row 100 the unit scatterer (f = c = 1) that stands for delta atoms."""
     unit = np.zeros((1, 9), np.float32)
     unit[0, 8] = 1.0
-    return np.ascontiguousarray(np.vstack([vals, unit]))
+    return np.ascontiguousarray(np.vstack([reference_atomsf(), unit]))
 
 
 def form_factor(d: float, Z: int, B: float = 0.0, ff: np.ndarray | None = None) -> float:

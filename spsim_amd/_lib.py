@@ -96,6 +96,41 @@ SIGNATURES = {
 _LIB: C.CDLL | None = None
 
 
+def hip_runtimes_mapped(maps_text: str | None = None) -> list[str]:
+    """Distinct HIP runtime files (libamdhip64*) mapped into this process,
+    by resolved path, from /proc/self/maps (or the given text in its format)."""
+    if maps_text is None:
+        try:
+            maps_text = Path("/proc/self/maps").read_text()
+        except OSError:
+            return []
+    found = set()
+    for line in maps_text.splitlines():
+        fields = line.split(None, 5)
+        if len(fields) < 6:
+            continue
+        path = fields[5].strip()
+        if path.endswith(" (deleted)"):
+            path = path[: -len(" (deleted)")]
+        if Path(path).name.startswith("libamdhip64"):
+            try:
+                found.add(str(Path(path).resolve()))
+            except OSError:
+                found.add(path)
+    return sorted(found)
+
+
+def check_single_hip_runtime(maps_text: str | None = None) -> None:
+    """Raises RuntimeError when two different HIP runtimes are mapped: device
+    pointers and streams of one are invalid in the other."""
+    libs = hip_runtimes_mapped(maps_text)
+    if len(libs) > 1:
+        raise RuntimeError(
+            "two HIP runtimes are loaded in this process (" + ", ".join(libs) + "); device pointers and streams "
+            "would cross between them. Import torch before anything loads libamdhip64 (in Python, import torch "
+            "or spsim_amd first), or load only one runtime.")
+
+
 def load() -> C.CDLL:
     """Loads libspsim_amd.so and declares the C-ABI signatures."""
     global _LIB
@@ -103,16 +138,21 @@ def load() -> C.CDLL:
         if not LIB_PATH.exists():
             raise RuntimeError(f"{LIB_PATH} is not built; run `python -m spsim_amd.build`")
         # One HIP runtime per process. The library needs libamdhip64.so.7;
-        # PyTorch ships its own copy and asks for it by another name
-        # (libamdhip64.so), so if this library were loaded first and torch
-        # afterwards, the process would hold two HIP and two HSA runtimes,
-        # and device pointers and streams would cross between them. Loading
-        # torch first makes this library bind to torch's runtime.
+        # PyTorch ships its own copy (torch/lib/libamdhip64.so, the same
+        # SONAME) and loads it by path, so if the system runtime were mapped
+        # first (this library or another HIP program loaded before torch),
+        # importing torch would map a second HIP and HSA runtime, and device
+        # pointers and streams would cross between them. Importing torch first
+        # makes this library bind to torch's runtime; the maps are checked
+        # before and after, so the mixed state is an error, not a crash.
+        check_single_hip_runtime()
         try:
             import torch  # noqa: F401
         except ImportError:
             pass
+        check_single_hip_runtime()
         L = C.CDLL(str(LIB_PATH))
+        check_single_hip_runtime()
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name, None)
             if fn is None:

@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 import oracle
-from spsim_amd import spsim
+import spsim  # the reference's module name (src/spsim.py), top-level spsim.py
 from spsim_amd import workloads as wl
 from tests.conftest import rel_l2
 
@@ -222,3 +222,50 @@ def test_simulate_shot_through_the_nfft_path(gpu):
                                  d.maximum_value)
     assert np.array_equal(ccd_array(d.noiseless_output, nb), nl)
     spsim.free_stuff(pat, opts)
+
+
+def test_atomsf_lib_override_intensity(gpu, tmp_path):
+    """compute_pattern_on_list with an ./atomsf.lib in the working directory
+    (src/diffraction.c:118-177): the table is read once per process, so a
+    child process runs the drop-in call there; its intensities match the FP64
+    oracle fed the same overridden rows (S and Fe replaced, a1 = -1 in every
+    other row)."""
+    import subprocess
+    import sys
+    import textwrap
+    from pathlib import Path
+
+    from tests.test_oracle_formfactor import atomsf_lib_text, override_table
+
+    root = Path(__file__).resolve().parents[1]
+    (tmp_path / "atomsf.lib").write_text(atomsf_lib_text())
+    rng = np.random.default_rng(17)
+    Z = rng.choice(np.array([6, 16, 26], np.int32), 300).astype(np.int32)
+    pos = (rng.uniform(-1, 1, (300, 3)) * 2e-9).astype(np.float32)
+    np.save(tmp_path / "Z.npy", Z)
+    np.save(tmp_path / "pos.npy", pos)
+    code = textwrap.dedent(f"""
+        import sys
+        sys.path.insert(0, {str(root)!r})
+        import torch  # noqa: F401  (the library binds to torch's HIP runtime)
+        import ctypes as C
+        import numpy as np
+        import spsim
+        from tests.test_gpu_dropin import make_opts, hkl_of
+        opts = make_opts(24)
+        mol = spsim.get_molecule_from_atoms(np.load("Z.npy"), np.load("pos.npy"))
+        ptr, n, q = hkl_of(opts)
+        print("atomsf.lib child: compute_pattern_on_list 300 atoms x", n, flush=True)
+        pat = spsim.compute_pattern_on_list(mol, ptr, n, 2.5, opts.experiment, opts)
+        assert pat is not None, spsim.last_error()
+        np.save("I.npy", spsim.float_array(pat.ints, n))
+        np.save("q.npy", q)
+    """)
+    r = subprocess.run([sys.executable, "-c", code], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    I, q = np.load(tmp_path / "I.npy"), np.load(tmp_path / "q.npy")
+    ff = override_table(oracle.default_form_factors())
+    _, I_ref = oracle.sf_f64(q, Z, pos, B=2.5, ff=ff)
+    assert rel_l2(I, I_ref) < 1e-5
+    _, I_builtin = oracle.sf_f64(q, Z, pos, B=2.5)
+    assert rel_l2(I_builtin, I_ref) > 1e-2  # the override changes the pattern
