@@ -55,6 +55,9 @@ def lib() -> C.CDLL:
                                        C.c_void_p, _f32p, C.c_int]
         L.oracle_sf_grad_f64.argtypes = [_f32p, C.c_int, _i32p, _f32p, C.c_void_p, C.c_void_p, C.c_int, _f32p,
                                          C.c_float, _f64p, _f64p, C.c_void_p, C.c_int]
+        L.oracle_sf_grad_given_f_f64.argtypes = [_f32p, C.c_int, _f64p, _i32p, _f32p, C.c_void_p, C.c_void_p,
+                                                 _f32p, C.c_float, _f64p, C.c_int, _i32p, _f64p, C.c_void_p,
+                                                 C.c_int]
         L.oracle_thomson.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int,
                                      _f32p]
         L.oracle_solid_angle.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float,
@@ -235,6 +238,30 @@ def sf_grad_f64(hkl, Z, pos, g, amp=None, atom_b=None, B=0.0, ff=None, threads=0
     lib().oracle_sf_grad_f64(hkl.ravel(), hkl.shape[0], Z, pos.ravel(), amp_p, b_p, N, ff.ravel(), float(B), g, gp,
                              gb.ctypes.data_as(C.c_void_p), int(threads))
     return gp.reshape(N, 3), gb
+
+
+def sf_grad_given_f_f64(hkl, F, Z, pos, g, atoms, amp=None, atom_b=None, B=0.0, ff=None, threads=0):
+    """The gradient of sum_p g_p I_p for the listed atoms only, given the FP64
+    forward F (complex [npix], e.g. from sf_f64 over every atom). Returns
+    (grad_pos [len(atoms), 3], grad_b [len(atoms)]) in the order of `atoms`."""
+    hkl = np.ascontiguousarray(hkl, np.float32).reshape(-1, 3)
+    Z = np.ascontiguousarray(Z, np.int32)
+    pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+    ff = default_form_factors() if ff is None else np.ascontiguousarray(ff, np.float32)
+    Fr = np.empty(2 * hkl.shape[0], np.float64)
+    Fr[0::2] = np.real(F)
+    Fr[1::2] = np.imag(F)
+    amp_a, amp_p = _opt(amp, np.float32)
+    b_a, b_p = _opt(atom_b, np.float32)
+    g = np.ascontiguousarray(g, np.float64)
+    sub = np.ascontiguousarray(atoms, np.int32)
+    assert sub.size == 0 or (sub.min() >= 0 and sub.max() < Z.shape[0])
+    gp = np.zeros(3 * sub.size, np.float64)
+    gb = np.zeros(sub.size, np.float64)
+    lib().oracle_sf_grad_given_f_f64(hkl.ravel(), hkl.shape[0], Fr, Z, pos.ravel(), amp_p, b_p, ff.ravel(),
+                                     float(B), g, int(sub.size), sub, gp, gb.ctypes.data_as(C.c_void_p),
+                                     int(threads))
+    return gp.reshape(-1, 3), gb
 
 
 # --------------------------------------------------------------- detector chain

@@ -42,9 +42,13 @@
  *   oracle_poisson            src/noise.c:32-51 restated with inversion (mean < 10), PTRS
  *                             (Hormann 1993; 10 <= mean < 1000) and the reference's Gaussian
  *                             branch. Pinned by mean/variance tests.
- *   oracle_sf_grad_f64        gradient of L = sum_p g_p I_p w.r.t. atom positions and B
- *                             factors in FP64 (no reference counterpart). Pinned by central
- *                             finite differences of oracle_sf_f64.
+ *   oracle_sf_grad_f64,
+ *   oracle_sf_grad_given_f_f64 gradient of L = sum_p g_p I_p w.r.t. atom positions and B
+ *                             factors in FP64 (no reference counterpart), for every atom or
+ *                             for a listed subset given the forward F. Pinned by central
+ *                             finite differences of oracle_sf_f64 (small problems, and the
+ *                             Poisson NLL of the 20,000-atom differentiation workload on
+ *                             64 atoms, tests/test_gpu_adjoint.py).
  */
 #include <math.h>
 #include <stdint.h>
@@ -347,22 +351,21 @@ int oracle_sf_refsem(const float *hkl, int npix, const int *Z, const float *pos,
  *   dI_p/dB_j = 2 Re(conj(F_p) (-s_p^2) a_pj e^{-i phi_pj})
  * grad_pos is [natoms][3] (original atom order), grad_b [natoms] (may be NULL).
  */
-int oracle_sf_grad_f64(const float *hkl, int npix, const int *Z, const float *pos, const float *amp,
-                       const float *atom_b, int natoms, const float *ff, float B, const double *g, double *grad_pos,
-                       double *grad_b, int nthreads) {
-  double *F = (double *)malloc(sizeof(double) * 2 * (size_t)(npix > 0 ? npix : 1));
-  double *I = (double *)malloc(sizeof(double) * (size_t)(npix > 0 ? npix : 1));
-  if (!F || !I) {
-    free(F);
-    free(I);
-    return -1;
-  }
-  oracle_sf_f64(hkl, npix, Z, pos, amp, atom_b, natoms, 0, 0, ff, B, F, I, nthreads, NULL);
+/*
+ * The same gradient for the atoms listed in sub[0..nsub) only, given F_p
+ * (F: [npix][2], FP64, the forward of every atom, e.g. from oracle_sf_f64).
+ * grad_pos is [nsub][3] and grad_b [nsub] (may be NULL), in the order of sub.
+ * Atoms with Z == 0 get zero (compute_pattern_on_list skips them).
+ */
+int oracle_sf_grad_given_f_f64(const float *hkl, int npix, const double *F, const int *Z, const float *pos,
+                               const float *amp, const float *atom_b, const float *ff, float B, const double *g,
+                               int nsub, const int *sub, double *grad_pos, double *grad_b, int nthreads) {
 #ifdef _OPENMP
   if (nthreads > 0) omp_set_num_threads(nthreads);
-#pragma omp parallel for schedule(dynamic, 4)
+#pragma omp parallel for schedule(dynamic, 1)
 #endif
-  for (int j = 0; j < natoms; ++j) {
+  for (int k = 0; k < nsub; ++k) {
+    const int j = sub[k];
     double gx = 0, gy = 0, gz = 0, gb = 0;
     if (Z[j] != 0) {
       for (int p = 0; p < npix; ++p) {
@@ -380,20 +383,39 @@ int oracle_sf_grad_f64(const float *hkl, int npix, const int *Z, const float *po
         const double u = -(Fr * s + Fi * c);
         /* Re(conj(F) e^{-i phi}) = Fr c - Fi s */
         const double v = Fr * c - Fi * s;
-        const double k = 2.0 * g[p] * a;
-        gx += k * TWO_PI * qx * u;
-        gy += k * TWO_PI * qy * u;
-        gz += k * TWO_PI * qz * u;
-        gb += k * (-s2) * v;
+        const double w = 2.0 * g[p] * a;
+        gx += w * TWO_PI * qx * u;
+        gy += w * TWO_PI * qy * u;
+        gz += w * TWO_PI * qz * u;
+        gb += w * (-s2) * v;
       }
     }
-    grad_pos[3 * j] = gx;
-    grad_pos[3 * j + 1] = gy;
-    grad_pos[3 * j + 2] = gz;
-    if (grad_b) grad_b[j] = gb;
+    grad_pos[3 * k] = gx;
+    grad_pos[3 * k + 1] = gy;
+    grad_pos[3 * k + 2] = gz;
+    if (grad_b) grad_b[k] = gb;
   }
+  return 0;
+}
+
+int oracle_sf_grad_f64(const float *hkl, int npix, const int *Z, const float *pos, const float *amp,
+                       const float *atom_b, int natoms, const float *ff, float B, const double *g, double *grad_pos,
+                       double *grad_b, int nthreads) {
+  double *F = (double *)malloc(sizeof(double) * 2 * (size_t)(npix > 0 ? npix : 1));
+  double *I = (double *)malloc(sizeof(double) * (size_t)(npix > 0 ? npix : 1));
+  int *all = (int *)malloc(sizeof(int) * (size_t)(natoms > 0 ? natoms : 1));
+  if (!F || !I || !all) {
+    free(F);
+    free(I);
+    free(all);
+    return -1;
+  }
+  oracle_sf_f64(hkl, npix, Z, pos, amp, atom_b, natoms, 0, 0, ff, B, F, I, nthreads, NULL);
+  for (int j = 0; j < natoms; ++j) all[j] = j;
+  oracle_sf_grad_given_f_f64(hkl, npix, F, Z, pos, amp, atom_b, ff, B, g, natoms, all, grad_pos, grad_b, nthreads);
   free(F);
   free(I);
+  free(all);
   return 0;
 }
 

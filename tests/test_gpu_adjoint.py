@@ -125,6 +125,130 @@ def test_config4_gradient_vs_dense_fp64_torch(gpu):
     assert rel_l2(Bt.grad.cpu().numpy(), Bd.grad.numpy()) < 1e-4
 
 
+def nll_central_differences(q, F64, Z, pos, B, y, K, atoms, h=5e-14, dB=0.01, threads=0):
+    """FP64 central differences of the Poisson NLL L = sum_p (K I_p - y_p log(K I_p))
+    with respect to the positions and B factors of the listed atoms, at the
+    FP32 inputs (pos, B) whose FP64 forward is F64. Moving one atom changes F by
+    that atom's own term only, so L(x +- h) is evaluated exactly from F64 and
+    the atom's term at the shifted inputs (oracle.sf_f64 of that one atom),
+    with I+ - I- and log(I+/I-) formed directly to avoid cancellation. Steps
+    are taken in FP32 (the step used is the FP64 difference of the two FP32
+    inputs). Returns (fd_pos [len(atoms), 3], fd_b [len(atoms)])."""
+    y = np.asarray(y, np.float64)
+
+    def term(j, p, b):
+        return oracle.sf_f64(q, Z[j:j + 1], p[None, :], atom_b=np.array([b], np.float32), threads=threads)[0]
+
+    def dL(f0, j, p_plus, b_plus, p_minus, b_minus):
+        fp, fm = term(j, p_plus, b_plus), term(j, p_minus, b_minus)
+        ep, em = fp - f0, fm - f0
+        dI = 2.0 * np.real(np.conj(F64) * (fp - fm)) + np.abs(ep) ** 2 - np.abs(em) ** 2
+        Im = np.abs(F64 + em) ** 2
+        return float(np.sum(K * dI - y * np.log1p(dI / Im)))
+
+    fd_p = np.zeros((len(atoms), 3))
+    fd_b = np.zeros(len(atoms))
+    for k, j in enumerate(atoms):
+        f0 = term(j, pos[j], B[j])
+        for c in range(3):
+            pp, pm = pos[j].copy(), pos[j].copy()
+            pp[c] += np.float32(h)
+            pm[c] -= np.float32(h)
+            fd_p[k, c] = dL(f0, j, pp, B[j], pm, B[j]) / (float(pp[c]) - float(pm[c]))
+        bp, bm = np.float32(B[j] + np.float32(dB)), np.float32(B[j] - np.float32(dB))
+        fd_b[k] = dL(f0, j, pos[j], bp, pos[j], bm) / (float(bp) - float(bm))
+    return fd_p, fd_b
+
+
+def test_config4_full_workload_gradient(gpu):
+    """Differentiation configuration at its full size: 20,000 C/N/O/S atoms
+    (radius 7 nm), B = 20 A^2, 512 x 512 detector (75 um at 0.1 m, 0.2 nm).
+    Counts y ~ Poisson(K I) at 1e3 mean photons per pixel for the true
+    coordinates; the model is evaluated at coordinates perturbed by
+    N(0, 0.05 nm). dL/dr and dL/dB of the GPU adjoint (behind autograd) on 64
+    random atoms against the FP64 oracle gradient, and both against FP64
+    central differences of the FP64 loss."""
+    import torch
+
+    det, Z, pos = wl.config4()
+    n = len(Z)
+    rng = np.random.default_rng(40)
+    B = np.full(n, 20.0, np.float32)
+    q = oracle.hkl_detector(det.n, det.n, det.width, det.width, det.distance, 0, 0, 0, det.wavelength)
+    layout = to.SpeciesLayout(Z, gpu)
+    hkl = torch.from_numpy(q).to(gpu)
+    ff = dv.form_factor_table(device=gpu)
+    Bt = torch.from_numpy(B).to(gpu)
+    with torch.no_grad():
+        I_true = to.intensity(torch.from_numpy(pos).to(gpu), hkl, layout, ff, atom_b=Bt).double().cpu().numpy()
+    K = float(np.float32(1e3 / I_true.mean()))
+    y = rng.poisson(K * I_true).astype(np.float32)
+    pert = (pos + rng.normal(0, 0.05e-9, pos.shape)).astype(np.float32)
+
+    P = torch.from_numpy(pert).to(gpu).requires_grad_(True)
+    Bv = Bt.clone().requires_grad_(True)
+    I = to.intensity(P, hkl, layout, ff, atom_b=Bv)
+    loss = to.poisson_nll(I, torch.from_numpy(y).to(gpu), torch.full_like(I, K))
+    loss.backward()
+    gp_all, gb_all = P.grad.cpu().numpy(), Bv.grad.cpu().numpy()
+    assert np.isfinite(gp_all).all() and np.isfinite(gb_all).all()
+
+    F64, I64 = oracle.sf_f64(q, Z, pert, atom_b=B, threads=oracle_threads())
+    L64 = float(np.sum(K * I64 - y * np.log(K * I64)))
+    assert float(loss) == pytest.approx(L64, rel=1e-6)
+    g = K - y.astype(np.float64) / I64
+    sub = np.sort(rng.choice(n, 64, replace=False)).astype(np.int32)
+    gp_ref, gb_ref = oracle.sf_grad_given_f_f64(q, F64, Z, pert, g, sub, atom_b=B, threads=oracle_threads())
+    fd_p, fd_b = nll_central_differences(q, F64, Z, pert, B, y, K, sub, threads=oracle_threads())
+    e_p, e_b = rel_l2(gp_all[sub], gp_ref), rel_l2(gb_all[sub], gb_ref)
+    print(f"config 4 (20000 atoms, 512^2): GPU vs FP64 oracle gradient rel L2 pos {e_p:.2e} B {e_b:.2e}; "
+          f"oracle vs central differences pos {rel_l2(gp_ref, fd_p):.2e} B {rel_l2(gb_ref, fd_b):.2e}")
+    assert rel_l2(gp_ref, fd_p) < 1e-6 and rel_l2(gb_ref, fd_b) < 1e-6
+    assert e_p < 1e-4 and e_b < 1e-4
+    assert rel_l2(gp_all[sub], fd_p) < 1e-4 and rel_l2(gb_all[sub], fd_b) < 1e-4
+
+
+def test_config4_n500_full_detector_vs_dense_fp64_torch(gpu):
+    """The dense FP64 PyTorch reference at N = 500 on the configuration's full
+    512 x 512 detector, evaluated in pixel chunks on the CPU (the loss is a sum
+    over pixels, so the gradients of the chunks add up)."""
+    import torch
+
+    rng = np.random.default_rng(45)
+    Z, pos, _ = wl.random_molecule(500, 7e-9 * (500 / 20000) ** (1 / 3), rng)
+    B = np.full(500, 20.0, np.float32)
+    det = wl.CFG4_DET
+    q = oracle.hkl_detector(det.n, det.n, det.width, det.width, det.distance, 0, 0, 0, det.wavelength)
+    ff = oracle.default_form_factors()
+    _, I_true = oracle.sf_f64(q, Z, pos, atom_b=B, threads=oracle_threads())
+    K = float(np.float32(1e3 / I_true.mean()))
+    y = rng.poisson(K * I_true).astype(np.float32)
+    pert = (pos + rng.normal(0, 0.05e-9, pos.shape)).astype(np.float32)
+
+    layout = to.SpeciesLayout(Z, gpu)
+    hkl = torch.from_numpy(q).to(gpu)
+    fft = dv.form_factor_table(device=gpu)
+    P = torch.from_numpy(pert).to(gpu).requires_grad_(True)
+    Bt = torch.from_numpy(B).to(gpu).requires_grad_(True)
+    I = to.intensity(P, hkl, layout, fft, atom_b=Bt)
+    loss = to.poisson_nll(I, torch.from_numpy(y).to(gpu), torch.full_like(I, K))
+    loss.backward()
+
+    Pd = torch.from_numpy(pert.astype(np.float64)).requires_grad_(True)
+    Bd = torch.from_numpy(B.astype(np.float64)).requires_grad_(True)
+    Ld = 0.0
+    chunk = 16384
+    for p0 in range(0, q.shape[0], chunk):
+        Id = dense_torch_intensity(q[p0:p0 + chunk], Z, Pd, Bd, ff, torch)
+        lam = K * Id
+        part = (lam - torch.from_numpy(y[p0:p0 + chunk].astype(np.float64)) * torch.log(lam)).sum()
+        part.backward()
+        Ld += float(part)
+    assert float(loss) == pytest.approx(Ld, rel=1e-6)
+    assert rel_l2(P.grad.cpu().numpy(), Pd.grad.numpy()) < 1e-4
+    assert rel_l2(Bt.grad.cpu().numpy(), Bd.grad.numpy()) < 1e-4
+
+
 def test_poisson_nll_kernel(gpu):
     import torch
 

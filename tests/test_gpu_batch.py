@@ -180,3 +180,56 @@ def test_invalid_arguments_are_rejected(gpu):
     rc = L.spsim_hip_batch_create(vp(q), 2, 2, 1, vp(offs), vp(badZ), vp(P), None, None, 0, 1, vp(pm), vp(R), None,
                                   0.0, None, C.byref(plan))
     assert rc == -1
+
+
+def test_config3_full_workload(gpu):
+    """Batched uneven instances at full size: 512 molecules with log-uniform
+    atom counts in [2e3, 2e5] (radius 5 nm (N/5000)^(1/3)) at uniform random
+    orientations on a 512 x 512 detector (75 um at 0.1 m, 0.2 nm). The
+    autotuned persistent kernel, replayed from a hipGraph, against the FP64
+    oracle on 2048 random pixels of 4 patterns (smallest, largest and two
+    between), against the plain per-pattern path on every pattern, and
+    identical over graph replays."""
+    import torch
+
+    det, mols, rots = wl.config3(512)
+    q = oracle.hkl_detector(det.n, det.n, det.width, det.width, det.distance, 0, 0, 0, det.wavelength)
+    npat = len(mols)
+    L = _lib.load()
+    plan, *_ = make_plan(mols, rots, np.arange(npat), q, det.n, det.n)
+    try:
+        stream = torch.cuda.Stream()
+        sp = C.c_void_p(stream.cuda_stream)
+        best = _lib.Tile()
+        _lib.check(L.spsim_hip_batch_autotune(plan, sp, C.byref(best), None), "autotune")
+        I = torch.zeros((npat, q.shape[0]), dtype=torch.float32, device=gpu)
+        with torch.cuda.stream(stream):
+            _lib.check(L.spsim_hip_batch_run(plan, None, C.c_void_p(I.data_ptr()), 1, sp), "graph")
+        stream.synchronize()
+        Ib = I.cpu().numpy()
+        sizes = np.array([len(m[0]) for m in mols])
+        rng = np.random.default_rng(33)
+        errs = []
+        for p in (int(sizes.argmin()), int(sizes.argmax()), 100, 400):
+            sub = np.sort(rng.choice(q.shape[0], 2048, replace=False))
+            Z, pos = mols[p]
+            _, ref = oracle.sf_f64(q[sub], Z, pos, R=rots[p], threads=oracle_threads())
+            errs.append(rel_l2(Ib[p, sub], ref))
+        print(f"config 3: tile {best.pixels_per_lane}/{best.chunk_atoms}, rel err vs FP64 (4 patterns x 2048 px) "
+              f"max {max(errs):.2e}")
+        assert max(errs) < 1e-5
+        for rep in range(2):
+            I.zero_()
+            torch.cuda.synchronize()
+            with torch.cuda.stream(stream):
+                _lib.check(L.spsim_hip_batch_run(plan, None, C.c_void_p(I.data_ptr()), 1, sp), "graph")
+            stream.synchronize()
+            assert np.array_equal(I.cpu().numpy(), Ib), rep
+        I2 = torch.zeros_like(I)
+        _lib.check(L.spsim_hip_batch_run_plain(plan, None, C.c_void_p(I2.data_ptr()), sp), "plain")
+        stream.synchronize()
+        I2h = I2.cpu().numpy()
+        worst = max(rel_l2(I2h[p], Ib[p]) for p in range(npat))
+        assert worst < 1e-6, worst
+    finally:
+        L.spsim_hip_batch_destroy(plan)

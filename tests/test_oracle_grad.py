@@ -28,3 +28,17 @@ def test_oracle_gradient_central_differences():
         bm[j] -= np.float32(0.01)
         fd = (L(pos, bp) - L(pos, bm)) / (float(bp[j]) - float(bm[j]))
         assert fd == pytest.approx(gb[j], rel=1e-5, abs=1e-8 * np.abs(gb).max())
+
+
+def test_oracle_subset_gradient_equals_full_gradient():
+    """The subset entry point given F (used at the full differentiation
+    workload) equals the full gradient on the same atoms, bit for bit."""
+    Z, pos, B, q = small_problem(natoms=40, n=10, seed=5)
+    Z[3] = 0
+    g = np.random.default_rng(6).normal(size=q.shape[0])
+    gp, gb = oracle.sf_grad_f64(q, Z, pos, g, atom_b=B, B=1.5)
+    F, _ = oracle.sf_f64(q, Z, pos, atom_b=B, B=1.5)
+    sub = np.array([39, 3, 0, 17], np.int32)
+    sp, sb = oracle.sf_grad_given_f_f64(q, F, Z, pos, g, sub, atom_b=B, B=1.5)
+    assert np.array_equal(sp, gp[sub]) and np.array_equal(sb, gb[sub])
+    assert np.all(sp[1] == 0) and sb[1] == 0
