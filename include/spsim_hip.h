@@ -284,9 +284,14 @@ typedef struct spsim_hip_batch spsim_hip_batch;
    [ny][nx][3] host q list. When that list is mirror-symmetric the plan runs the
    quad kernel (tile->pixels_per_lane = quads per lane), or, when it is also
    square and transpose-symmetric, the octant kernel (pixels_per_lane = 8, the
-   default there); otherwise the per-pixel kernel. The plan copies everything to the device, builds the cost-sorted
-   (pattern, tile) work list and owns its device memory until
-   spsim_hip_batch_destroy. tile may be NULL (default tiling). */
+   default there); otherwise the per-pixel kernel. The plan copies everything
+   to the device, builds the cost-sorted (pattern, tile, atom range) work list
+   and owns its device memory until spsim_hip_batch_destroy. tile may be NULL
+   (default tiling). A pattern whose molecule has more atoms than the plan's
+   item bound (spsim_hip_batch_set_item_atoms) is split into near-equal atom
+   ranges, one work item per range and tile; the tile's last item to finish
+   adds the ranges' FP64 partial sums in range order, so results do not depend
+   on the order in which items finish. */
 int spsim_hip_batch_create(const float *hkl, int nx, int ny, int nmol, const int *mol_offsets, const int *Z,
                            const float *pos, const float *amp, const float *atom_b, int delta_atoms,
                            int npatterns, const int *pattern_mol, const double *rot9, const float *ff,
@@ -303,6 +308,15 @@ int spsim_hip_batch_uses_grid(const spsim_hip_batch *b);
 /* Tile of the persistent kernel; setting it rebuilds the work list. */
 int spsim_hip_batch_set_tile(spsim_hip_batch *b, const spsim_hip_tile *tile);
 int spsim_hip_batch_get_tile(const spsim_hip_batch *b, spsim_hip_tile *tile, int *blocks);
+/* Bound on the atoms of one work item; rebuilds the work list. 0 (the default):
+   automatic, about an eighth of one resident workgroup's share of the whole
+   run and at least 4 LDS chunks; > 0: that many atoms; < 0: never split a
+   pattern's atoms. */
+int spsim_hip_batch_set_item_atoms(spsim_hip_batch *b, int item_atoms);
+/* Work items of one run, the item bound in effect (0: no pattern is split) and
+   the number of (pattern, tile) pairs split over several items. Any pointer
+   may be NULL. */
+int spsim_hip_batch_describe(const spsim_hip_batch *b, int *nitems, int *item_atoms, int *split_tiles);
 /* Autotuner: times every supported tile (pixels per lane x LDS chunk) on a
    slice of the plan's own work (patterns spread over its atom-count range),
    keeps the fastest in pixel-atom terms/s and returns it (best_rate may be

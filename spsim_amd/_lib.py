@@ -89,6 +89,8 @@ SIGNATURES = {
     "spsim_hip_batch_terms": (_i, [_vp, C.POINTER(C.c_double)]),
     "spsim_hip_batch_set_tile": (_i, [_vp, C.POINTER(Tile)]),
     "spsim_hip_batch_get_tile": (_i, [_vp, C.POINTER(Tile), C.POINTER(_i)]),
+    "spsim_hip_batch_set_item_atoms": (_i, [_vp, _i]),
+    "spsim_hip_batch_describe": (_i, [_vp, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i)]),
     "spsim_hip_batch_autotune": (_i, [_vp, _vp, C.POINTER(Tile), C.POINTER(C.c_double)]),
     "spsim_hip_batch_destroy": (None, [_vp]),
 }

@@ -1,7 +1,17 @@
 // Batched structure factors: many molecules and orientations in one persistent
-// launch over a cost-sorted (pattern, pixel-tile) work list, optionally
-// replayed from a captured hipGraph, plus the plain one-launch-per-pattern path
-// and the tile autotuner.
+// launch over a cost-sorted (pattern, pixel-tile, atom-range) work list,
+// optionally replayed from a captured hipGraph, plus the plain
+// one-launch-per-pattern path and the tile autotuner.
+//
+// Atom ranges: a pattern whose molecule has more than item_atoms atoms is
+// split into ceil(N / item_atoms) near-equal ranges, one work item each per
+// tile, whose FP64 partial sums the tile's last item adds in range order. By
+// default item_atoms bounds the largest item to about an eighth of one
+// workgroup slot's share of the whole run (total atom-tiles / (8 x resident
+// workgroups)), and at least 4 LDS chunks, so a few very large molecules
+// among many small ones no longer leave most of the GPU idle at the end of
+// the run, while runs whose molecules are all small against that bound keep
+// one item per (pattern, tile) and write their outputs directly.
 #include <algorithm>
 #include <cstring>
 #include <numeric>
@@ -10,6 +20,7 @@
 #include "common.hpp"
 #include "sf_kernels.hpp"
 
+using spsim::BatchItem;
 using spsim::MolInfo;
 
 struct spsim_hip_batch {
@@ -22,6 +33,9 @@ struct spsim_hip_batch {
   bool has_b = false;
   float b_global = 0.0f;
   int nitems = 0;
+  int item_atoms_req = 0;  // 0: automatic, > 0: fixed bound, < 0: never split
+  int item_atoms = 0;      // in effect (0: no pattern split)
+  int split_tiles = 0;     // (pattern, tile) pairs with more than one item
   double terms = 0.0;
 
   // device memory
@@ -32,8 +46,10 @@ struct spsim_hip_batch {
   MolInfo *mols = nullptr;
   int *pattern_mol = nullptr;
   double *rot = nullptr;
-  int2 *items = nullptr;
+  BatchItem *items = nullptr;
   unsigned *queue = nullptr;
+  unsigned *counters = nullptr;  // one per split (pattern, tile), zero between runs
+  double2 *partials = nullptr;   // partial slots of split tiles
   float *ff = nullptr;
 
   // host copies
@@ -68,17 +84,53 @@ int dupload(T **p, const T *h, size_t n) {
 void free_batch(spsim_hip_batch *b) {
   if (!b) return;
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
-  void *ptrs[] = {b->hkl, b->atoms, b->atom_b, b->segs, b->mols, b->pattern_mol, b->rot, b->items, b->queue, b->ff};
+  void *ptrs[] = {b->hkl,  b->atoms, b->atom_b,   b->segs,     b->mols, b->pattern_mol,
+                  b->rot,  b->items, b->counters, b->partials, b->queue, b->ff};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   delete b;
 }
 
-// Work items (pattern, tile) for tile size `tp`, heaviest pattern first so the
-// persistent workgroups finish together (longest-processing-time order).
-std::vector<int2> build_items(const spsim_hip_batch &b, const spsim_hip_tile &t,
-                              const std::vector<int> *only_patterns = nullptr,
-                              int max_tiles = -1) {
+int natoms_of(const spsim_hip_batch &b, int p) {
+  const MolInfo &m = b.h_mols[b.h_pattern_mol[p]];
+  return m.atom_end - m.atom_begin;
+}
+
+int tiles_of(const spsim_hip_batch &b, const spsim_hip_tile &t) {
+  const int units = b.grid ? grid_units(t, b.nx, b.ny) : b.npix;
+  const int per_tile = b.grid ? grid_units_per_tile(t) : tile_units(t);
+  return (units + per_tile - 1) / per_tile;
+}
+
+// Largest atom range per work item for tile t with `blocks` resident
+// workgroups (0: no pattern is split).
+int item_atoms_for(const spsim_hip_batch &b, const spsim_hip_tile &t, int blocks) {
+  if (b.item_atoms_req < 0) return 0;
+  int bound = b.item_atoms_req;
+  if (bound == 0) {
+    double atom_tiles = 0.0;
+    for (int p = 0; p < b.npatterns; ++p) atom_tiles += (double)natoms_of(b, p);
+    atom_tiles *= tiles_of(b, t);
+    const double share = atom_tiles / (8.0 * std::max(blocks, 1));
+    bound = (int)std::min<double>(std::max<double>(std::ceil(share), 4.0 * t.chunk_atoms), 1 << 30);
+  }
+  int biggest = 0;
+  for (int p = 0; p < b.npatterns; ++p) biggest = std::max(biggest, natoms_of(b, p));
+  return biggest > bound ? bound : 0;
+}
+
+struct ItemList {
+  std::vector<BatchItem> items;
+  int groups = 0;       // split (pattern, tile) pairs
+  long long slots = 0;  // partial slots
+};
+
+// Work items for tile t, atoms split into ranges of at most item_atoms (0: no
+// split), heaviest first so the persistent workgroups finish together
+// (longest-processing-time order). only_patterns / max_tiles restrict the
+// list to a sample (autotuner).
+ItemList build_items(const spsim_hip_batch &b, const spsim_hip_tile &t, int item_atoms,
+                     const std::vector<int> *only_patterns = nullptr, int max_tiles = -1) {
   std::vector<int> pats;
   if (only_patterns) {
     pats = *only_patterns;
@@ -86,20 +138,51 @@ std::vector<int2> build_items(const spsim_hip_batch &b, const spsim_hip_tile &t,
     pats.resize(b.npatterns);
     std::iota(pats.begin(), pats.end(), 0);
   }
-  auto natoms = [&](int p) { const MolInfo &m = b.h_mols[b.h_pattern_mol[p]]; return m.atom_end - m.atom_begin; };
-  std::stable_sort(pats.begin(), pats.end(), [&](int a, int c) { return natoms(a) > natoms(c); });
-  const int units = b.grid ? grid_units(t, b.nx, b.ny) : b.npix;
-  const int per_tile = b.grid ? grid_units_per_tile(t) : tile_units(t);
-  int tiles = (units + per_tile - 1) / per_tile;
+  std::stable_sort(pats.begin(), pats.end(), [&](int a, int c) { return natoms_of(b, a) > natoms_of(b, c); });
+  int tiles = tiles_of(b, t);
   if (max_tiles > 0) tiles = std::min(tiles, max_tiles);
-  std::vector<int2> items;
-  items.reserve((size_t)pats.size() * tiles);
-  for (int p : pats)
-    for (int t = 0; t < tiles; ++t) items.push_back(make_int2(p, t));
-  return items;
+  ItemList L;
+  for (int p : pats) {
+    const int n = natoms_of(b, p);
+    const int nch = (item_atoms > 0 && n > item_atoms) ? (n + item_atoms - 1) / item_atoms : 1;
+    for (int tl = 0; tl < tiles; ++tl) {
+      const int group = nch > 1 ? L.groups++ : -1;
+      const int slot = nch > 1 ? (int)L.slots : -1;
+      if (nch > 1) L.slots += nch;
+      for (int c = 0; c < nch; ++c) {
+        BatchItem it;
+        it.pattern = p;
+        it.tile = tl;
+        it.a_begin = (int)((long long)n * c / nch);
+        it.a_end = (int)((long long)n * (c + 1) / nch);
+        it.group = group;
+        it.chunk = c;
+        it.nchunks = nch;
+        it.slot = slot;
+        L.items.push_back(it);
+      }
+    }
+  }
+  std::stable_sort(L.items.begin(), L.items.end(), [](const BatchItem &x, const BatchItem &y) {
+    return x.a_end - x.a_begin > y.a_end - y.a_begin;
+  });
+  return L;
 }
 
-BatchLaunch launch_args(const spsim_hip_batch *b, const int2 *items, int nitems, unsigned *queue, float *F, float *I) {
+// Device counters (zeroed) and partial slots for an item list.
+int alloc_split_state(const ItemList &L, const spsim_hip_tile &t, bool grid, unsigned **counters,
+                      double2 **partials) {
+  *counters = nullptr;
+  *partials = nullptr;
+  if (L.groups == 0) return SPSIM_HIP_OK;
+  SPSIM_TRY(dmalloc(counters, (size_t)L.groups));
+  SPSIM_HIP_TRY(hipMemsetAsync(*counters, 0, sizeof(unsigned) * (size_t)L.groups, nullptr));
+  SPSIM_HIP_TRY(hipStreamSynchronize(nullptr));
+  return dmalloc(partials, (size_t)L.slots * slot_entries(t, grid));
+}
+
+BatchLaunch launch_args(const spsim_hip_batch *b, const BatchItem *items, int nitems, unsigned *queue,
+                        unsigned *counters, double2 *partials, float *F, float *I) {
   BatchLaunch a;
   a.hkl = b->hkl;
   a.npix = b->npix;
@@ -114,6 +197,8 @@ BatchLaunch launch_args(const spsim_hip_batch *b, const int2 *items, int nitems,
   a.items = items;
   a.nitems = nitems;
   a.queue = queue;
+  a.counters = counters;
+  a.partials = partials;
   a.ff = b->ff;
   a.b_global = b->b_global;
   a.F = (float2 *)F;
@@ -126,17 +211,29 @@ int set_tile(spsim_hip_batch *b, const spsim_hip_tile &t) {
     return fail(SPSIM_HIP_ERR_INVALID, "unsupported tile (octants need a square, transpose-symmetric q list)");
   int per_cu = 1;
   SPSIM_TRY(batch_blocks_per_cu(t, b->has_b, b->grid, &per_cu));
-  std::vector<int2> items = build_items(*b, t);
-  if (b->items) SPSIM_HIP_TRY(hipFree(b->items));
-  b->items = nullptr;
-  SPSIM_TRY(dupload(&b->items, items.data(), items.size()));
-  b->nitems = (int)items.size();
-  b->tile = t;
-  b->blocks = b->cus * per_cu;
+  const int blocks = b->cus * per_cu;
+  const int item_atoms = item_atoms_for(*b, t, blocks);
+  ItemList L = build_items(*b, t, item_atoms);
+  if (L.slots > 0x7fffffffLL || L.items.size() > 0x7fffffffULL)
+    return fail(SPSIM_HIP_ERR_INVALID, "work list too large (%zu items, %lld partial slots); raise item_atoms",
+                L.items.size(), L.slots);
   if (b->exec) {
     (void)hipGraphExecDestroy(b->exec);
     b->exec = nullptr;
   }
+  void *old[] = {b->items, b->counters, b->partials};
+  for (void *p : old)
+    if (p) SPSIM_HIP_TRY(hipFree(p));
+  b->items = nullptr;
+  b->counters = nullptr;
+  b->partials = nullptr;
+  SPSIM_TRY(dupload(&b->items, L.items.data(), L.items.size()));
+  SPSIM_TRY(alloc_split_state(L, t, b->grid, &b->counters, &b->partials));
+  b->nitems = (int)L.items.size();
+  b->item_atoms = item_atoms;
+  b->split_tiles = L.groups;
+  b->tile = t;
+  b->blocks = blocks;
   return SPSIM_HIP_OK;
 }
 
@@ -249,11 +346,30 @@ int spsim_hip_batch_get_tile(const spsim_hip_batch *b, spsim_hip_tile *tile, int
   return SPSIM_HIP_OK;
 }
 
+int spsim_hip_batch_set_item_atoms(spsim_hip_batch *b, int item_atoms) {
+  using namespace spsim;
+  if (!b) return fail(SPSIM_HIP_ERR_INVALID, "NULL plan");
+  const int old = b->item_atoms_req;
+  b->item_atoms_req = item_atoms;
+  const int rc = set_tile(b, b->tile);
+  if (rc != SPSIM_HIP_OK) b->item_atoms_req = old;
+  return rc;
+}
+
+int spsim_hip_batch_describe(const spsim_hip_batch *b, int *nitems, int *item_atoms, int *split_tiles) {
+  using namespace spsim;
+  if (!b) return fail(SPSIM_HIP_ERR_INVALID, "NULL plan");
+  if (nitems) *nitems = b->nitems;
+  if (item_atoms) *item_atoms = b->item_atoms;
+  if (split_tiles) *split_tiles = b->split_tiles;
+  return SPSIM_HIP_OK;
+}
+
 int spsim_hip_batch_run(spsim_hip_batch *b, float *d_F, float *d_I, int use_graph, void *stream) {
   using namespace spsim;
   if (!b || !d_I) return fail(SPSIM_HIP_ERR_INVALID, "NULL argument");
   hipStream_t s = (hipStream_t)stream;
-  BatchLaunch a = launch_args(b, b->items, b->nitems, b->queue, d_F, d_I);
+  BatchLaunch a = launch_args(b, b->items, b->nitems, b->queue, b->counters, b->partials, d_F, d_I);
   if (!use_graph) return launch_batch(a, b->tile, b->grid, b->blocks, s);
   if (!s) return fail(SPSIM_HIP_ERR_INVALID, "graph replay needs a non-default stream");
   if (!b->exec || b->g_stream != s || b->g_F != d_F || b->g_I != d_I) {
@@ -314,10 +430,11 @@ int spsim_hip_batch_terms(const spsim_hip_batch *b, double *terms) {
 }
 
 // Times each supported tile of the plan's kernel on a representative slice of
-// the work (up to 24 patterns spread over the cost range, all of their tiles:
-// tens of milliseconds per candidate at large plans, so that coarse tiles are
-// not judged on a launch dominated by its tail) and keeps the fastest in
-// pixel-atom terms per second. Leaves the plan on the best tile.
+// the work (up to 24 patterns spread over the cost range, all of their tiles
+// and atom ranges as the whole plan would split them: tens of milliseconds per
+// candidate at large plans, so that coarse tiles are not judged on a launch
+// dominated by its tail) and keeps the fastest. Leaves the plan on the best
+// tile.
 int spsim_hip_batch_autotune(spsim_hip_batch *b, void *stream, spsim_hip_tile *best_out, double *best_rate) {
   using namespace spsim;
   if (!b) return fail(SPSIM_HIP_ERR_INVALID, "NULL plan");
@@ -325,11 +442,14 @@ int spsim_hip_batch_autotune(spsim_hip_batch *b, void *stream, spsim_hip_tile *b
   // Sample patterns at cost quantiles.
   std::vector<int> order(b->npatterns);
   std::iota(order.begin(), order.end(), 0);
-  auto natoms = [&](int p) { const MolInfo &m = b->h_mols[b->h_pattern_mol[p]]; return m.atom_end - m.atom_begin; };
-  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return natoms(x) > natoms(y); });
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return natoms_of(*b, x) > natoms_of(*b, y); });
   std::vector<int> sample;
   const int want = std::min(b->npatterns, 24);
-  for (int i = 0; i < want; ++i) sample.push_back(order[(size_t)i * b->npatterns / want]);
+  double sample_terms = 0.0;  // every candidate computes these patterns in full
+  for (int i = 0; i < want; ++i) {
+    sample.push_back(order[(size_t)i * b->npatterns / want]);
+    sample_terms += (double)natoms_of(*b, sample.back()) * b->npix;
+  }
   // quads per lane {1, 2} (+ octants on a square list) or pixels per lane {1, 2, 4, 8}
   const int ks_grid[3] = {1, 2, kOctantLane}, ks_pix[4] = {1, 2, 4, 8};
   const int *ks = b->grid ? ks_grid : ks_pix;
@@ -351,23 +471,21 @@ int spsim_hip_batch_autotune(spsim_hip_batch *b, void *stream, spsim_hip_tile *b
   for (int ki = 0; ki < nks && rc == SPSIM_HIP_OK; ++ki) {
     for (int ci = 0; ci < 3 && rc == SPSIM_HIP_OK; ++ci) {
       spsim_hip_tile t{ks[ki], 4, cs[ci], 0};
-      const int tu = b->grid ? grid_units_per_tile(t) : tile_units(t);
       int per_cu = 1;
       if ((rc = batch_blocks_per_cu(t, b->has_b, b->grid, &per_cu)) != SPSIM_HIP_OK) break;
       const int blocks = b->cus * per_cu;
-      const int units = b->grid ? grid_units(t, b->nx, b->ny) : b->npix;
-      const int tiles = (units + tu - 1) / tu;
-      std::vector<int2> items = build_items(*b, t, &sample, tiles);
-      double terms = 0.0;
-      for (const int2 &it : items) {
-        // pixel-atom terms covered by the item (a quad counts 4 pixels, an
-        // octant 8; the duplicated middle rows and diagonal are ignored here)
-        const int n_units = std::min(tu, units - it.y * tu);
-        terms += (double)natoms(it.x) * n_units * (b->grid ? grid_unit_pixels(t) : 1);
+      // the sampled patterns in full (every tile), split as the whole plan would split them
+      ItemList L = build_items(*b, t, item_atoms_for(*b, t, blocks), &sample);
+      BatchItem *ditems = nullptr;
+      unsigned *dcount = nullptr;
+      double2 *dpart = nullptr;
+      if ((rc = dupload(&ditems, L.items.data(), L.items.size())) != SPSIM_HIP_OK) break;
+      if ((rc = alloc_split_state(L, t, b->grid, &dcount, &dpart)) != SPSIM_HIP_OK) {
+        (void)hipFree(ditems);
+        if (dcount) (void)hipFree(dcount);
+        break;
       }
-      int2 *ditems = nullptr;
-      if ((rc = dupload(&ditems, items.data(), items.size())) != SPSIM_HIP_OK) break;
-      BatchLaunch a = launch_args(b, ditems, (int)items.size(), q, nullptr, dI);
+      BatchLaunch a = launch_args(b, ditems, (int)L.items.size(), q, dcount, dpart, nullptr, dI);
       float ms = 0.0f;
       rc = launch_batch(a, t, b->grid, blocks, s);  // warm-up
       if (rc == SPSIM_HIP_OK) {
@@ -378,9 +496,11 @@ int spsim_hip_batch_autotune(spsim_hip_batch *b, void *stream, spsim_hip_tile *b
         if (e != hipSuccess) rc = hip_fail(e, "hipEventSynchronize");
         else hipEventElapsedTime(&ms, e0, e1);
       }
+      if (dcount) (void)hipFree(dcount);
+      if (dpart) (void)hipFree(dpart);
       (void)hipFree(ditems);
       if (rc == SPSIM_HIP_OK && ms > 0.0f) {
-        const double r = terms / (ms * 1e-3);
+        const double r = sample_terms / (ms * 1e-3);
         if (r > best_r) {
           best_r = r;
           best = t;

@@ -687,10 +687,17 @@ __global__ void reset_queue_kernel(unsigned *q) { *q = 0u; }
 
 // ---- persistent batched kernels ----
 //
-// Workgroups take the next (pattern, tile) item from the queue until the list
-// is exhausted. The item index is uniform across the workgroup, so every wave
-// leaves the loop together. Each pattern's orientation R is applied to the
-// atoms while they are staged.
+// Workgroups take the next (pattern, tile, atom range) item from the queue
+// until the list is exhausted. The item index is uniform across the
+// workgroup, so every wave leaves the loop together. Each pattern's
+// orientation R is applied to the atoms while they are staged.
+//
+// A tile whose pattern has its atoms split over several items (nchunks > 1)
+// leaves one FP64 partial slot per item; the workgroup that finishes the
+// tile's last item (counted with one atomic per tile) adds the slots in chunk
+// order and writes F and I, so the result does not depend on which workgroup
+// finished last. It then sets the tile's counter back to zero for the next
+// run, so replays need no clearing of the counters.
 
 __device__ __forceinline__ int next_item(unsigned *queue, int *s_item) {
   __syncthreads();
@@ -699,79 +706,111 @@ __device__ __forceinline__ int next_item(unsigned *queue, int *s_item) {
   return *s_item;
 }
 
+// Writes the NP pixels of each lane (pix[i] < 0: none) of one work item:
+// directly, or through the item's partial slot and the tile's last-finisher
+// reduction. Must be called by every thread of the workgroup.
+template <int NP>
+__device__ __forceinline__ void finish_item(const BatchLaunch &a, const BatchItem &it, const int (&pix)[NP],
+                                            const double (&Fr)[NP], const double (&Fi)[NP], int *s_last) {
+  const size_t off = (size_t)it.pattern * a.npix;
+  float2 *F = a.F ? a.F + off : nullptr;
+  float *I = a.I + off;
+  if (it.nchunks <= 1) {
+#pragma unroll
+    for (int i = 0; i < NP; ++i)
+      if (pix[i] >= 0) store_pixel(pix[i], Fr[i], Fi[i], a.npix, 0, nullptr, F, I);
+    return;
+  }
+  constexpr int kSlot = NP * kThreads;  // double2 entries per slot, [i][lane]
+  double2 *slots = a.partials + (size_t)it.slot * kSlot;
+  double2 *mine = slots + (size_t)it.chunk * kSlot;
+#pragma unroll
+  for (int i = 0; i < NP; ++i) mine[i * kThreads + threadIdx.x] = make_double2(Fr[i], Fi[i]);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __syncthreads();
+  if (threadIdx.x == 0) *s_last = atomicAdd(a.counters + it.group, 1u) == (unsigned)(it.nchunks - 1);
+  __syncthreads();
+  if (!*s_last) return;
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  double re[NP], im[NP];
+#pragma unroll
+  for (int i = 0; i < NP; ++i) re[i] = im[i] = 0.0;
+  for (int c = 0; c < it.nchunks; ++c) {
+    const double2 *sc = slots + (size_t)c * kSlot;
+#pragma unroll
+    for (int i = 0; i < NP; ++i) {
+      const double2 v = sc[i * kThreads + threadIdx.x];
+      re[i] += v.x;
+      im[i] += v.y;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < NP; ++i)
+    if (pix[i] >= 0) store_pixel(pix[i], re[i], im[i], a.npix, 0, nullptr, F, I);
+  if (threadIdx.x == 0) a.counters[it.group] = 0u;
+}
+
 template <int K, int CHUNK, bool HAS_B>
 __global__ __launch_bounds__(kThreads) void sf_batch_kernel(BatchLaunch a) {
   __shared__ AtomLds sm[CHUNK];
-  __shared__ int s_item;
+  __shared__ int s_item, s_last;
   for (;;) {
     const int item = next_item(a.queue, &s_item);
     if (item >= a.nitems) break;
-    const int2 it = a.items[item];
-    const int p = it.x;
-    const MolInfo m = a.mols[a.pattern_mol[p]];
-    const int base = it.y * (kThreads * K) + threadIdx.x;
+    const BatchItem it = a.items[item];
+    const MolInfo m = a.mols[a.pattern_mol[it.pattern]];
+    const int base = it.tile * (kThreads * K) + threadIdx.x;
     double qx[K], qy[K], qz[K];
     float s2[K];
+    int pix[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      const int pix = base + k * kThreads;
-      load_q(a.hkl, pix, pix < a.npix, qx[k], qy[k], qz[k], s2[k]);
+      const int p = base + k * kThreads;
+      pix[k] = p < a.npix ? p : -1;
+      load_q(a.hkl, p, p < a.npix, qx[k], qy[k], qz[k], s2[k]);
     }
     double Fr[K], Fi[K];
     // segment starts are relative to the molecule's first packed atom
     tile_sum<K, CHUNK, HAS_B>(qx, qy, qz, s2, a.atoms + m.atom_begin, HAS_B ? a.atom_b + m.atom_begin : nullptr,
-                              a.segs + m.seg_begin, m.nseg, 0, m.atom_end - m.atom_begin, a.ff, a.b_global,
-                              a.rot + 9 * (size_t)p, sm, Fr, Fi);
-    const size_t off = (size_t)p * a.npix;
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const int pix = base + k * kThreads;
-      if (pix < a.npix) store_pixel(pix, Fr[k], Fi[k], a.npix, 0, nullptr, a.F ? a.F + off : nullptr, a.I + off);
-    }
+                              a.segs + m.seg_begin, m.nseg, it.a_begin, it.a_end, a.ff, a.b_global,
+                              a.rot + 9 * (size_t)it.pattern, sm, Fr, Fi);
+    finish_item<K>(a, it, pix, Fr, Fi, &s_last);
   }
 }
 
 template <int K, int CHUNK, bool HAS_B>
 __global__ __launch_bounds__(kThreads) void sf_batch_grid_kernel(BatchLaunch a) {
   __shared__ AtomLds sm[CHUNK];
-  __shared__ int s_item;
+  __shared__ int s_item, s_last;
   for (;;) {
     const int item = next_item(a.queue, &s_item);
     if (item >= a.nitems) break;
-    const int2 it = a.items[item];
-    const int p = it.x;
-    const MolInfo m = a.mols[a.pattern_mol[p]];
+    const BatchItem it = a.items[item];
+    const MolInfo m = a.mols[a.pattern_mol[it.pattern]];
     int pix[4 * K];
     double Fr[4 * K], Fi[4 * K];
-    grid_tile<K, CHUNK, HAS_B>(a.hkl, a.nx, a.ny, it.y * (kThreads * K), a.atoms + m.atom_begin,
-                               HAS_B ? a.atom_b + m.atom_begin : nullptr, a.segs + m.seg_begin, m.nseg, 0,
-                               m.atom_end - m.atom_begin, a.ff, a.b_global, a.rot + 9 * (size_t)p, sm, pix, Fr, Fi);
-    const size_t off = (size_t)p * a.npix;
-#pragma unroll
-    for (int i = 0; i < 4 * K; ++i)
-      if (pix[i] >= 0) store_pixel(pix[i], Fr[i], Fi[i], a.npix, 0, nullptr, a.F ? a.F + off : nullptr, a.I + off);
+    grid_tile<K, CHUNK, HAS_B>(a.hkl, a.nx, a.ny, it.tile * (kThreads * K), a.atoms + m.atom_begin,
+                               HAS_B ? a.atom_b + m.atom_begin : nullptr, a.segs + m.seg_begin, m.nseg, it.a_begin,
+                               it.a_end, a.ff, a.b_global, a.rot + 9 * (size_t)it.pattern, sm, pix, Fr, Fi);
+    finish_item<4 * K>(a, it, pix, Fr, Fi, &s_last);
   }
 }
 
 template <int CHUNK, bool HAS_B>
 __global__ __launch_bounds__(kThreads) void sf_batch_octant_kernel(BatchLaunch a) {
   __shared__ AtomLds sm[CHUNK];
-  __shared__ int s_item;
+  __shared__ int s_item, s_last;
   for (;;) {
     const int item = next_item(a.queue, &s_item);
     if (item >= a.nitems) break;
-    const int2 it = a.items[item];
-    const int p = it.x;
-    const MolInfo m = a.mols[a.pattern_mol[p]];
+    const BatchItem it = a.items[item];
+    const MolInfo m = a.mols[a.pattern_mol[it.pattern]];
     int pix[8];
     double Fr[8], Fi[8];
-    octant_tile<CHUNK, HAS_B>(a.hkl, a.nx, it.y * kThreads, a.atoms + m.atom_begin,
-                              HAS_B ? a.atom_b + m.atom_begin : nullptr, a.segs + m.seg_begin, m.nseg, 0,
-                              m.atom_end - m.atom_begin, a.ff, a.b_global, a.rot + 9 * (size_t)p, sm, pix, Fr, Fi);
-    const size_t off = (size_t)p * a.npix;
-#pragma unroll
-    for (int i = 0; i < 8; ++i)
-      if (pix[i] >= 0) store_pixel(pix[i], Fr[i], Fi[i], a.npix, 0, nullptr, a.F ? a.F + off : nullptr, a.I + off);
+    octant_tile<CHUNK, HAS_B>(a.hkl, a.nx, it.tile * kThreads, a.atoms + m.atom_begin,
+                              HAS_B ? a.atom_b + m.atom_begin : nullptr, a.segs + m.seg_begin, m.nseg, it.a_begin,
+                              it.a_end, a.ff, a.b_global, a.rot + 9 * (size_t)it.pattern, sm, pix, Fr, Fi);
+    finish_item<8>(a, it, pix, Fr, Fi, &s_last);
   }
 }
 
@@ -819,8 +858,10 @@ const void *kernel_for(Kind kind, const spsim_hip_tile &t, bool has_b) {
                : kernel_ptr<false>(kind, t.pixels_per_lane, t.chunk_atoms);
 }
 
+// partial_buf, when given, holds the atom-range partial sums (at least
+// splits * npix entries) instead of the library workspace.
 int launch_single(Kind kind, const PatternLaunch &a, int nx, int ny, const spsim_hip_tile &t, int splits,
-                  hipStream_t s, const Spectrum *spec = nullptr) {
+                  hipStream_t s, const Spectrum *spec = nullptr, double2 *partial_buf = nullptr) {
   if (a.npix <= 0) return SPSIM_HIP_OK;
   const void *k = kernel_for(kind, t, a.atom_b != nullptr);
   if (!k) return fail(SPSIM_HIP_ERR_INVALID, "unsupported tile (per lane %d, chunk %d)", t.pixels_per_lane, t.chunk_atoms);
@@ -829,7 +870,9 @@ int launch_single(Kind kind, const PatternLaunch &a, int nx, int ny, const spsim
   const int tiles = (units + per_tile - 1) / per_tile;
   if (splits < 1) splits = 1;
   double2 *partial = nullptr;
-  if (splits > 1) {
+  if (splits > 1 && partial_buf) {
+    partial = partial_buf;
+  } else if (splits > 1) {
     void *ws = nullptr;
     SPSIM_TRY(workspace(sizeof(double2) * (size_t)splits * a.npix, &ws));
     partial = (double2 *)ws;
@@ -991,16 +1034,20 @@ extern "C" int spsim_hip_sf_pattern(const float *d_hkl, int npix, const spsim_hi
   return launch_pattern(a, t, splits, (hipStream_t)stream);
 }
 
-extern "C" int spsim_hip_sf_grid(const float *d_hkl, int nx, int ny, const spsim_hip_atom *d_atoms,
-                                 const float *d_atom_b, const spsim_hip_segment *d_segs, int nsegs, int natoms,
-                                 const float *d_ff, float b_global, const double *rot9, float *d_F, float *d_I,
-                                 const spsim_hip_tile *tile, void *stream) {
-  using namespace spsim;
+namespace spsim {
+namespace {
+
+// Checks the arguments of a grid call and picks its tile and atom-split count
+// (explicit, or chosen to fill the GPU). *npix_out = 0 means nothing to do.
+int grid_setup(const float *d_hkl, int nx, int ny, const spsim_hip_atom *d_atoms, const spsim_hip_segment *d_segs,
+               int nsegs, int natoms, const float *d_ff, const spsim_hip_tile *tile, spsim_hip_tile *t_out,
+               int *splits_out, bool *explicit_splits, int *npix_out) {
   if (nx < 0 || ny < 0) return fail(SPSIM_HIP_ERR_INVALID, "negative detector size %d x %d", nx, ny);
   const long long npix = (long long)nx * ny;
   SPSIM_TRY(check_common(d_hkl, npix, d_atoms, d_segs, nsegs, natoms, d_ff));
   int cus = 0;
   SPSIM_TRY(ensure_device(&cus));
+  *npix_out = (int)npix;
   if (npix == 0) return SPSIM_HIP_OK;
   spsim_hip_tile t = tile ? *tile : default_tile(natoms, (int)npix, true);
   if (!tile && nx == ny) t.pixels_per_lane = kOctantLane;
@@ -1009,9 +1056,28 @@ extern "C" int spsim_hip_sf_grid(const float *d_hkl, int nx, int ny, const spsim
                 "unsupported grid tile (quads_per_lane=%d waves=%d chunk_atoms=%d; octants need nx == ny)",
                 t.pixels_per_lane, t.waves, t.chunk_atoms);
   const int tiles = (grid_units(t, nx, ny) + grid_units_per_tile(t) - 1) / grid_units_per_tile(t);
-  const int splits = t.atom_splits > 0 ? t.atom_splits : auto_splits(tiles, natoms, t.chunk_atoms, cus);
+  *explicit_splits = t.atom_splits > 0;
+  *splits_out = t.atom_splits > 0 ? t.atom_splits : auto_splits(tiles, natoms, t.chunk_atoms, cus);
+  *t_out = t;
+  return SPSIM_HIP_OK;
+}
+
+}  // namespace
+}  // namespace spsim
+
+extern "C" int spsim_hip_sf_grid(const float *d_hkl, int nx, int ny, const spsim_hip_atom *d_atoms,
+                                 const float *d_atom_b, const spsim_hip_segment *d_segs, int nsegs, int natoms,
+                                 const float *d_ff, float b_global, const double *rot9, float *d_F, float *d_I,
+                                 const spsim_hip_tile *tile, void *stream) {
+  using namespace spsim;
+  spsim_hip_tile t{};
+  int splits = 1, npix = 0;
+  bool explicit_splits = false;
+  SPSIM_TRY(grid_setup(d_hkl, nx, ny, d_atoms, d_segs, nsegs, natoms, d_ff, tile, &t, &splits, &explicit_splits,
+                       &npix));
+  if (npix == 0) return SPSIM_HIP_OK;
   const PatternLaunch a =
-      make_launch(d_hkl, (int)npix, d_atoms, d_atom_b, d_segs, nsegs, natoms, d_ff, b_global, rot9, d_F, d_I);
+      make_launch(d_hkl, npix, d_atoms, d_atom_b, d_segs, nsegs, natoms, d_ff, b_global, rot9, d_F, d_I);
   return launch_grid(a, nx, ny, t, splits, (hipStream_t)stream);
 }
 

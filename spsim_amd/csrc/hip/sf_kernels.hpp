@@ -41,6 +41,16 @@ struct PatternLaunch {
   Rot rot;
 };
 
+// One work item of the batched kernels: the atoms [a_begin, a_end) (relative
+// to the molecule's first packed atom) of one pattern on one tile. When the
+// pattern's atoms are split over nchunks > 1 items, the tile's items share the
+// counter counters[group] and the partial slots slot .. slot + nchunks - 1
+// (slot_entries(tile, grid) double2 entries each), this item's being slot + chunk.
+struct BatchItem {
+  int pattern, tile, a_begin, a_end;
+  int group, chunk, nchunks, slot;
+};
+
 struct BatchLaunch {
   const float *hkl;
   int npix;
@@ -51,9 +61,11 @@ struct BatchLaunch {
   const MolInfo *mols;
   const int *pattern_mol;
   const double *rot;  // [npatterns][9]
-  const int2 *items;  // {pattern, tile}, cost-sorted
+  const BatchItem *items;  // cost-sorted
   int nitems;
-  unsigned *queue;    // work counter, reset by launch_batch
+  unsigned *queue;       // work counter, reset by launch_batch
+  unsigned *counters;    // per split tile, zero between runs
+  double2 *partials;     // partial slots of split tiles
   const float *ff;
   float b_global;
   float2 *F;  // [npatterns][npix] or null
@@ -80,6 +92,12 @@ inline int grid_units_per_tile(const spsim_hip_tile &t) {
 }
 inline int grid_units(const spsim_hip_tile &t, int nx, int ny) {
   return t.pixels_per_lane == kOctantLane ? grid_octants(nx) : grid_quads(nx, ny);
+}
+// double2 entries of one partial slot of the batched kernels: each lane's
+// pixels of a tile (K per lane, 4 K for quads, 8 for an octant).
+inline int slot_entries(const spsim_hip_tile &t, bool grid) {
+  const int per_lane = !grid ? t.pixels_per_lane : (t.pixels_per_lane == kOctantLane ? 8 : 4 * t.pixels_per_lane);
+  return per_lane * t.waves * 64;
 }
 // Pixels per grid work unit.
 inline int grid_unit_pixels(const spsim_hip_tile &t) { return t.pixels_per_lane == kOctantLane ? 8 : 4; }

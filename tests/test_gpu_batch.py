@@ -233,3 +233,116 @@ def test_config3_full_workload(gpu):
         assert worst < 1e-6, worst
     finally:
         L.spsim_hip_batch_destroy(plan)
+
+
+@pytest.mark.parametrize("shape", ["octants", "quads", "pixels"])
+def test_atom_ranges_match_unsplit_plan_and_oracle(gpu, shape):
+    """Patterns split into atom ranges (item bound 700 atoms: the 2000- and
+    5000-atom molecules become 3 and 8 ranges per tile) against the same plan
+    with no split, against the FP64 oracle, and over hipGraph replays (the
+    tiles' counters clear themselves, so every replay equals the first)."""
+    import torch
+
+    rng = np.random.default_rng(31)
+    sizes = [0, 1, 333, 2000, 5000]
+    mols = [wl.random_molecule(n, 2e-9 + 1e-9 * (n / 5000.0) ** (1 / 3), rng)[:2] if n else
+            (np.zeros(0, np.int32), np.zeros((0, 3), np.float32)) for n in sizes]
+    pattern_mol = np.array([4, 3, 2, 1, 0, 4, 3], np.int32)
+    npat = len(pattern_mol)
+    rots = wl.uniform_rotations(npat, rng)
+    nx, ny, cx = {"octants": (40, 40, 0.0), "quads": (48, 40, 0.0), "pixels": (40, 40, 1.5e-4)}[shape]
+    q = oracle.hkl_detector(nx, ny, nx * 5e-4 / 40, ny * 5e-4 / 40, 0.1, cx, 0, 0, 0.2e-9)
+    L = _lib.load()
+    plan, *_ = make_plan(mols, rots, pattern_mol, q, nx, ny)
+    try:
+        stream = torch.cuda.Stream()
+        sp = C.c_void_p(stream.cuda_stream)
+        npix = q.shape[0]
+        _lib.check(L.spsim_hip_batch_set_item_atoms(plan, -1), "no split")
+        nitems, bound, split = C.c_int(), C.c_int(), C.c_int()
+        L.spsim_hip_batch_describe(plan, C.byref(nitems), C.byref(bound), C.byref(split))
+        assert bound.value == 0 and split.value == 0
+        I0 = torch.zeros((npat, npix), dtype=torch.float32, device=gpu)
+        _lib.check(L.spsim_hip_batch_run(plan, None, C.c_void_p(I0.data_ptr()), 0, sp), "run unsplit")
+        _lib.check(L.spsim_hip_batch_set_item_atoms(plan, 700), "split")
+        L.spsim_hip_batch_describe(plan, C.byref(nitems), C.byref(bound), C.byref(split))
+        assert bound.value == 700 and split.value > 0
+        tile, blocks = _lib.Tile(), C.c_int()
+        L.spsim_hip_batch_get_tile(plan, C.byref(tile), C.byref(blocks))
+        print(f"{shape}: tile {tile.pixels_per_lane}/{tile.chunk_atoms}, {nitems.value} items, "
+              f"{split.value} split tiles", flush=True)
+        I1 = torch.full((npat, npix), float("nan"), dtype=torch.float32, device=gpu)
+        _lib.check(L.spsim_hip_batch_run(plan, None, C.c_void_p(I1.data_ptr()), 0, sp), "run split")
+        stream.synchronize()
+        a, b = I0.cpu().numpy(), I1.cpu().numpy()
+        for p in range(npat):
+            Z, pos = mols[pattern_mol[p]]
+            if len(Z) == 0:
+                assert np.all(b[p] == 0)
+                continue
+            assert rel_l2(b[p], a[p]) < 1e-6, p
+            ref = oracle.sf_f64(q, Z, pos, R=rots[p], threads=oracle_threads())[1]
+            assert rel_l2(b[p], ref) < 1e-5, p
+        # graph replays with eager work on the device between them
+        I2 = torch.full((npat, npix), float("nan"), dtype=torch.float32, device=gpu)
+        first = None
+        for rep in range(3):
+            with torch.cuda.stream(stream):
+                _lib.check(L.spsim_hip_batch_run(plan, None, C.c_void_p(I2.data_ptr()), 1, sp), "graph")
+            stream.synchronize()
+            got = I2.cpu().numpy().copy()
+            if first is None:
+                first = got
+                assert np.array_equal(got.view(np.uint32), b.view(np.uint32))
+            else:
+                assert np.array_equal(got.view(np.uint32), first.view(np.uint32)), rep
+            I2.fill_(float("nan"))
+            torch.cuda.synchronize()
+    finally:
+        L.spsim_hip_batch_destroy(plan)
+
+
+def test_automatic_atom_ranges_on_a_skewed_batch(gpu):
+    """One 60,000-atom molecule among fifteen 400-atom ones on a 128 x 128
+    detector: the automatic item bound splits the large molecule's tiles (so
+    its work spreads over the GPU) and leaves the small ones whole; every
+    pattern against the FP64 oracle on a pixel sample, and the split plan
+    against the unsplit one."""
+    import torch
+
+    rng = np.random.default_rng(32)
+    big = wl.random_molecule(60000, 6e-9, rng)[:2]
+    small = [wl.random_molecule(400, 1.5e-9, rng)[:2] for _ in range(15)]
+    mols = [big] + small
+    npat = len(mols)
+    pattern_mol = np.arange(npat, dtype=np.int32)
+    rots = wl.uniform_rotations(npat, rng)
+    n = 128
+    q = oracle.hkl_detector(n, n, n * 75e-6, n * 75e-6, 0.1, 0, 0, 0, 0.2e-9)
+    L = _lib.load()
+    plan, *_ = make_plan(mols, rots, pattern_mol, q, n, n)
+    try:
+        stream = torch.cuda.Stream()
+        sp = C.c_void_p(stream.cuda_stream)
+        nitems, bound, split = C.c_int(), C.c_int(), C.c_int()
+        L.spsim_hip_batch_describe(plan, C.byref(nitems), C.byref(bound), C.byref(split))
+        print(f"skewed batch: item bound {bound.value} atoms, {nitems.value} items, {split.value} split tiles",
+              flush=True)
+        assert 0 < bound.value < 60000 and split.value > 0
+        I = torch.zeros((npat, n * n), dtype=torch.float32, device=gpu)
+        with torch.cuda.stream(stream):
+            _lib.check(L.spsim_hip_batch_run(plan, None, C.c_void_p(I.data_ptr()), 1, sp), "graph")
+        stream.synchronize()
+        Ib = I.cpu().numpy()
+        sub = np.sort(rng.choice(n * n, 1024, replace=False))
+        for p in range(npat):
+            Z, pos = mols[p]
+            ref = oracle.sf_f64(q[sub], Z, pos, R=rots[p], threads=oracle_threads())[1]
+            assert rel_l2(Ib[p, sub], ref) < 1e-5, p
+        _lib.check(L.spsim_hip_batch_set_item_atoms(plan, -1), "no split")
+        I0 = torch.zeros_like(I)
+        _lib.check(L.spsim_hip_batch_run(plan, None, C.c_void_p(I0.data_ptr()), 0, sp), "unsplit")
+        stream.synchronize()
+        assert rel_l2(Ib, I0.cpu().numpy()) < 1e-6
+    finally:
+        L.spsim_hip_batch_destroy(plan)
