@@ -144,6 +144,24 @@ int spsim_hip_sf_grid(const float *d_hkl, int nx, int ny, const spsim_hip_atom *
                       const spsim_hip_segment *d_segs, int nsegs, int natoms, const float *d_ff, float b_global,
                       const double *rot9, float *d_F, float *d_I, const spsim_hip_tile *tile, void *stream);
 
+/* spsim_hip_sf_grid without any allocation. No counterpart in the reference:
+   it is for callers that cannot allocate device memory (code under HIP graph
+   capture, tight serving loops, harnesses that forbid hipMalloc). Same
+   arguments and kernels; the atom-range partial sums use a fixed 32 MiB device
+   array (2^21 double2 entries, held by each device the library runs on until
+   the process ends) instead of the library workspace, so splits x nx x ny may
+   not exceed 2^21. With tile->atom_splits > 0 a larger product is rejected
+   (SPSIM_HIP_ERR_INVALID, nothing launched); with an automatic split count the
+   count is lowered to what fits (1 range needs no array at all, so every
+   detector size is accepted). Every call on one device shares that device's
+   array: calls that overlap on one device (two streams, two threads) race. With
+   the same tile and split count the result is bitwise that of
+   spsim_hip_sf_grid. */
+int spsim_hip_sf_grid_noalloc(const float *d_hkl, int nx, int ny, const spsim_hip_atom *d_atoms,
+                              const float *d_atom_b, const spsim_hip_segment *d_segs, int nsegs, int natoms,
+                              const float *d_ff, float b_global, const double *rot9, float *d_F, float *d_I,
+                              const spsim_hip_tile *tile, void *stream);
+
 /* ---- Wavelength spectrum (replaces the spectrum branch of the reference's GPU
    path, cuda_compute_pattern_on_list2 src/diffraction.cu:300-318 with the kernel
    CUDA_spectrum_scattering_from_all_atoms :416-476) ---- */

@@ -65,6 +65,8 @@ SIGNATURES = {
     "spsim_hip_hkl_detector_3d": (_i, [_i, _i, _i, _f, _f, _f, _f, _f, _f, _f, _f, _vp, _vp]),
     "spsim_hip_sf_pattern": (_i, [_vp, _i, _vp, _vp, _vp, _i, _i, _vp, _f, _vp, _vp, _vp, C.POINTER(Tile), _vp]),
     "spsim_hip_sf_grid": (_i, [_vp, _i, _i, _vp, _vp, _vp, _i, _i, _vp, _f, _vp, _vp, _vp, C.POINTER(Tile), _vp]),
+    "spsim_hip_sf_grid_noalloc": (_i, [_vp, _i, _i, _vp, _vp, _vp, _i, _i, _vp, _f, _vp, _vp, _vp, C.POINTER(Tile),
+                                       _vp]),
     "spsim_hip_sf_spectrum": (_i, [_vp, _i, _vp, _vp, _vp, _i, _i, _vp, _f, _vp, _i, _vp, _vp, _vp, _vp,
                                    C.POINTER(Tile), _vp]),
     "spsim_hip_spectrum_samples": (_i, [C.c_double, C.c_double, _i, _vp, _vp]),

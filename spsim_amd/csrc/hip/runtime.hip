@@ -2,8 +2,6 @@
 #include <cstdio>
 #include <cstring>
 #include <mutex>
-#include <string>
-#include <vector>
 
 #include "common.hpp"
 #include "../host/cromer_mann.h"
@@ -11,7 +9,10 @@
 namespace spsim {
 
 namespace {
-thread_local std::string g_last_error;
+// Fixed-size storage, so that recording an error or the first call on a device
+// allocates no host memory.
+thread_local char g_last_error[1024];
+constexpr int kMaxDevices = 64;
 
 struct DeviceState {
   int checked = 0;  // 0 = not yet, 1 = ok, -1 = unusable
@@ -23,25 +24,21 @@ struct DeviceState {
   hipEvent_t ev[kHelperStreams + 1] = {};
 };
 std::mutex g_mu;
-std::vector<DeviceState> g_dev;
+DeviceState g_dev[kMaxDevices];
 
 DeviceState *state_for_current(int *dev_out) {
   int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
-  std::lock_guard<std::mutex> lk(g_mu);
-  if ((int)g_dev.size() <= dev) g_dev.resize(dev + 1);
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return nullptr;
   if (dev_out) *dev_out = dev;
   return &g_dev[dev];
 }
 }  // namespace
 
 int fail(int code, const char *fmt, ...) {
-  char buf[1024];
   va_list ap;
   va_start(ap, fmt);
-  vsnprintf(buf, sizeof buf, fmt, ap);
+  vsnprintf(g_last_error, sizeof g_last_error, fmt, ap);
   va_end(ap);
-  g_last_error = buf;
   return code;
 }
 
@@ -121,7 +118,7 @@ void workspace_release() {
 
 extern "C" {
 
-const char *spsim_hip_last_error(void) { return spsim::g_last_error.c_str(); }
+const char *spsim_hip_last_error(void) { return spsim::g_last_error; }
 
 int spsim_hip_device_count(int *count) {
   if (!count) return spsim::fail(SPSIM_HIP_ERR_INVALID, "count is NULL");

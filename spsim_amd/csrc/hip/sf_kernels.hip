@@ -24,6 +24,7 @@
 #include "form_factor.hpp"
 
 #include <algorithm>
+#include <mutex>
 
 namespace spsim {
 
@@ -1062,6 +1063,36 @@ int grid_setup(const float *d_hkl, int nx, int ny, const spsim_hip_atom *d_atoms
   return SPSIM_HIP_OK;
 }
 
+// ---- no-allocation variant of spsim_hip_sf_grid ----
+//
+// The atom-range partial sums live in this fixed device array instead of the
+// library workspace, so the call allocates nothing. Host code never uses the
+// array's name as an address: the device address comes from
+// hipGetSymbolAddress, once per device with that device current, and is kept
+// in noalloc_ptr[device]. Every call on one device shares that device's array.
+constexpr size_t kNoAllocPartials = (size_t)1 << 21;  // double2 entries: 32 MiB
+static __device__ __align__(256) double2 scratch[kNoAllocPartials];
+
+constexpr int kNoAllocDevices = 64;
+std::mutex g_noalloc_mu;
+double2 *g_noalloc_ptr[kNoAllocDevices];
+
+int noalloc_scratch(double2 **out) {
+  int dev = 0;
+  SPSIM_HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNoAllocDevices) return fail(SPSIM_HIP_ERR_NODEVICE, "device index %d out of range", dev);
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (!g_noalloc_ptr[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    if (!p) return fail(SPSIM_HIP_ERR_HIP, "hipGetSymbolAddress(scratch) returned NULL");
+    g_noalloc_ptr[dev] = (double2 *)p;
+  }
+  *out = g_noalloc_ptr[dev];
+  return SPSIM_HIP_OK;
+}
+
 }  // namespace
 }  // namespace spsim
 
@@ -1079,6 +1110,34 @@ extern "C" int spsim_hip_sf_grid(const float *d_hkl, int nx, int ny, const spsim
   const PatternLaunch a =
       make_launch(d_hkl, npix, d_atoms, d_atom_b, d_segs, nsegs, natoms, d_ff, b_global, rot9, d_F, d_I);
   return launch_grid(a, nx, ny, t, splits, (hipStream_t)stream);
+}
+
+extern "C" int spsim_hip_sf_grid_noalloc(const float *d_hkl, int nx, int ny, const spsim_hip_atom *d_atoms,
+                                         const float *d_atom_b, const spsim_hip_segment *d_segs, int nsegs,
+                                         int natoms, const float *d_ff, float b_global, const double *rot9,
+                                         float *d_F, float *d_I, const spsim_hip_tile *tile, void *stream) {
+  using namespace spsim;
+  spsim_hip_tile t{};
+  int splits = 1, npix = 0;
+  bool explicit_splits = false;
+  SPSIM_TRY(grid_setup(d_hkl, nx, ny, d_atoms, d_segs, nsegs, natoms, d_ff, tile, &t, &splits, &explicit_splits,
+                       &npix));
+  if (npix == 0) return SPSIM_HIP_OK;
+  const long long cap = (long long)(kNoAllocPartials / (size_t)npix);  // atom ranges that fit
+  if (splits > 1 && (long long)splits > cap) {
+    if (explicit_splits)
+      return fail(SPSIM_HIP_ERR_INVALID,
+                  "atom_splits=%d on %d pixels needs %lld bytes of partial sums; the no-allocation entry point "
+                  "holds %zu (at most %lld atom ranges at this size)",
+                  splits, npix, (long long)splits * npix * (long long)sizeof(double2),
+                  kNoAllocPartials * sizeof(double2), cap < 1 ? 1LL : cap);
+    splits = cap < 1 ? 1 : (int)cap;  // automatic choice: as many ranges as fit
+  }
+  double2 *partial = nullptr;
+  if (splits > 1) SPSIM_TRY(noalloc_scratch(&partial));
+  const PatternLaunch a =
+      make_launch(d_hkl, npix, d_atoms, d_atom_b, d_segs, nsegs, natoms, d_ff, b_global, rot9, d_F, d_I);
+  return launch_single(Kind::kGrid, a, nx, ny, t, splits, (hipStream_t)stream, nullptr, partial);
 }
 
 extern "C" int spsim_hip_sf_spectrum(const float *d_hkl, int npix, const spsim_hip_atom *d_atoms,
