@@ -1,26 +1,42 @@
 #!/usr/bin/env python3
 """Benchmark of the structure-factor hot path on one MI355X.
 
-Headline (`value`): G pixel-atom terms/s of one large pattern, the
-"large single pattern" configuration: a 100,000-atom C/N/O/S molecule in a
-12 nm-radius sphere with per-atom B factors U[10, 40] A^2, on a 1024 x 1024
-detector (75 um pixels at 0.1 m, lambda 0.15 nm). A step is one
-spsim_hip_sf_pattern call (q list, packed atoms and output already in HBM).
+Headline (`value`, patterns/s): the batched uneven-instances workload, 512
+molecules with log-uniform atom counts in [2e3, 2e5] (C/N/O/S, radius
+5 nm (N/5000)^(1/3)) at uniform random orientations on a 512 x 512 detector
+(75 um pixels at 0.1 m, lambda 0.2 nm). A step is one replay of the captured
+hipGraph of the persistent (pattern, tile, atom range) work-list kernel over
+all 512 patterns, inputs and outputs already in HBM; the tile is chosen by the
+plan's autotuner. Beside it: G pixel-atom terms/s, the plain path (one
+single-pattern launch per pattern with a host synchronisation after each),
+the FP64 error on 4,096 random pixels of 4 patterns, and the roofline of the
+timed kernel.
 
-Also reported, from the "batched uneven instances" configuration (512
-molecules with log-uniform atom counts in [2e3, 2e5] at random orientations on
-a 512 x 512 detector): patterns/s of the persistent work-list kernel replayed
-from a hipGraph versus the plain one-launch-per-pattern path.
+Roofline: the structure-factor kernels are bound by VALU issue. `achieved` is
+the VALU wave instructions the timed kernel executes (its inner loop's count
+per wave and atom, read from this build's gfx950 assembly, times the
+wave-atom iterations of the launch), expressed as FP32 flop-equivalents at the
+peak's 4 flops per lane and issue slot, so frac = achieved / 157.3 is the
+fraction of the VALU issue slots used. The textbook count (10 FP32 flops and
+2 transcendentals at 8 flop-equivalents per pixel-atom term) is reported
+under `textbook`: the mirror and octant kernels share sines between pixels,
+so that figure can exceed the peak.
 
-The CPU baseline times the FP64 oracle (oracle/oracle.c, all host threads) on
-a bounded pixel sample of the headline workload; it is a reference point, not
-the target.
+Further sections: the large single pattern (100,000 atoms with per-atom B,
+1024 x 1024), the small-molecule slice (2e3-1e4 atoms) and a skewed batch
+(batched vs plain, and the atom-range split), the full pipeline (256
+orientations of a 50,000-atom molecule plus the detector chain), the
+differentiation workload, the wavelength spectrum and the NUFFT paths, and the
+CPU baseline: the oracle's restatement of the reference's plain path
+(compute_pattern_on_list arithmetic: FP32 dot, FP64 libm cos/sin, FP32 sums),
+timed on all host cores on a bounded sample, with the FP64 oracle beside it.
 """
 from __future__ import annotations
 
 import argparse
 import ctypes as C
 import json
+import math
 import os
 import sys
 import time
@@ -31,14 +47,32 @@ import numpy as np
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
-# Flop-equivalents per pixel-atom term: 10 FP32 flops + sin + cos
-# at 8 flop-equivalents each (transcendentals run at 32/clk/CU on gfx950 against
-# 256 FP32 flops/clk/CU), plus exp2 and a multiply for the per-atom Debye-Waller factor.
-W_EQ_PLAIN = 26.0
-W_EQ_DW = 36.0
-PEAK_FP32_TFLOPS = 157.3
+PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector: 256 CUs x 2.4 GHz x 256 flop/clk (4 flop per lane and issue slot)
 HBM_PEAK_GBS = 8000.0
 COPY_CAL_BYTES = 1 << 29  # 512 MiB
+# Textbook work per pixel-atom term: 10 FP32 flops + sin + cos at 8 flop-equivalents
+# each (32 transcendentals/clk/CU against 256 flops/clk/CU), plus exp2 and a multiply
+# with a per-atom Debye-Waller factor.
+W_TEXTBOOK = 26.0
+W_TEXTBOOK_DW = 36.0
+
+# VALU wave instructions per wave and atom in the inner loop of each kernel
+# family, counted in the gfx950 assembly of this build (loop body up to its
+# back edge; v_sin/v_cos/v_exp counted once each) and checked against the
+# SQ_INSTS_VALU counter of profiled launches. Keys: (family, per-atom B).
+#   octant: 8 pixels per lane (5 phases: 10 sin/cos), quad1/quad2: 1 or 2
+#   mirror quads per lane (3 phases per quad), pixelK: K pixels per lane.
+VALU_PER_WAVE_ATOM = {
+    ("octant", False): 46, ("octant", True): 48,
+    ("quad1", False): 27, ("quad1", True): 29,
+    ("quad2", False): 53, ("quad2", True): 57,
+    ("pixel1", False): 10, ("pixel1", True): 12,
+    ("pixel2", False): 19, ("pixel2", True): 23,
+    ("pixel4", False): 37, ("pixel4", True): 45,
+    ("pixel8", False): 73, ("pixel8", True): 89,
+}
+WAVES_PER_TILE = 4
+LANES = 64
 
 
 def parse():
@@ -46,14 +80,18 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=3)
-    p.add_argument("--no-batched", action="store_true", help="skip the batched vs plain measurement")
-    p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--batched-patterns", type=int, default=512)
     p.add_argument("--pipeline-patterns", type=int, default=256)
+    p.add_argument("--batch-tile", default=None,
+                   help="K,C: run the headline plan on this tile (pixels or quads per lane, or 8 for octants; "
+                        "LDS chunk) instead of the autotuner's choice")
+    p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    p.add_argument("--no-single", action="store_true", help="skip the large-single-pattern section")
+    p.add_argument("--no-slices", action="store_true", help="skip the small-molecule slice and the skewed batch")
     p.add_argument("--no-pipeline", action="store_true", help="skip the full-pipeline measurement")
     p.add_argument("--no-adjoint", action="store_true", help="skip the differentiation measurement")
     p.add_argument("--no-spectrum", action="store_true", help="skip the wavelength-spectrum measurement")
-    p.add_argument("--no-nfft", action="store_true", help="skip the non-uniform FFT measurement")
+    p.add_argument("--no-nfft", action="store_true", help="skip the non-uniform FFT measurements")
     p.add_argument("--nfft-only", action="store_true",
                    help="only the non-uniform FFT section on the large-single-pattern geometry (for profiling)")
     p.add_argument("--spectrum-only", action="store_true",
@@ -62,9 +100,11 @@ def parse():
                    help="only the batched (many molecules) workload through the NUFFT (for profiling)")
     p.add_argument("--pipeline-nfft-only", action="store_true",
                    help="only the pipeline workload through the non-uniform FFT (for profiling)")
-    p.add_argument("--profile", action="store_true", help="headline workload only (for rocprofv3)")
-    p.add_argument("--kernel", choices=("auto", "pattern", "grid"), default="auto",
-                   help="restrict the headline sweep to one structure-factor kernel")
+    p.add_argument("--pipeline-only", action="store_true", help="only the full-pipeline section (for profiling)")
+    p.add_argument("--single-only", action="store_true",
+                   help="only the large-single-pattern section (for profiling)")
+    p.add_argument("--profile", action="store_true",
+                   help="headline workload only, preceded by a plain copy of known size (for rocprofv3)")
     return p.parse_args()
 
 
@@ -77,29 +117,295 @@ def rel_l2(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
-def recorded_traffic(kernel, tile):
-    """HBM bytes per launch of the headline kernel from the FETCH_SIZE /
-    WRITE_SIZE counter runs recorded in profiles/ (scaled by the copy
-    calibration recorded with them); None when the chosen kernel or tile is not
-    the one that was profiled."""
+def oracle_threads():
+    return min(16, os.cpu_count() or 1)
+
+
+# ---- kernel bookkeeping -----------------------------------------------------
+
+def grid_octants(n):
+    h = (n + 1) // 2
+    return h * (h + 1) // 2
+
+
+def kernel_family(grid, per_lane):
+    """Kernel family of a tile: octant / quadK on a mirror-symmetric list, pixelK otherwise."""
+    if not grid:
+        return f"pixel{per_lane}"
+    return "octant" if per_lane == 8 else f"quad{per_lane}"
+
+
+def tiles_per_pattern(family, nx, ny, per_lane):
+    lanes = WAVES_PER_TILE * LANES
+    if family == "octant":
+        units, per_tile = grid_octants(nx), lanes
+    elif family.startswith("quad"):
+        units, per_tile = ((nx + 1) // 2) * ((ny + 1) // 2), per_lane * lanes
+    else:
+        units, per_tile = nx * ny, per_lane * lanes
+    return -(-units // per_tile)
+
+
+def executed_roofline(family, has_b, wave_atoms, kernel_ms, terms, kernel_name):
+    """Roofline of a structure-factor launch from the VALU instructions it
+    executes: wave_atoms inner-loop iterations (one wave, one atom) of
+    VALU_PER_WAVE_ATOM[family] instructions each, in kernel_ms."""
+    valu = VALU_PER_WAVE_ATOM[(family, has_b)]
+    wave_instr = float(wave_atoms) * valu
+    achieved = wave_instr * LANES * 4 / (kernel_ms * 1e-3) / 1e12
+    frac = achieved / PEAK_FP32_TFLOPS
+    if frac > 1.02:
+        raise AssertionError(f"{kernel_name}: executed VALU rate {achieved:.1f} TFLOP/s-eq exceeds the peak; "
+                             f"the instruction count per wave-atom ({valu}) is wrong for this build")
+    w_tb = W_TEXTBOOK_DW if has_b else W_TEXTBOOK
+    tb = terms * w_tb / (kernel_ms * 1e-3) / 1e12
+    return {
+        "bound": "valu", "achieved": round(achieved, 2), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+        "frac": round(frac, 4), "traffic": None, "kernel": kernel_name,
+        "work": {"valu_wave_instr_per_wave_atom": valu, "wave_atoms_per_launch": float(wave_atoms),
+                 "flop_equiv_per_lane_slot": 4, "kernel_ms": round(kernel_ms, 4)},
+        "textbook": {"w_eq_per_term": w_tb, "terms_per_launch": terms,
+                     "tflops_equiv": round(tb, 2), "frac_of_peak": round(tb / PEAK_FP32_TFLOPS, 4),
+                     "note": "textbook sin+cos per term; the mirror and octant kernels share sines between "
+                             "pixels, so this figure is not bounded by the peak"},
+    }
+
+
+def recorded_traffic(name, kernel, per_lane, chunk, splits=None):
+    """HBM bytes per launch of a kernel from the FETCH_SIZE / WRITE_SIZE counter
+    runs recorded in profiles/<name> (each scaled by the same counter's reading
+    for a plain copy of known size); None when this tile was not profiled."""
     try:
-        rec = json.loads((ROOT / "profiles" / "traffic_large_single_pattern.json").read_text())
+        recs = json.loads((ROOT / "profiles" / name).read_text())
     except (OSError, ValueError):
         return None
-    if (rec.get("kernel"), rec.get("per_lane"), rec.get("chunk_atoms"), rec.get("atom_splits")) != \
-            (kernel, tile.pixels_per_lane, tile.chunk_atoms, tile.atom_splits):
-        return None
-    return round(1024 * (rec["fetch_kib_per_launch"] * rec["fetch_scale"] +
-                         rec["write_kib_per_launch"] * rec["write_scale"]))
+    for rec in recs if isinstance(recs, list) else [recs]:
+        if (rec.get("kernel"), rec.get("per_lane"), rec.get("chunk_atoms")) != (kernel, per_lane, chunk):
+            continue
+        if splits is not None and rec.get("atom_splits") not in (None, splits):
+            continue
+        return round(1024 * (rec["fetch_kib_per_launch"] * rec["fetch_scale"] +
+                             rec["write_kib_per_launch"] * rec["write_scale"]))
+    return None
 
 
-def kernel_name(kind, tile):
-    if kind == "grid":
-        return "sf_octant_kernel" if tile.pixels_per_lane == 8 else "sf_grid_kernel"
-    return "sf_pattern_kernel"
+# ---- batched plans ----------------------------------------------------------
 
+def make_plan(L, _lib, q, nx, ny, mols, rots, pattern_mol=None, atom_b=None):
+    vp = lambda a: a.ctypes.data_as(C.c_void_p)
+    offs = np.zeros(len(mols) + 1, np.int32)
+    offs[1:] = np.cumsum([len(m[0]) for m in mols])
+    Z = np.ascontiguousarray(np.concatenate([m[0] for m in mols]), np.int32)
+    P = np.ascontiguousarray(np.concatenate([m[1] for m in mols]), np.float32)
+    pm = np.arange(len(mols), dtype=np.int32) if pattern_mol is None else np.ascontiguousarray(pattern_mol, np.int32)
+    R = np.ascontiguousarray(np.asarray(rots).reshape(-1, 9), np.float64)
+    B = None if atom_b is None else np.ascontiguousarray(atom_b, np.float32)
+    plan = C.c_void_p()
+    _lib.check(L.spsim_hip_batch_create(vp(q), nx, ny, len(mols), vp(offs), vp(Z), vp(P), None,
+                                        None if B is None else vp(B), 0, len(pm), vp(pm), vp(R), None,
+                                        C.c_float(0.0), None, C.byref(plan)), "spsim_hip_batch_create")
+    return plan
+
+
+def plan_state(L, plan):
+    from spsim_amd._lib import Tile
+
+    tile, blocks = Tile(), C.c_int()
+    L.spsim_hip_batch_get_tile(plan, C.byref(tile), C.byref(blocks))
+    nitems, bound, split = C.c_int(), C.c_int(), C.c_int()
+    L.spsim_hip_batch_describe(plan, C.byref(nitems), C.byref(bound), C.byref(split))
+    grid = bool(L.spsim_hip_batch_uses_grid(plan))
+    fam = kernel_family(grid, tile.pixels_per_lane)
+    kname = {"octant": "sf_batch_octant_kernel"}.get(fam, "sf_batch_grid_kernel" if grid else "sf_batch_kernel")
+    return {"kernel": kname, "family": fam, "per_lane": tile.pixels_per_lane, "chunk_atoms": tile.chunk_atoms,
+            "blocks": blocks.value, "items": nitems.value, "item_atoms": bound.value, "split_tiles": split.value}
+
+
+def time_graph(L, _lib, torch, plan, I, stream, reps):
+    """Mean ms of `reps` graph replays (HIP events on the replay stream)."""
+    sp = C.c_void_p(stream.cuda_stream)
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+    with torch.cuda.stream(stream):
+        for e0, e1 in ev:
+            e0.record(stream)
+            _lib.check(L.spsim_hip_batch_run(plan, None, C.c_void_p(I.data_ptr()), 1, sp), "batch_run")
+            e1.record(stream)
+    stream.synchronize()
+    return float(np.mean([a.elapsed_time(b) for a, b in ev]))
+
+
+def time_plain(L, _lib, plan, I, stream):
+    """Wall ms of one plain pass (one launch per pattern, host sync after each)."""
+    sp = C.c_void_p(stream.cuda_stream)
+    stream.synchronize()
+    t0 = time.perf_counter()
+    _lib.check(L.spsim_hip_batch_run_plain(plan, None, C.c_void_p(I.data_ptr()), sp), "batch_run_plain")
+    stream.synchronize()
+    return (time.perf_counter() - t0) * 1e3
+
+
+def fp64_check(I_host, q, mols, rots, patterns, npix_sample, seed):
+    """Max relative L2 error of the listed patterns on random pixel samples
+    against the FP64 oracle (q rotated exactly, q' = R q)."""
+    import oracle
+
+    rng = np.random.default_rng(seed)
+    errs = []
+    for p in patterns:
+        sub = np.sort(rng.choice(q.shape[0], npix_sample, replace=False))
+        Z, pos = mols[p]
+        _, ref = oracle.sf_f64(q[sub], Z, pos, R=rots[p], threads=oracle_threads())
+        errs.append(rel_l2(I_host[p][sub], ref))
+    return max(errs), errs
+
+
+def wave_atoms_of(mols, pattern_mol, family, nx, ny, per_lane):
+    natoms = sum(len(mols[m][0]) for m in pattern_mol)
+    return float(natoms) * tiles_per_pattern(family, nx, ny, per_lane) * WAVES_PER_TILE
+
+
+# ---- sections ---------------------------------------------------------------
 
 def headline(args, torch, dv, wl, _lib):
+    """The batched uneven-instances workload: autotune, graph replays (the
+    timed steps), the plain path, FP64 check, roofline."""
+    det, mols, rots = wl.config3(args.batched_patterns)
+    npat = len(mols)
+    q = dv.hkl_detector(det.n, det.n, det.width, det.width, det.distance, det.wavelength,
+                        device=torch.device("cuda", 0)).cpu().numpy()
+    L = _lib.load()
+    log(f"headline: packing {npat} molecules ({sum(len(m[0]) for m in mols)} atoms)")
+    plan = make_plan(L, _lib, q, det.n, det.n, mols, rots)
+    stream = torch.cuda.Stream()
+    sp = C.c_void_p(stream.cuda_stream)
+    try:
+        t0 = time.perf_counter()
+        if args.batch_tile:
+            k, c = (int(x) for x in args.batch_tile.split(","))
+            _lib.check(L.spsim_hip_batch_set_tile(plan, C.byref(_lib.Tile(k, 4, c, 0))), "set_tile")
+            tuned = "fixed by --batch-tile"
+        else:
+            _lib.check(L.spsim_hip_batch_autotune(plan, sp, None, None), "spsim_hip_batch_autotune")
+            tuned = "autotuner"
+        tune_s = time.perf_counter() - t0
+        st = plan_state(L, plan)
+        log(f"headline: tile {st['family']}/{st['chunk_atoms']} ({tuned}, {tune_s:.1f} s), {st['items']} items")
+        terms = C.c_double(0)
+        L.spsim_hip_batch_terms(plan, C.byref(terms))
+        terms = terms.value
+        I = torch.empty((npat, det.npix), dtype=torch.float32, device="cuda")
+        with torch.cuda.stream(stream):
+            for _ in range(max(1, args.warmup)):  # the first replay captures the graph
+                _lib.check(L.spsim_hip_batch_run(plan, None, C.c_void_p(I.data_ptr()), 1, sp), "batch_run")
+        stream.synchronize()
+        torch.cuda.synchronize()
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+        t0 = time.perf_counter()
+        with torch.cuda.stream(stream):
+            for e0, e1 in ev:
+                e0.record(stream)
+                _lib.check(L.spsim_hip_batch_run(plan, None, C.c_void_p(I.data_ptr()), 1, sp), "batch_run")
+                e1.record(stream)
+        stream.synchronize()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        kernel_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+        ms_per_step = wall * 1e3 / args.steps
+        I_host = I.cpu().numpy()
+        res = {"plan": st, "tuned_by": tuned, "autotune_s": round(tune_s, 2), "terms": terms,
+               "kernel_ms": kernel_ms, "ms_per_step": ms_per_step, "npatterns": npat,
+               "natoms_mean": float(np.mean([len(m[0]) for m in mols]))}
+        if not args.profile:
+            log("headline: plain path (one launch per pattern)")
+            I2 = torch.empty_like(I)
+            ms_plain = time_plain(L, _lib, plan, I2, stream)
+            res["plain_ms"] = ms_plain
+            res["batched_vs_plain_rel_l2"] = rel_l2(I2.cpu().numpy(), I_host)
+            del I2
+            sizes = np.array([len(m[0]) for m in mols])
+            pats = [int(sizes.argmin()), int(sizes.argmax()), int(np.argsort(sizes)[npat // 4]),
+                    int(np.argsort(sizes)[3 * npat // 4])]
+            log("headline: FP64 check on 4 patterns x 4096 pixels")
+            res["rel_err"], res["rel_errs"] = fp64_check(I_host, q, mols, rots, pats, 4096, 7)
+            res["rel_err_patterns"] = [{"pattern": p, "natoms": int(sizes[p])} for p in pats]
+        wave_atoms = wave_atoms_of(mols, range(npat), st["family"], det.n, det.n, st["per_lane"])
+        roof = executed_roofline(st["family"], False, wave_atoms, kernel_ms, terms, st["kernel"])
+        roof["traffic"] = recorded_traffic("traffic_batched.json", st["kernel"], st["per_lane"], st["chunk_atoms"])
+        roof["algorithmic_bytes_per_launch"] = float(16 * sum(len(m[0]) for m in mols) + 12 * det.npix +
+                                                     4 * det.npix * npat)
+        res["roofline"] = roof
+        res["workload"] = (f"{npat} molecules, N log-uniform in [2e3, 2e5] (mean {res['natoms_mean']:.0f}), C/N/O/S "
+                           f"62:17:20:1, radius 5 nm (N/5000)^(1/3), uniform random orientations, "
+                           f"{det.n}x{det.n} detector 75 um at 0.1 m (assumed), lambda 0.2 nm; FP64 phase, FP32 "
+                           f"sincos and chunk sums, FP64 per-chunk accumulation")
+        return res
+    finally:
+        L.spsim_hip_batch_destroy(plan)
+
+
+def batched_vs_plain(name, det, mols, rots, args, torch, dv, _lib, fp64_patterns, extra=None):
+    """Autotuned batched graph replays against the plain path on one workload."""
+    npat = len(mols)
+    q = dv.hkl_detector(det.n, det.n, det.width, det.width, det.distance, det.wavelength,
+                        device=torch.device("cuda", 0)).cpu().numpy()
+    L = _lib.load()
+    plan = make_plan(L, _lib, q, det.n, det.n, mols, rots)
+    stream = torch.cuda.Stream()
+    sp = C.c_void_p(stream.cuda_stream)
+    try:
+        _lib.check(L.spsim_hip_batch_autotune(plan, sp, None, None), "autotune")
+        st = plan_state(L, plan)
+        terms = C.c_double(0)
+        L.spsim_hip_batch_terms(plan, C.byref(terms))
+        I = torch.empty((npat, det.npix), dtype=torch.float32, device="cuda")
+        time_graph(L, _lib, torch, plan, I, stream, 1)
+        ms_b = time_graph(L, _lib, torch, plan, I, stream, 3)
+        I_host = I.cpu().numpy()
+        ms_p = time_plain(L, _lib, plan, I, stream)
+        out = {"plan": st, "batched_ms": round(ms_b, 3), "plain_ms": round(ms_p, 3),
+               "patterns_per_s_batched": round(npat / (ms_b * 1e-3), 1),
+               "patterns_per_s_plain": round(npat / (ms_p * 1e-3), 1),
+               "speedup_batched_vs_plain": round(ms_p / ms_b, 3),
+               "gterms_per_s_batched": round(terms.value / (ms_b * 1e-3) / 1e9, 1)}
+        wave_atoms = wave_atoms_of(mols, range(npat), st["family"], det.n, det.n, st["per_lane"])
+        out["valu_issue_frac"] = executed_roofline(st["family"], False, wave_atoms, ms_b, terms.value,
+                                                   st["kernel"])["frac"]
+        out["rel_err_vs_fp64"], _ = fp64_check(I_host, q, mols, rots, fp64_patterns, 2048, 11)
+        if extra:
+            extra(L, plan, I, stream, out)
+        return out
+    finally:
+        L.spsim_hip_batch_destroy(plan)
+
+
+def small_slice(args, torch, dv, wl, _lib):
+    det, mols, rots = wl.small_slice()
+    sizes = np.array([len(m[0]) for m in mols])
+    out = batched_vs_plain("small", det, mols, rots, args, torch, dv, _lib, [int(sizes.argmin()), int(sizes.argmax())])
+    out["workload"] = (f"{len(mols)} molecules, N log-uniform in [2e3, 1e4] (mean {sizes.mean():.0f}), radius "
+                       f"5 nm (N/5000)^(1/3), random orientations, {det.n}x{det.n} detector as the headline")
+    return out
+
+
+def skewed(args, torch, dv, wl, _lib):
+    det, mols, rots = wl.skewed_batch()
+
+    def unsplit(L, plan, I, stream, out):
+        _lib.check(L.spsim_hip_batch_set_item_atoms(plan, -1), "item_atoms")
+        time_graph(L, _lib, torch, plan, I, stream, 1)
+        ms = time_graph(L, _lib, torch, plan, I, stream, 3)
+        out["batched_unsplit_ms"] = round(ms, 3)
+        out["speedup_split_vs_unsplit"] = round(ms / out["batched_ms"], 3)
+
+    out = batched_vs_plain("skewed", det, mols, rots, args, torch, dv, _lib, [0, 1], extra=unsplit)
+    out["workload"] = (f"one 200000-atom molecule among 63 of 2000 atoms (radius 5 nm (N/5000)^(1/3)), random "
+                       f"orientations, {det.n}x{det.n} detector as the headline")
+    return out
+
+
+def single_pattern(args, torch, dv, wl, _lib):
+    """Large single pattern: 100,000 atoms with per-atom B on 1024 x 1024; tile
+    sweep, then timed launches; FP64 check on 4,096 random pixels."""
     dev = torch.device("cuda", 0)
     det, Z, pos, B = wl.config2()
     q = dv.hkl_detector(det.n, det.n, det.width, det.width, det.distance, det.wavelength, device=dev)
@@ -110,9 +416,6 @@ def headline(args, torch, dv, wl, _lib):
     F = torch.empty((npix, 2), dtype=torch.float32, device=dev)
     terms = float(npix) * atoms.natoms
 
-    # Kernel and tile choice: a short sweep over the per-pixel kernel
-    # (pixels per lane x LDS chunk) and the mirror-symmetric detector kernel
-    # (quads per lane x LDS chunk), one atom range per tile (one kernel per step).
     def run(kind, t):
         if kind == "grid":
             dv.structure_factor_grid(q, det.n, det.n, atoms, ff, F=F, I=I, tile=t)
@@ -120,13 +423,8 @@ def headline(args, torch, dv, wl, _lib):
             dv.structure_factor(q, atoms, ff, F=F, I=I, tile=t)
 
     candidates = [("pattern", dv.make_tile(k, c, 1)) for k in (2, 4) for c in (128, 256)]
-    # grid kernels: 1 or 2 mirror quads per lane, or one octant (8 pixels) per
-    # lane; an octant tile covers 8 pixels per lane, so it needs 2-4 atom
-    # ranges (a second, reducing launch) to give each SIMD 4 waves
     candidates += [("grid", dv.make_tile(k, c, 1)) for k in (1, 2) for c in (128, 256)]
     candidates += [("grid", dv.make_tile(8, c, s)) for c in (128, 256) for s in (2, 4)]
-    if args.kernel != "auto":
-        candidates = [c for c in candidates if c[0] == args.kernel]
     best, best_ms, sweep = None, 1e30, []
     for kind, t in candidates:
         run(kind, t)
@@ -142,120 +440,163 @@ def headline(args, torch, dv, wl, _lib):
         if ms < best_ms:
             best, best_ms = (kind, t), ms
     kind, tile = best
-
-    for _ in range(args.warmup):
+    for _ in range(3):
         run(kind, tile)
     torch.cuda.synchronize()
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
-    t0 = time.perf_counter()
+    reps = 10
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
     for e0, e1 in ev:
         e0.record()
         run(kind, tile)
         e1.record()
     torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
     kernel_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
-    ms_per_step = wall * 1e3 / args.steps
+    import oracle
 
-    # accuracy of the timed output on a pixel sample against the FP64 oracle
-    err = None
-    try:
-        import oracle
-
-        rng = np.random.default_rng(0)
-        sub = np.sort(rng.choice(npix, 256, replace=False))
-        hkl = q.cpu().numpy()
-        _, I_ref = oracle.sf_f64(hkl[sub], Z, pos, atom_b=B, threads=min(16, os.cpu_count() or 1))
-        err = rel_l2(I.cpu().numpy()[sub], I_ref)
-    except Exception as e:  # the oracle is a checker only
-        print(f"accuracy check skipped: {e}", file=sys.stderr)
-
-    achieved = terms * W_EQ_DW / (kernel_ms * 1e-3) / 1e12
-    traffic = recorded_traffic(kernel_name(kind, tile), tile)
-    res = {
-        "terms": terms,
-        "kernel_ms": kernel_ms,
-        "ms_per_step": ms_per_step,
-        "value": terms / (ms_per_step * 1e-3) / 1e9,
-        "tile": {"kernel": kind, "per_lane": tile.pixels_per_lane, "waves": tile.waves,
-                 "chunk_atoms": tile.chunk_atoms, "atom_splits": tile.atom_splits, "sweep": sweep},
-        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": traffic,
-                     "kernel": kernel_name(kind, tile),
-                     "w_eq_flops_per_term": W_EQ_DW,
-                     "terms_per_launch": terms, "kernel_ms": round(kernel_ms, 4)},
-        "rel_err_vs_fp64": err,
-        "natoms": atoms.natoms, "npix": npix, "Z": Z, "pos": pos, "B": B, "hkl": q,
+    rng = np.random.default_rng(0)
+    sub = np.sort(rng.choice(npix, 4096, replace=False))
+    hkl = q.cpu().numpy()
+    _, I_ref = oracle.sf_f64(hkl[sub], Z, pos, atom_b=B, threads=oracle_threads())
+    err = rel_l2(I.cpu().numpy()[sub], I_ref)
+    family = kernel_family(kind == "grid", tile.pixels_per_lane)
+    kname = {"octant": "sf_octant_kernel"}.get(family, "sf_grid_kernel" if kind == "grid" else "sf_pattern_kernel")
+    wave_atoms = float(atoms.natoms) * tiles_per_pattern(family, det.n, det.n, tile.pixels_per_lane) * WAVES_PER_TILE
+    roof = executed_roofline(family, True, wave_atoms, kernel_ms, terms, kname)
+    roof["traffic"] = recorded_traffic("traffic_large_single_pattern.json", kname, tile.pixels_per_lane,
+                                       tile.chunk_atoms, tile.atom_splits)
+    return {
+        "workload": "100000-atom C/N/O/S (62:17:20:1) molecule uniform in a 12 nm-radius sphere, per-atom B "
+                    "U[10,40] A^2, 1024x1024 detector, 75 um pixels at 0.1 m, lambda 0.15 nm",
+        "kernel_ms": round(kernel_ms, 4), "gterms_per_s": round(terms / (kernel_ms * 1e-3) / 1e9, 1),
+        "tile": {"kernel": kind, "per_lane": tile.pixels_per_lane, "chunk_atoms": tile.chunk_atoms,
+                 "atom_splits": tile.atom_splits, "sweep": sweep},
+        "roofline": roof, "rel_err_vs_fp64": err, "rel_err_sample": "4096 random pixels",
+        "Z": Z, "pos": pos, "B": B, "hkl": q, "npix": npix,
     }
-    return res
 
 
-def batch_kernel_name(L, plan, tile):
-    if not L.spsim_hip_batch_uses_grid(plan):
-        return "sf_batch_kernel"
-    return "sf_batch_octant_kernel" if tile.pixels_per_lane == 8 else "sf_batch_grid_kernel"
-
-
-def batched(args, torch, dv, wl, _lib):
+def pipeline(args, torch, dv, wl, _lib):
+    """Full pipeline, batched: 256 orientations of one 50,000-atom molecule on a
+    1024 x 1024 detector (20 um pixels at 0.05 m, lambda 1.3 nm), then the fused
+    Thomson / solid angle / Poisson / readout chain with 4 x 4 binning."""
     dev = torch.device("cuda", 0)
-    det, mols, rots = wl.config3(args.batched_patterns)
-    q = dv.hkl_detector(det.n, det.n, det.width, det.width, det.distance, det.wavelength, device=dev).cpu().numpy()
-    offs = np.zeros(len(mols) + 1, np.int32)
-    offs[1:] = np.cumsum([len(m[0]) for m in mols])
-    Zc = np.ascontiguousarray(np.concatenate([m[0] for m in mols]), np.int32)
-    Pc = np.ascontiguousarray(np.concatenate([m[1] for m in mols]), np.float32)
-    pm = np.arange(len(mols), dtype=np.int32)
-    R = np.ascontiguousarray(rots.reshape(-1, 9), np.float64)
+    det, Z, pos, rots = wl.config5(args.pipeline_patterns)
+    n, npat = det.n, len(rots)
+    q = dv.hkl_detector(n, n, det.width, det.width, det.distance, det.wavelength, device=dev).cpu().numpy()
     L = _lib.load()
-    plan = C.c_void_p()
-    vp = lambda a: a.ctypes.data_as(C.c_void_p)
-    _lib.check(L.spsim_hip_batch_create(vp(q), det.n, det.n, len(mols), vp(offs), vp(Zc), vp(Pc), None, None, 0,
-                                        len(mols), vp(pm), vp(R), None, C.c_float(0.0), None, C.byref(plan)),
-               "spsim_hip_batch_create")
+    plan = make_plan(L, _lib, q, n, n, [(Z, pos)], rots, pattern_mol=np.zeros(npat, np.int32))
+    detp = dv.detector_params(nx=n, ny=n, bin_x=4, bin_y=4, width=det.width, height=det.width, distance=det.distance,
+                              pixel_width=det.pixel, pixel_height=det.pixel, beam_intensity=1e25,
+                              quantum_efficiency=0.15, wavelength=det.wavelength,
+                              electron_hole_production_energy=5.8e-19, dark_current=0.1, exposure_time=1e-13,
+                              readout_noise=10.0, linear_full_well=2e5, maximum_value=65535.0)
     stream = torch.cuda.Stream()
     sp = C.c_void_p(stream.cuda_stream)
     try:
-        best = _lib.Tile()
-        rate = C.c_double(0)
-        _lib.check(L.spsim_hip_batch_autotune(plan, sp, C.byref(best), C.byref(rate)), "spsim_hip_batch_autotune")
+        _lib.check(L.spsim_hip_batch_autotune(plan, sp, None, None), "autotune")
+        st = plan_state(L, plan)
         terms = C.c_double(0)
         L.spsim_hip_batch_terms(plan, C.byref(terms))
-        I = torch.empty((len(mols), det.npix), dtype=torch.float32, device=dev)
+        I = torch.empty((npat, n * n), dtype=torch.float32, device=dev)
+        nb = (n // 4) * (n // 4)
+        real = torch.empty((npat, nb), dtype=torch.float32, device=dev)
+        noiseless = torch.empty((npat, nb), dtype=torch.float32, device=dev)
+
+        def chain():
+            _lib.check(L.spsim_hip_detector_chain(C.byref(detp), C.c_void_p(I.data_ptr()), npat, 5, 0, None, None,
+                                                  None, None, None, C.c_void_p(real.data_ptr()),
+                                                  C.c_void_p(noiseless.data_ptr()), sp), "detector_chain")
+
         with torch.cuda.stream(stream):
-            _lib.check(L.spsim_hip_batch_run(plan, None, C.c_void_p(I.data_ptr()), 1, sp), "batch_run")  # capture + warm
+            _lib.check(L.spsim_hip_batch_run(plan, None, C.c_void_p(I.data_ptr()), 1, sp), "batch_run")
+            chain()
             stream.synchronize()
-            reps = 2
-            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            e0.record(stream)
-            for _ in range(reps):
-                _lib.check(L.spsim_hip_batch_run(plan, None, C.c_void_p(I.data_ptr()), 1, sp), "batch_run")
-            e1.record(stream)
-            e1.synchronize()
-            ms_batched = e0.elapsed_time(e1) / reps
-            I_b = I.clone()
-            t0 = time.perf_counter()
-            _lib.check(L.spsim_hip_batch_run_plain(plan, None, C.c_void_p(I.data_ptr()), sp), "batch_run_plain")
-            stream.synchronize()
-            ms_plain = (time.perf_counter() - t0) * 1e3
-        agree = rel_l2(I.cpu().numpy(), I_b.cpu().numpy())
-        n = len(mols)
+            ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+            ev[0].record(stream)
+            _lib.check(L.spsim_hip_batch_run(plan, None, C.c_void_p(I.data_ptr()), 1, sp), "batch_run")
+            ev[1].record(stream)
+            chain()
+            ev[2].record(stream)
+            ev[2].synchronize()
+            ms_sf, ms_chain = ev[0].elapsed_time(ev[1]), ev[1].elapsed_time(ev[2])
+            chain_ms = []
+            for _ in range(5):
+                e = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+                e[0].record(stream)
+                chain()
+                e[1].record(stream)
+                e[1].synchronize()
+                chain_ms.append(e[0].elapsed_time(e[1]))
+            ms_chain = float(np.median(chain_ms))
+        I_host = I.cpu().numpy()
+        ms_plain = time_plain(L, _lib, plan, I, stream)
+        err, _ = fp64_check(I_host, q, [(Z, pos)] * npat, rots, [0, npat // 2, npat - 1], 4096, 13)
+        chain_bytes = npat * (n * n * 4 + 2 * nb * 4)
+        wave_atoms = wave_atoms_of([(Z, pos)], [0] * npat, st["family"], n, n, st["per_lane"])
+        roof = executed_roofline(st["family"], False, wave_atoms, ms_sf, terms.value, st["kernel"])
         return {
-            "workload": f"{n} molecules, N log-uniform in [2e3, 2e5] (mean {offs[-1] / n:.0f}), radius 5 nm (N/5000)^(1/3), "
-                        f"uniform random orientations, 512x512 detector 75 um at 0.1 m, lambda 0.2 nm",
-            "terms": terms.value,
-            "batched_ms": round(ms_batched, 2), "plain_ms": round(ms_plain, 2),
-            "patterns_per_s_batched": round(n / (ms_batched * 1e-3), 2),
-            "patterns_per_s_plain": round(n / (ms_plain * 1e-3), 2),
-            "speedup_batched_vs_plain": round(ms_plain / ms_batched, 3),
-            "gterms_per_s_batched": round(terms.value / (ms_batched * 1e-3) / 1e9, 1),
-            "roofline_frac_batched": round(terms.value * W_EQ_PLAIN / (ms_batched * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, 4),
-            "tile": {"kernel": batch_kernel_name(L, plan, best), "per_lane": best.pixels_per_lane,
-                     "chunk_atoms": best.chunk_atoms},
-            "batched_vs_plain_rel_l2": agree,
-            "graph": True,
+            "workload": f"{npat} orientations of one 50000-atom C/N/O/S molecule (radius 10 nm), 1024x1024 detector "
+                        f"20 um at 0.05 m, lambda 1.3 nm, 4x4 binning, QE 0.15, readout 10 e-, dark 0.1, "
+                        f"full well 2e5, max 65535, beam 1e25 photons/m^2 (e-h 5.8e-19 J, exposure 1e-13 s assumed)",
+            "patterns_per_s": round(npat / ((ms_sf + ms_chain) * 1e-3), 2),
+            "structure_factor_ms": round(ms_sf, 2), "detector_chain_ms": round(ms_chain, 3),
+            "gterms_per_s": round(terms.value / (ms_sf * 1e-3) / 1e9, 1),
+            "structure_factor_roofline": {k: roof[k] for k in ("achieved", "peak", "unit", "frac", "kernel")},
+            "detector_chain_gbs": round(chain_bytes / (ms_chain * 1e-3) / 1e9, 1),
+            "detector_chain_hbm_frac": round(chain_bytes / (ms_chain * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "detector_chain_bytes": chain_bytes,
+            "plain_structure_factor_ms": round(ms_plain, 2),
+            "speedup_batched_vs_plain": round(ms_plain / ms_sf, 3),
+            "rel_err_vs_fp64": err, "rel_err_sample": "4096 random pixels of patterns 0, 128, 255",
+            "plan": st,
         }
     finally:
         L.spsim_hip_batch_destroy(plan)
+
+
+def cpu_baseline(args, wl):
+    """The oracle's restatement of the reference's plain path (FP32 dot, FP64
+    libm cos/sin, FP32 sums), on all host cores, on a bounded sample of the
+    headline workload: random pixels of three of its patterns (the 25th, 50th
+    and 75th percentile of atom count) with q rotated as the reference rotates
+    it; the FP64 oracle on the same sample beside it."""
+    import oracle
+
+    threads = os.cpu_count() or 1
+    det, mols, rots = wl.config3(args.batched_patterns)
+    q = oracle.hkl_detector(det.n, det.n, det.width, det.width, det.distance, 0, 0, 0, det.wavelength)
+    sizes = np.array([len(m[0]) for m in mols])
+    order = np.argsort(sizes)
+    pats = [int(order[len(order) // 4]), int(order[len(order) // 2]), int(order[3 * len(order) // 4])]
+    rng = np.random.default_rng(17)
+    mean_terms = float(sizes.mean()) * det.npix
+
+    def run(fn, budget_s):
+        npx, spent, done = 64, 0.0, 0.0
+        while spent < budget_s:
+            for p in pats:
+                sub = rng.choice(q.shape[0], npx, replace=False)
+                qr = oracle.rotate_hkl(q[sub], rots[p])
+                Z, pos = mols[p]
+                t0 = time.perf_counter()
+                fn(qr, Z, pos)
+                spent += time.perf_counter() - t0
+                done += float(npx) * len(Z)
+            npx = min(1 << 16, npx * 2)
+        return done / spent, spent, done
+
+    rate, spent, done = run(lambda qr, Z, pos: oracle.sf_refsem(qr, Z, pos, threads=threads), 12.0)
+    rate64, spent64, _ = run(lambda qr, Z, pos: oracle.sf_f64(qr, Z, pos, threads=threads), 6.0)
+    return {
+        "value": round(rate / mean_terms, 5), "unit": "patterns/s", "cores": threads, "kind": "port",
+        "gterms_per_s": round(rate / 1e9, 4),
+        "sample": f"random pixels of headline patterns {pats} (N = {[int(sizes[p]) for p in pats]}), "
+                  f"{done:.3g} pixel-atom terms in {spent:.1f} s; oracle_sf_refsem (the reference's "
+                  f"compute_pattern_on_list arithmetic, OpenMP over pixels); patterns/s = terms/s over the headline's "
+                  f"mean terms per pattern ({mean_terms:.3g})",
+        "fp64_oracle": {"gterms_per_s": round(rate64 / 1e9, 4), "patterns_per_s": round(rate64 / mean_terms, 5),
+                        "seconds": round(spent64, 1)},
+    }
 
 
 def batched_nfft(args, torch, dv, wl, _lib, direct=None):
@@ -333,82 +674,6 @@ def batched_nfft(args, torch, dv, wl, _lib, direct=None):
     if direct:
         out["speedup_vs_batched_direct"] = round(direct["batched_ms"] / ms, 2)
     return out
-
-
-def pipeline(args, torch, dv, wl, _lib):
-    """Full pipeline, batched: 256 orientations of one 50,000-atom molecule on a
-    1024 x 1024 detector (20 um pixels at 0.05 m, lambda 1.3 nm), then the fused
-    Thomson / solid angle / Poisson / readout chain with 4 x 4 binning."""
-    dev = torch.device("cuda", 0)
-    det, Z, pos, rots = wl.config5(args.pipeline_patterns)
-    n, npat = det.n, len(rots)
-    q = dv.hkl_detector(n, n, det.width, det.width, det.distance, det.wavelength, device=dev).cpu().numpy()
-    offs = np.array([0, len(Z)], np.int32)
-    pm = np.zeros(npat, np.int32)
-    R = np.ascontiguousarray(rots.reshape(-1, 9), np.float64)
-    L = _lib.load()
-    plan = C.c_void_p()
-    vp = lambda a: a.ctypes.data_as(C.c_void_p)
-    _lib.check(L.spsim_hip_batch_create(vp(q), n, n, 1, vp(offs), vp(np.ascontiguousarray(Z, np.int32)),
-                                        vp(np.ascontiguousarray(pos, np.float32)), None, None, 0, npat, vp(pm), vp(R),
-                                        None, C.c_float(0.0), None, C.byref(plan)), "spsim_hip_batch_create")
-    # detector of the full-pipeline configuration; e-h energy and exposure from the reference README
-    detp = dv.detector_params(nx=n, ny=n, bin_x=4, bin_y=4, width=det.width, height=det.width, distance=det.distance,
-                              pixel_width=det.pixel, pixel_height=det.pixel, beam_intensity=1e25,
-                              quantum_efficiency=0.15, wavelength=det.wavelength,
-                              electron_hole_production_energy=5.8e-19, dark_current=0.1, exposure_time=1e-13,
-                              readout_noise=10.0, linear_full_well=2e5, maximum_value=65535.0)
-    stream = torch.cuda.Stream()
-    sp = C.c_void_p(stream.cuda_stream)
-    try:
-        best = _lib.Tile()
-        _lib.check(L.spsim_hip_batch_autotune(plan, sp, C.byref(best), None), "autotune")
-        terms = C.c_double(0)
-        L.spsim_hip_batch_terms(plan, C.byref(terms))
-        I = torch.empty((npat, n * n), dtype=torch.float32, device=dev)
-        nb = (n // 4) * (n // 4)
-        real = torch.empty((npat, nb), dtype=torch.float32, device=dev)
-        noiseless = torch.empty((npat, nb), dtype=torch.float32, device=dev)
-
-        def chain():
-            _lib.check(L.spsim_hip_detector_chain(C.byref(detp), C.c_void_p(I.data_ptr()), npat, 5, 0, None, None,
-                                                  None, None, None, C.c_void_p(real.data_ptr()),
-                                                  C.c_void_p(noiseless.data_ptr()), sp), "detector_chain")
-
-        with torch.cuda.stream(stream):
-            _lib.check(L.spsim_hip_batch_run(plan, None, C.c_void_p(I.data_ptr()), 1, sp), "batch_run")
-            chain()
-            stream.synchronize()
-            ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
-            ev[0].record(stream)
-            _lib.check(L.spsim_hip_batch_run(plan, None, C.c_void_p(I.data_ptr()), 1, sp), "batch_run")
-            ev[1].record(stream)
-            chain()
-            ev[2].record(stream)
-            ev[2].synchronize()
-            ms_sf, ms_chain = ev[0].elapsed_time(ev[1]), ev[1].elapsed_time(ev[2])
-            t0 = time.perf_counter()
-            _lib.check(L.spsim_hip_batch_run_plain(plan, None, C.c_void_p(I.data_ptr()), sp), "plain")
-            stream.synchronize()
-            ms_plain = (time.perf_counter() - t0) * 1e3
-        chain_bytes = npat * (n * n * 4 + 2 * nb * 4)
-        return {
-            "workload": f"{npat} orientations of one 50000-atom C/N/O/S molecule (radius 10 nm), 1024x1024 detector "
-                        f"20 um at 0.05 m, lambda 1.3 nm, 4x4 binning, QE 0.15, readout 10 e-, dark 0.1, "
-                        f"full well 2e5, max 65535, beam 1e25 photons/m^2 (e-h 5.8e-19 J, exposure 1e-13 s assumed)",
-            "patterns_per_s": round(npat / ((ms_sf + ms_chain) * 1e-3), 2),
-            "structure_factor_ms": round(ms_sf, 2), "detector_chain_ms": round(ms_chain, 3),
-            "gterms_per_s": round(terms.value / (ms_sf * 1e-3) / 1e9, 1),
-            "roofline_frac_structure_factor": round(terms.value * W_EQ_PLAIN / (ms_sf * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, 4),
-            "detector_chain_gbs": round(chain_bytes / (ms_chain * 1e-3) / 1e9, 1),
-            "detector_chain_hbm_frac": round(chain_bytes / (ms_chain * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-            "plain_structure_factor_ms": round(ms_plain, 2),
-            "speedup_batched_vs_plain": round(ms_plain / ms_sf, 3),
-            "kernel": batch_kernel_name(L, plan, best),
-            "tile": {"per_lane": best.pixels_per_lane, "chunk_atoms": best.chunk_atoms},
-        }
-    finally:
-        L.spsim_hip_batch_destroy(plan)
 
 
 def pipeline_nfft(args, torch, dv, wl, _lib, direct=None):
@@ -554,12 +819,26 @@ def differentiation(args, torch, dv, wl, _lib):
         gn = gpn
     grad_agree = rel_l2(gn.cpu().numpy(), gp.cpu().numpy())
     terms = float(det.npix) * len(Z)
+    # FP64 check of the direct forward's gradients on 16 random atoms: the
+    # oracle's gradient given the FP64 forward over every pixel
+    import oracle
+
+    q = hkl.cpu().numpy()
+    ph = pert.cpu().numpy()
+    F64, I64 = oracle.sf_f64(q, Z, ph, atom_b=B, threads=oracle_threads())
+    Kh = float(K[0])
+    g = Kh - y.cpu().numpy().astype(np.float64) / I64
+    sub = np.sort(rng.choice(len(Z), 16, replace=False)).astype(np.int32)
+    gp_ref, gb_ref = oracle.sf_grad_given_f_f64(q, F64, Z, ph, g, sub, atom_b=B, threads=oracle_threads())
+    gpa, gba = gp.cpu().numpy(), gb.cpu().numpy()
     return {
         "workload": "20000-atom C/N/O/S molecule (radius 7 nm), per-atom B 20 A^2, 512x512 detector 75 um at 0.1 m, "
                     "lambda 0.2 nm (pixel size, distance, wavelength assumed); Poisson NLL at 1e3 mean photons/pixel",
         "forward_ms": round(f, 3), "backward_ms": round(b, 3), "backward_over_forward": round(b / f, 3),
         "forward_gterms_per_s": round(terms / (f * 1e-3) / 1e9, 1),
         "grad_finite": bool(torch.isfinite(gp).all().item() and torch.isfinite(gb).all().item()),
+        "grad_rel_err_vs_fp64": {"position": rel_l2(gpa[sub], gp_ref), "b_factor": rel_l2(gba[sub], gb_ref),
+                                 "sample": "16 random atoms, oracle gradient given the FP64 forward"},
         "nufft_forward": {"forward_ms": round(float(np.median(fwn)), 3),
                           "backward_ms": round(float(np.median(bwn)), 3),
                           "position_gradient_rel_l2_vs_direct_forward": grad_agree},
@@ -682,8 +961,10 @@ def nfft(args, torch, dv, wl, _lib, h):
         def run_b():
             info_b.update(dv.structure_factor_nfft(q, atoms_b, ff, eps=eps, I=I_b, want_F=False)[2])
 
+        I_db = torch.empty(h["npix"], dtype=torch.float32, device=dev)
+
         def run_db():
-            dv.structure_factor_grid(q, n, n, atoms_b, ff, I=I_d, want_F=False, tile=dv.make_tile(8, 256, 4))
+            dv.structure_factor_grid(q, n, n, atoms_b, ff, I=I_db, want_F=False, tile=dv.make_tile(8, 256, 4))
 
         ms_b, wall_b = timed(run_b)
         ms_db, _ = timed(run_db)
@@ -722,32 +1003,6 @@ def nfft(args, torch, dv, wl, _lib, h):
     return out
 
 
-def cpu_baseline(h):
-    import oracle
-
-    threads = os.cpu_count() or 1
-    Z, pos, B = h["Z"], h["pos"], h["B"]
-    hkl = h["hkl"].cpu().numpy()
-    rng = np.random.default_rng(1)
-    n = 128
-    spent = 0.0
-    done_terms = 0.0
-    while True:
-        sub = rng.choice(hkl.shape[0], n, replace=False)
-        t0 = time.perf_counter()
-        oracle.sf_f64(hkl[sub], Z, pos, atom_b=B, threads=threads)
-        dt = time.perf_counter() - t0
-        spent += dt
-        done_terms += n * len(Z)
-        if spent >= 8.0 or n >= 65536:
-            break
-        n = int(min(65536, max(n * 2, n * 8.0 / max(dt, 1e-3) / 2)))
-    rate = done_terms / spent
-    return {"value": round(rate / 1e9, 4), "unit": "G pixel-atom terms/s", "cores": threads, "kind": "port",
-            "sample": f"{int(done_terms / len(Z))} random pixels of the 1024x1024 headline pattern x {len(Z)} atoms "
-                      f"with per-atom B, FP64 oracle (oracle/oracle.c, OpenMP over pixels), {spent:.1f} s"}
-
-
 def main():
     args = parse()
     if args.gpus != 1:
@@ -773,6 +1028,15 @@ def main():
         log("pipeline_nfft only: the pipeline workload through the NUFFT path")
         print(json.dumps({"pipeline_nfft": pipeline_nfft(args, torch, dv, wl, _lib)}))
         return
+    if args.pipeline_only:
+        log("pipeline only: batched orientations + fused detector chain")
+        print(json.dumps({"pipeline": pipeline(args, torch, dv, wl, _lib)}))
+        return
+    if args.single_only:
+        log("single pattern only")
+        sp = single_pattern(args, torch, dv, wl, _lib)
+        print(json.dumps({"single_pattern": {k: v for k, v in sp.items() if k not in ("Z", "pos", "B", "hkl", "npix")}}))
+        return
     if args.nfft_only:
         det, Z, pos, B = wl.config2()
         dev = torch.device("cuda", 0)
@@ -790,56 +1054,76 @@ def main():
         torch.cuda.synchronize()
         log(f"copy calibration: {COPY_CAL_BYTES} bytes read and written")
         del src, dst
-    log("headline: large single pattern (tile sweep, then timed steps)")
+    log("headline: batched uneven instances (autotune, graph replays)")
     h = headline(args, torch, dv, wl, _lib)
-    log(f"headline done: {h['value']:.1f} G terms/s, kernel {h['kernel_ms']:.3f} ms")
+    npat = h["npatterns"]
+    value = npat / (h["ms_per_step"] * 1e-3)
+    log(f"headline done: {value:.1f} patterns/s, {h['ms_per_step']:.2f} ms per step")
     out = {
         "metric": "G pixel-atom terms/s + patterns/s, batched vs plain, at <=1e-5 rel err vs FP64",
-        "value": round(h["value"], 2),
-        "unit": "G pixel-atom terms/s",
+        "value": round(value, 2),
+        "unit": "patterns/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(h["ms_per_step"], 4),
+        "ms_per_step": round(h["ms_per_step"], 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": "large single pattern: 100000-atom C/N/O/S (62:17:20:1) molecule uniform in a 12 nm-radius "
-                        "sphere, per-atom B U[10,40] A^2, 1024x1024 detector, 75 um pixels at 0.1 m, lambda 0.15 nm; "
-                        "FP64 phase, FP32 sincos and sums",
-            "natoms": h["natoms"], "npix": h["npix"], "terms_per_step": h["terms"], "tile": h["tile"],
-            "parallelism": "1 GPU",
+            "workload": h["workload"], "npatterns": npat, "terms_per_step": h["terms"],
+            "tile": {k: h["plan"][k] for k in ("kernel", "per_lane", "chunk_atoms", "blocks", "items", "item_atoms",
+                                               "split_tiles")},
+            "tile_chosen_by": h["tuned_by"], "graph": True, "parallelism": "1 GPU",
         },
+        "gterms_per_s": round(h["terms"] / (h["ms_per_step"] * 1e-3) / 1e9, 1),
+        "kernel_ms": round(h["kernel_ms"], 3),
         "roofline": h["roofline"],
-        "rel_err_vs_fp64": h["rel_err_vs_fp64"],
     }
-    if not args.profile and not args.no_batched:
-        log("batched: many molecules, autotune, graph vs plain")
-        out["batched"] = batched(args, torch, dv, wl, _lib)
-    if not args.profile and not args.no_batched and not args.no_nfft:
-        log("batched_nfft: the batched workload through the NUFFT path")
-        out["batched_nfft"] = batched_nfft(args, torch, dv, wl, _lib, out.get("batched"))
-    if not args.profile and not args.no_pipeline:
+    if "plain_ms" in h:
+        out["plain"] = {"ms": round(h["plain_ms"], 2), "patterns_per_s": round(npat / (h["plain_ms"] * 1e-3), 2),
+                        "speedup_batched_vs_plain": round(h["plain_ms"] / h["ms_per_step"], 3),
+                        "batched_vs_plain_rel_l2": h["batched_vs_plain_rel_l2"],
+                        "path": "one spsim_hip_sf_grid launch per pattern with a host synchronisation after each"}
+        out["rel_err_vs_fp64"] = h["rel_err"]
+        out["rel_err_sample"] = {"pixels_per_pattern": 4096, "patterns": h["rel_err_patterns"],
+                                 "per_pattern": h["rel_errs"]}
+    if args.profile:
+        print(json.dumps(out))
+        return
+    sp = None
+    if not args.no_single:
+        log("single pattern: 100k atoms with per-atom B on 1024x1024")
+        sp = single_pattern(args, torch, dv, wl, _lib)
+        out["single_pattern"] = {k: v for k, v in sp.items() if k not in ("Z", "pos", "B", "hkl", "npix")}
+    if not args.no_slices:
+        log("small-molecule slice: batched vs plain")
+        out["small_slice"] = small_slice(args, torch, dv, wl, _lib)
+        log("skewed batch: atom-range split vs none vs plain")
+        out["skewed_batch"] = skewed(args, torch, dv, wl, _lib)
+    if not args.no_nfft:
+        log("batched_nfft: the headline workload through the NUFFT path")
+        out["batched_nfft"] = batched_nfft(args, torch, dv, wl, _lib, {"batched_ms": h["ms_per_step"]})
+    if not args.no_pipeline:
         log("pipeline: batched orientations + fused detector chain")
         out["pipeline"] = pipeline(args, torch, dv, wl, _lib)
-    if not args.profile and not args.no_pipeline and not args.no_nfft:
-        log("pipeline_nfft: the pipeline workload through the NUFFT path")
-        out["pipeline_nfft"] = pipeline_nfft(args, torch, dv, wl, _lib, out.get("pipeline"))
-    if not args.profile and not args.no_adjoint:
+        if not args.no_nfft:
+            log("pipeline_nfft: the pipeline workload through the NUFFT path")
+            out["pipeline_nfft"] = pipeline_nfft(args, torch, dv, wl, _lib, out["pipeline"])
+    if not args.no_adjoint:
         log("differentiation: forward and adjoint")
         out["differentiation"] = differentiation(args, torch, dv, wl, _lib)
-    if not args.profile and not args.no_spectrum:
+    if not args.no_spectrum:
         log("spectrum: wavelength-sampled pattern")
         out["spectrum"] = spectrum(args, torch, dv, wl, _lib)
-    if not args.profile and not args.no_nfft:
-        log("nfft: non-uniform FFT vs direct sum")
-        out["nfft"] = nfft(args, torch, dv, wl, _lib, h)
-    if not args.profile and not args.no_cpu:
-        log("cpu baseline: FP64 oracle on a pixel sample")
-        out["cpu_baseline"] = cpu_baseline(h)
+    if not args.no_nfft and sp is not None:
+        log("nfft: non-uniform FFT vs direct sum on the single-pattern geometry")
+        out["nfft"] = nfft(args, torch, dv, wl, _lib, sp)
+    if not args.no_cpu:
+        log("cpu baseline: reference-arithmetic restatement on a pixel sample")
+        out["cpu_baseline"] = cpu_baseline(args, wl)
     log("done")
     print(json.dumps(out))
 

@@ -304,15 +304,16 @@ static float ref_scatt_factor(float d, int Z, float B, const float *ff) {
  */
 int oracle_sf_refsem(const float *hkl, int npix, const int *Z, const float *pos, const float *illum, int natoms,
                      int delta, const float *ff, float B, float *F_out, float *I_out, int nthreads) {
+  /* elements present, once per call (src/diffraction.c:794-800) */
+  int present[NSPEC];
+  memset(present, 0, sizeof present);
+  for (int j = 0; j < natoms; ++j) present[Z[j]] = 1;
 #ifdef _OPENMP
   if (nthreads > 0) omp_set_num_threads(nthreads);
 #pragma omp parallel for schedule(dynamic, 16)
 #endif
   for (int i = 0; i < npix; ++i) {
-    int present[NSPEC];
     float cache[NSPEC];
-    memset(present, 0, sizeof present);
-    for (int j = 0; j < natoms; ++j) present[Z[j]] = 1;
     const float *h = hkl + 3 * (size_t)i;
     const float len = sqrt(h[0] * h[0] + h[1] * h[1] + h[2] * h[2]);
     for (int z = 0; z < 100; ++z)

@@ -429,26 +429,33 @@ int spsim_hip_batch_terms(const spsim_hip_batch *b, double *terms) {
   return SPSIM_HIP_OK;
 }
 
-// Times each supported tile of the plan's kernel on a representative slice of
-// the work (up to 24 patterns spread over the cost range, all of their tiles
-// and atom ranges as the whole plan would split them: tens of milliseconds per
-// candidate at large plans, so that coarse tiles are not judged on a launch
-// dominated by its tail) and keeps the fastest. Leaves the plan on the best
-// tile.
+// Times each supported tile of the plan's kernel on representative slices of
+// the work (patterns spread over the cost range, all of their tiles and atom
+// ranges as the whole plan would split them) and keeps the one with the
+// highest marginal rate. Leaves the plan on the best tile.
 int spsim_hip_batch_autotune(spsim_hip_batch *b, void *stream, spsim_hip_tile *best_out, double *best_rate) {
   using namespace spsim;
   if (!b) return fail(SPSIM_HIP_ERR_INVALID, "NULL plan");
   hipStream_t s = (hipStream_t)stream;
-  // Sample patterns at cost quantiles.
+  // Two samples of patterns at cost quantiles: `big` (up to 48) and `small`
+  // (every third of them). Each candidate runs both in full; the marginal
+  // rate (extra terms over extra time) cancels the run's tail, whose length
+  // depends on the tile's item size and would otherwise decide the choice on
+  // samples much smaller than the plan.
   std::vector<int> order(b->npatterns);
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return natoms_of(*b, x) > natoms_of(*b, y); });
-  std::vector<int> sample;
-  const int want = std::min(b->npatterns, 24);
-  double sample_terms = 0.0;  // every candidate computes these patterns in full
+  std::vector<int> big, small;
+  const int want = std::min(b->npatterns, 48);
+  double big_terms = 0.0, small_terms = 0.0;
   for (int i = 0; i < want; ++i) {
-    sample.push_back(order[(size_t)i * b->npatterns / want]);
-    sample_terms += (double)natoms_of(*b, sample.back()) * b->npix;
+    const int p = order[(size_t)i * b->npatterns / want];
+    big.push_back(p);
+    big_terms += (double)natoms_of(*b, p) * b->npix;
+    if (i % 3 == 1) {
+      small.push_back(p);
+      small_terms += (double)natoms_of(*b, p) * b->npix;
+    }
   }
   // quads per lane {1, 2} (+ octants on a square list) or pixels per lane {1, 2, 4, 8}
   const int ks_grid[3] = {1, 2, kOctantLane}, ks_pix[4] = {1, 2, 4, 8};
@@ -465,6 +472,33 @@ int spsim_hip_batch_autotune(spsim_hip_batch *b, void *stream, spsim_hip_tile *b
   hipEvent_t e0, e1;
   SPSIM_HIP_TRY(hipEventCreate(&e0));
   SPSIM_HIP_TRY(hipEventCreate(&e1));
+  // Runs the given patterns in full (every tile, split as the whole plan
+  // would split them) with tile t; *ms is the second of two runs.
+  auto time_sample = [&](const spsim_hip_tile &t, int blocks, const std::vector<int> &pats, float *ms) -> int {
+    *ms = 0.0f;
+    ItemList L = build_items(*b, t, item_atoms_for(*b, t, blocks), &pats);
+    BatchItem *ditems = nullptr;
+    unsigned *dcount = nullptr;
+    double2 *dpart = nullptr;
+    SPSIM_TRY(dupload(&ditems, L.items.data(), L.items.size()));
+    int r = alloc_split_state(L, t, b->grid, &dcount, &dpart);
+    if (r == SPSIM_HIP_OK) {
+      BatchLaunch a = launch_args(b, ditems, (int)L.items.size(), q, dcount, dpart, nullptr, dI);
+      r = launch_batch(a, t, b->grid, blocks, s);  // warm-up
+      if (r == SPSIM_HIP_OK) {
+        hipEventRecord(e0, s);
+        r = launch_batch(a, t, b->grid, blocks, s);
+        hipEventRecord(e1, s);
+        const hipError_t e = hipEventSynchronize(e1);
+        if (e != hipSuccess) r = hip_fail(e, "hipEventSynchronize");
+        else hipEventElapsedTime(ms, e0, e1);
+      }
+    }
+    if (dcount) (void)hipFree(dcount);
+    if (dpart) (void)hipFree(dpart);
+    (void)hipFree(ditems);
+    return r;
+  };
   spsim_hip_tile best = b->tile;
   double best_r = -1.0;
   int rc = SPSIM_HIP_OK;
@@ -474,37 +508,17 @@ int spsim_hip_batch_autotune(spsim_hip_batch *b, void *stream, spsim_hip_tile *b
       int per_cu = 1;
       if ((rc = batch_blocks_per_cu(t, b->has_b, b->grid, &per_cu)) != SPSIM_HIP_OK) break;
       const int blocks = b->cus * per_cu;
-      // the sampled patterns in full (every tile), split as the whole plan would split them
-      ItemList L = build_items(*b, t, item_atoms_for(*b, t, blocks), &sample);
-      BatchItem *ditems = nullptr;
-      unsigned *dcount = nullptr;
-      double2 *dpart = nullptr;
-      if ((rc = dupload(&ditems, L.items.data(), L.items.size())) != SPSIM_HIP_OK) break;
-      if ((rc = alloc_split_state(L, t, b->grid, &dcount, &dpart)) != SPSIM_HIP_OK) {
-        (void)hipFree(ditems);
-        if (dcount) (void)hipFree(dcount);
-        break;
-      }
-      BatchLaunch a = launch_args(b, ditems, (int)L.items.size(), q, dcount, dpart, nullptr, dI);
-      float ms = 0.0f;
-      rc = launch_batch(a, t, b->grid, blocks, s);  // warm-up
-      if (rc == SPSIM_HIP_OK) {
-        hipEventRecord(e0, s);
-        rc = launch_batch(a, t, b->grid, blocks, s);
-        hipEventRecord(e1, s);
-        hipError_t e = hipEventSynchronize(e1);
-        if (e != hipSuccess) rc = hip_fail(e, "hipEventSynchronize");
-        else hipEventElapsedTime(&ms, e0, e1);
-      }
-      if (dcount) (void)hipFree(dcount);
-      if (dpart) (void)hipFree(dpart);
-      (void)hipFree(ditems);
-      if (rc == SPSIM_HIP_OK && ms > 0.0f) {
-        const double r = sample_terms / (ms * 1e-3);
-        if (r > best_r) {
-          best_r = r;
-          best = t;
-        }
+      float ms_big = 0.0f, ms_small = 0.0f;
+      if ((rc = time_sample(t, blocks, big, &ms_big)) != SPSIM_HIP_OK) break;
+      if (!small.empty() && (rc = time_sample(t, blocks, small, &ms_small)) != SPSIM_HIP_OK) break;
+      double r = -1.0;
+      if (!small.empty() && ms_big > ms_small)
+        r = (big_terms - small_terms) / ((ms_big - ms_small) * 1e-3);
+      else if (ms_big > 0.0f)
+        r = big_terms / (ms_big * 1e-3);
+      if (r > best_r) {
+        best_r = r;
+        best = t;
       }
     }
   }

@@ -117,6 +117,33 @@ def config3(npatterns: int = 512, seed: int = 3):
     return CFG3_DET, mols, rots
 
 
+def molecules(counts, rng: np.random.Generator):
+    """One C/N/O/S molecule per count, radius 5 nm (N/5000)^(1/3) (the batched
+    configuration's size rule), and a uniform random rotation each."""
+    mols = []
+    for n in counts:
+        Z, pos, _ = random_molecule(int(n), 5e-9 * (n / 5000.0) ** (1.0 / 3.0), rng)
+        mols.append((Z, pos))
+    return mols, uniform_rotations(len(counts), rng)
+
+
+def small_slice(npatterns: int = 256, seed: int = 33):
+    """The batched configuration's small end: N log-uniform in [2e3, 1e4] on the
+    same 512 x 512 detector (where one launch per pattern is short)."""
+    rng = np.random.default_rng(seed)
+    counts = np.floor(np.exp(rng.uniform(np.log(2e3), np.log(1e4), size=npatterns))).astype(np.int64)
+    mols, rots = molecules(counts, rng)
+    return CFG3_DET, mols, rots
+
+
+def skewed_batch(seed: int = 34):
+    """The batched configuration's extremes in one run: one 200,000-atom
+    molecule among 63 of 2,000 atoms, 512 x 512 detector."""
+    rng = np.random.default_rng(seed)
+    mols, rots = molecules([200_000] + [2_000] * 63, rng)
+    return CFG3_DET, mols, rots
+
+
 def config4(seed: int = 4):
     rng = np.random.default_rng(seed)
     Z, pos, _ = random_molecule(20_000, 7e-9, rng)
