@@ -503,6 +503,11 @@ def pipeline(args, torch, dv, wl, _lib):
         noiseless = torch.empty((npat, nb), dtype=torch.float32, device=dev)
 
         def chain():
+            _lib.check(L.spsim_hip_detector_chain_binned(C.byref(detp), C.c_void_p(I.data_ptr()), npat, 5, 0,
+                                                         C.c_void_p(real.data_ptr()),
+                                                         C.c_void_p(noiseless.data_ptr()), sp), "detector_chain_binned")
+
+        def chain_per_pixel():
             _lib.check(L.spsim_hip_detector_chain(C.byref(detp), C.c_void_p(I.data_ptr()), npat, 5, 0, None, None,
                                                   None, None, None, C.c_void_p(real.data_ptr()),
                                                   C.c_void_p(noiseless.data_ptr()), sp), "detector_chain")
@@ -519,15 +524,20 @@ def pipeline(args, torch, dv, wl, _lib):
             ev[2].record(stream)
             ev[2].synchronize()
             ms_sf, ms_chain = ev[0].elapsed_time(ev[1]), ev[1].elapsed_time(ev[2])
-            chain_ms = []
-            for _ in range(5):
-                e = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
-                e[0].record(stream)
-                chain()
-                e[1].record(stream)
-                e[1].synchronize()
-                chain_ms.append(e[0].elapsed_time(e[1]))
-            ms_chain = float(np.median(chain_ms))
+            def chain_time(fn):
+                fn()
+                ms = []
+                for _ in range(5):
+                    e = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+                    e[0].record(stream)
+                    fn()
+                    e[1].record(stream)
+                    e[1].synchronize()
+                    ms.append(e[0].elapsed_time(e[1]))
+                return float(np.median(ms))
+
+            ms_chain_px = chain_time(chain_per_pixel)
+            ms_chain = chain_time(chain)
         I_host = I.cpu().numpy()
         ms_plain = time_plain(L, _lib, plan, I, stream)
         err, _ = fp64_check(I_host, q, [(Z, pos)] * npat, rots, [0, npat // 2, npat - 1], 4096, 13)
@@ -545,6 +555,12 @@ def pipeline(args, torch, dv, wl, _lib):
             "detector_chain_gbs": round(chain_bytes / (ms_chain * 1e-3) / 1e9, 1),
             "detector_chain_hbm_frac": round(chain_bytes / (ms_chain * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "detector_chain_bytes": chain_bytes,
+            "detector_chain_kernel": "chain_bin_draws_kernel (binned outputs, one Poisson draw per bin)",
+            "detector_chain_per_pixel": {"kernel": "chain_binned_runs_kernel (one Poisson draw per raw pixel)",
+                                         "ms": round(ms_chain_px, 3),
+                                         "gbs": round(chain_bytes / (ms_chain_px * 1e-3) / 1e9, 1),
+                                         "hbm_frac": round(chain_bytes / (ms_chain_px * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                                                           4)},
             "plain_structure_factor_ms": round(ms_plain, 2),
             "speedup_batched_vs_plain": round(ms_plain / ms_sf, 3),
             "rel_err_vs_fp64": err, "rel_err_sample": "4096 random pixels of patterns 0, 128, 255",

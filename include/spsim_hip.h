@@ -16,6 +16,8 @@
  *   spsim_hip_detector_chain    <- calculate_thomson_correction .. calculate_noiseless_detector_output
  *                                  src/diffraction.c:1109-1204, src/noise.c:32-51,
  *                                  src/amplification.c:54-153 (fused)
+ *   spsim_hip_detector_chain_binned <- the same chain when only the binned outputs
+ *                                  (src/amplification.c:77-153) are kept
  *   spsim_hip_sf_nfft           <- compute_pattern_on_list_by_nfft src/diffraction.c:528-675 (NNFFT path)
  *   spsim_hip_sf_adjoint        <- (no reference counterpart) gradients of a Poisson
  *                                  negative log-likelihood w.r.t. atom positions and B factors
@@ -381,6 +383,19 @@ int spsim_hip_detector_chain(const spsim_hip_detector *det, const float *d_I, in
                              uint64_t pattern_base, float *d_thomson, float *d_solid_angle, float *d_photons,
                              float *d_count, float *d_electrons, float *d_real_output,
                              float *d_noiseless_output, void *stream);
+
+/* The binned outputs of spsim_hip_detector_chain only (real and noiseless,
+   either may be NULL), with one photon count per bin instead of one per raw
+   pixel: the pixels of a bin whose mean photons x QE is below 1000 have
+   independent Poisson counts, and their sum is one Poisson variable with the
+   summed mean, which is drawn once per bin (counters (pattern_base + p, bin));
+   pixels on the reference's Gaussian branch (mean >= 1000, src/noise.c:44-46)
+   keep their own draw. The binned real output therefore has the distribution
+   of the per-pixel chain's, not its values; the noiseless output is bitwise
+   the per-pixel chain's. */
+int spsim_hip_detector_chain_binned(const spsim_hip_detector *det, const float *d_I, int npatterns, uint64_t seed,
+                                    uint64_t pattern_base, float *d_real_output, float *d_noiseless_output,
+                                    void *stream);
 
 /* ---- Adjoint ---- */
 

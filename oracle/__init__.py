@@ -74,6 +74,11 @@ def lib() -> C.CDLL:
         L.oracle_std_normal.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32]
         L.oracle_poisson.restype = C.c_double
         L.oracle_poisson.argtypes = [C.c_double, C.c_uint64, C.c_uint32, C.c_uint32]
+        L.oracle_poisson_exact.restype = C.c_double
+        L.oracle_poisson_exact.argtypes = [C.c_double, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32]
+        L.oracle_real_output_bin_draws.argtypes = [_f32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                                   C.c_float, C.c_float, C.c_float, C.c_float, C.c_float,
+                                                   C.c_float, C.c_float, C.c_uint64, C.c_uint32, _f32p]
         _LIB = L
     return _LIB
 
@@ -318,3 +323,17 @@ def std_normal(seed, idx, pattern, purpose):
 
 def poisson(mean, seed, idx, pattern):
     return lib().oracle_poisson(float(mean), int(seed), int(idx), int(pattern))
+
+
+def poisson_exact(mean, seed, idx, pattern, purpose):
+    return lib().oracle_poisson_exact(float(mean), int(seed), int(idx), int(pattern), int(purpose))
+
+
+def real_output_bin_draws(ph, nx, ny, bx, by, qe, epp, dark, exposure, readout_noise, full_well, max_value, seed,
+                          pattern, nz=1, bz=1):
+    """Binned real output with one Poisson draw per bin (see oracle.c)."""
+    ph = np.ascontiguousarray(ph, np.float32)
+    out = np.zeros((nx // bx) * (ny // by) * (nz // bz), np.float32)
+    lib().oracle_real_output_bin_draws(ph, nx, ny, nz, bx, by, bz, qe, epp, dark, exposure, readout_noise, full_well,
+                                       max_value, int(seed), int(pattern), out)
+    return out

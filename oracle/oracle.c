@@ -42,6 +42,12 @@
  *   oracle_poisson            src/noise.c:32-51 restated with inversion (mean < 10), PTRS
  *                             (Hormann 1993; 10 <= mean < 1000) and the reference's Gaussian
  *                             branch. Pinned by mean/variance tests.
+ *   oracle_poisson_exact,
+ *   oracle_real_output_bin_draws  the binned chain with one count per bin: the sum of
+ *                             independent Poisson counts is Poisson with the summed mean,
+ *                             so a bin's Poisson pixels are drawn together; pixels on the
+ *                             reference's Gaussian branch keep their own draw. Pinned by
+ *                             mean/variance tests of the joint draw over bins of known mean.
  *   oracle_sf_grad_f64,
  *   oracle_sf_grad_given_f_f64 gradient of L = sum_p g_p I_p w.r.t. atom positions and B
  *                             factors in FP64 (no reference counterpart), for every atom or
@@ -578,30 +584,25 @@ double oracle_std_normal(uint64_t seed, uint32_t idx, uint32_t pattern, uint32_t
 }
 
 /* Poisson draw for the counters (idx, pattern), purpose 1. */
-double oracle_poisson(double mean, uint64_t seed, uint32_t idx, uint32_t pattern) {
-  if (!(mean > 0.0)) return 0.0;
-  if (mean >= 1000.0) {
-    const double v = mean + sqrt(mean) * oracle_std_normal(seed, idx, pattern, 1);
-    return v >= 2147483647.0 ? 2147483647.0 : trunc(v);
+static double inversion(double mean, double u) {
+  double p = exp(-mean), F = p;
+  int k = 0;
+  while (u > F && k < 256) {
+    ++k;
+    p *= mean / k;
+    F += p;
   }
-  if (mean < 10.0) {
-    double u, unused;
-    uniforms(seed, idx, pattern, 1, 0, &u, &unused);
-    double p = exp(-mean), F = p;
-    int k = 0;
-    while (u > F && k < 256) {
-      ++k;
-      p *= mean / k;
-      F += p;
-    }
-    return k;
-  }
+  return k;
+}
+
+/* PTRS (Hormann 1993) on the attempts att0 .. att0 + 63. */
+static double ptrs(double mean, uint64_t seed, uint32_t idx, uint32_t pattern, uint32_t purpose, uint32_t att0) {
   const double slam = sqrt(mean), loglam = log(mean);
   const double b = 0.931 + 2.53 * slam, a = -0.059 + 0.02483 * b;
   const double invalpha = 1.1239 + 1.1328 / (b - 3.4), vr = 0.9277 - 3.6224 / (b - 2.0);
   for (uint32_t att = 0; att < 64; ++att) {
     double u, v;
-    uniforms(seed, idx, pattern, 1, att, &u, &v);
+    uniforms(seed, idx, pattern, purpose, att0 + att, &u, &v);
     u -= 0.5;
     const double us = 0.5 - fabs(u);
     const double k = floor((2.0 * a / us + b) * u + mean + 0.43);
@@ -610,4 +611,80 @@ double oracle_poisson(double mean, uint64_t seed, uint32_t idx, uint32_t pattern
     if (log(v) + log(invalpha) - log(a / (us * us) + b) <= -mean + k * loglam - lgamma(k + 1.0)) return k;
   }
   return floor(mean);
+}
+
+/* Poisson(mean) with no Gaussian branch: inversion below 10 on the first
+   uniform of attempt 0, PTRS above, on the counters (idx, pattern, purpose,
+   attempt). */
+double oracle_poisson_exact(double mean, uint64_t seed, uint32_t idx, uint32_t pattern, uint32_t purpose) {
+  if (!(mean > 0.0)) return 0.0;
+  if (mean >= 10.0) return ptrs(mean, seed, idx, pattern, purpose, 0);
+  double u, unused;
+  uniforms(seed, idx, pattern, purpose, 0, &u, &unused);
+  return inversion(mean, u);
+}
+
+/* One pixel's count as src/noise.c:32-51 draws it: (int)(mean + sqrt(mean) N(0,1))
+   truncated toward zero from mean 1000 up, Poisson below, purpose 1. */
+double oracle_poisson(double mean, uint64_t seed, uint32_t idx, uint32_t pattern) {
+  if (!(mean > 0.0)) return 0.0;
+  if (mean >= 1000.0) {
+    const double v = mean + sqrt(mean) * oracle_std_normal(seed, idx, pattern, 1);
+    return v >= 2147483647.0 ? 2147483647.0 : trunc(v);
+  }
+  return oracle_poisson_exact(mean, seed, idx, pattern, 1);
+}
+
+/* Binned real output with one count per bin (spsim_hip_detector_chain_binned):
+   per bin, the pixels with mean photons x QE >= 1000 take their own Gaussian
+   count (purpose 1, flat raw index) as oracle_poisson does, and the others'
+   counts are drawn together as one Poisson variable with their summed mean
+   (FP32 sum in (z, y, x) order) from the block (bin, pattern, purpose 3,
+   attempt 0): inversion on the 53-bit uniform of its words 0-1 below 10,
+   PTRS on attempts 1..64 above. acc sums, in (z, y, x) order, count x
+   electrons-per-photon + dark for the Gaussian pixels and dark for the
+   others, then adds the joint count x electrons-per-photon and the readout
+   noise, readout_noise x Box-Muller of the block's words 2 and 3 as uniforms
+   (w + 1/2) 2^-32; clamp and ADC as src/amplification.c:104-108. */
+void oracle_real_output_bin_draws(const float *photons, int nx, int ny, int nz, int bx, int by, int bz, float qe,
+                                  float e_per_photon, float dark_current, float exposure, float readout_noise,
+                                  float full_well, float max_value, uint64_t seed, uint32_t pattern, float *out) {
+  const float adc = adc_constant(full_well, max_value);
+  const float dark = dark_current * exposure;
+  const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  uint32_t bin = 0;
+  for (int z = 0; z < nz / bz; ++z)
+    for (int y = 0; y < ny / by; ++y)
+      for (int x = 0; x < nx / bx; ++x, ++bin) {
+        float acc = 0, lam = 0;
+        for (int zi = 0; zi < bz; ++zi)
+          for (int yi = 0; yi < by; ++yi)
+            for (int xi = 0; xi < bx; ++xi) {
+              const size_t i = ((size_t)(z * bz + zi) * ny + (y * by + yi)) * nx + (x * bx + xi);
+              const float mean = photons[i] * qe;
+              if (mean >= 1000.0f) {
+                const float cnt = (float)oracle_poisson((double)mean, seed, (uint32_t)i, pattern);
+                acc += (cnt * e_per_photon + dark);
+              } else {
+                lam += mean;
+                acc += dark;
+              }
+            }
+        const uint32_t ctr[4] = {bin, pattern, 3, 0};
+        uint32_t r[4];
+        oracle_philox4x32_10(ctr, key, r);
+        double n = 0.0;
+        if (lam >= 10.0f) {
+          n = ptrs((double)lam, seed, bin, pattern, 3, 1);
+        } else if (lam > 0.0f) {
+          const double u = ((double)((((uint64_t)(r[0] >> 5)) << 26) | (r[1] >> 6)) + 0.5) / 9007199254740992.0;
+          n = inversion((double)lam, u);
+        }
+        acc += (float)n * e_per_photon;
+        const double ua = ((double)r[2] + 0.5) / 4294967296.0, ub = ((double)r[3] + 0.5) / 4294967296.0;
+        acc += (float)((double)readout_noise * (sqrt(-2.0 * log(ua)) * cos(TWO_PI * ub)));
+        if (acc < 0) acc = 0;
+        acc *= adc;
+        out[bin] = acc;
+      }
 }

@@ -81,6 +81,8 @@ SIGNATURES = {
     "spsim_hip_release": (_i, []),
     "spsim_hip_detector_chain": (_i, [C.POINTER(DetectorParams), _vp, _i, C.c_uint64, C.c_uint64,
                                       _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "spsim_hip_detector_chain_binned": (_i, [C.POINTER(DetectorParams), _vp, _i, C.c_uint64, C.c_uint64, _vp, _vp,
+                                             _vp]),
     "spsim_hip_sf_adjoint": (_i, [_vp, _i, _vp, _vp, _vp, _i, _i, _vp, _f, _vp, _vp, _vp, _vp, _vp]),
     "spsim_hip_poisson_nll": (_i, [_vp, _vp, _vp, _i, _vp, _vp, _vp]),
     "spsim_hip_batch_create": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp, _vp, _vp, _f,

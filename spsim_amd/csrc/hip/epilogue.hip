@@ -89,8 +89,9 @@ __device__ __forceinline__ RawPixel raw_pixel(const DetConst &d, int x, int y, u
   return raw_from_geometry(d, thomson_at(d, x, y), solid_angle_at(d, x, y), idx, I, seed, pattern);
 }
 
+// Readout noise of one binned pixel: readout_noise x N(0, 1) (FP32 Box-Muller).
 __device__ __forceinline__ float readout_noise(const DetConst &d, uint64_t seed, uint32_t bin, uint32_t pattern) {
-  return (float)((double)d.readout_noise * std_normal(seed, bin, pattern, kRngReadout, 0u));
+  return d.readout_noise * std_normal_f32(seed, bin, pattern, kRngReadout);
 }
 
 // src/amplification.c:104-108: noise once per binned pixel, clamp at 0, ADC.
@@ -206,6 +207,173 @@ __global__ __launch_bounds__(kT) void chain_binned_runs_kernel(DetConst d, const
     if (o.real) o.real[ob] = finish_real(d, acc, seed, (uint32_t)bin, pat);
     if (o.noiseless) o.noiseless[ob] = accn;
   }
+}
+
+// Binned outputs only, one count per bin. The pixels of a bin whose mean
+// (photons x QE) is below 1000 have independent Poisson counts, and their sum
+// is one Poisson variable with the summed mean (FP32 sum in (z, y, x) order):
+// it is drawn once per bin from the block (seed, bin, pattern,
+// kRngPoissonBin, 0), whose words 0-1 also feed the inversion and words 2-3
+// the readout noise (bin_count, box_muller_f32). Pixels on the reference's
+// Gaussian branch keep their own draw (src/noise.c:44-46). The real output's
+// sum adds count x electrons-per-photon + dark for the Gaussian pixels and
+// dark for the others in (z, y, x) order, then the joint count x
+// electrons-per-photon and the readout noise; clamp and ADC as
+// src/amplification.c:104-108. The noiseless output is the per-pixel sum, bit
+// for bit that of the other chains.
+//
+// Two passes, so that the common pass carries no register-hungry branch:
+// chain_bin_draws_kernel finishes every bin whose summed mean is below 10 and
+// that has no Gaussian pixel, and marks the others' real output with a NaN;
+// chain_bin_fixup_kernel recomputes the marked bins with PTRS and the
+// Gaussian branch. Each lane of the first pass keeps the Thomson factor x
+// solid angle of its bin's raw pixels in LDS for a run of patterns; VEC4
+// reads a bin row of 4 pixels as one 16-byte load (bin_x = 4, nx a multiple
+// of 4, I 16-byte aligned), B44 fixes 4 x 4 x 1 bins at compile time.
+constexpr int kBinDrawMaxRaw = 64;  // raw pixels per bin kept in LDS (64 KiB per workgroup)
+
+template <bool VEC4, bool B44, bool GEO_LDS>
+__global__ __launch_bounds__(kT) void chain_bin_draws_kernel(DetConst d, const float *__restrict__ I, int npatterns,
+                                                             int run, uint64_t seed, uint64_t pattern_base,
+                                                             float *__restrict__ real, float *__restrict__ noiseless) {
+#pragma clang fp contract(off)
+  extern __shared__ float thom[];
+  if constexpr (B44) {
+    d.bin_x = 4;
+    d.bin_y = 4;
+    d.bin_z = 1;
+  }
+  const int nbx = d.nx / d.bin_x, nby = d.ny / d.bin_y, nbz = d.nz / d.bin_z;
+  const int nraw = d.bin_x * d.bin_y * d.bin_z;
+  const long long nb = (long long)nbx * nby * nbz;
+  const long long gid = (long long)blockIdx.x * kT + threadIdx.x;
+  if (gid >= nb) return;  // lanes use only their own LDS slots: no barrier
+  const int bin = (int)gid;
+  const int bx = bin % nbx, by = (bin / nbx) % nby, bz = bin / (nbx * nby);
+  constexpr bool in_lds = GEO_LDS;  // nraw <= kBinDrawMaxRaw
+  const float dark = d.dark_current * d.exposure_time;
+  float dark_sum = 0.0f;  // the real output's sum over a bin without Gaussian pixels
+  {
+    int t = 0;
+    for (int zi = 0; zi < d.bin_z; ++zi)
+      for (int yi = 0; yi < d.bin_y; ++yi)
+        for (int xi = 0; xi < d.bin_x; ++xi, ++t) {
+          dark_sum += dark;
+          if constexpr (in_lds) {
+            const int x = bx * d.bin_x + xi, y = by * d.bin_y + yi;
+            thom[t * kT + threadIdx.x] = thomson_at(d, x, y) * solid_angle_at(d, x, y);
+          }
+        }
+  }
+  const size_t npix = (size_t)d.nx * d.ny * d.nz;
+  const int p0 = (int)blockIdx.y * run, p1 = min(npatterns, p0 + run);
+  for (int p = p0; p < p1; ++p) {
+    const uint32_t pat = (uint32_t)(pattern_base + p);
+    const float *Ip = I + (size_t)p * npix;
+    float accn = 0.0f, lam = 0.0f, peak = 0.0f;
+    int t = 0;
+    for (int zi = 0; zi < d.bin_z; ++zi) {
+      for (int yi = 0; yi < d.bin_y; ++yi) {
+        const int y = by * d.bin_y + yi, z = bz * d.bin_z + zi;
+        const size_t row = ((size_t)z * d.ny + y) * d.nx + (size_t)bx * d.bin_x;
+        float v4[4];
+        if constexpr (VEC4) {
+          const float4 q = *reinterpret_cast<const float4 *>(Ip + row);
+          v4[0] = q.x;
+          v4[1] = q.y;
+          v4[2] = q.z;
+          v4[3] = q.w;
+        }
+        for (int xi = 0; xi < d.bin_x; ++xi, ++t) {
+          const float Ii = VEC4 ? v4[xi & 3] : Ip[row + xi];
+          const int x = bx * d.bin_x + xi;
+          float g;
+          if constexpr (in_lds)
+            g = thom[t * kT + threadIdx.x];
+          else
+            g = thomson_at(d, x, y) * solid_angle_at(d, x, y);
+          const float mean = g * Ii * d.beam_intensity * d.qe;  // (((th om) I) beam) QE
+          accn += mean * d.e_per_photon_noiseless * d.adc;
+          lam += mean;
+          peak = fmaxf(peak, mean);
+        }
+      }
+    }
+    const size_t ob = (size_t)p * nb + bin;
+    if (noiseless) noiseless[ob] = accn;
+    if (!real) continue;
+    if (peak >= 1000.0f || lam >= 10.0f) {
+      real[ob] = __builtin_nanf("");  // for chain_bin_fixup_kernel
+      continue;
+    }
+    const U4 r = draw(seed, (uint32_t)bin, pat, kRngPoissonBin, 0u);
+    float acc = dark_sum;
+    acc += (float)bin_count_small(lam, r) * d.e_per_photon;
+    acc += d.readout_noise * box_muller_f32(uniform32(r.v[2]), uniform32(r.v[3]));
+    if (acc < 0) acc = 0;
+    acc *= d.adc;
+    real[ob] = acc;
+  }
+}
+
+// Second pass: the bins the first pass marked (summed mean >= 10 or a
+// Gaussian pixel), with every branch. Each lane scans 16 consecutive entries
+// of the [npatterns][nb] real output (16 loads in flight) and recomputes the
+// marked ones.
+constexpr int kFixupPerLane = 16;
+__device__ __noinline__ void fixup_bin(DetConst d, const float *__restrict__ I, long long nb, long long gid,
+                                       uint64_t seed, uint64_t pattern_base, float *__restrict__ real);
+
+// nb binned pixels per pattern, total = nb x npatterns entries.
+__global__ __launch_bounds__(kT) void chain_bin_fixup_kernel(DetConst d, const float *__restrict__ I, long long nb,
+                                                             long long total, uint64_t seed, uint64_t pattern_base,
+                                                             float *__restrict__ real) {
+  const long long base = ((long long)blockIdx.x * kT + threadIdx.x) * kFixupPerLane;
+  if (base >= total) return;
+  float v[kFixupPerLane];
+#pragma unroll
+  for (int k = 0; k < kFixupPerLane; ++k) v[k] = real[min(base + k, total - 1)];  // all loads in flight at once
+  unsigned marked = 0;
+#pragma unroll
+  for (int k = 0; k < kFixupPerLane; ++k)
+    if (base + k < total && __builtin_isnan(v[k])) marked |= 1u << k;
+  while (marked) {
+    const int k = __builtin_ctz(marked);
+    marked &= marked - 1;
+    fixup_bin(d, I, nb, base + k, seed, pattern_base, real);
+  }
+}
+
+__device__ __noinline__ void fixup_bin(DetConst d, const float *__restrict__ I, long long nb, long long gid,
+                                       uint64_t seed, uint64_t pattern_base, float *__restrict__ real) {
+#pragma clang fp contract(off)
+  const int nbx = d.nx / d.bin_x, nby = d.ny / d.bin_y;
+  const int p = (int)(gid / nb), bin = (int)(gid % nb);
+  const int bx = bin % nbx, by = (bin / nbx) % nby, bz = bin / (nbx * nby);
+  const uint32_t pat = (uint32_t)(pattern_base + p);
+  const float *Ip = I + (size_t)p * d.nx * d.ny * d.nz;
+  const float dark = d.dark_current * d.exposure_time;
+  float acc = 0.0f, lam = 0.0f;
+  for (int zi = 0; zi < d.bin_z; ++zi)
+    for (int yi = 0; yi < d.bin_y; ++yi)
+      for (int xi = 0; xi < d.bin_x; ++xi) {
+        const int x = bx * d.bin_x + xi, y = by * d.bin_y + yi, z = bz * d.bin_z + zi;
+        const size_t i = ((size_t)z * d.ny + y) * d.nx + x;
+        const float mean = thomson_at(d, x, y) * solid_angle_at(d, x, y) * Ip[i] * d.beam_intensity * d.qe;
+        if (mean >= 1000.0f) {
+          const float cnt = (float)poisson_gaussian((double)mean, seed, (uint32_t)i, pat);
+          acc += (cnt * d.e_per_photon + dark);
+        } else {
+          lam += mean;
+          acc += dark;
+        }
+      }
+  const U4 r = draw(seed, (uint32_t)bin, pat, kRngPoissonBin, 0u);
+  acc += (float)bin_count(lam, r, seed, (uint32_t)bin, pat) * d.e_per_photon;
+  acc += d.readout_noise * box_muller_f32(uniform32(r.v[2]), uniform32(r.v[3]));
+  if (acc < 0) acc = 0;
+  acc *= d.adc;
+  real[gid] = acc;
 }
 
 // Raw outputs for pixels outside complete bins (only when nx, ny or nz is not
@@ -411,7 +579,64 @@ int launch_chain(const DetConst &d, const float *I, int npatterns, uint64_t seed
   return SPSIM_HIP_OK;
 }
 
+int launch_chain_bin_draws(const DetConst &d, const float *I, int npatterns, uint64_t seed, uint64_t pattern_base,
+                           float *real, float *noiseless, hipStream_t s) {
+  const long long nb = (long long)(d.nx / d.bin_x) * (d.ny / d.bin_y) * (d.nz / d.bin_z);
+  if (npatterns <= 0 || nb <= 0 || (!real && !noiseless)) return SPSIM_HIP_OK;
+  const int nraw = d.bin_x * d.bin_y * d.bin_z;
+  // runs of patterns per lane: about 2048 workgroups in all
+  const long long gx = (nb + kT - 1) / kT;
+  if (gx > 0x7fffffffLL) return fail(SPSIM_HIP_ERR_INVALID, "too many bins");
+  long long groups = (2048 + gx - 1) / gx;
+  if (groups > npatterns) groups = npatterns;
+  const int run = (int)((npatterns + groups - 1) / groups);
+  groups = (npatterns + run - 1) / run;
+  const size_t lds = nraw <= kBinDrawMaxRaw ? (size_t)nraw * kT * sizeof(float) : 0;
+  const bool vec4 = d.bin_x == 4 && d.nx % 4 == 0 && ((uintptr_t)I & 15) == 0;
+  const bool b44 = vec4 && d.bin_y == 4 && d.bin_z == 1;
+  const bool geo_lds = nraw <= kBinDrawMaxRaw;
+  const dim3 grid((unsigned)gx, (unsigned)groups);
+#define SPSIM_BIN_DRAWS(V, B, G)                                                                                \
+  hipLaunchKernelGGL((chain_bin_draws_kernel<V, B, G>), grid, dim3(kT), G ? lds : 0, s, d, I, npatterns, run, \
+                     seed, pattern_base, real, noiseless)
+  if (b44)
+    SPSIM_BIN_DRAWS(true, true, true);
+  else if (vec4 && geo_lds)
+    SPSIM_BIN_DRAWS(true, false, true);
+  else if (vec4)
+    SPSIM_BIN_DRAWS(true, false, false);
+  else if (geo_lds)
+    SPSIM_BIN_DRAWS(false, false, true);
+  else
+    SPSIM_BIN_DRAWS(false, false, false);
+#undef SPSIM_BIN_DRAWS
+  SPSIM_HIP_TRY(hipGetLastError());
+  if (real) {
+    const long long lanes = (nb * npatterns + kFixupPerLane - 1) / kFixupPerLane;
+    hipLaunchKernelGGL(chain_bin_fixup_kernel, dim3(blocks_for(lanes)), dim3(kT), 0, s, d, I, nb, nb * npatterns,
+                       seed, pattern_base, real);
+    SPSIM_HIP_TRY(hipGetLastError());
+  }
+  return SPSIM_HIP_OK;
+}
+
 }  // namespace spsim
+
+extern "C" int spsim_hip_detector_chain_binned(const spsim_hip_detector *det, const float *d_I, int npatterns,
+                                               uint64_t seed, uint64_t pattern_base, float *d_real_output,
+                                               float *d_noiseless_output, void *stream) {
+  using namespace spsim;
+  DetConst d;
+  SPSIM_TRY(make_det_const(det, &d));
+  if (npatterns < 0) return fail(SPSIM_HIP_ERR_INVALID, "negative npatterns");
+  if ((long long)npatterns * d.nx * d.ny * d.nz > (1LL << 31))
+    return fail(SPSIM_HIP_ERR_INVALID, "npatterns * npix exceeds 2^31");
+  if (npatterns > 0 && (long long)d.nx * d.ny * d.nz > 0 && !d_I)
+    return fail(SPSIM_HIP_ERR_INVALID, "d_I is NULL");
+  SPSIM_TRY(ensure_device(nullptr));
+  return launch_chain_bin_draws(d, d_I, npatterns, seed, pattern_base, d_real_output, d_noiseless_output,
+                                (hipStream_t)stream);
+}
 
 extern "C" int spsim_hip_detector_chain(const spsim_hip_detector *det, const float *d_I, int npatterns, uint64_t seed,
                                         uint64_t pattern_base, float *d_thomson, float *d_solid_angle, float *d_photons,

@@ -13,7 +13,9 @@
 //    random variables", Insurance: Mathematics and Economics 12 (1993) 39-45.
 //  * mean >= 1000: (int)(mean + sqrt(mean) N(0,1)), truncated toward zero, the
 //    reference's Gaussian branch (src/noise.c:26-28,44-46).
-//  * N(0,1): Box-Muller on two 53-bit uniforms.
+//  * The count of a whole bin (binned outputs only): one Poisson draw with the
+//    summed mean of the bin's pixels below 1000, inversion or PTRS at any mean.
+//  * N(0,1): Box-Muller on two 53-bit uniforms (FP64; FP32 for the readout noise).
 #pragma once
 
 #include <cstdint>
@@ -21,13 +23,17 @@
 
 #ifdef __HIPCC__
 #define SPSIM_HD __host__ __device__ __forceinline__
+// Rare branches (PTRS, the Gaussian branch) stay out of line, so their
+// registers do not limit the occupancy of the kernels' common path.
+#define SPSIM_HD_RARE __host__ __device__ __noinline__
 #else
 #define SPSIM_HD inline
+#define SPSIM_HD_RARE inline
 #endif
 
 namespace spsim {
 
-enum RngPurpose : uint32_t { kRngPoisson = 1u, kRngReadout = 2u };
+enum RngPurpose : uint32_t { kRngPoisson = 1u, kRngReadout = 2u, kRngPoissonBin = 3u };
 
 struct U4 {
   uint32_t v[4];
@@ -80,6 +86,9 @@ SPSIM_HD double std_normal(uint64_t seed, uint32_t idx, uint32_t pattern, uint32
   return sqrt(-2.0 * log(u0)) * cos(6.283185307179586476925286766559 * u1);
 }
 
+// A uniform in (0, 1) from one 32-bit word, exact in FP64: (w + 1/2) 2^-32.
+SPSIM_HD double uniform32(uint32_t w) { return ((double)w + 0.5) * (1.0 / 4294967296.0); }
+
 // log(k!) for integer k >= 0: exact table below 10, Stirling series above.
 SPSIM_HD double log_factorial(double k) {
   if (k < 10.0) {
@@ -101,32 +110,10 @@ SPSIM_HD double log_factorial(double k) {
          in * (1.0 / 12.0 - in2 * (1.0 / 360.0 - in2 * (1.0 / 1260.0 - in2 * (1.0 / 1680.0))));
 }
 
-// Poisson(mean) as the reference's generate_poisson_noise would draw it, with
-// the sampler addressed by (seed, idx, pattern). Bounded loops: inversion stops
-// after 256 steps (probability of reaching them below 1e-100 for mean < 10)
-// and PTRS after 64 rejections (probability < 0.2^64), returning the mode.
-SPSIM_HD double poisson_sample(double mean, uint64_t seed, uint32_t idx, uint32_t pattern) {
-  if (!(mean > 0.0)) return 0.0;
-  if (mean >= 1000.0) {
-    const double g = std_normal(seed, idx, pattern, kRngPoisson, 0u);
-    // (int) conversion: truncation toward zero, saturated at INT_MAX where the
-    // reference's conversion would overflow.
-    const double v = mean + sqrt(mean) * g;
-    return v >= 2147483647.0 ? 2147483647.0 : trunc(v);
-  }
-  if (mean < 10.0) {
-    double u0, u1;
-    uniform2(draw(seed, idx, pattern, kRngPoisson, 0u), u0, u1);
-    (void)u1;
-    double p = exp(-mean), F = p;
-    int k = 0;
-    while (u0 > F && k < 256) {
-      ++k;
-      p *= mean / k;
-      F += p;
-    }
-    return (double)k;
-  }
+// PTRS for mean >= 10 on the attempts att0 .. att0 + 63 (64 rejections at
+// most, probability < 0.2^64, then the mode).
+SPSIM_HD_RARE double poisson_ptrs(double mean, uint64_t seed, uint32_t idx, uint32_t pattern, uint32_t purpose,
+                                  uint32_t att0 = 0u) {
   const double slam = sqrt(mean), loglam = log(mean);
   const double b = 0.931 + 2.53 * slam;
   const double a = -0.059 + 0.02483 * b;
@@ -134,7 +121,7 @@ SPSIM_HD double poisson_sample(double mean, uint64_t seed, uint32_t idx, uint32_
   const double vr = 0.9277 - 3.6224 / (b - 2.0);
   for (uint32_t att = 0; att < 64u; ++att) {
     double u, v;
-    uniform2(draw(seed, idx, pattern, kRngPoisson, att), u, v);
+    uniform2(draw(seed, idx, pattern, purpose, att0 + att), u, v);
     u -= 0.5;
     const double us = 0.5 - fabs(u);
     const double k = floor((2.0 * a / us + b) * u + mean + 0.43);
@@ -144,5 +131,104 @@ SPSIM_HD double poisson_sample(double mean, uint64_t seed, uint32_t idx, uint32_
   }
   return floor(mean);
 }
+
+// Inversion by sequential search for mean < 10: the smallest k whose CDF
+// reaches u (stops after 256 steps, whose probability is below 1e-100 there).
+SPSIM_HD double poisson_inversion(double mean, double u) {
+  double p = exp(-mean), F = p;
+  int k = 0;
+  while (u > F && k < 256) {
+    ++k;
+    p *= mean / k;
+    F += p;
+  }
+  return (double)k;
+}
+
+// Poisson(mean) with no Gaussian branch, addressed by (seed, idx, pattern,
+// purpose): inversion below 10 on the first uniform of attempt 0, PTRS above.
+SPSIM_HD double poisson_exact(double mean, uint64_t seed, uint32_t idx, uint32_t pattern, uint32_t purpose) {
+  if (!(mean > 0.0)) return 0.0;
+  if (mean >= 10.0) return poisson_ptrs(mean, seed, idx, pattern, purpose);
+  double u0, u1;
+  uniform2(draw(seed, idx, pattern, purpose, 0u), u0, u1);
+  (void)u1;
+  return poisson_inversion(mean, u0);
+}
+
+// The reference's Gaussian branch, (int)(mean + sqrt(mean) N(0,1)) truncated
+// toward zero (src/noise.c:26-28,44-46), saturated at INT_MAX where the
+// reference's conversion would overflow.
+SPSIM_HD_RARE double poisson_gaussian(double mean, uint64_t seed, uint32_t idx, uint32_t pattern) {
+  const double g = std_normal(seed, idx, pattern, kRngPoisson, 0u);
+  const double v = mean + sqrt(mean) * g;
+  return v >= 2147483647.0 ? 2147483647.0 : trunc(v);
+}
+
+// Poisson(mean) as the reference's generate_poisson_noise would draw it, with
+// the sampler addressed by (seed, idx, pattern): from mean 1000 up,
+// (int)(mean + sqrt(mean) N(0,1)) truncated toward zero (src/noise.c:26-28,44-46),
+// below that poisson_exact.
+SPSIM_HD double poisson_sample(double mean, uint64_t seed, uint32_t idx, uint32_t pattern) {
+  if (!(mean > 0.0)) return 0.0;
+  if (mean >= 1000.0) return poisson_gaussian(mean, seed, idx, pattern);
+  return poisson_exact(mean, seed, idx, pattern, kRngPoisson);
+}
+
+#ifdef __HIPCC__
+// The same Box-Muller draw in FP32 on the hardware's log2, square root and
+// cosine (argument in turns), from the same two 53-bit uniforms rounded to
+// FP32: for the readout noise, where a relative error of about 1e-6 in a
+// N(0, sigma) sample is far below any use of it.
+__device__ __forceinline__ float std_normal_f32(uint64_t seed, uint32_t idx, uint32_t pattern, uint32_t purpose) {
+  double u0, u1;
+  uniform2(draw(seed, idx, pattern, purpose, 0u), u0, u1);
+  const float l2 = __builtin_amdgcn_logf((float)u0);  // log2(u0)
+  const float r = __builtin_sqrtf(-2.0f * 0.693147180559945309f * l2);
+  return r * __builtin_amdgcn_cosf((float)u1);
+}
+
+// FP32 Box-Muller from two given uniforms.
+__device__ __forceinline__ float box_muller_f32(double u0, double u1) {
+  const float l2 = __builtin_amdgcn_logf((float)u0);
+  return __builtin_sqrtf(-2.0f * 0.693147180559945309f * l2) * __builtin_amdgcn_cosf((float)u1);
+}
+
+// The photon count of a bin with summed mean < 10 from its block (seed, bin,
+// pattern, kRngPoissonBin, 0): inversion on the 53-bit uniform u of words
+// 0-1 (bins with larger means take PTRS on attempts 1..64, bin_count). The
+// inversion runs in FP32 with
+// certainty margins and returns the FP64 inversion's k exactly: the FP32
+// exp2 of the rounded argument is within 2e-6 of e^-mean, each recurrence
+// step p *= mean / k adds at most 2e-7, and the running sum and the rounded
+// uniform 6e-8 each, so for k <= 40 the FP32 CDF is within 1.3e-5 of the
+// FP64 one; k is accepted when u_f clears both neighbouring CDF values by
+// 3e-5 relative, and any closer case (or k > 40) takes the FP64 inversion.
+__device__ __forceinline__ double bin_count_small(float mean, const U4 &r) {
+  if (!(mean > 0.0f)) return 0.0;
+  double u0, u1;
+  uniform2(r, u0, u1);
+  (void)u1;
+  const float uf = (float)u0;
+  float p = __builtin_amdgcn_exp2f(-mean * 1.44269504088896341f);
+  float F = p, lo = 0.0f;  // lo: CDF below the candidate k
+  int k = 0;
+  while (uf >= F * 0.99997f && k < 40) {
+    if (uf <= F * 1.00003f) return poisson_inversion((double)mean, u0);  // too close to F_k
+    ++k;
+    lo = F;
+    p *= mean / (float)k;
+    F += p;
+  }
+  if (uf >= F * 0.99997f || (k > 0 && uf <= lo * 1.00003f)) return poisson_inversion((double)mean, u0);
+  return (double)k;
+}
+
+// Any bin's count: PTRS on attempts 1..64 from mean 10 up.
+__device__ __forceinline__ double bin_count(float mean, const U4 &r, uint64_t seed, uint32_t bin, uint32_t pattern) {
+  if (mean >= 10.0f) return poisson_ptrs((double)mean, seed, bin, pattern, kRngPoissonBin, 1u);
+  return bin_count_small(mean, r);
+}
+#endif
 
 }  // namespace spsim

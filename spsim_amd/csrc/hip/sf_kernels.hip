@@ -948,13 +948,29 @@ spsim_hip_tile default_tile(int natoms, int npix, bool grid) {
   return t;
 }
 
-int auto_splits(int tiles, int natoms, int chunk_atoms, int cus) {
-  const long long want_items = 4LL * cus * 2;  // ~4 rounds of 2 workgroups per CU
-  long long s = (want_items + tiles - 1) / std::max(tiles, 1);
-  const long long max_s = std::max(1, natoms / std::max(1, 2 * chunk_atoms));
-  s = std::min(s, max_s);
-  s = std::max(1LL, std::min(s, 64LL));
-  return (int)s;
+int auto_splits(int tiles, int natoms, int chunk_atoms, int slots, int max_splits) {
+  const int hi = std::max(1, std::min(std::min(64, max_splits), natoms / std::max(1, chunk_atoms)));
+  int best = 1;
+  double best_cost = 1e300;
+  for (int s = 1; s <= hi; ++s) {
+    const long long items = (long long)tiles * s;
+    const long long rounds = (items + std::max(slots, 1) - 1) / std::max(slots, 1);
+    // rounds of resident workgroups x (atoms per range + a per-item cost of
+    // about 64 atoms), plus the partial sums' write and read (~0.02 % per range)
+    const double cost = (double)rounds * ((double)natoms / s + 64.0) * (1.0 + 2e-4 * s);
+    if (cost < best_cost) {
+      best_cost = cost;
+      best = s;
+    }
+  }
+  return best;
+}
+
+int resident_slots(const void *kernel, int cus) {
+  int nb = 0;
+  if (!kernel || hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, kThreads, 0) != hipSuccess || nb < 1)
+    nb = 1;
+  return nb * std::max(cus, 1);
 }
 
 bool host_transpose_symmetric(const float *hkl, int n) {
@@ -1029,7 +1045,11 @@ extern "C" int spsim_hip_sf_pattern(const float *d_hkl, int npix, const spsim_hi
     return fail(SPSIM_HIP_ERR_INVALID, "unsupported tile (pixels_per_lane=%d waves=%d chunk_atoms=%d)",
                 t.pixels_per_lane, t.waves, t.chunk_atoms);
   const int tiles = (npix + tile_units(t) - 1) / tile_units(t);
-  const int splits = t.atom_splits > 0 ? t.atom_splits : auto_splits(tiles, natoms, t.chunk_atoms, cus);
+  const int splits = t.atom_splits > 0
+                         ? t.atom_splits
+                         : auto_splits(tiles, natoms, t.chunk_atoms,
+                                       resident_slots(kernel_for(Kind::kPattern, t, d_atom_b != nullptr), cus),
+                                       (int)std::min<size_t>(64, kWorkspacePartials / (size_t)npix));
   const PatternLaunch a =
       make_launch(d_hkl, npix, d_atoms, d_atom_b, d_segs, nsegs, natoms, d_ff, b_global, rot9, d_F, d_I);
   return launch_pattern(a, t, splits, (hipStream_t)stream);
@@ -1039,10 +1059,11 @@ namespace spsim {
 namespace {
 
 // Checks the arguments of a grid call and picks its tile and atom-split count
-// (explicit, or chosen to fill the GPU). *npix_out = 0 means nothing to do.
-int grid_setup(const float *d_hkl, int nx, int ny, const spsim_hip_atom *d_atoms, const spsim_hip_segment *d_segs,
-               int nsegs, int natoms, const float *d_ff, const spsim_hip_tile *tile, spsim_hip_tile *t_out,
-               int *splits_out, bool *explicit_splits, int *npix_out) {
+// (explicit, or chosen to fill the GPU with at most max_partials partial sums,
+// splits x npix). *npix_out = 0 means nothing to do.
+int grid_setup(const float *d_hkl, int nx, int ny, const spsim_hip_atom *d_atoms, bool has_b,
+               const spsim_hip_segment *d_segs, int nsegs, int natoms, const float *d_ff, const spsim_hip_tile *tile,
+               size_t max_partials, spsim_hip_tile *t_out, int *splits_out, bool *explicit_splits, int *npix_out) {
   if (nx < 0 || ny < 0) return fail(SPSIM_HIP_ERR_INVALID, "negative detector size %d x %d", nx, ny);
   const long long npix = (long long)nx * ny;
   SPSIM_TRY(check_common(d_hkl, npix, d_atoms, d_segs, nsegs, natoms, d_ff));
@@ -1058,7 +1079,10 @@ int grid_setup(const float *d_hkl, int nx, int ny, const spsim_hip_atom *d_atoms
                 t.pixels_per_lane, t.waves, t.chunk_atoms);
   const int tiles = (grid_units(t, nx, ny) + grid_units_per_tile(t) - 1) / grid_units_per_tile(t);
   *explicit_splits = t.atom_splits > 0;
-  *splits_out = t.atom_splits > 0 ? t.atom_splits : auto_splits(tiles, natoms, t.chunk_atoms, cus);
+  *splits_out = t.atom_splits > 0
+                    ? t.atom_splits
+                    : auto_splits(tiles, natoms, t.chunk_atoms, resident_slots(kernel_for(Kind::kGrid, t, has_b), cus),
+                                  (int)std::min<size_t>(64, max_partials / (size_t)npix));
   *t_out = t;
   return SPSIM_HIP_OK;
 }
@@ -1104,8 +1128,8 @@ extern "C" int spsim_hip_sf_grid(const float *d_hkl, int nx, int ny, const spsim
   spsim_hip_tile t{};
   int splits = 1, npix = 0;
   bool explicit_splits = false;
-  SPSIM_TRY(grid_setup(d_hkl, nx, ny, d_atoms, d_segs, nsegs, natoms, d_ff, tile, &t, &splits, &explicit_splits,
-                       &npix));
+  SPSIM_TRY(grid_setup(d_hkl, nx, ny, d_atoms, d_atom_b != nullptr, d_segs, nsegs, natoms, d_ff, tile,
+                       kWorkspacePartials, &t, &splits, &explicit_splits, &npix));
   if (npix == 0) return SPSIM_HIP_OK;
   const PatternLaunch a =
       make_launch(d_hkl, npix, d_atoms, d_atom_b, d_segs, nsegs, natoms, d_ff, b_global, rot9, d_F, d_I);
@@ -1120,8 +1144,8 @@ extern "C" int spsim_hip_sf_grid_noalloc(const float *d_hkl, int nx, int ny, con
   spsim_hip_tile t{};
   int splits = 1, npix = 0;
   bool explicit_splits = false;
-  SPSIM_TRY(grid_setup(d_hkl, nx, ny, d_atoms, d_segs, nsegs, natoms, d_ff, tile, &t, &splits, &explicit_splits,
-                       &npix));
+  SPSIM_TRY(grid_setup(d_hkl, nx, ny, d_atoms, d_atom_b != nullptr, d_segs, nsegs, natoms, d_ff, tile,
+                       kNoAllocPartials, &t, &splits, &explicit_splits, &npix));
   if (npix == 0) return SPSIM_HIP_OK;
   const long long cap = (long long)(kNoAllocPartials / (size_t)npix);  // atom ranges that fit
   if (splits > 1 && (long long)splits > cap) {
@@ -1164,7 +1188,11 @@ extern "C" int spsim_hip_sf_spectrum(const float *d_hkl, int npix, const spsim_h
     sp.w[k] = k < nsamples ? (float)weights[k] : 0.0f;
   }
   const int tiles = (npix + tile_units(t) - 1) / tile_units(t);
-  const int splits = t.atom_splits > 0 ? t.atom_splits : auto_splits(tiles, natoms, t.chunk_atoms, cus);
+  const int splits = t.atom_splits > 0
+                         ? t.atom_splits
+                         : auto_splits(tiles, natoms, t.chunk_atoms,
+                                       resident_slots(kernel_for(Kind::kSpectrum, t, d_atom_b != nullptr), cus),
+                                       (int)std::min<size_t>(64, kWorkspacePartials / (size_t)npix));
   const PatternLaunch a =
       make_launch(d_hkl, npix, d_atoms, d_atom_b, d_segs, nsegs, natoms, d_ff, b_global, rot9, d_F, d_I);
   return launch_spectrum(a, sp, t, splits, (hipStream_t)stream);

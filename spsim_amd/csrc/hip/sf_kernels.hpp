@@ -113,9 +113,17 @@ int launch_spectrum(const PatternLaunch &a, const Spectrum &sp, const spsim_hip_
 int launch_batch(const BatchLaunch &a, const spsim_hip_tile &t, bool grid, int blocks, hipStream_t s);
 // Workgroups of the batch kernel that fit on one CU at once for this tile.
 int batch_blocks_per_cu(const spsim_hip_tile &t, bool has_b, bool grid, int *blocks);
-// Default tile and split count for a single pattern of npix pixels and natoms atoms.
+// Default tile for a single pattern of npix pixels and natoms atoms.
 spsim_hip_tile default_tile(int natoms, int npix, bool grid);
-int auto_splits(int tiles, int natoms, int chunk_atoms, int cus);
+// Atom ranges per pixel tile of one pattern (at most max_splits, each at
+// least one LDS chunk): the count that minimises the launch's length in rounds
+// of `slots` resident workgroups.
+int auto_splits(int tiles, int natoms, int chunk_atoms, int slots, int max_splits);
+// Workgroups of `kernel` resident on the device at once (cus CUs).
+int resident_slots(const void *kernel, int cus);
+// Largest automatic partial-sum buffer of a single pattern (splits x npix double2
+// entries in the library workspace): 256 MiB.
+constexpr size_t kWorkspacePartials = (size_t)1 << 24;
 // True when the [ny][nx][3] host list is exactly mirror-symmetric in x and y.
 bool host_mirror_symmetric(const float *hkl, int nx, int ny);
 // True when the n x n host list is also transpose-symmetric: q(y, x) = (qy, qx, qz)(x, y).

@@ -376,3 +376,17 @@ def detector_chain(det: DetectorParams, I: torch.Tensor, npatterns: int = 1, see
     check(_lib.load().spsim_hip_detector_chain(C.byref(det), _ptr(I), npatterns, seed, pattern_base, *ptrs,
                                                _stream(stream)), "spsim_hip_detector_chain")
     return res
+
+
+def detector_chain_binned(det: DetectorParams, I: torch.Tensor, npatterns: int = 1, seed: int = 0,
+                          pattern_base: int = 0, outputs=("real_output", "noiseless_output"), stream=None) -> dict:
+    """Binned outputs of the detector chain with one photon count drawn per bin
+    (spsim_hip_detector_chain_binned); returns the requested outputs."""
+    nz, bz = max(det.nz, 1), max(det.bin_z, 1)
+    nb = (det.nx // det.bin_x) * (det.ny // det.bin_y) * (nz // bz)
+    res = {n: torch.empty((npatterns, nb), dtype=torch.float32, device=I.device)
+           for n in ("real_output", "noiseless_output") if n in outputs}
+    check(_lib.load().spsim_hip_detector_chain_binned(C.byref(det), _ptr(I), npatterns, seed, pattern_base,
+                                                      _ptr(res.get("real_output")), _ptr(res.get("noiseless_output")),
+                                                      _stream(stream)), "spsim_hip_detector_chain_binned")
+    return res

@@ -194,3 +194,73 @@ def test_three_d_volume_chain(gpu, shape, bins):
     ro = oracle.real_output(o["electrons"][0], nx, ny, bx, by, p["dark_current"], p["exposure_time"], noise,
                             p["linear_full_well"], p["maximum_value"], nz=nz, bz=bz)
     np.testing.assert_allclose(o["real_output"][0], ro, rtol=1e-6, atol=1e-3)
+
+
+def run_binned(gpu, p, I, seed=3, base=0, npatterns=1):
+    import torch
+
+    det = dv.detector_params(**p)
+    It = torch.from_numpy(np.ascontiguousarray(I, np.float32)).to(gpu)
+    out = dv.detector_chain_binned(det, It, npatterns, seed, base)
+    torch.cuda.synchronize()
+    return {k: v.cpu().numpy().reshape(npatterns, -1) for k, v in out.items()}
+
+
+@pytest.mark.parametrize("shape,bins", [((96, 80, 1), (4, 4, 1)),    # 16-byte row loads
+                                        ((30, 21, 1), (4, 4, 1)),    # ragged, nx not a multiple of 4
+                                        ((96, 80, 1), (8, 16, 1)),   # 128 raw pixels per bin: geometry not in LDS
+                                        ((16, 12, 6), (4, 4, 2))])   # 3-D volume
+def test_bin_draws_match_oracle(gpu, shape, bins):
+    """spsim_hip_detector_chain_binned against the oracle's restatement with the
+    same counters, over intensities spanning the three Poisson branches; its
+    noiseless output bitwise equals the per-pixel chain's."""
+    nx, ny, nz = shape
+    bx, by, bz = bins
+    p = params(nx=nx, ny=ny, bx=bx, by=by, nz=nz, bin_z=bz, spherical=int(nz > 1), beam_intensity=1e32)
+    rng = np.random.default_rng(19)
+    npat = 3
+    I = (10.0 ** rng.uniform(4.2, 8.6, (npat, nx * ny * nz))).astype(np.float32)
+    seed, base = 41, 9
+    o = run_binned(gpu, p, I.ravel(), seed=seed, base=base, npatterns=npat)
+    full = run_chain(gpu, p, I.ravel(), seed=seed, base=base, npatterns=npat)
+    epp = oracle.electrons_per_photon(p["wavelength"], p["electron_hole_production_energy"])
+    for q in range(npat):
+        assert np.array_equal(o["noiseless_output"][q], full["noiseless_output"][q])
+        ref = oracle.real_output_bin_draws(full["photons"][q], nx, ny, bx, by, p["quantum_efficiency"], epp,
+                                           p["dark_current"], p["exposure_time"], p["readout_noise"],
+                                           p["linear_full_well"], p["maximum_value"], seed, base + q, nz=nz, bz=bz)
+        close = np.isclose(o["real_output"][q], ref, rtol=1e-6, atol=1e-3)
+        assert np.mean(close) > 0.999 or close.size < 1000 and np.all(close), (q, np.mean(close))
+
+
+def test_bin_draws_patterns_match_one_at_a_time(gpu):
+    n, npat = 128, 11
+    p = params(nx=n, ny=n, beam_intensity=1e30)
+    rng = np.random.default_rng(23)
+    I = (10.0 ** rng.uniform(3.0, 7.5, (npat, n * n))).astype(np.float32)
+    o = run_binned(gpu, p, I.ravel(), seed=5, base=2, npatterns=npat)
+    for q in (0, 5, 10):
+        o1 = run_binned(gpu, p, I[q], seed=5, base=2 + q)
+        for k in o1:
+            assert np.array_equal(o[k][q], o1[k][0]), (k, q)
+
+
+@pytest.mark.parametrize("bin_mean", [0.2, 3.2, 40.0, 9000.0])
+def test_bin_draws_moments_on_gpu(gpu, bin_mean):
+    """Joint counts recovered from the real output (no dark current, no readout
+    noise, unit ADC) have the moments of Poisson(bin mean)."""
+    p = params(nx=256, ny=256, dark_current=0.0, readout_noise=0.0, linear_full_well=0.0)
+    o0 = run_chain(gpu, p, np.ones(256 * 256, np.float32))
+    ph1 = o0["photons"][0].astype(np.float64)
+    I = (bin_mean / 16.0 / (ph1 * p["quantum_efficiency"])).astype(np.float32)
+    full = run_chain(gpu, p, I)
+    lam_px = (full["photons"][0] * np.float32(p["quantum_efficiency"])).astype(np.float64)
+    lam = lam_px.reshape(64, 4, 64, 4).sum(axis=(1, 3)).ravel()
+    o = run_binned(gpu, p, I, seed=8)
+    epp = float(oracle.electrons_per_photon(p["wavelength"], p["electron_hole_production_energy"]))
+    x = np.rint(o["real_output"][0].astype(np.float64) / epp)
+    n = x.size
+    z = (x - lam).sum() / math.sqrt(lam.sum())
+    assert abs(z) < 5, z
+    vr = ((x - lam) ** 2).sum() / lam.sum()
+    assert abs(vr - 1) < 5 * math.sqrt(2.0 / n) * 1.5, vr

@@ -151,3 +151,82 @@ def test_three_d_binning_hand_sums():
     two_d = oracle.noiseless_output(ph[: nx * ny], nx, ny, bx, by, 1.0, 1.0, 0.0, 0.0)
     slab = oracle.noiseless_output(ph, nx, ny, bx, by, 1.0, 1.0, 0.0, 0.0, nz=nz, bz=1)
     assert np.array_equal(slab[: two_d.size], two_d)
+
+
+def _bin_counts(ph, nx, ny, b, qe, epp, seed, pattern):
+    """Joint counts per bin recovered from oracle.real_output_bin_draws with no
+    dark current, no readout noise and unit ADC (acc = count x epp exactly for
+    integer epp)."""
+    out = oracle.real_output_bin_draws(ph, nx, ny, b, b, qe, epp, 0.0, 0.0, 0.0, 0.0, 0.0, seed, pattern)
+    return out.astype(np.float64) / epp
+
+
+@pytest.mark.parametrize("bin_mean", [0.5, 3.2, 40.0, 14400.0])
+def test_bin_draws_poisson_moments(bin_mean):
+    """One draw per bin over 4 x 4 bins of pixels below the Gaussian branch:
+    the bin count is Poisson with the summed mean (inversion below 10, PTRS
+    above, up to 16 x 900 photons)."""
+    nx = ny = 256
+    ph = np.full(nx * ny, bin_mean / 16.0, np.float32)  # QE 1
+    lam = float(np.float32(bin_mean / 16.0)) * 16
+    x = _bin_counts(ph, nx, ny, 4, 1.0, 1.0, 5, 3)
+    n = x.size
+    z = (x - lam).sum() / math.sqrt(lam * n)
+    assert abs(z) < 5, z
+    vr = ((x - x.mean()) ** 2).sum() / (n - 1) / lam
+    assert abs(vr - 1) < 5 * math.sqrt(2.0 / n), vr
+    assert np.all(x == np.rint(x)) and x.min() >= 0
+
+
+def test_bin_draws_small_mean_pmf():
+    ph = np.full(512 * 512, 0.2 / 16, np.float32)
+    x = _bin_counts(ph, 512, 512, 4, 1.0, 1.0, 9, 0)
+    lam = float(np.float32(0.2 / 16)) * 16
+    for k in range(4):
+        p = math.exp(-lam) * lam ** k / math.factorial(k)
+        assert abs(np.mean(x == k) - p) < 5 * math.sqrt(p * (1 - p) / x.size) + 1e-6, k
+
+
+def test_bin_draws_with_gaussian_pixels():
+    """One pixel of each bin at mean 5000 (the reference's truncated Gaussian:
+    mean 4999.5, variance 5000 + 1/12) and fifteen at 0.2: the bin count has
+    the summed moments."""
+    nx = ny = 256
+    ph = np.full((ny, nx), 0.2, np.float32)
+    ph[::4, ::4] = 5000.0
+    x = _bin_counts(ph.ravel(), nx, ny, 4, 1.0, 1.0, 6, 1)
+    m = 4999.5 + 15 * float(np.float32(0.2))
+    v = 5000.0 + 1.0 / 12 + 15 * float(np.float32(0.2))
+    n = x.size
+    assert abs((x - m).sum() / math.sqrt(v * n)) < 5
+    assert abs(((x - x.mean()) ** 2).sum() / (n - 1) / v - 1) < 5 * math.sqrt(2.0 / n)
+
+
+def test_bin_draws_all_gaussian_equals_per_pixel_chain():
+    """When every pixel takes the Gaussian branch, the joint Poisson count is
+    0 and, without readout noise, the binned real output is bitwise the
+    per-pixel chain's with the same draws."""
+    nx, ny, b = 32, 24, 4
+    rng = np.random.default_rng(4)
+    ph = rng.uniform(2000, 9000, nx * ny).astype(np.float32)
+    qe, epp, dark, t, fw, mv, seed, pat = 0.5, 263.5, 0.1, 1e-3, 2e5, 65535.0, 12, 7
+    got = oracle.real_output_bin_draws(ph, nx, ny, b, b, qe, epp, dark, t, 0.0, fw, mv, seed, pat)
+    lam = (ph * np.float32(qe)).astype(np.float64)
+    cnt = np.array([oracle.poisson(l, seed, i, pat) for i, l in enumerate(lam)], np.float32)
+    el = cnt * np.float32(epp)
+    nb = (nx // b) * (ny // b)
+    ref = oracle.real_output(el, nx, ny, b, b, dark, t, np.zeros(nb, np.float32), fw, mv)
+    assert np.array_equal(got, ref)
+
+
+def test_bin_draws_readout_noise_moments():
+    """No photons: the real output is max(N(0, sigma), 0) x ADC per bin, from
+    the Box-Muller pair of each bin's block."""
+    nx = ny = 512
+    s = 10.0
+    out = oracle.real_output_bin_draws(np.zeros(nx * ny, np.float32), nx, ny, 4, 4, 1.0, 263.5, 0.0, 0.0, s, 0.0,
+                                       0.0, 3, 1).astype(np.float64)
+    n = out.size
+    assert abs(np.mean(out == 0) - 0.5) < 5 * math.sqrt(0.25 / n)
+    assert abs(out.mean() - s / math.sqrt(2 * math.pi)) < 5 * s * math.sqrt(0.5 / n)
+    assert abs((out ** 2).mean() - s * s / 2) < 5 * s * s * math.sqrt(2.0 / n)
