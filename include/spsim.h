@@ -34,8 +34,6 @@ void output_array(char *basename, int index, float *array, SpRotation *rot, Opti
 void output_carray(char *basename, int index, Complex *array, SpRotation *rot, Options *opts);
 /* src/spsim.c:230-279 (include/spsim.h:53): the files the output_* flags select. */
 void output_files(Diffraction_Pattern *pattern, Options *opts, int index);
-/* src/io.c:63-93 (include/io.h:23): VTK legacy structured points. */
-void write_3D_array_to_vtk(float *F, int nx, int ny, int nz, char *filename);
 
 /* src/spsim.c:57-81 */
 void gaussian_blur_pattern(Options *opts, Diffraction_Pattern *pattern);

@@ -27,8 +27,8 @@
  * are written with their binned dimensions; the reference wrote them as
  * nx x ny x nz images, reading past the binned arrays whenever binning > 1.
  * The real-space outputs need an FFT of the structure factor and are not
- * written (a message says so). output_sf_vtk writes the intensities as a VTK
- * legacy structured-points file (write_3D_array_to_vtk).
+ * written (a message says so), nor is VTK output (output_sf_vtk): the
+ * reference's VTK writer (src/io.c) is outside this library's scope.
  */
 #include <dlfcn.h>
 #include <stdint.h>
@@ -259,34 +259,13 @@ void output_carray(char *basename, int index, Complex *array, SpRotation *rot, O
   write_named(basename, index, array, 1, rot, opts, 0);
 }
 
-/* src/io.c:63-93: VTK legacy structured points, ASCII. */
-void write_3D_array_to_vtk(float *F, int nx, int ny, int nz, char *filename) {
-  FILE *f = filename ? fopen(filename, "w") : NULL;
-  if (!f || !F) {
-    if (f) fclose(f);
-    fprintf(stderr, "spsim: cannot write %s\n", filename ? filename : "(null)");
-    return;
-  }
-  const long long n = (long long)nx * ny * nz;
-  fprintf(f, "# vtk DataFile Version 2.0\nspsim intensities\nASCII\nDATASET STRUCTURED_POINTS\n");
-  fprintf(f, "DIMENSIONS %d %d %d\nORIGIN 0 0 0\nSPACING 1 1 1\nPOINT_DATA %lld\n", nx, ny, nz, n);
-  fprintf(f, "SCALARS intensity float 1\nLOOKUP_TABLE default\n");
-  for (long long i = 0; i < n; ++i) fprintf(f, i ? " %e" : "%e", (double)F[i]);
-  fprintf(f, "\n");
-  fclose(f);
-}
-
 void output_files(Diffraction_Pattern *pattern, Options *opts, int index) {
   if (!pattern || !opts || !opts->detector) return;
   CCD *d = opts->detector;
   const SpRotation *rot = pattern->rot;
   if (opts->output_scatt_int || opts->output_intensities)
     write_named("scattering_int", index, pattern->ints, 0, rot, opts, 0);
-  if (opts->output_sf_vtk) {
-    char buffer[64];
-    snprintf(buffer, sizeof buffer, "scattering_factor-%05d.vtk", index);
-    write_3D_array_to_vtk(pattern->ints, d->nx, d->ny, d->nz > 0 ? d->nz : 1, buffer);
-  }
+  if (opts->output_sf_vtk) fprintf(stderr, "spsim: VTK output (src/io.c) is not built; nothing written\n");
   if (opts->output_noiseless_photons)
     write_named("noiseless_photon_out", index, d->photons_per_pixel, 0, rot, opts, 0);
   if (opts->output_scattering_factors) write_named("scattering_factors", index, pattern->F, 1, rot, opts, 0);

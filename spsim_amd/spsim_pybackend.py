@@ -268,7 +268,6 @@ _SIGS = {
     "output_array": (None, [C.c_char_p, C.c_int, _P(C.c_float), _P(sp_matrix), _P(Options)]),
     "output_carray": (None, [C.c_char_p, C.c_int, _P(Complex), _P(sp_matrix), _P(Options)]),
     "output_files": (None, [_P(Diffraction_Pattern), _P(Options), C.c_int]),
-    "write_3D_array_to_vtk": (None, [_P(C.c_float), C.c_int, C.c_int, C.c_int, C.c_char_p]),
     # libspimage subset
     "sp_image_alloc": (_P(Image), [C.c_int, C.c_int, C.c_int]),
     "sp_image_free": (None, [_P(Image)]),

