@@ -424,7 +424,7 @@ def single_pattern(args, torch, dv, wl, _lib):
 
     candidates = [("pattern", dv.make_tile(k, c, 1)) for k in (2, 4) for c in (128, 256)]
     candidates += [("grid", dv.make_tile(k, c, 1)) for k in (1, 2) for c in (128, 256)]
-    candidates += [("grid", dv.make_tile(8, c, s)) for c in (128, 256) for s in (2, 4)]
+    candidates += [("grid", dv.make_tile(8, c, s)) for c in (128, 256) for s in (0, 4)]  # 0: automatic ranges
     best, best_ms, sweep = None, 1e30, []
     for kind, t in candidates:
         run(kind, t)
@@ -1019,6 +1019,17 @@ def nfft(args, torch, dv, wl, _lib, h):
     return out
 
 
+def calibration_copy(torch):
+    """A plain device copy of known size, for scaling the HBM byte counters
+    (FETCH_SIZE / WRITE_SIZE) read in the same profiled run."""
+    src = torch.ones(COPY_CAL_BYTES // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    log(f"copy calibration: {COPY_CAL_BYTES} bytes read and written")
+    del src, dst
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -1045,6 +1056,7 @@ def main():
         print(json.dumps({"pipeline_nfft": pipeline_nfft(args, torch, dv, wl, _lib)}))
         return
     if args.pipeline_only:
+        calibration_copy(torch)
         log("pipeline only: batched orientations + fused detector chain")
         print(json.dumps({"pipeline": pipeline(args, torch, dv, wl, _lib)}))
         return
@@ -1062,14 +1074,7 @@ def main():
                                         {"Z": Z, "pos": pos, "B": B, "hkl": q, "npix": det.npix})}))
         return
     if args.profile:
-        # A plain copy of known size, for scaling the HBM byte counters
-        # (FETCH_SIZE / WRITE_SIZE) read in the same profiled run.
-        src = torch.ones(COPY_CAL_BYTES // 4, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        log(f"copy calibration: {COPY_CAL_BYTES} bytes read and written")
-        del src, dst
+        calibration_copy(torch)
     log("headline: batched uneven instances (autotune, graph replays)")
     h = headline(args, torch, dv, wl, _lib)
     npat = h["npatterns"]
