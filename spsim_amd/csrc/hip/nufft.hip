@@ -126,10 +126,11 @@ struct FftPass {
   long long nlines;  // lines = outer * inner
   long long inner;   // stride between consecutive points of a line (1: lines are contiguous)
   int M, nf, L, m_lo, l_lo;
+  int m0, l0;        // slots of point 0 and output 0: m_lo, l_lo mod nf
   int T;             // lines per workgroup (<= kMaxT)
   int nsum;          // input = the sum of nsum grids sum_stride apart (fixed order; 1: one grid)
   long long sum_stride;
-  FastDiv dM, dL, dT;
+  FastDiv dT;
   int nstages;
   FftStage st[kMaxStages];
 };
@@ -709,23 +710,9 @@ __device__ __forceinline__ void dft4(float2 &u0, float2 &u1, float2 &u2, float2 
   u3 = csub(b, d);
 }
 
-// One Stockham stage of radix R on a line: butterfly i (of nb = n/R) reads
-// x[i + t nb], twiddles element t by exp(-2 pi i t k / (span R)) =
-// tw[t k stride] with k = i mod span, applies the R-point DFT and writes
-// y[(i - k) R + k + s span].
+// The R-point DFT of u in place (R in {2, 3, 4, 5, 8}).
 template <int R>
-__device__ __forceinline__ void stockham_bfly(const float2 *__restrict__ x, float2 *__restrict__ y, int i,
-                                              const FftStage &st, const float2 *__restrict__ tw) {
-  const int nb = (int)st.nb.d, span = (int)st.span.d;
-  const int k = i - (int)fdiv((unsigned)i, st.span) * span;
-  float2 u[R];
-#pragma unroll
-  for (int t = 0; t < R; ++t) u[t] = x[i + t * nb];
-  if (k) {
-    const int step = k * st.stride;
-#pragma unroll
-    for (int t = 1; t < R; ++t) u[t] = cmul(u[t], tw[t * step]);
-  }
+__device__ __forceinline__ void dft_r(float2 (&u)[R]) {
   if constexpr (R == 2) {
     const float2 a = u[0];
     u[0] = cadd(a, u[1]);
@@ -776,9 +763,83 @@ __device__ __forceinline__ void stockham_bfly(const float2 *__restrict__ x, floa
     u[2] = cadd(m2, r2);
     u[3] = csub(m2, r2);
   }
-  const int j = (i - k) * R + k;
+}
+
+// Where a line's data sit in global memory for one FFT pass.
+struct LineIo {
+  const float2 *__restrict__ in;  // input point m of the line at in[bin + m inner] (+ nsum - 1 more grids)
+  float2 *__restrict__ out;       // kept output l at out[bout + l inner]
+  long long bin, bout;            // < 0: no such line (padding of the last workgroup)
+};
+
+// One Stockham stage of radix R on a line: butterfly i (of nb = nf/R) reads
+// x[i + t nb], twiddles element t by exp(-2 pi i t k / (span R)) =
+// tw[t k stride] with k = i mod span, applies the R-point DFT and writes
+// y[(i - k) R + k + s span]. GLOAD: the first stage (span 1, no twiddles)
+// reads the line's points straight from global memory instead, slot
+// (m0 + m) mod nf holding point m (zero outside the M points); GSTORE: the
+// last stage writes only the kept outputs, slot (l0 + l) mod nf to output l
+// (every such l < L), to global memory instead of LDS.
+template <int R, bool GLOAD, bool GSTORE>
+__device__ __forceinline__ void stockham_bfly(const FftPass &p, const FftStage &st, int i,
+                                              const float2 *__restrict__ x, float2 *__restrict__ y,
+                                              const float2 *__restrict__ tw, const LineIo &io) {
+  const int nf = p.nf, nb = (int)st.nb.d, span = (int)st.span.d;
+  float2 u[R];
+  int k = 0;
+  if constexpr (GLOAD) {
 #pragma unroll
-  for (int s2 = 0; s2 < R; ++s2) y[j + s2 * span] = u[s2];
+    for (int t = 0; t < R; ++t) {
+      int m = i + t * nb - p.m0;
+      m += m < 0 ? nf : 0;
+      float2 v = make_float2(0.0f, 0.0f);
+      if (m < p.M && io.bin >= 0) {
+        const long long at = io.bin + (long long)m * p.inner;
+        v = io.in[at];
+        for (int g = 1; g < p.nsum; ++g) {
+          const float2 w = io.in[at + g * p.sum_stride];
+          v.x += w.x;
+          v.y += w.y;
+        }
+      }
+      u[t] = v;
+    }
+  } else {
+    k = i - (int)fdiv((unsigned)i, st.span) * span;
+#pragma unroll
+    for (int t = 0; t < R; ++t) u[t] = x[i + t * nb];
+    if (k) {
+      const int step = k * st.stride;
+#pragma unroll
+      for (int t = 1; t < R; ++t) u[t] = cmul(u[t], tw[t * step]);
+    }
+  }
+  dft_r<R>(u);
+  const int j = (i - k) * R + k;
+  if constexpr (GSTORE) {
+    if (io.bout < 0) return;
+#pragma unroll
+    for (int s2 = 0; s2 < R; ++s2) {
+      int l = j + s2 * span - p.l0;
+      l += l < 0 ? nf : 0;
+      for (; l < p.L; l += nf) io.out[io.bout + (long long)l * p.inner] = u[s2];
+    }
+  } else {
+#pragma unroll
+    for (int s2 = 0; s2 < R; ++s2) y[j + s2 * span] = u[s2];
+  }
+}
+
+template <bool GLOAD, bool GSTORE>
+__device__ __forceinline__ void stockham_stage(const FftPass &p, const FftStage &st, int i, const float2 *x,
+                                               float2 *y, const float2 *tw, const LineIo &io) {
+  switch (st.R) {
+    case 8: stockham_bfly<8, GLOAD, GSTORE>(p, st, i, x, y, tw, io); break;
+    case 4: stockham_bfly<4, GLOAD, GSTORE>(p, st, i, x, y, tw, io); break;
+    case 2: stockham_bfly<2, GLOAD, GSTORE>(p, st, i, x, y, tw, io); break;
+    case 3: stockham_bfly<3, GLOAD, GSTORE>(p, st, i, x, y, tw, io); break;
+    default: stockham_bfly<5, GLOAD, GSTORE>(p, st, i, x, y, tw, io); break;
+  }
 }
 
 // tw[j] = exp(-2 pi i j / nf), j < nf.
@@ -793,8 +854,11 @@ __global__ __launch_bounds__(kThreads) void twiddle_table(int nf, float2 *__rest
 // Pruned forward FFT of T lines per workgroup. A line of the input view
 // [outer][M][inner] holds points m = m_lo + i at FFT slot m mod nf (the rest
 // zero); the output view [outer][L][inner] receives FFT values at l = l_lo + i
-// (slot l mod nf). The host keeps m_lo, l_lo in [-nf, nf] and the last
-// point and output below 2 nf, so one conditional add or subtract wraps a slot.
+// (slot l mod nf).
+// The first Stockham stage reads the points from global memory and the last
+// one writes the kept outputs there; the stages between run on two LDS line
+// buffers. Lines of a strided view (inner > 1) are adjacent in memory, so the
+// global stages give consecutive lanes consecutive lines.
 // TWL: the twiddle table is copied into LDS after the two line buffers
 // (lines up to kFftLds points); otherwise it is read from global memory.
 template <bool TWL>
@@ -821,67 +885,48 @@ __global__ __launch_bounds__(kThreads) void fft_lines(FftPass p, const float2 *_
       base_in[t] = base_out[t] = -1;
     }
   }
-  for (int e = threadIdx.x; e < T * nf; e += kThreads) x[e] = make_float2(0.0f, 0.0f);
   __syncthreads();
-  for (int e = threadIdx.x; e < T * p.M; e += kThreads) {
-    int t, i;
-    if (p.inner == 1) {
-      t = (int)fdiv((unsigned)e, p.dM);
-      i = e - t * p.M;
-    } else {
-      i = (int)fdiv((unsigned)e, p.dT);
-      t = e - i * T;
+  const int ns = p.nstages;
+  // Global stages map butterfly b to (line t, butterfly i): line-major for
+  // contiguous lines, line-minor for strided views (coalesced across lines).
+  const bool contiguous = p.inner == 1;
+  {
+    const FftStage &st = p.st[0];
+    const int nb = (int)st.nb.d;
+    for (int b = threadIdx.x; b < T * nb; b += kThreads) {
+      const int q = (int)fdiv((unsigned)b, contiguous ? st.nb : p.dT);
+      const int t = contiguous ? q : b - q * T, i = contiguous ? b - q * nb : q;
+      const LineIo io{in, out, base_in[t], base_out[t]};
+      if (ns == 1)
+        stockham_stage<true, true>(p, st, i, nullptr, nullptr, tw, io);
+      else
+        stockham_stage<true, false>(p, st, i, nullptr, x + t * nf, tw, io);
     }
-    const long long b = base_in[t];
-    if (b < 0) continue;
-    int slot = p.m_lo + i;
-    slot += slot < 0 ? nf : 0;
-    slot -= slot >= nf ? nf : 0;
-    const long long at = b + (long long)i * p.inner;
-    float2 v = in[at];
-    for (int k = 1; k < p.nsum; ++k) {
-      const float2 w = in[at + k * p.sum_stride];
-      v.x += w.x;
-      v.y += w.y;
-    }
-    x[t * nf + slot] = v;
   }
+  if (ns == 1) return;
   __syncthreads();
-  for (int s = 0; s < p.nstages; ++s) {
+  for (int s = 1; s < ns - 1; ++s) {
     const FftStage &st = p.st[s];
     const int nb = (int)st.nb.d;
     for (int b = threadIdx.x; b < T * nb; b += kThreads) {
       const int t = (int)fdiv((unsigned)b, st.nb), i = b - t * nb;
-      const float2 *xs = x + t * nf;
-      float2 *ys = y + t * nf;
-      switch (st.R) {
-        case 8: stockham_bfly<8>(xs, ys, i, st, tw); break;
-        case 4: stockham_bfly<4>(xs, ys, i, st, tw); break;
-        case 2: stockham_bfly<2>(xs, ys, i, st, tw); break;
-        case 3: stockham_bfly<3>(xs, ys, i, st, tw); break;
-        default: stockham_bfly<5>(xs, ys, i, st, tw); break;
-      }
+      const LineIo io{in, out, -1, -1};
+      stockham_stage<false, false>(p, st, i, x + t * nf, y + t * nf, tw, io);
     }
     __syncthreads();
     float2 *tmp = x;
     x = y;
     y = tmp;
   }
-  for (int e = threadIdx.x; e < T * p.L; e += kThreads) {
-    int t, i;
-    if (p.inner == 1) {
-      t = (int)fdiv((unsigned)e, p.dL);
-      i = e - t * p.L;
-    } else {
-      i = (int)fdiv((unsigned)e, p.dT);
-      t = e - i * T;
+  {
+    const FftStage &st = p.st[ns - 1];
+    const int nb = (int)st.nb.d;
+    for (int b = threadIdx.x; b < T * nb; b += kThreads) {
+      const int q = (int)fdiv((unsigned)b, contiguous ? st.nb : p.dT);
+      const int t = contiguous ? q : b - q * T, i = contiguous ? b - q * nb : q;
+      const LineIo io{in, out, base_in[t], base_out[t]};
+      stockham_stage<false, true>(p, st, i, x + t * nf, nullptr, tw, io);
     }
-    const long long b = base_out[t];
-    if (b < 0) continue;
-    int slot = p.l_lo + i;
-    slot += slot < 0 ? nf : 0;
-    slot -= slot >= nf ? nf : 0;
-    out[b + (long long)i * p.inner] = x[t * nf + slot];
   }
 }
 
@@ -1325,13 +1370,13 @@ FftPass make_pass(const NufftDim &D, long long outer, long long inner) {
   p.L = D.L;
   p.m_lo = D.m_lo;
   p.l_lo = D.l_lo;
+  p.m0 = ((D.m_lo % D.nf) + D.nf) % D.nf;
+  p.l0 = ((D.l_lo % D.nf) + D.nf) % D.nf;
   p.T = kFftLds / D.nf;
   p.nsum = 1;
   p.sum_stride = 0;
   if (p.T < 1) p.T = 1;
   if (p.T > kMaxT) p.T = kMaxT;
-  p.dM = fast_div((unsigned)p.M);
-  p.dL = fast_div((unsigned)p.L);
   p.dT = fast_div((unsigned)p.T);
   int r[kMaxStages];
   p.nstages = radices(D.nf, r);
@@ -1354,7 +1399,7 @@ int launch_twiddles(int nf, float2 *tw, hipStream_t stream) {
 }
 
 int launch_fft(const FftPass &p, const float2 *in, float2 *out, const float2 *tw, hipStream_t stream) {
-  // slot wrapping in fft_lines assumes these ranges; every index stays below 2^16
+  // every slot and line index of fft_lines stays below 2^16
   if (p.m_lo < -p.nf || p.m_lo + p.M > 2 * p.nf || p.l_lo < -p.nf || p.l_lo + p.L > 2 * p.nf || p.M > p.nf ||
       p.L >= 65536 || p.nf > kMaxNf)
     return fail(SPSIM_HIP_ERR_INVALID, "FFT line layout out of range (M %d at %d, L %d at %d, nf %d)", p.M, p.m_lo,

@@ -31,7 +31,7 @@ def fft_reference(x, m_lo, nf, L, l_lo):
     (96, 48, -24, 60, -30, 6, 1), (100, 50, -25, 60, -30, 2, 9), (125, 60, -30, 70, -35, 3, 1),
     (480, 240, -120, 250, -125, 4, 1), (480, 240, -120, 250, -125, 2, 33), (1000, 500, -250, 520, -260, 2, 1),
     (2048, 1000, -500, 1030, -515, 2, 3), (3840, 1900, -950, 1930, -965, 1, 2),
-    (64, 10, 60, 12, 110, 3, 2)])
+    (64, 10, 60, 12, 110, 3, 2), (8, 5, -2, 20, -8, 3, 4), (16, 16, 16, 40, -16, 2, 1)])
 def test_fft_pruned_matches_numpy(gpu, nf, M, m_lo, L, l_lo, outer, inner):
     import torch
 
