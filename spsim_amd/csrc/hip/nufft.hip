@@ -51,9 +51,9 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kMinW = 4, kMaxW = 10;
 constexpr int kMaxQuad = 2 * kMaxW + 6;
-// LDS per FFT workgroup: two buffers of T lines (T nf <= kFftLds complex values,
-// 30 KiB) and the twiddle table (up to 15 KiB), or one line of up to kMaxNf
-// points (60 KiB, twiddles read from global memory); plus the line offsets (4 KiB).
+// LDS per FFT workgroup: one buffer of T lines (T nf <= kFftLds complex values,
+// 15 KiB) and the twiddle table (up to 15 KiB), or one line of up to kMaxNf
+// points (30 KiB, twiddles read from global memory); plus two pointers per line.
 constexpr int kMaxNf = 3840;      // longest FFT line
 constexpr int kFftLds = 1920;     // complex values per LDS buffer for several lines
 constexpr int kMaxStages = 16;
@@ -110,10 +110,16 @@ struct NufftGeom {
   float corr[kCorrDeg + 1];
 };
 
-// Exact n / d for n, d < 2^16: q = umulhi(n, m) with m = floor(2^32 / d) + 1
-// (the error n (1 - frac(2^32/d)) / 2^32 stays below 1/d); d == 1 is m == 0.
+// Exact n / d for n, d < 2^16, two ways:
+//  * q = umulhi(n, m) with m = floor(2^32 / d) + 1 (the error
+//    n (1 - frac(2^32/d)) / 2^32 stays below 1/d); d == 1 is m == 0;
+//  * q = trunc(fma(n, fi, fh)) in FP32 with fi = 1/d rounded, fh = fi / 2:
+//    (n + 1/2) / d lies at least 1/(2d) from an integer, and the relative error
+//    of the FP32 result (below 2^-23) is under (2^16 / d + 1) 2^-23 < 1/(2d)
+//    in absolute terms. Full-rate instructions only, for the FFT's indices.
 struct FastDiv {
   unsigned d, m;
+  float fi, fh;
 };
 
 struct FftStage {
@@ -127,6 +133,9 @@ struct FftPass {
   long long inner;   // stride between consecutive points of a line (1: lines are contiguous)
   int M, nf, L, m_lo, l_lo;
   int m0, l0;        // slots of point 0 and output 0: m_lo, l_lo mod nf
+  // the fast modes' offsets in points: inner, first-stage nb, m0, M, last-stage
+  // span, l0, L and nf, each times inner
+  int inner32, nb_in, m0_in, M_in, span_in, l0_in, L_in, nf_in;
   int T;             // lines per workgroup (<= kMaxT)
   int nsum;          // input = the sum of nsum grids sum_stride apart (fixed order; 1: one grid)
   long long sum_stride;
@@ -700,6 +709,13 @@ __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(
 __device__ __forceinline__ float2 mul_mi(float2 a) { return make_float2(a.y, -a.x); }
 
 __device__ __forceinline__ unsigned fdiv(unsigned n, FastDiv f) { return f.m ? __umulhi(n, f.m) : n; }
+// n / d for n < 2^16 by the FP32 form of FastDiv.
+__device__ __forceinline__ int sdiv(int n, const FastDiv &f) { return (int)__builtin_fmaf((float)n, f.fi, f.fh); }
+// n - (n / d) d for n < 2^16.
+__device__ __forceinline__ int smod(int n, const FastDiv &f) { return n - (int)__umul24((unsigned)sdiv(n, f), f.d); }
+// Hides v from the optimiser, so that a running index stays a chain of adds
+// instead of being rewritten as multiplies by the unrolled step count.
+__device__ __forceinline__ void opaque(int &v) { asm("" : "+v"(v)); }
 
 __device__ __forceinline__ void dft4(float2 &u0, float2 &u1, float2 &u2, float2 &u3) {
   const float2 a = cadd(u0, u2), b = csub(u0, u2);
@@ -765,12 +781,12 @@ __device__ __forceinline__ void dft_r(float2 (&u)[R]) {
   }
 }
 
-// Where a line's data sit in global memory for one FFT pass.
-struct LineIo {
-  const float2 *__restrict__ in;  // input point m of the line at in[bin + m inner] (+ nsum - 1 more grids)
-  float2 *__restrict__ out;       // kept output l at out[bout + l inner]
-  long long bin, bout;            // < 0: no such line (padding of the last workgroup)
-};
+// How the global stages of a pass address memory. kFftGeneral: 64-bit point
+// offsets, an input summed over nsum grids, outputs that wrap (L > nf).
+// kFftContig / kFftStrided (one grid, L <= nf, every offset of a line below
+// 2^31): 32-bit offsets formed incrementally from per-pass products, for
+// contiguous lines (inner == 1) and strided ones.
+enum FftMode { kFftGeneral = 0, kFftContig = 1, kFftStrided = 2 };
 
 // One Stockham stage of radix R on a line: butterfly i (of nb = nf/R) reads
 // x[i + t nb], twiddles element t by exp(-2 pi i t k / (span R)) =
@@ -778,67 +794,96 @@ struct LineIo {
 // y[(i - k) R + k + s span]. GLOAD: the first stage (span 1, no twiddles)
 // reads the line's points straight from global memory instead, slot
 // (m0 + m) mod nf holding point m (zero outside the M points); GSTORE: the
-// last stage writes only the kept outputs, slot (l0 + l) mod nf to output l
-// (every such l < L), to global memory instead of LDS.
-template <int R, bool GLOAD, bool GSTORE>
+// last stage (k = i, so output slot i + s span) writes only the kept outputs,
+// slot (l0 + l) mod nf to output l (every such l < L), to global memory
+// instead of LDS. lin / lout: the line's point 0 and output 0; has: false for
+// the padding lines of the last workgroup, which load zeros and store nothing.
+template <int R, bool GLOAD, bool GSTORE, int MODE>
 __device__ __forceinline__ void stockham_bfly(const FftPass &p, const FftStage &st, int i,
                                               const float2 *__restrict__ x, float2 *__restrict__ y,
-                                              const float2 *__restrict__ tw, const LineIo &io) {
+                                              const float2 *__restrict__ tw, const float2 *__restrict__ lin,
+                                              float2 *__restrict__ lout, bool has) {
   const int nf = p.nf, nb = (int)st.nb.d, span = (int)st.span.d;
   float2 u[R];
   int k = 0;
   if constexpr (GLOAD) {
+    if constexpr (MODE == kFftGeneral) {
 #pragma unroll
-    for (int t = 0; t < R; ++t) {
-      int m = i + t * nb - p.m0;
-      m += m < 0 ? nf : 0;
-      float2 v = make_float2(0.0f, 0.0f);
-      if (m < p.M && io.bin >= 0) {
-        const long long at = io.bin + (long long)m * p.inner;
-        v = io.in[at];
-        for (int g = 1; g < p.nsum; ++g) {
-          const float2 w = io.in[at + g * p.sum_stride];
-          v.x += w.x;
-          v.y += w.y;
+      for (int t = 0; t < R; ++t) {
+        int m = i + t * nb - p.m0;
+        m += m < 0 ? nf : 0;
+        float2 v = make_float2(0.0f, 0.0f);
+        if (m < p.M && has) {
+          const long long at = (long long)m * p.inner;
+          v = lin[at];
+          for (int g = 1; g < p.nsum; ++g) {
+            const float2 w = lin[at + g * p.sum_stride];
+            v.x += w.x;
+            v.y += w.y;
+          }
         }
+        u[t] = v;
       }
-      u[t] = v;
+    } else {
+      // slot offset o in (-nf_in, nf_in) wraps by one add
+      const int base = (MODE == kFftContig ? i : (int)__umul24((unsigned)i, (unsigned)p.inner32)) - p.m0_in;
+#pragma unroll
+      for (int t = 0; t < R; ++t) {
+        int o = base + t * p.nb_in;
+        o += o < 0 ? p.nf_in : 0;
+        u[t] = o < p.M_in && has ? lin[(unsigned)o] : make_float2(0.0f, 0.0f);
+      }
     }
   } else {
-    k = i - (int)fdiv((unsigned)i, st.span) * span;
+    // tw[0] = (1, -0) leaves k == 0 exact, so every butterfly twiddles
+    k = smod(i, st.span);
+    int a = i;
 #pragma unroll
-    for (int t = 0; t < R; ++t) u[t] = x[i + t * nb];
-    if (k) {
-      const int step = k * st.stride;
+    for (int t = 0; t < R; ++t, a += nb) u[t] = x[a];
+    const int step = (int)__umul24((unsigned)k, (unsigned)st.stride);
+    int w = step;
 #pragma unroll
-      for (int t = 1; t < R; ++t) u[t] = cmul(u[t], tw[t * step]);
+    for (int t = 1; t < R; ++t, w += step) {
+      opaque(w);
+      u[t] = cmul(u[t], tw[w]);
     }
   }
   dft_r<R>(u);
-  const int j = (i - k) * R + k;
   if constexpr (GSTORE) {
-    if (io.bout < 0) return;
+    if (!has) return;
+    if constexpr (MODE == kFftGeneral) {
 #pragma unroll
-    for (int s2 = 0; s2 < R; ++s2) {
-      int l = j + s2 * span - p.l0;
-      l += l < 0 ? nf : 0;
-      for (; l < p.L; l += nf) io.out[io.bout + (long long)l * p.inner] = u[s2];
+      for (int s2 = 0; s2 < R; ++s2) {
+        int l = i + s2 * span - p.l0;
+        l += l < 0 ? nf : 0;
+        for (; l < p.L; l += nf) lout[(long long)l * p.inner] = u[s2];
+      }
+    } else {
+      const int base = (MODE == kFftContig ? i : (int)__umul24((unsigned)i, (unsigned)p.inner32)) - p.l0_in;
+#pragma unroll
+      for (int s2 = 0; s2 < R; ++s2) {
+        int o = base + s2 * p.span_in;
+        o += o < 0 ? p.nf_in : 0;
+        if (o < p.L_in) lout[(unsigned)o] = u[s2];
+      }
     }
   } else {
+    const int j = (i - k) * R + k;
 #pragma unroll
     for (int s2 = 0; s2 < R; ++s2) y[j + s2 * span] = u[s2];
   }
 }
 
-template <bool GLOAD, bool GSTORE>
+template <bool GLOAD, bool GSTORE, int MODE>
 __device__ __forceinline__ void stockham_stage(const FftPass &p, const FftStage &st, int i, const float2 *x,
-                                               float2 *y, const float2 *tw, const LineIo &io) {
+                                               float2 *y, const float2 *tw, const float2 *lin, float2 *lout,
+                                               bool has) {
   switch (st.R) {
-    case 8: stockham_bfly<8, GLOAD, GSTORE>(p, st, i, x, y, tw, io); break;
-    case 4: stockham_bfly<4, GLOAD, GSTORE>(p, st, i, x, y, tw, io); break;
-    case 2: stockham_bfly<2, GLOAD, GSTORE>(p, st, i, x, y, tw, io); break;
-    case 3: stockham_bfly<3, GLOAD, GSTORE>(p, st, i, x, y, tw, io); break;
-    default: stockham_bfly<5, GLOAD, GSTORE>(p, st, i, x, y, tw, io); break;
+    case 8: stockham_bfly<8, GLOAD, GSTORE, MODE>(p, st, i, x, y, tw, lin, lout, has); break;
+    case 4: stockham_bfly<4, GLOAD, GSTORE, MODE>(p, st, i, x, y, tw, lin, lout, has); break;
+    case 2: stockham_bfly<2, GLOAD, GSTORE, MODE>(p, st, i, x, y, tw, lin, lout, has); break;
+    case 3: stockham_bfly<3, GLOAD, GSTORE, MODE>(p, st, i, x, y, tw, lin, lout, has); break;
+    default: stockham_bfly<5, GLOAD, GSTORE, MODE>(p, st, i, x, y, tw, lin, lout, has); break;
   }
 }
 
@@ -851,81 +896,151 @@ __global__ __launch_bounds__(kThreads) void twiddle_table(int nf, float2 *__rest
   tw[j] = make_float2((float)cs, (float)sn);
 }
 
+// Workgroup b of G in an order that keeps neighbouring workgroups on one XCD:
+// the hardware deals workgroup IDs round-robin over the 8 XCDs, so ID b runs
+// on XCD b mod 8 as that XCD's (b / 8)-th; giving each XCD a contiguous range
+// of workgroups lets the few that share a cache line of a strided view meet
+// in one L2.
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned G) {
+  const unsigned x = b & 7u, s = b >> 3, q = G >> 3, r = G & 7u;
+  return x < r ? x * (q + 1u) + s : r * (q + 1u) + (x - r) * q + s;
+}
+
+// A middle Stockham stage in place on the one LDS buffer of T lines: every
+// thread first loads and twiddles the inputs of its butterflies (at most KB,
+// kept in registers), then, after a barrier, applies the DFTs and writes the
+// outputs over the same buffer. Same indexing as stockham_bfly.
+template <int R, int KB>
+__device__ __forceinline__ void stockham_inplace(const FftPass &p, const FftStage &st, float2 *__restrict__ buf,
+                                                 const float2 *__restrict__ tw) {
+  const int nf = p.nf, nb = (int)st.nb.d, span = (int)st.span.d, nbt = p.T * nb;
+  float2 u[KB][R];
+  int at[KB];
+#pragma unroll
+  for (int kb = 0; kb < KB; ++kb) {
+    const int b = threadIdx.x + kb * kThreads;
+    at[kb] = -1;
+    if (b < nbt) {
+      const int t = sdiv(b, st.nb);
+      const int i = b - (int)__umul24((unsigned)t, (unsigned)nb);
+      const int k = smod(i, st.span);
+      const int line = (int)__umul24((unsigned)t, (unsigned)nf);
+      int a = line + i;
+#pragma unroll
+      for (int r = 0; r < R; ++r, a += nb) u[kb][r] = buf[a];
+      const int step = (int)__umul24((unsigned)k, (unsigned)st.stride);
+      int w = step;
+#pragma unroll
+      for (int r = 1; r < R; ++r, w += step) {
+        opaque(w);
+        u[kb][r] = cmul(u[kb][r], tw[w]);
+      }
+      at[kb] = line + (i - k) * R + k;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int kb = 0; kb < KB; ++kb) {
+    if (at[kb] >= 0) {
+      dft_r<R>(u[kb]);
+      int a = at[kb];
+#pragma unroll
+      for (int s2 = 0; s2 < R; ++s2, a += span) buf[a] = u[kb][s2];
+    }
+  }
+  __syncthreads();
+}
+
+// Butterflies per thread of a radix-R stage on lines of at most P points in
+// all (T nf <= P): ceil(P / (R kThreads)).
+template <int R, int P>
+constexpr int fft_kb() {
+  return (P + R * kThreads - 1) / (R * kThreads);
+}
+
 // Pruned forward FFT of T lines per workgroup. A line of the input view
 // [outer][M][inner] holds points m = m_lo + i at FFT slot m mod nf (the rest
 // zero); the output view [outer][L][inner] receives FFT values at l = l_lo + i
 // (slot l mod nf).
 // The first Stockham stage reads the points from global memory and the last
-// one writes the kept outputs there; the stages between run on two LDS line
-// buffers. Lines of a strided view (inner > 1) are adjacent in memory, so the
-// global stages give consecutive lanes consecutive lines.
-// TWL: the twiddle table is copied into LDS after the two line buffers
-// (lines up to kFftLds points); otherwise it is read from global memory.
-template <bool TWL>
+// one writes the kept outputs there; the stages between run in place on one
+// LDS buffer of T lines. Lines of a strided view (inner > 1) are adjacent in
+// memory, so the global stages give consecutive lanes consecutive lines.
+// TWL: lines of up to kFftLds points in all, the twiddle table copied into LDS
+// after the buffer; otherwise one line of up to kMaxNf points, twiddles read
+// from global memory. The T line offsets (input, output; -1: no line) follow
+// in LDS.
+template <bool TWL, int MODE>
 __global__ __launch_bounds__(kThreads) void fft_lines(FftPass p, const float2 *__restrict__ in,
                                                      float2 *__restrict__ out, const float2 *__restrict__ tw_g) {
+  constexpr int P = TWL ? kFftLds : kMaxNf;
   extern __shared__ float2 lds[];
-  __shared__ long long base_in[kMaxT], base_out[kMaxT];
   const int nf = p.nf, T = p.T;
   float2 *x = lds;
-  float2 *y = lds + T * nf;
-  float2 *tw_l = lds + 2 * T * nf;
+  float2 *tw_l = lds + T * nf;
+  long long *lin_s = (long long *)(lds + T * nf + (TWL ? nf : 0));
+  long long *lout_s = lin_s + T;
   if constexpr (TWL)
     for (int e = threadIdx.x; e < nf; e += kThreads) tw_l[e] = tw_g[e];
   const float2 *tw = TWL ? (const float2 *)tw_l : tw_g;
-  const long long line0 = (long long)blockIdx.x * T;
+  const unsigned blk = MODE == kFftStrided ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+  const long long line0 = (long long)blk * T;
   for (int t = threadIdx.x; t < T; t += kThreads) {
     const long long line = line0 + t;
     if (line < p.nlines) {
       const long long o = line / p.inner;
       const long long c = line - o * p.inner;
-      base_in[t] = o * p.M * p.inner + c;
-      base_out[t] = o * p.L * p.inner + c;
+      lin_s[t] = o * p.M * p.inner + c;
+      lout_s[t] = o * p.L * p.inner + c;
     } else {
-      base_in[t] = base_out[t] = -1;
+      lin_s[t] = lout_s[t] = -1;
     }
   }
   __syncthreads();
   const int ns = p.nstages;
   // Global stages map butterfly b to (line t, butterfly i): line-major for
   // contiguous lines, line-minor for strided views (coalesced across lines).
-  const bool contiguous = p.inner == 1;
+  const bool contiguous = MODE == kFftGeneral ? p.inner == 1 : MODE == kFftContig;
   {
     const FftStage &st = p.st[0];
     const int nb = (int)st.nb.d;
     for (int b = threadIdx.x; b < T * nb; b += kThreads) {
-      const int q = (int)fdiv((unsigned)b, contiguous ? st.nb : p.dT);
-      const int t = contiguous ? q : b - q * T, i = contiguous ? b - q * nb : q;
-      const LineIo io{in, out, base_in[t], base_out[t]};
+      const int q = sdiv(b, contiguous ? st.nb : p.dT);
+      const int r = b - (int)__umul24((unsigned)q, contiguous ? (unsigned)nb : (unsigned)T);
+      const int t = contiguous ? q : r, i = contiguous ? r : q;
+      const long long li = lin_s[t], lo = lout_s[t];
+      const bool has = li >= 0;
       if (ns == 1)
-        stockham_stage<true, true>(p, st, i, nullptr, nullptr, tw, io);
+        stockham_stage<true, true, MODE>(p, st, i, nullptr, nullptr, tw, in + (has ? li : 0), out + (has ? lo : 0),
+                                         has);
       else
-        stockham_stage<true, false>(p, st, i, nullptr, x + t * nf, tw, io);
+        stockham_stage<true, false, MODE>(p, st, i, nullptr, x + __umul24((unsigned)t, (unsigned)nf), tw,
+                                          in + (has ? li : 0), nullptr, has);
     }
   }
   if (ns == 1) return;
   __syncthreads();
   for (int s = 1; s < ns - 1; ++s) {
     const FftStage &st = p.st[s];
-    const int nb = (int)st.nb.d;
-    for (int b = threadIdx.x; b < T * nb; b += kThreads) {
-      const int t = (int)fdiv((unsigned)b, st.nb), i = b - t * nb;
-      const LineIo io{in, out, -1, -1};
-      stockham_stage<false, false>(p, st, i, x + t * nf, y + t * nf, tw, io);
+    switch (st.R) {
+      case 8: stockham_inplace<8, fft_kb<8, P>()>(p, st, x, tw); break;
+      case 4: stockham_inplace<4, fft_kb<4, P>()>(p, st, x, tw); break;
+      case 2: stockham_inplace<2, fft_kb<2, P>()>(p, st, x, tw); break;
+      case 3: stockham_inplace<3, fft_kb<3, P>()>(p, st, x, tw); break;
+      default: stockham_inplace<5, fft_kb<5, P>()>(p, st, x, tw); break;
     }
-    __syncthreads();
-    float2 *tmp = x;
-    x = y;
-    y = tmp;
   }
   {
     const FftStage &st = p.st[ns - 1];
     const int nb = (int)st.nb.d;
     for (int b = threadIdx.x; b < T * nb; b += kThreads) {
-      const int q = (int)fdiv((unsigned)b, contiguous ? st.nb : p.dT);
-      const int t = contiguous ? q : b - q * T, i = contiguous ? b - q * nb : q;
-      const LineIo io{in, out, base_in[t], base_out[t]};
-      stockham_stage<false, true>(p, st, i, x + t * nf, nullptr, tw, io);
+      const int q = sdiv(b, contiguous ? st.nb : p.dT);
+      const int r = b - (int)__umul24((unsigned)q, contiguous ? (unsigned)nb : (unsigned)T);
+      const int t = contiguous ? q : r, i = contiguous ? r : q;
+      const long long lo = lout_s[t];
+      const bool has = lo >= 0;
+      stockham_stage<false, true, MODE>(p, st, i, x + __umul24((unsigned)t, (unsigned)nf), nullptr, tw, nullptr,
+                                        out + (has ? lo : 0), has);
     }
   }
 }
@@ -1236,6 +1351,8 @@ FastDiv fast_div(unsigned d) {
   FastDiv f;
   f.d = d;
   f.m = d <= 1 ? 0u : (unsigned)((1ull << 32) / d + 1);
+  f.fi = d ? (float)(1.0 / (double)d) : 0.0f;
+  f.fh = 0.5f * f.fi;
   return f;
 }
 
@@ -1398,7 +1515,8 @@ int launch_twiddles(int nf, float2 *tw, hipStream_t stream) {
   return SPSIM_HIP_OK;
 }
 
-int launch_fft(const FftPass &p, const float2 *in, float2 *out, const float2 *tw, hipStream_t stream) {
+int launch_fft(const FftPass &pass, const float2 *in, float2 *out, const float2 *tw, hipStream_t stream) {
+  FftPass p = pass;
   // every slot and line index of fft_lines stays below 2^16
   if (p.m_lo < -p.nf || p.m_lo + p.M > 2 * p.nf || p.l_lo < -p.nf || p.l_lo + p.L > 2 * p.nf || p.M > p.nf ||
       p.L >= 65536 || p.nf > kMaxNf)
@@ -1407,13 +1525,30 @@ int launch_fft(const FftPass &p, const float2 *in, float2 *out, const float2 *tw
   const long long blocks = (p.nlines + p.T - 1) / p.T;
   if (blocks <= 0) return SPSIM_HIP_OK;
   if (blocks > 0x7fffffffLL) return fail(SPSIM_HIP_ERR_INVALID, "too many FFT lines");
-  if (p.nf <= kFftLds) {
-    const size_t lds = (2 * (size_t)p.T + 1) * p.nf * sizeof(float2);
-    hipLaunchKernelGGL(fft_lines<true>, dim3((unsigned)blocks), dim3(kThreads), lds, stream, p, in, out, tw);
-  } else {
-    const size_t lds = 2 * (size_t)p.T * p.nf * sizeof(float2);
-    hipLaunchKernelGGL(fft_lines<false>, dim3((unsigned)blocks), dim3(kThreads), lds, stream, p, in, out, tw);
+  // the fast modes need one grid, no wrapped outputs, offsets within a line
+  // (up to 2 nf inner, before the wrap) below 2^31 and a stride below 2^24
+  // (24-bit multiplies)
+  const bool fast = p.nsum == 1 && p.L <= p.nf && (long long)p.nf * p.inner < 0x40000000LL && p.inner < (1 << 24);
+  int mode = kFftGeneral;
+  if (fast) {
+    const int in32 = (int)p.inner;
+    p.inner32 = in32;
+    p.nb_in = (int)p.st[0].nb.d * in32;
+    p.m0_in = p.m0 * in32;
+    p.M_in = p.M * in32;
+    p.span_in = (int)p.st[p.nstages - 1].span.d * in32;
+    p.l0_in = p.l0 * in32;
+    p.L_in = p.L * in32;
+    p.nf_in = p.nf * in32;
+    mode = p.inner == 1 ? kFftContig : kFftStrided;
   }
+  const bool twl = p.nf <= kFftLds;
+  const size_t lds = ((size_t)p.T + (twl ? 1 : 0)) * p.nf * sizeof(float2) + 2 * (size_t)p.T * sizeof(long long);
+  using Kernel = void (*)(FftPass, const float2 *, float2 *, const float2 *);
+  static const Kernel kernels[2][3] = {
+      {fft_lines<false, kFftGeneral>, fft_lines<false, kFftContig>, fft_lines<false, kFftStrided>},
+      {fft_lines<true, kFftGeneral>, fft_lines<true, kFftContig>, fft_lines<true, kFftStrided>}};
+  hipLaunchKernelGGL(kernels[twl ? 1 : 0][mode], dim3((unsigned)blocks), dim3(kThreads), lds, stream, p, in, out, tw);
   SPSIM_HIP_TRY(hipGetLastError());
   return SPSIM_HIP_OK;
 }
