@@ -38,8 +38,10 @@
 // The spread sums each grid cell's atoms in registers without FP atomics, but
 // in an order that integer atomics of the binning decide, so the last bits
 // of F can vary from run to run.
+#include <array>
 #include <cmath>
 #include <mutex>
+#include <utility>
 #include <vector>
 
 #include "common.hpp"
@@ -726,9 +728,9 @@ __device__ __forceinline__ void dft4(float2 &u0, float2 &u1, float2 &u2, float2 
   u3 = csub(b, d);
 }
 
-// The R-point DFT of u in place (R in {2, 3, 4, 5, 8}).
+// The R-point DFT of u in place, exp(-2 pi i n k / R), R in {2, 3, 4, 5, 8}.
 template <int R>
-__device__ __forceinline__ void dft_r(float2 (&u)[R]) {
+__device__ __forceinline__ void dft_small(float2 (&u)[R]) {
   if constexpr (R == 2) {
     const float2 a = u[0];
     u[0] = cadd(a, u[1]);
@@ -779,6 +781,114 @@ __device__ __forceinline__ void dft_r(float2 (&u)[R]) {
     u[2] = cadd(m2, r2);
     u[3] = csub(m2, r2);
   }
+}
+
+// exp(-2 pi i j / N) at compile time: the angle reduced to [-pi, pi] and a
+// Taylor series to double precision.
+constexpr double kTwoPi = 6.283185307179586476925286766559;
+constexpr double ct_sin(double x) {
+  double term = x, sum = x;
+  for (int n = 1; n < 16; ++n) {
+    term *= -x * x / ((2.0 * n) * (2.0 * n + 1.0));
+    sum += term;
+  }
+  return sum;
+}
+constexpr double ct_cos(double x) {
+  double term = 1.0, sum = 1.0;
+  for (int n = 1; n < 16; ++n) {
+    term *= -x * x / ((2.0 * n - 1.0) * (2.0 * n));
+    sum += term;
+  }
+  return sum;
+}
+constexpr int ct_mod(int j, int n) { return ((j % n) + n) % n; }
+constexpr double ct_angle(int j, int n) {
+  const int r = ct_mod(j, n);
+  return kTwoPi * (2 * r <= n ? r : r - n) / n;  // in [-pi, pi]
+}
+
+// a exp(-2 pi i J / N) with J, N known at compile time: exact for multiples
+// of pi/2, two multiplies for odd multiples of pi/4, four otherwise.
+template <int J, int N>
+__device__ __forceinline__ float2 twiddle_ct(float2 a) {
+  constexpr int j = ct_mod(J, N);
+  if constexpr (j == 0) {
+    return a;
+  } else if constexpr (4 * j == N) {
+    return mul_mi(a);
+  } else if constexpr (2 * j == N) {
+    return make_float2(-a.x, -a.y);
+  } else if constexpr (4 * j == 3 * N) {
+    return make_float2(-a.y, a.x);
+  } else if constexpr (8 * j == N || 8 * j == 3 * N || 8 * j == 5 * N || 8 * j == 7 * N) {
+    constexpr float h = 0.70710678118654752440f;
+    // (1 - i) h, (-1 - i) h, (-1 + i) h, (1 + i) h
+    constexpr float cr = (8 * j == N || 8 * j == 7 * N) ? h : -h;
+    constexpr float ci = (8 * j == N || 8 * j == 3 * N) ? -h : h;
+    return make_float2(cr * a.x - ci * a.y, cr * a.y + ci * a.x);
+  } else {
+    constexpr float c = (float)ct_cos(ct_angle(j, N)), s = (float)ct_sin(ct_angle(j, N));
+    return make_float2(a.x * c + a.y * s, a.y * c - a.x * s);  // a (c - i s)
+  }
+}
+
+template <int... I, class F>
+__device__ __forceinline__ void static_for_impl(std::integer_sequence<int, I...>, F &&f) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
+// f(std::integral_constant<int, i>) for i = 0 .. N-1, each index a constant.
+template <int N, class F>
+__device__ __forceinline__ void static_for(F &&f) {
+  static_for_impl(std::make_integer_sequence<int, N>{}, f);
+}
+
+// First factor of a composite radix: 4 for multiples of 4 above 8 (R = 4 R2),
+// else the smallest prime factor.
+constexpr int radix_split(int R) {
+  if (R % 4 == 0 && R > 8) return 4;
+  for (int p = 2; p <= R; ++p)
+    if (R % p == 0) return p;
+  return R;
+}
+
+template <int R>
+__device__ __forceinline__ void dft_r(float2 (&u)[R]);
+
+// The R1 R2-point DFT by Cooley-Tukey in registers: R1 DFTs of size R2 over
+// u[n1 + R1 n2], twiddles exp(-2 pi i n1 k2 / R), R2 DFTs of size R1; output
+// k2 + R2 k1.
+template <int R1, int R2>
+__device__ __forceinline__ void dft_composite(float2 (&u)[R1 * R2]) {
+  constexpr int R = R1 * R2;
+  float2 a[R1][R2];
+  static_for<R1>([&](auto n1c) {
+    constexpr int n1 = decltype(n1c)::value;
+    float2 v[R2];
+    static_for<R2>([&](auto n2c) { v[decltype(n2c)::value] = u[n1 + R1 * decltype(n2c)::value]; });
+    dft_r<R2>(v);
+    static_for<R2>([&](auto k2c) {
+      constexpr int k2 = decltype(k2c)::value;
+      a[n1][k2] = twiddle_ct<n1 * k2, R>(v[k2]);
+    });
+  });
+  static_for<R2>([&](auto k2c) {
+    constexpr int k2 = decltype(k2c)::value;
+    float2 v[R1];
+    static_for<R1>([&](auto n1c) { v[decltype(n1c)::value] = a[decltype(n1c)::value][k2]; });
+    dft_r<R1>(v);
+    static_for<R1>([&](auto k1c) { u[k2 + R2 * decltype(k1c)::value] = v[decltype(k1c)::value]; });
+  });
+}
+
+// The R-point DFT of u in place: hand-written for 2, 3, 4, 5 and 8, composite
+// (radix_split) for other 2,3,5-smooth R.
+template <int R>
+__device__ __forceinline__ void dft_r(float2 (&u)[R]) {
+  if constexpr (R == 2 || R == 3 || R == 4 || R == 5 || R == 8)
+    dft_small<R>(u);
+  else
+    dft_composite<radix_split(R), R / radix_split(R)>(u);
 }
 
 // How the global stages of a pass address memory. kFftGeneral: 64-bit point
@@ -1043,6 +1153,255 @@ __global__ __launch_bounds__(kThreads) void fft_lines(FftPass p, const float2 *_
                                         out + (has ? lo : 0), has);
     }
   }
+}
+
+// Line lengths with a kernel of their own (fft_fixed): every size and index
+// step is a compile-time constant, and the radices (up to kFixedMaxRadix,
+// composite ones included) are chosen for the fewest stages. Radices above 16
+// cost more in registers (fewer waves per SIMD) than their saved stage gains.
+constexpr int kFixedMaxRadix = 16;
+
+struct FixedPlan {
+  int ns;
+  int R[4];
+};
+
+// 2,3,5-smooth radices up to kFixedMaxRadix, largest first.
+constexpr int kRadixCount = 11;
+constexpr int kRadices[kRadixCount] = {16, 15, 12, 10, 9, 8, 6, 5, 4, 3, 2};
+
+constexpr bool smooth_ct(int n) {
+  for (int p : {2, 3, 5})
+    while (n > 1 && n % p == 0) n /= p;
+  return n == 1;
+}
+
+// The plan of nf with the fewest stages (at most 4), then the smallest largest
+// radix, then radices in decreasing order; ns == 0 when there is none.
+constexpr FixedPlan fixed_plan(int nf) {
+  FixedPlan best{0, {0, 0, 0, 0}};
+  if (!smooth_ct(nf)) return best;
+  for (int ns = 1; ns <= 4; ++ns) {
+    // radices in decreasing order a >= b >= c >= d; the first hit in a
+    // decreasing scan of a from the smallest feasible one is the best plan
+    for (int ia = kRadixCount - 1; ia >= 0; --ia) {
+      const int a = kRadices[ia];
+      if (a > kFixedMaxRadix || nf % a) continue;
+      const int n1 = nf / a;
+      if (ns == 1) {
+        if (n1 == 1) return {1, {a, 0, 0, 0}};
+        continue;
+      }
+      for (int ib = ia; ib < kRadixCount; ++ib) {
+        const int b = kRadices[ib];
+        if (n1 % b) continue;
+        const int n2 = n1 / b;
+        if (ns == 2) {
+          if (n2 == 1) return {2, {a, b, 0, 0}};
+          continue;
+        }
+        for (int ic = ib; ic < kRadixCount; ++ic) {
+          const int c = kRadices[ic];
+          if (n2 % c) continue;
+          const int n3 = n2 / c;
+          if (ns == 3) {
+            if (n3 == 1) return {3, {a, b, c, 0}};
+            continue;
+          }
+          for (int id = ic; id < kRadixCount; ++id)
+            if (kRadices[id] == n3) return {4, {a, b, c, n3}};
+        }
+      }
+    }
+  }
+  return best;
+}
+
+template <int NF>
+struct FixedPlanOf {
+  static constexpr FixedPlan plan = fixed_plan(NF);
+  static constexpr int span(int s) { return s == 0 ? 1 : span(s - 1) * plan.R[s - 1]; }
+};
+
+// The lengths with a fixed kernel: even, 2,3,5-smooth, 16 .. kFftLds (several
+// lines per workgroup), at most 3 stages.
+constexpr bool fixed_length(int n) {
+  if (n < 16 || n > kFftLds || n % 2 || !smooth_ct(n)) return false;
+  const FixedPlan p = fixed_plan(n);
+  return p.ns >= 1 && p.ns <= 3;
+}
+constexpr int count_fixed_lengths() {
+  int c = 0;
+  for (int n = 16; n <= kFftLds; ++n) c += fixed_length(n) ? 1 : 0;
+  return c;
+}
+constexpr int kNumFixed = count_fixed_lengths();
+struct FixedLengths {
+  int n[kNumFixed];
+};
+constexpr FixedLengths list_fixed_lengths() {
+  FixedLengths f{};
+  int c = 0;
+  for (int n = 16; n <= kFftLds; ++n)
+    if (fixed_length(n)) f.n[c++] = n;
+  return f;
+}
+constexpr FixedLengths kFixed = list_fixed_lengths();
+
+// q = n / D and n - q D for 0 <= n < 2^16 (the FP32 division of FastDiv with
+// constant operands).
+template <int D>
+__device__ __forceinline__ int cdiv(int n) {
+  return (int)__builtin_fmaf((float)n, 1.0f / D, 0.5f / D);
+}
+
+// One stage of radix R (span SPAN) of fft_fixed on the workgroup's T lines of
+// NF points. FIRST: reads the points from global memory (like stockham_bfly's
+// GLOAD) and writes the LDS buffer; LAST: reads the buffer and writes the
+// kept outputs to global memory; otherwise in place on the buffer.
+template <int NF, int T, int MODE, int R, int SPAN, bool FIRST, bool LAST>
+__device__ __forceinline__ void fixed_stage(const FftPass &p, float2 *__restrict__ x, const float2 *__restrict__ tw,
+                                            const long long *lin_s, const long long *lout_s,
+                                            const float2 *__restrict__ in, float2 *__restrict__ out) {
+  constexpr int nb = NF / R, nbt = T * nb, stride = NF / (SPAN * R);
+  constexpr bool contiguous = MODE == kFftContig;
+  // butterfly b -> (line t, butterfly i): line-major for contiguous lines,
+  // line-minor for strided views
+  auto split = [](int b, int &t, int &i) {
+    if constexpr (contiguous) {
+      t = cdiv<nb>(b);
+      i = b - t * nb;
+    } else {
+      i = cdiv<T>(b);
+      t = b - i * T;
+    }
+  };
+  auto twiddled = [&](float2 (&u)[R], const float2 *xl, int i) {
+    int a = i;
+#pragma unroll
+    for (int r = 0; r < R; ++r, a += nb) u[r] = xl[a];
+    if constexpr (SPAN > 1) {
+      const int k = i - cdiv<SPAN>(i) * SPAN;
+      const int step = k * stride;
+      int w = step;
+#pragma unroll
+      for (int r = 1; r < R; ++r, w += step) {
+        opaque(w);
+        u[r] = cmul(u[r], tw[w]);
+      }
+    }
+  };
+  auto store = [&](const float2 (&u)[R], int i, bool has, long long lo) {
+    if (!has) return;
+    float2 *lout = out + lo;
+    const int base = (contiguous ? i : i * p.inner32) - p.l0_in;
+    const int span_in = SPAN * p.inner32;
+#pragma unroll
+    for (int s2 = 0; s2 < R; ++s2) {
+      int o = base + s2 * span_in;
+      o += o < 0 ? p.nf_in : 0;
+      if (o < p.L_in) lout[(unsigned)o] = u[s2];
+    }
+  };
+  if constexpr (FIRST) {
+    const int nb_in = nb * p.inner32;
+    for (int b = threadIdx.x; b < nbt; b += kThreads) {
+      int t, i;
+      split(b, t, i);
+      const long long li = lin_s[t];
+      const bool has = li >= 0;
+      const float2 *lin = in + (has ? li : 0);
+      float2 u[R];
+      const int base = (contiguous ? i : i * p.inner32) - p.m0_in;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        int o = base + r * nb_in;
+        o += o < 0 ? p.nf_in : 0;
+        u[r] = o < p.M_in && has ? lin[(unsigned)o] : make_float2(0.0f, 0.0f);
+      }
+      dft_r<R>(u);
+      if constexpr (LAST) {
+        store(u, i, has, lout_s[t]);
+      } else {
+        float2 *yl = x + t * NF + i * R;
+#pragma unroll
+        for (int s2 = 0; s2 < R; ++s2) yl[s2] = u[s2];
+      }
+    }
+    if constexpr (!LAST) __syncthreads();
+  } else if constexpr (LAST) {
+    for (int b = threadIdx.x; b < nbt; b += kThreads) {
+      int t, i;
+      split(b, t, i);
+      float2 u[R];
+      twiddled(u, x + t * NF, i);
+      dft_r<R>(u);
+      const long long lo = lout_s[t];
+      store(u, i, lo >= 0, lo);
+    }
+  } else {
+    constexpr int KB = (nbt + kThreads - 1) / kThreads;
+    float2 u[KB][R];
+    int at[KB];
+#pragma unroll
+    for (int kb = 0; kb < KB; ++kb) {
+      const int b = threadIdx.x + kb * kThreads;
+      at[kb] = -1;
+      if (b < nbt) {
+        const int t = cdiv<nb>(b), i = b - t * nb;
+        twiddled(u[kb], x + t * NF, i);
+        const int k = i - cdiv<SPAN>(i) * SPAN;
+        at[kb] = t * NF + (i - k) * R + k;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kb = 0; kb < KB; ++kb) {
+      if (at[kb] >= 0) {
+        dft_r<R>(u[kb]);
+        float2 *yl = x + at[kb];
+#pragma unroll
+        for (int s2 = 0; s2 < R; ++s2) yl[s2 * SPAN] = u[kb][s2];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// fft_lines for lines of exactly NF points (a single grid, L <= NF, 32-bit
+// offsets, MODE kFftContig or kFftStrided): the same layout and results, with
+// the stages of FixedPlanOf<NF> unrolled at compile time.
+template <int NF, int MODE>
+__global__ __launch_bounds__(kThreads) void fft_fixed(FftPass p, const float2 *__restrict__ in,
+                                                     float2 *__restrict__ out, const float2 *__restrict__ tw_g) {
+  using Plan = FixedPlanOf<NF>;
+  constexpr int T = kFftLds / NF;
+  static_assert(Plan::plan.ns > 0 && T >= 1 && T <= kMaxT, "fixed FFT length");
+  extern __shared__ float2 lds[];
+  float2 *x = lds;
+  float2 *tw = lds + T * NF;
+  long long *lin_s = (long long *)(lds + T * NF + NF);
+  long long *lout_s = lin_s + T;
+  for (int e = threadIdx.x; e < NF; e += kThreads) tw[e] = tw_g[e];
+  const unsigned blk = MODE == kFftStrided ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+  const long long line0 = (long long)blk * T;
+  for (int t = threadIdx.x; t < T; t += kThreads) {
+    const long long line = line0 + t;
+    if (line < p.nlines) {
+      const long long o = line / p.inner;
+      const long long c = line - o * p.inner;
+      lin_s[t] = o * p.M * p.inner + c;
+      lout_s[t] = o * p.L * p.inner + c;
+    } else {
+      lin_s[t] = lout_s[t] = -1;
+    }
+  }
+  __syncthreads();
+  static_for<Plan::plan.ns>([&](auto sc) {
+    constexpr int s = decltype(sc)::value;
+    fixed_stage<NF, T, MODE, Plan::plan.R[s], Plan::span(s), s == 0, s == Plan::plan.ns - 1>(p, x, tw, lin_s,
+                                                                                              lout_s, in, out);
+  });
 }
 
 // Per-q state of the interpolation: the (rotated) q in FP64, the window
@@ -1331,9 +1690,13 @@ bool smooth_length(int n) {
   return n == 1;
 }
 
+// The smallest even 2,3,5-smooth length >= at_least, or a length with a
+// fixed-plan kernel up to 10 % above it.
 int fft_length(int at_least) {
   int n = at_least + (at_least & 1);
   while (!smooth_length(n)) n += 2;
+  for (int m = n; m <= n + n / 10; m += 2)
+    if (smooth_length(m) && fixed_length(m)) return m;
   return n;
 }
 
@@ -1515,6 +1878,24 @@ int launch_twiddles(int nf, float2 *tw, hipStream_t stream) {
   return SPSIM_HIP_OK;
 }
 
+struct FixedKernels {
+  void (*contig)(FftPass, const float2 *, float2 *, const float2 *);
+  void (*strided)(FftPass, const float2 *, float2 *, const float2 *);
+};
+template <size_t... I>
+constexpr std::array<FixedKernels, sizeof...(I)> fixed_kernel_table(std::index_sequence<I...>) {
+  return {FixedKernels{fft_fixed<kFixed.n[I], kFftContig>, fft_fixed<kFixed.n[I], kFftStrided>}...};
+}
+constexpr std::array<FixedKernels, kNumFixed> kFixedKernels =
+    fixed_kernel_table(std::make_index_sequence<kNumFixed>{});
+
+// Position of nf in kFixed, or -1.
+int fixed_index(int nf) {
+  for (int i = 0; i < kNumFixed; ++i)
+    if (kFixed.n[i] == nf) return i;
+  return -1;
+}
+
 int launch_fft(const FftPass &pass, const float2 *in, float2 *out, const float2 *tw, hipStream_t stream) {
   FftPass p = pass;
   // every slot and line index of fft_lines stays below 2^16
@@ -1548,7 +1929,12 @@ int launch_fft(const FftPass &pass, const float2 *in, float2 *out, const float2 
   static const Kernel kernels[2][3] = {
       {fft_lines<false, kFftGeneral>, fft_lines<false, kFftContig>, fft_lines<false, kFftStrided>},
       {fft_lines<true, kFftGeneral>, fft_lines<true, kFftContig>, fft_lines<true, kFftStrided>}};
-  hipLaunchKernelGGL(kernels[twl ? 1 : 0][mode], dim3((unsigned)blocks), dim3(kThreads), lds, stream, p, in, out, tw);
+  Kernel k = kernels[twl ? 1 : 0][mode];
+  if (mode != kFftGeneral && p.T == kFftLds / p.nf) {
+    const int f = fixed_index(p.nf);
+    if (f >= 0) k = mode == kFftContig ? kFixedKernels[f].contig : kFixedKernels[f].strided;
+  }
+  hipLaunchKernelGGL(k, dim3((unsigned)blocks), dim3(kThreads), lds, stream, p, in, out, tw);
   SPSIM_HIP_TRY(hipGetLastError());
   return SPSIM_HIP_OK;
 }

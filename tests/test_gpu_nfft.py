@@ -42,6 +42,41 @@ def test_fft_pruned_matches_numpy(gpu, nf, M, m_lo, L, l_lo, outer, inner):
     assert rel_l2(np.c_[out.real.ravel(), out.imag.ravel()], np.c_[ref.real.ravel(), ref.imag.ravel()]) < 2e-6
 
 
+def smooth_even_lengths(lo, hi):
+    out = []
+    for n in range(lo, hi + 1, 2):
+        m = n
+        for f in (2, 3, 5):
+            while m % f == 0:
+                m //= f
+        if m == 1:
+            out.append(n)
+    return out
+
+
+@pytest.mark.parametrize("inner", [1, 6])
+def test_fft_pruned_every_smooth_length(gpu, inner):
+    """Every even 2,3,5-smooth line length from 16 to 1920 (each has a kernel of
+    its own when its plan takes at most 3 stages of radices up to 16, the
+    general kernel otherwise), contiguous and strided, on half-filled lines
+    with about half the outputs kept, against numpy's FFT."""
+    import torch
+
+    rng = np.random.default_rng(inner)
+    worst = 0.0
+    for nf in smooth_even_lengths(16, 1920):
+        M, L = nf // 2 - 1, nf // 2 + 3
+        m_lo, l_lo = -(M // 2), -(L // 2) + 1
+        outer = 3
+        x = (rng.normal(size=(outer, M, inner)) + 1j * rng.normal(size=(outer, M, inner))).astype(np.complex64)
+        out = dv.fft_pruned(torch.from_numpy(x).to(gpu), M, m_lo, nf, L, l_lo).cpu().numpy()
+        ref = fft_reference(x.astype(np.complex128), m_lo, nf, L, l_lo)
+        err = rel_l2(np.c_[out.real.ravel(), out.imag.ravel()], np.c_[ref.real.ravel(), ref.imag.ravel()])
+        assert err < 2e-6, (nf, err)
+        worst = max(worst, err)
+    print("worst relative L2 error", worst)
+
+
 def test_fft_pruned_rejects_bad_lengths(gpu):
     import torch
 
