@@ -1256,10 +1256,10 @@ __device__ __forceinline__ int cdiv(int n) {
 }
 
 // One stage of radix R (span SPAN) of fft_fixed on the workgroup's T lines of
-// NF points. FIRST: reads the points from global memory (like stockham_bfly's
+// NF points, PITCH = NF + 1 apart in LDS. FIRST: reads the points from global memory (like stockham_bfly's
 // GLOAD) and writes the LDS buffer; LAST: reads the buffer and writes the
 // kept outputs to global memory; otherwise in place on the buffer.
-template <int NF, int T, int MODE, int R, int SPAN, bool FIRST, bool LAST>
+template <int NF, int PITCH, int T, int MODE, int R, int SPAN, bool FIRST, bool LAST>
 __device__ __forceinline__ void fixed_stage(const FftPass &p, float2 *__restrict__ x, const float2 *__restrict__ tw,
                                             const long long *lin_s, const long long *lout_s,
                                             const float2 *__restrict__ in, float2 *__restrict__ out) {
@@ -1323,7 +1323,7 @@ __device__ __forceinline__ void fixed_stage(const FftPass &p, float2 *__restrict
       if constexpr (LAST) {
         store(u, i, has, lout_s[t]);
       } else {
-        float2 *yl = x + t * NF + i * R;
+        float2 *yl = x + t * PITCH + i * R;
 #pragma unroll
         for (int s2 = 0; s2 < R; ++s2) yl[s2] = u[s2];
       }
@@ -1334,7 +1334,7 @@ __device__ __forceinline__ void fixed_stage(const FftPass &p, float2 *__restrict
       int t, i;
       split(b, t, i);
       float2 u[R];
-      twiddled(u, x + t * NF, i);
+      twiddled(u, x + t * PITCH, i);
       dft_r<R>(u);
       const long long lo = lout_s[t];
       store(u, i, lo >= 0, lo);
@@ -1349,9 +1349,9 @@ __device__ __forceinline__ void fixed_stage(const FftPass &p, float2 *__restrict
       at[kb] = -1;
       if (b < nbt) {
         const int t = cdiv<nb>(b), i = b - t * nb;
-        twiddled(u[kb], x + t * NF, i);
+        twiddled(u[kb], x + t * PITCH, i);
         const int k = i - cdiv<SPAN>(i) * SPAN;
-        at[kb] = t * NF + (i - k) * R + k;
+        at[kb] = t * PITCH + (i - k) * R + k;
       }
     }
     __syncthreads();
@@ -1370,17 +1370,19 @@ __device__ __forceinline__ void fixed_stage(const FftPass &p, float2 *__restrict
 
 // fft_lines for lines of exactly NF points (a single grid, L <= NF, 32-bit
 // offsets, MODE kFftContig or kFftStrided): the same layout and results, with
-// the stages of FixedPlanOf<NF> unrolled at compile time.
+// the stages of FixedPlanOf<NF> unrolled at compile time. Lines sit NF + 1
+// points apart in LDS: an odd pitch puts the same point of consecutive lines
+// (consecutive lanes of a strided view's global stages) in different banks.
 template <int NF, int MODE>
 __global__ __launch_bounds__(kThreads) void fft_fixed(FftPass p, const float2 *__restrict__ in,
                                                      float2 *__restrict__ out, const float2 *__restrict__ tw_g) {
   using Plan = FixedPlanOf<NF>;
-  constexpr int T = kFftLds / NF;
+  constexpr int T = kFftLds / NF, PITCH = NF + 1;
   static_assert(Plan::plan.ns > 0 && T >= 1 && T <= kMaxT, "fixed FFT length");
   extern __shared__ float2 lds[];
   float2 *x = lds;
-  float2 *tw = lds + T * NF;
-  long long *lin_s = (long long *)(lds + T * NF + NF);
+  float2 *tw = lds + T * PITCH;
+  long long *lin_s = (long long *)(lds + T * PITCH + NF);
   long long *lout_s = lin_s + T;
   for (int e = threadIdx.x; e < NF; e += kThreads) tw[e] = tw_g[e];
   const unsigned blk = MODE == kFftStrided ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
@@ -1399,8 +1401,8 @@ __global__ __launch_bounds__(kThreads) void fft_fixed(FftPass p, const float2 *_
   __syncthreads();
   static_for<Plan::plan.ns>([&](auto sc) {
     constexpr int s = decltype(sc)::value;
-    fixed_stage<NF, T, MODE, Plan::plan.R[s], Plan::span(s), s == 0, s == Plan::plan.ns - 1>(p, x, tw, lin_s,
-                                                                                              lout_s, in, out);
+    fixed_stage<NF, PITCH, T, MODE, Plan::plan.R[s], Plan::span(s), s == 0, s == Plan::plan.ns - 1>(
+        p, x, tw, lin_s, lout_s, in, out);
   });
 }
 
@@ -1833,12 +1835,12 @@ int plan_axis(double xmin, double xmax, double smin, double smax, int W, double 
 }
 
 // Cost of a plan in units of one window gather: the three pruned FFT passes
-// (about 18 gathers per line point per pass, measured on MI355X) and the
+// (about 10 gathers per line point per pass, measured on MI355X) and the
 // interpolation (W2^3 gathers per q and species).
 double plan_cost(const NufftDim d[3], int W2, double nq) {
   const double fft = (double)d[0].nf * d[1].M * d[2].M + (double)d[1].nf * d[0].L * d[2].M +
                      (double)d[2].nf * d[0].L * d[1].L;
-  return 18.0 * fft + nq * W2 * W2 * W2;
+  return 10.0 * fft + nq * W2 * W2 * W2;
 }
 
 FftPass make_pass(const NufftDim &D, long long outer, long long inner) {
@@ -1930,11 +1932,15 @@ int launch_fft(const FftPass &pass, const float2 *in, float2 *out, const float2 
       {fft_lines<false, kFftGeneral>, fft_lines<false, kFftContig>, fft_lines<false, kFftStrided>},
       {fft_lines<true, kFftGeneral>, fft_lines<true, kFftContig>, fft_lines<true, kFftStrided>}};
   Kernel k = kernels[twl ? 1 : 0][mode];
+  size_t lds_k = lds;
   if (mode != kFftGeneral && p.T == kFftLds / p.nf) {
     const int f = fixed_index(p.nf);
-    if (f >= 0) k = mode == kFftContig ? kFixedKernels[f].contig : kFixedKernels[f].strided;
+    if (f >= 0) {
+      k = mode == kFftContig ? kFixedKernels[f].contig : kFixedKernels[f].strided;
+      lds_k += (size_t)p.T * sizeof(float2);  // line pitch nf + 1
+    }
   }
-  hipLaunchKernelGGL(k, dim3((unsigned)blocks), dim3(kThreads), lds, stream, p, in, out, tw);
+  hipLaunchKernelGGL(k, dim3((unsigned)blocks), dim3(kThreads), lds_k, stream, p, in, out, tw);
   SPSIM_HIP_TRY(hipGetLastError());
   return SPSIM_HIP_OK;
 }
