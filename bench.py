@@ -968,6 +968,7 @@ def nfft(args, torch, dv, wl, _lib, h):
 
     ms_n, wall_n = timed(run_n)
     ms_d, wall_d = timed(run_d)
+    fft = fft_passes(torch, dv, _lib, info, len(np.unique(Z)))
     with_b = None
     if h.get("B") is not None:
         atoms_b = dv.DeviceAtoms(dv.pack_atoms(Z, pos, atom_b=h["B"]), dev)
@@ -1014,8 +1015,57 @@ def nfft(args, torch, dv, wl, _lib, h):
         "effective_gterms_per_s": round(terms / (ms_n * 1e-3) / 1e9, 1),
         "rel_err_vs_fp64": err_n, "rel_err_direct_vs_fp64": err_d,
     }
+    out["fft_passes"] = fft
     if with_b is not None:
         out["per_atom_b"] = with_b
+    return out
+
+
+def fft_passes(torch, dv, _lib, info, nch):
+    """The three pruned FFT passes of one NUFFT call, timed through
+    spsim_hip_fft_pruned on the call's grid (nch species grids; the x pass reads
+    [nch Mz My][Mx] lines, the y pass [nch Mz][My][Lx], the z pass [nch][Mz][Ly Lx],
+    each pass the previous one's output) with HIP events on the launching stream.
+    Algorithmic bytes per pass: the M input points and the L kept outputs of
+    every line, 8 bytes each, read and written once. Twenty calls run back to
+    back between one pair of events, so that the host's call overhead hides
+    behind the GPU work; each call also fills its twiddle table (one small
+    kernel), which the times include."""
+    M, nf, L = info["M"], info["nf"], info["L"]
+    dev = torch.device("cuda", 0)
+    g = torch.Generator(device=dev).manual_seed(9)
+    x = torch.complex(torch.randn(nch * M[2] * M[1], M[0], 1, device=dev, generator=g),
+                      torch.randn(nch * M[2] * M[1], M[0], 1, device=dev, generator=g))
+    passes = [("x", lambda a: a, M[0], nf[0], L[0]),
+              ("y", lambda a: a.reshape(nch * M[2], M[1], L[0]), M[1], nf[1], L[1]),
+              ("z", lambda a: a.reshape(nch, M[2], L[1] * L[0]), M[2], nf[2], L[2])]
+    out, total_b, total_us = {}, 0.0, 0.0
+    cur = x
+    for name, view, m, n, l in passes:
+        src = view(cur).contiguous()
+        dst = dv.fft_pruned(src, m, -(m // 2), n, l, -(l // 2))
+        # the entry point itself with prepared arguments: a Python wrapper per
+        # call would take longer than the pass
+        fn = _lib.load().spsim_hip_fft_pruned
+        args = (C.c_void_p(src.data_ptr()), src.shape[0], src.shape[2], m, -(m // 2), n, l, -(l // 2),
+                C.c_void_p(dst.data_ptr()), C.c_void_p(torch.cuda.current_stream().cuda_stream))
+        reps = 20
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            _lib.check(fn(*args), "spsim_hip_fft_pruned")
+        e1.record()
+        e1.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        nbytes = 8.0 * src.shape[0] * src.shape[2] * (m + l)
+        out[name] = {"lines": int(src.shape[0] * src.shape[2]), "M": m, "nf": n, "L": l, "us": round(ms * 1e3, 1),
+                     "algorithmic_mb": round(nbytes / 1e6, 1),
+                     "hbm_frac": round(nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 3)}
+        total_b += nbytes
+        total_us += ms * 1e3
+        cur = dst
+    out["all"] = {"us": round(total_us, 1), "hbm_frac": round(total_b / (total_us * 1e-6) / 1e9 / HBM_PEAK_GBS, 3)}
     return out
 
 
