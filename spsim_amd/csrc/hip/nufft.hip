@@ -1463,9 +1463,19 @@ __device__ __forceinline__ void interp_prepare(const float *__restrict__ hkl, in
 template <int W, bool kRollZ = false, typename Row>
 __device__ __forceinline__ void window_sum(const InterpQ<W> &s, Row row, float &ar, float &ai) {
   ar = ai = 0.0f;
+  // Every weight index is a constant after unrolling, so the weights stay in
+  // registers: the z weights shift down one place per z iteration (the
+  // current one is wz[0]) when that loop stays rolled.
+  float wz[W];
+#pragma unroll
+  for (int j = 0; j < W; ++j) wz[j] = s.wt[2][j];
   constexpr int kUnrollZ = kRollZ ? 1 : W;
 #pragma unroll kUnrollZ
   for (int iz = 0; iz < W; ++iz) {
+    const float w2 = wz[0];
+#pragma unroll
+    for (int j = 0; j + 1 < W; ++j) wz[j] = wz[j + 1];
+#pragma unroll
     for (int iy = 0; iy < W; ++iy) {
       const float2 *r = row(iz, iy);
       float rr = 0.0f, ri = 0.0f;
@@ -1475,7 +1485,7 @@ __device__ __forceinline__ void window_sum(const InterpQ<W> &s, Row row, float &
         rr += h.x * s.wt[0][ix];
         ri += h.y * s.wt[0][ix];
       }
-      const float wzy = s.wt[2][iz] * s.wt[1][iy];
+      const float wzy = w2 * s.wt[1][iy];
       ar += wzy * rr;
       ai += wzy * ri;
     }
@@ -1564,15 +1574,15 @@ __global__ __launch_bounds__(kThreads) void nufft_interp(const float *__restrict
 // per axis, at most kBoxCap points) from global memory into their own LDS
 // region and then gather from LDS, which serves twice the bytes per clock of
 // the vector L1. A wave whose box is larger gathers from global memory.
+// At least 4 waves per SIMD (at most 128 VGPRs): measured faster than the 3
+// that the unconstrained allocation (136 VGPRs at W = 7) leaves.
 constexpr int kBoxCap = 1024;  // grid points per wave (8 KiB of LDS per species)
 
 template <int W>
-__global__ __launch_bounds__(kThreads) void nufft_interp_lds(const float *__restrict__ hkl, int npix, NufftGeom g,
-                                                            const float2 *__restrict__ H, int nch,
-                                                            const spsim_hip_segment *__restrict__ segs,
-                                                            const float *__restrict__ ff, float b_global,
-                                                            const double *__restrict__ rots, int p0,
-                                                            float2 *__restrict__ F, float *__restrict__ I) {
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void nufft_interp_lds(
+    const float *__restrict__ hkl, int npix, NufftGeom g, const float2 *__restrict__ H, int nch,
+    const spsim_hip_segment *__restrict__ segs, const float *__restrict__ ff, float b_global,
+    const double *__restrict__ rots, int p0, float2 *__restrict__ F, float *__restrict__ I) {
   extern __shared__ float2 box[];  // g.box_cap points per wave
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   const int k = interp_q_index(g, npix);  // npix: no q
