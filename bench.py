@@ -103,6 +103,9 @@ def parse():
     p.add_argument("--pipeline-only", action="store_true", help="only the full-pipeline section (for profiling)")
     p.add_argument("--single-only", action="store_true",
                    help="only the large-single-pattern section (for profiling)")
+    p.add_argument("--single-tile", default=None,
+                   help="L,C,S: the single pattern's grid tile (pixels per lane, atom chunk, atom ranges, 0 for "
+                        "automatic) instead of the sweep, after the counter calibration copy (for rocprofv3)")
     p.add_argument("--profile", action="store_true",
                    help="headline workload only, preceded by a plain copy of known size (for rocprofv3)")
     return p.parse_args()
@@ -425,6 +428,10 @@ def single_pattern(args, torch, dv, wl, _lib):
     candidates = [("pattern", dv.make_tile(k, c, 1)) for k in (2, 4) for c in (128, 256)]
     candidates += [("grid", dv.make_tile(k, c, 1)) for k in (1, 2) for c in (128, 256)]
     candidates += [("grid", dv.make_tile(8, c, s)) for c in (128, 256) for s in (0, 4)]  # 0: automatic ranges
+    if args.single_tile:
+        calibration_copy(torch)
+        k, c, s = (int(v) for v in args.single_tile.split(","))
+        candidates = [("grid", dv.make_tile(k, c, s))]
     best, best_ms, sweep = None, 1e30, []
     for kind, t in candidates:
         run(kind, t)
