@@ -20,8 +20,11 @@
 //     grid, where phi(z) = exp(beta (sqrt(1 - z^2) - 1)) (beta1 = 2.30 W) and
 //     D_m deconvolves the second window in advance.
 //  3. Pruned FFT: along x, y, then z, each grid line (M points) is zero-padded
-//     to nf >= sigma2 M points, transformed in LDS (Stockham, radices
-//     8/4/2/3/5) and only the L outputs the targets can reach are kept.
+//     to nf >= sigma2 M points, transformed in LDS (Stockham) and only the L
+//     outputs the targets can reach are kept. Lengths up to 1920 with a plan
+//     of at most 3 stages run a kernel compiled for that length (fft_fixed,
+//     composite radices up to 16); others the general kernel (fft_lines,
+//     radices 8/4/2/3/5 chosen at run time).
 //  4. Interpolate: each q gathers W2^3 FFT values per species with the
 //     second window phi2 (width W2, beta2), divides by the first window's
 //     transform at s', multiplies by f_Z(|q|), sums the species and applies
@@ -35,14 +38,16 @@
 // whose transform carries the atom's Debye-Waller factor (see GaussSpread).
 // The error of the centred sum is about 10^{1-W} relative to the number of
 // atoms (W = 7 by default); the grids and FFT run in FP32, the phases in FP64.
-// The spread sums each grid cell's atoms in registers without FP atomics, but
-// in an order that integer atomics of the binning decide, so the last bits
-// of F can vary from run to run.
+// The spread sums each grid cell's atoms in registers without FP atomics, in
+// an order fixed by the input (a stable sort bins the atoms by index; the
+// candidates of a block are taken in lane order), so F repeats bit for bit.
 #include <array>
 #include <cmath>
 #include <mutex>
 #include <utility>
 #include <vector>
+
+#include <rocprim/device/device_radix_sort.hpp>
 
 #include "common.hpp"
 #include "form_factor.hpp"
@@ -112,15 +117,13 @@ struct NufftGeom {
   float corr[kCorrDeg + 1];
 };
 
-// Exact n / d for n, d < 2^16, two ways:
-//  * q = umulhi(n, m) with m = floor(2^32 / d) + 1 (the error
-//    n (1 - frac(2^32/d)) / 2^32 stays below 1/d); d == 1 is m == 0;
-//  * q = trunc(fma(n, fi, fh)) in FP32 with fi = 1/d rounded, fh = fi / 2:
-//    (n + 1/2) / d lies at least 1/(2d) from an integer, and the relative error
-//    of the FP32 result (below 2^-23) is under (2^16 / d + 1) 2^-23 < 1/(2d)
-//    in absolute terms. Full-rate instructions only, for the FFT's indices.
+// Exact n / d for n, d < 2^16 as q = trunc(fma(n, fi, fh)) in FP32, with
+// fi = 1/d rounded and fh = fi / 2: (n + 1/2) / d lies at least 1/(2d) from an
+// integer, and the FP32 result's error, below (2^16 / d + 1) 2^-23 in absolute
+// terms, is smaller than that. Full-rate instructions only, for the FFT's
+// indices.
 struct FastDiv {
-  unsigned d, m;
+  unsigned d;
   float fi, fh;
 };
 
@@ -248,7 +251,7 @@ __global__ __launch_bounds__(kThreads) void bounds_molecules(const spsim_hip_ato
 //    back to back: dec[0 .. Mx) | dec[Mx .. Mx+My) | dec[.. + Mz);
 //  - the twiddles tw[j] = exp(-2 pi i j / nf) of the three FFT lengths, back to
 //    back (nfx | nfy | nfz);
-//  - n4 zeroed 16-byte words at zero (the bin counters), grid-strided.
+//  - n4 zeroed 16-byte words at zero (the bins' start and end), grid-strided.
 __device__ __forceinline__ void setup_tables(const NufftGeom &g, int W, float *__restrict__ dec,
                                              float2 *__restrict__ tw, float4 *__restrict__ zero, size_t n4, int i,
                                              int stride) {
@@ -301,15 +304,34 @@ struct GaussSpread {
 };
 constexpr double kBToSigma2 = 2.5e-21 / (2.0 * M_PI * M_PI);  // m^2 per A^2 of B
 
+// Atoms are binned by the kSpreadBlock^3 grid block that holds the first point
+// of their window (their home block). One 64-lane workgroup per output block
+// then gathers the atoms of the home blocks whose windows can reach it; each
+// lane owns one (x, y) column of the block and keeps its kSpreadBlock cells
+// in registers, so the species grids need no zeroing and no atomics.
+constexpr int kSpreadBlock = 8;
+static_assert(kSpreadBlock * kSpreadBlock == 64, "one lane per (x, y) column of a spread block");
+
+struct SpreadBins {
+  int nbx, nby, nbz;  // spread blocks per axis
+  long long nbins;    // nch * nbz * nby * nbx
+};
+
+__device__ __forceinline__ long long home_bin(const SpreadAtom &r, const SpreadBins &b) {
+  return (((long long)r.ch * b.nbz + r.bz / kSpreadBlock) * b.nby + r.by / kSpreadBlock) * b.nbx + r.bx / kSpreadBlock;
+}
+
 // One thread per packed atom: the record above (ch = -1 marks an atom outside
-// the grid, which the bounds exclude).
+// the grid, which the bounds exclude), and the binning's sort key (home block;
+// nbins for an atom outside the grid) and value (the atom's index).
 // atom_b (packed order, may be NULL): Gaussian windows of per-atom width
 // (at most W points); else the exponential-of-semicircle window of width W.
 __device__ __forceinline__ void spread_prep(const spsim_hip_atom *__restrict__ atoms, int natoms,
                                             const spsim_hip_segment *__restrict__ segs, int nsegs,
                                             const AtomRot &R, const NufftGeom &g, int W,
                                             const float *__restrict__ atom_b, GaussSpread gs,
-                                            SpreadAtom *__restrict__ rec, int j) {
+                                            SpreadAtom *__restrict__ rec, SpreadBins bins,
+                                            unsigned *__restrict__ keys, int *__restrict__ index, int j) {
   if (j >= natoms) return;
   int ch = 0;
   for (int s = 1; s < nsegs; ++s)
@@ -375,6 +397,8 @@ __device__ __forceinline__ void spread_prep(const spsim_hip_atom *__restrict__ a
   r.kz = (float)kz;
   r.pad1 = r.pad2 = 0.0f;
   rec[j] = r;
+  keys[j] = r.ch < 0 ? (unsigned)bins.nbins : (unsigned)home_bin(r, bins);
+  index[j] = j;
 }
 
 // The tables and the atom records in one launch (they are independent): the
@@ -386,88 +410,98 @@ __global__ __launch_bounds__(kThreads) void tables_and_prep(NufftGeom g, int W2,
                                                            const spsim_hip_atom *__restrict__ atoms, int natoms,
                                                            const spsim_hip_segment *__restrict__ segs, int nsegs,
                                                            AtomRot R, int Wsp, const float *__restrict__ atom_b,
-                                                           GaussSpread gs, SpreadAtom *__restrict__ rec) {
+                                                           GaussSpread gs, SpreadAtom *__restrict__ rec,
+                                                           SpreadBins bins, unsigned *__restrict__ keys,
+                                                           int *__restrict__ index) {
   if ((int)blockIdx.x < tb)
     setup_tables(g, W2, dec, tw, zero, n4, blockIdx.x * kThreads + threadIdx.x, tb * kThreads);
   else
-    spread_prep(atoms, natoms, segs, nsegs, R, g, Wsp, atom_b, gs, rec,
+    spread_prep(atoms, natoms, segs, nsegs, R, g, Wsp, atom_b, gs, rec, bins, keys, index,
                 ((int)blockIdx.x - tb) * kThreads + threadIdx.x);
 }
 
-// Atoms are binned by the kSpreadBlock^3 grid block that holds the first point
-// of their window (their home block). One 64-lane workgroup per output block
-// then gathers the atoms of the home blocks whose windows can reach it; each
-// lane owns one (x, y) column of the block and keeps its kSpreadBlock cells
-// in registers, so the species grids need no zeroing and no atomics.
-constexpr int kSpreadBlock = 8;
-static_assert(kSpreadBlock * kSpreadBlock == 64, "one lane per (x, y) column of a spread block");
-
-struct SpreadBins {
-  int nbx, nby, nbz;  // spread blocks per axis
-  long long nbins;    // nch * nbz * nby * nbx
-};
-
-__device__ __forceinline__ long long home_bin(const SpreadAtom &r, const SpreadBins &b) {
-  return (((long long)r.ch * b.nbz + r.bz / kSpreadBlock) * b.nby + r.by / kSpreadBlock) * b.nbx + r.bx / kSpreadBlock;
-}
-
-// counts[home bin] += 1 for every atom inside the grid.
-__global__ __launch_bounds__(kThreads) void bin_count(const SpreadAtom *__restrict__ rec, int natoms, SpreadBins b,
-                                                     int *__restrict__ counts) {
-  const int j = blockIdx.x * kThreads + threadIdx.x;
-  if (j >= natoms) return;
-  const SpreadAtom r = rec[j];
-  if (r.ch < 0) return;
-  atomicAdd(counts + home_bin(r, b), 1);
-}
-
-// Few bins (small grids): each workgroup counts a run of kBinRun atoms in an
-// LDS histogram and adds it to the global counts with one atomic per bin,
-// instead of one contended global atomic per atom.
+// The binning is a stable radix sort of (home bin, atom index) pairs (the keys
+// come from spread_prep): every bin lists its atoms in index order, so the
+// spread sums them in an order that depends on the input alone and the results
+// repeat bit for bit. Atoms outside the grid take the key nbins and sort after
+// every bin.
+using BinSortConfig = rocprim::default_config;
+//
+// Grids of at most kLocalBins spread blocks (every small-grid call) take a
+// counting sort of the same order instead, in as many launches as an atomic
+// one: each run of kBinRun atoms counts its atoms per bin in LDS
+// (bin_count_runs, a bin-major (bin, run) matrix), an exclusive scan of that
+// matrix gives every (bin, run) its first slot, and bin_place_runs places each
+// run's atoms chunk by chunk in index order.
 constexpr int kLocalBins = 4096;
-constexpr int kBinRun = 1024;
+constexpr int kBinRun = 256;
 
-__global__ __launch_bounds__(kThreads) void bin_count_local(const SpreadAtom *__restrict__ rec, int natoms,
-                                                           SpreadBins b, int *__restrict__ counts) {
+__global__ __launch_bounds__(kThreads) void bin_count_runs(const unsigned *__restrict__ keys, int natoms, int nbins,
+                                                          int runs, int *__restrict__ cnt) {
   __shared__ int hist[kLocalBins];
-  const int nb = (int)b.nbins;
-  for (int i = threadIdx.x; i < nb; i += kThreads) hist[i] = 0;
+  for (int i = threadIdx.x; i < nbins; i += kThreads) hist[i] = 0;
   __syncthreads();
-  const int j0 = blockIdx.x * kBinRun, j1 = min(natoms, j0 + kBinRun);
+  const int r = blockIdx.x, j0 = r * kBinRun, j1 = min(natoms, j0 + kBinRun);
   for (int j = j0 + threadIdx.x; j < j1; j += kThreads) {
-    const SpreadAtom r = rec[j];
-    if (r.ch >= 0) atomicAdd(hist + home_bin(r, b), 1);
+    const unsigned k = keys[j];
+    if (k < (unsigned)nbins) atomicAdd(hist + k, 1);
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < nb; i += kThreads)
-    if (hist[i]) atomicAdd(counts + i, hist[i]);
+  for (int i = threadIdx.x; i < nbins; i += kThreads) cnt[(size_t)i * runs + r] = hist[i];
+  if (r == 0 && threadIdx.x == 0) cnt[(size_t)nbins * runs] = 0;  // the scan's total
 }
 
-// The matching scatter: the workgroup reserves a range of each bin for its run
-// (one global atomic per bin) and places its atoms there by local rank.
-__global__ __launch_bounds__(kThreads) void bin_scatter_local(const SpreadAtom *__restrict__ rec, int natoms,
-                                                             SpreadBins b, const int *__restrict__ start,
-                                                             int *__restrict__ cursor,
-                                                             SpreadAtom *__restrict__ sorted) {
-  __shared__ int base[kLocalBins], rank[kLocalBins];
-  const int nb = (int)b.nbins;
-  for (int i = threadIdx.x; i < nb; i += kThreads) base[i] = rank[i] = 0;
+// One wave per run: chunks of 64 atoms in order; an atom goes to its bin's
+// cursor plus its rank among the chunk's atoms of that bin (lanes in order,
+// by ballots over the chunk's distinct keys).
+__global__ __launch_bounds__(64) void bin_place_runs(const SpreadAtom *__restrict__ rec,
+                                                    const unsigned *__restrict__ keys, int natoms, int nbins,
+                                                    int runs, const int *__restrict__ pos,
+                                                    SpreadAtom *__restrict__ sorted) {
+  __shared__ int cur[kLocalBins];
+  const int lane = threadIdx.x, r = blockIdx.x;
+  for (int i = lane; i < nbins; i += 64) cur[i] = pos[(size_t)i * runs + r];
   __syncthreads();
-  const int j0 = blockIdx.x * kBinRun, j1 = min(natoms, j0 + kBinRun);
-  for (int j = j0 + threadIdx.x; j < j1; j += kThreads) {
-    const SpreadAtom r = rec[j];
-    if (r.ch >= 0) atomicAdd(base + home_bin(r, b), 1);
+  const int j0 = r * kBinRun, j1 = min(natoms, j0 + kBinRun);
+  for (int base = j0; base < j1; base += 64) {
+    const int j = base + lane;
+    const unsigned k = j < j1 ? keys[j] : (unsigned)nbins;
+    const bool valid = k < (unsigned)nbins;
+    int rank = 0, count = 0;
+    unsigned long long todo = __ballot(valid);
+    while (todo) {  // one round per distinct key of the chunk
+      const int leader = __ffsll((long long)todo) - 1;
+      const unsigned kl = (unsigned)__shfl((int)k, leader);
+      const unsigned long long m = __ballot(valid && k == kl);
+      if (valid && k == kl) {
+        rank = __popcll(m & ((1ull << lane) - 1ull));
+        count = __popcll(m);
+      }
+      todo &= ~m;
+    }
+    const int p = valid ? cur[k] + rank : 0;
+    __syncthreads();
+    if (valid && rank == 0) cur[k] += count;
+    if (valid) sorted[p] = rec[j];
+    __syncthreads();
   }
-  __syncthreads();
-  for (int i = threadIdx.x; i < nb; i += kThreads)
-    if (base[i]) base[i] = start[i] + atomicAdd(cursor + i, base[i]);
-  __syncthreads();
-  for (int j = j0 + threadIdx.x; j < j1; j += kThreads) {
-    const SpreadAtom r = rec[j];
-    if (r.ch < 0) continue;
-    const int h = (int)home_bin(r, b);
-    sorted[base[h] + atomicAdd(rank + h, 1)] = r;
-  }
+}
+
+
+// After the sort: the records in bin order, and each bin's atoms at
+// [start[bin], end[bin]) from the boundaries of the sorted keys' runs (bins
+// without atoms keep their cleared 0, 0).
+__global__ __launch_bounds__(kThreads) void bin_gather(const SpreadAtom *__restrict__ rec, int natoms, unsigned nbins,
+                                                      const unsigned *__restrict__ keys,
+                                                      const int *__restrict__ index, SpreadAtom *__restrict__ sorted,
+                                                      int *__restrict__ start, int *__restrict__ end) {
+  const int i = blockIdx.x * kThreads + threadIdx.x;
+  if (i >= natoms) return;
+  const unsigned k = keys[i];
+  if (k >= nbins) return;
+  sorted[i] = rec[index[i]];
+  if (i == 0 || keys[i - 1] != k) start[k] = i;
+  if (i == natoms - 1 || keys[i + 1] != k) end[k] = i + 1;
 }
 
 // One workgroup: out[i] = sum_{k < i} in[k] over n entries, in chunks of
@@ -556,18 +590,6 @@ __global__ __launch_bounds__(kThreads) void scan_chunks(const int *__restrict__ 
   }
 }
 
-// sorted[start[bin] + k] = the bin's atoms (k from a per-bin cursor).
-__global__ __launch_bounds__(kThreads) void bin_scatter(const SpreadAtom *__restrict__ rec, int natoms, SpreadBins b,
-                                                       const int *__restrict__ start, int *__restrict__ cursor,
-                                                       SpreadAtom *__restrict__ sorted) {
-  const int j = blockIdx.x * kThreads + threadIdx.x;
-  if (j >= natoms) return;
-  const SpreadAtom r = rec[j];
-  if (r.ch < 0) return;
-  const long long h = home_bin(r, b);
-  sorted[start[h] + atomicAdd(cursor + h, 1)] = r;
-}
-
 // A staged candidate: its window weights (times the deconvolution factor) at
 // the block's kSpreadBlock points per axis (zero outside the window) and its
 // phased strength.
@@ -591,15 +613,15 @@ struct __align__(16) SpreadCand {
 // exp(-2 pi i r.kz (m - u)) of a Gaussian on a z axis centred in frequency).
 template <int W, bool GAUSS, bool CZ>
 __global__ __launch_bounds__(64) void spread_gather(const SpreadAtom *__restrict__ sorted,
-                                                   const int *__restrict__ start, const int *__restrict__ counts,
-                                                   SpreadBins b, NufftGeom g, const float *__restrict__ dec,
-                                                   float2 *__restrict__ grid, size_t grid_stride) {
+                                                   const int *__restrict__ start, const int *__restrict__ end,
+                                                   int bin_stride, SpreadBins b, NufftGeom g,
+                                                   const float *__restrict__ dec, float2 *__restrict__ grid,
+                                                   size_t grid_stride) {
   constexpr int kB = kSpreadBlock;
   constexpr int NB = (W + kB - 2) / kB;
   constexpr int NH = (NB + 1) * (NB + 1) * (NB + 1);
   __shared__ SpreadCand cand[64];
   __shared__ int hs[NH], hp[NH + 1];
-  __shared__ int ncand;
   long long blk = blockIdx.x;
   const int kx = (int)(blk % b.nbx);
   blk /= b.nbx;
@@ -616,8 +638,8 @@ __global__ __launch_bounds__(64) void spread_gather(const SpreadAtom *__restrict
     int st = 0, n = 0;
     if (kx >= dx && ky >= dy && kz >= dz) {
       const long long h = (((long long)ch * b.nbz + kz - dz) * b.nby + ky - dy) * b.nbx + kx - dx;
-      st = start[h];
-      n = counts[h];
+      st = start[h * bin_stride];
+      n = end[h * bin_stride] - st;
     }
     hs[i] = st;
     hp[i + 1] = n;
@@ -633,17 +655,23 @@ __global__ __launch_bounds__(64) void spread_gather(const SpreadAtom *__restrict
 #pragma unroll
   for (int z = 0; z < kB; ++z) ar[z] = ai[z] = 0.0f;
   for (int base = 64 * (int)blockIdx.y; base < total; base += 64 * (int)gridDim.y) {
-    if (lane == 0) ncand = 0;
-    __syncthreads();
     const int v = base + lane;
+    SpreadAtom r;
+    bool keep = false;
     if (v < total) {
       int bi = 0;
       while (v >= hp[bi + 1]) ++bi;
-      const SpreadAtom r = sorted[hs[bi] + v - hp[bi]];
+      r = sorted[hs[bi] + v - hp[bi]];
       const int rw = GAUSS ? r.w : W;
-      if (r.bx + rw > x0 && r.bx < x0 + kB && r.by + rw > y0 && r.by < y0 + kB && r.bz + rw > z0 &&
-          r.bz < z0 + kB) {
-        SpreadCand &c = cand[atomicAdd(&ncand, 1)];
+      keep = r.bx + rw > x0 && r.bx < x0 + kB && r.by + rw > y0 && r.by < y0 + kB && r.bz + rw > z0 &&
+             r.bz < z0 + kB;
+    }
+    // the kept candidates in lane order (the workgroup is one wave)
+    const unsigned long long kept = __ballot(keep);
+    {
+      if (keep) {
+        [[maybe_unused]] const int rw = GAUSS ? r.w : W;
+        SpreadCand &c = cand[__popcll(kept & ((1ull << lane) - 1ull))];
 #pragma unroll
         for (int i = 0; i < kB; ++i) {
           const int ix = x0 + i - r.bx, iy = y0 + i - r.by, iz = z0 + i - r.bz;
@@ -674,7 +702,7 @@ __global__ __launch_bounds__(64) void spread_gather(const SpreadAtom *__restrict
       }
     }
     __syncthreads();
-    const int nc = ncand;
+    const int nc = __popcll(kept);
     for (int a = 0; a < nc; ++a) {
       const float w = cand[a].wx[tx] * cand[a].wy[ty];
       const float vr = cand[a].cr * w, vi = cand[a].ci * w;
@@ -710,8 +738,7 @@ __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(
 // -i * a
 __device__ __forceinline__ float2 mul_mi(float2 a) { return make_float2(a.y, -a.x); }
 
-__device__ __forceinline__ unsigned fdiv(unsigned n, FastDiv f) { return f.m ? __umulhi(n, f.m) : n; }
-// n / d for n < 2^16 by the FP32 form of FastDiv.
+// n / d for n < 2^16 (FastDiv).
 __device__ __forceinline__ int sdiv(int n, const FastDiv &f) { return (int)__builtin_fmaf((float)n, f.fi, f.fh); }
 // n - (n / d) d for n < 2^16.
 __device__ __forceinline__ int smod(int n, const FastDiv &f) { return n - (int)__umul24((unsigned)sdiv(n, f), f.d); }
@@ -1248,8 +1275,8 @@ constexpr FixedLengths list_fixed_lengths() {
 }
 constexpr FixedLengths kFixed = list_fixed_lengths();
 
-// q = n / D and n - q D for 0 <= n < 2^16 (the FP32 division of FastDiv with
-// constant operands).
+// q = n / D for 0 <= n < 2^16 (the FP32 division of FastDiv with constant
+// operands).
 template <int D>
 __device__ __forceinline__ int cdiv(int n) {
   return (int)__builtin_fmaf((float)n, 1.0f / D, 0.5f / D);
@@ -1725,7 +1752,6 @@ int radices(int n, int *r) {
 FastDiv fast_div(unsigned d) {
   FastDiv f;
   f.d = d;
-  f.m = d <= 1 ? 0u : (unsigned)((1ull << 32) / d + 1);
   f.fi = d ? (float)(1.0 / (double)d) : 0.0f;
   f.fh = 0.5f * f.fi;
   return f;
@@ -1957,11 +1983,11 @@ int launch_fft(const FftPass &pass, const float2 *in, float2 *out, const float2 
 
 template <int W, bool GAUSS, bool CZ>
 int launch_spread(const NufftGeom &g, const SpreadAtom *rec, const SpreadBins &bins, const int *start,
-                  const int *counts, int nsplit, size_t grid_stride, const float *dec, float2 *grid,
+                  const int *end, int bin_stride, int nsplit, size_t grid_stride, const float *dec, float2 *grid,
                   hipStream_t stream) {
   if (bins.nbins > 0x7fffffffLL) return fail(SPSIM_HIP_ERR_INVALID, "too many spread blocks");
   hipLaunchKernelGGL((spread_gather<W, GAUSS, CZ>), dim3((unsigned)bins.nbins, (unsigned)nsplit), dim3(64), 0, stream,
-                     rec, start, counts, bins, g, dec, grid, grid_stride);
+                     rec, start, end, bin_stride, bins, g, dec, grid, grid_stride);
   SPSIM_HIP_TRY(hipGetLastError());
   return SPSIM_HIP_OK;
 }
@@ -1992,11 +2018,11 @@ constexpr int kGaussWidths[] = {17, 25, 33};
 
 // gauss: Gaussian windows; cz: with complex z weights (z centred in frequency).
 int dispatch_spread(int W, bool gauss, bool cz, const NufftGeom &g, const SpreadAtom *rec, const SpreadBins &bins,
-                    const int *start, const int *counts, int nsplit, size_t grid_stride, const float *dec,
-                    float2 *grid, hipStream_t s) {
+                    const int *start, const int *end, int bin_stride, int nsplit, size_t grid_stride,
+                    const float *dec, float2 *grid, hipStream_t s) {
 #define SPSIM_SPREAD_W(w, ga, c) \
   case w:                        \
-    return launch_spread<w, ga, c>(g, rec, bins, start, counts, nsplit, grid_stride, dec, grid, s)
+    return launch_spread<w, ga, c>(g, rec, bins, start, end, bin_stride, nsplit, grid_stride, dec, grid, s)
   if (gauss && cz) {
     switch (W) {
       SPSIM_SPREAD_W(17, true, true);
@@ -2318,9 +2344,29 @@ int nufft_run(double *hb, const float *d_hkl, int npix, const spsim_hip_atom *d_
     if (nsplit < 1) nsplit = 1;
   }
   const size_t b_part = nsplit > 1 ? align256(sizeof(float2) * nG * nsplit) : 0;
-  const long long nchunks = (bins.nbins + kScanChunk - 1) / kScanChunk;
+  // the binning: keys and atom indices; for few bins the (bin, run) counts,
+  // their scan and the scan's chunk sums; else the sorted keys and indices,
+  // the sort's scratch and the bins' [start, end)
+  if (bins.nbins >= 0x7fffffffLL) return fail(SPSIM_HIP_ERR_INVALID, "too many spread blocks");
+  const bool local_bins = bins.nbins <= kLocalBins;
+  const long long runs = (natoms + kBinRun - 1) / kBinRun;
+  const long long ncells = local_bins ? bins.nbins * runs + 1 : 0;
+  const long long nchunks = (ncells + kScanChunk - 1) / kScanChunk;
+  if (ncells > 0x7fffffffLL) return fail(SPSIM_HIP_ERR_INVALID, "too many atoms for the spread's bins");
+  const unsigned key_bits = 32u - (unsigned)__builtin_clz((unsigned)bins.nbins);  // keys 0 .. nbins
+  const size_t b_key = align256(sizeof(unsigned) * (size_t)natoms);
+  size_t sort_bytes = 0;
+  if (!local_bins)
+    SPSIM_HIP_TRY(rocprim::radix_sort_pairs<BinSortConfig>(nullptr, sort_bytes, (const unsigned *)nullptr,
+                                                           (unsigned *)nullptr, (const int *)nullptr, (int *)nullptr,
+                                                           natoms, 0u, key_bits, s));
+  const size_t b_key2 = local_bins ? 0 : b_key;
+  const size_t b_sort = local_bins ? 0 : align256(sort_bytes);
+  const size_t b_rng = local_bins ? 0 : b_bin;
+  const size_t b_cell = align256(sizeof(int) * (size_t)ncells);
   const size_t b_scan = align256(sizeof(int) * (size_t)nchunks);
-  const size_t b_all = b_dec + 2 * b_rec + 3 * b_bin + b_tw + b_rot + b1 + b2 + b_part + 2 * b_scan;
+  const size_t b_all = b_dec + 2 * b_rec + 2 * b_rng + 2 * b_key + 2 * b_key2 + b_sort + 2 * b_cell + 2 * b_scan +
+                       b_tw + b_rot + b1 + b2 + b_part;
   if (need) {
     *need = b_all;
     return SPSIM_HIP_OK;
@@ -2336,10 +2382,21 @@ int nufft_run(double *hb, const float *d_hkl, int npix, const spsim_hip_atom *d_
   wp += b_rec;
   SpreadAtom *sorted = (SpreadAtom *)wp;
   wp += b_rec;
-  int *counts = (int *)wp;  // counts and cursor are adjacent, cleared together
-  int *cursor = (int *)(wp + b_bin);
-  int *start = (int *)(wp + 2 * b_bin);
-  wp += 3 * b_bin;
+  int *bin_end = (int *)wp;  // end and start are adjacent, cleared together
+  int *bin_start = (int *)(wp + b_rng);
+  wp += 2 * b_rng;
+  unsigned *keys = (unsigned *)wp;
+  int *index = (int *)(wp + b_key);
+  wp += 2 * b_key;
+  unsigned *keys_sorted = (unsigned *)wp;
+  int *index_sorted = (int *)(wp + b_key2);
+  void *sort_tmp = wp + 2 * b_key2;
+  wp += 2 * b_key2 + b_sort;
+  int *cnt = (int *)wp;
+  int *pos = (int *)(wp + b_cell);
+  int *csum = (int *)(wp + 2 * b_cell);
+  int *coff = (int *)(wp + 2 * b_cell + b_scan);
+  wp += 2 * b_cell + 2 * b_scan;
   float2 *tw = (float2 *)wp;
   wp += b_tw;
   double *d_rots = npat > 0 ? (double *)wp : nullptr;
@@ -2347,15 +2404,13 @@ int nufft_run(double *hb, const float *d_hkl, int npix, const spsim_hip_atom *d_
   float2 *buf1 = (float2 *)wp;
   float2 *buf2 = (float2 *)(wp + b1);
   float2 *gpart = nsplit > 1 ? (float2 *)(wp + b1 + b2) : nullptr;
-  int *csum = (int *)(wp + b1 + b2 + b_part);
-  int *coff = (int *)(wp + b1 + b2 + b_part + b_scan);
   if (npat > 0) {
     // pageable source: the copy has completed when the call returns
     SPSIM_HIP_TRY(hipMemcpyAsync(d_rots, rots, sizeof(double) * 9 * (size_t)npat, hipMemcpyHostToDevice, s));
   }
 
-  // 3. Tables (deconvolution, twiddles, cleared bin counters), atom records
-  // binned by spread block, spread.
+  // 3. Tables (deconvolution, twiddles, cleared bin ranges), atom records
+  // binned by spread block (stable sort by home block), spread.
   float2 *twx = tw, *twy = tw + X.nf, *twz = tw + X.nf + Y.nf;
   {
     int most_t = X.M > X.nf ? X.M : X.nf;
@@ -2363,41 +2418,47 @@ int nufft_run(double *hb, const float *d_hkl, int npix, const spsim_hip_atom *d_
       if (D->M > most_t) most_t = D->M;
       if (D->nf > most_t) most_t = D->nf;
     }
-    const size_t n4 = 2 * b_bin / 16;
+    const size_t n4 = 2 * b_rng / 16;
     size_t tb = (size_t)(most_t + kThreads - 1) / kThreads;
     const size_t zb = (n4 + kThreads - 1) / kThreads;
     if (zb > tb) tb = zb < 4096 ? zb : 4096;
     const unsigned pblocks = (unsigned)((natoms + kThreads - 1) / kThreads);
     hipLaunchKernelGGL(tables_and_prep, dim3((unsigned)tb + pblocks), dim3(kThreads), 0, s, g, W2, dec, tw,
-                       (float4 *)counts, n4, (int)tb, d_atoms, natoms, d_segs, nsegs, R, Wsp, d_atom_b, gs, rec);
+                       (float4 *)bin_end, n4, (int)tb, d_atoms, natoms, d_segs, nsegs, R, Wsp, d_atom_b, gs, rec,
+                       bins, keys, index);
     SPSIM_HIP_TRY(hipGetLastError());
   }
-  const unsigned ablocks = (unsigned)((natoms + kThreads - 1) / kThreads);
-  const bool local_bins = bins.nbins <= kLocalBins;
-  const unsigned runs = (unsigned)((natoms + kBinRun - 1) / kBinRun);
-  if (local_bins)
-    hipLaunchKernelGGL(bin_count_local, dim3(runs), dim3(kThreads), 0, s, rec, natoms, bins, counts);
-  else
-    hipLaunchKernelGGL(bin_count, dim3(ablocks), dim3(kThreads), 0, s, rec, natoms, bins, counts);
-  SPSIM_HIP_TRY(hipGetLastError());
-  if (nchunks <= 1) {
-    hipLaunchKernelGGL(scan_exclusive, dim3(1), dim3(kThreads), 0, s, counts, bins.nbins, start);
+  const bool cz = gauss && g.d[2].cs != 0.0;
+  float2 *spread_out = nsplit > 1 ? gpart : buf1;
+  if (local_bins) {
+    const int nb = (int)bins.nbins, nr = (int)runs;
+    hipLaunchKernelGGL(bin_count_runs, dim3((unsigned)runs), dim3(kThreads), 0, s, keys, natoms, nb, nr, cnt);
+    SPSIM_HIP_TRY(hipGetLastError());
+    if (nchunks <= 1) {
+      hipLaunchKernelGGL(scan_exclusive, dim3(1), dim3(kThreads), 0, s, cnt, ncells, pos);
+    } else {
+      hipLaunchKernelGGL(chunk_sums, dim3((unsigned)nchunks), dim3(kThreads), 0, s, cnt, ncells, csum);
+      SPSIM_HIP_TRY(hipGetLastError());
+      hipLaunchKernelGGL(scan_exclusive, dim3(1), dim3(kThreads), 0, s, csum, nchunks, coff);
+      SPSIM_HIP_TRY(hipGetLastError());
+      hipLaunchKernelGGL(scan_chunks, dim3((unsigned)nchunks), dim3(kThreads), 0, s, cnt, ncells, coff, pos);
+    }
+    SPSIM_HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(bin_place_runs, dim3((unsigned)runs), dim3(64), 0, s, rec, keys, natoms, nb, nr, pos, sorted);
+    SPSIM_HIP_TRY(hipGetLastError());
+    // bin h: [pos[h runs], pos[(h + 1) runs])
+    SPSIM_TRY(dispatch_spread(Wsp, gauss, cz, g, sorted, bins, pos, pos + runs, nr, nsplit, nG, dec, spread_out, s));
   } else {
-    if (nchunks > 0x7fffffffLL) return fail(SPSIM_HIP_ERR_INVALID, "too many spread blocks");
-    hipLaunchKernelGGL(chunk_sums, dim3((unsigned)nchunks), dim3(kThreads), 0, s, counts, bins.nbins, csum);
+    const unsigned ablocks = (unsigned)((natoms + kThreads - 1) / kThreads);
+    size_t sort_have = sort_bytes;
+    SPSIM_HIP_TRY(rocprim::radix_sort_pairs<BinSortConfig>(sort_tmp, sort_have, keys, keys_sorted, index,
+                                                           index_sorted, natoms, 0u, key_bits, s));
+    hipLaunchKernelGGL(bin_gather, dim3(ablocks), dim3(kThreads), 0, s, rec, natoms, (unsigned)bins.nbins,
+                       keys_sorted, index_sorted, sorted, bin_start, bin_end);
     SPSIM_HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(scan_exclusive, dim3(1), dim3(kThreads), 0, s, csum, nchunks, coff);
-    SPSIM_HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(scan_chunks, dim3((unsigned)nchunks), dim3(kThreads), 0, s, counts, bins.nbins, coff, start);
+    SPSIM_TRY(dispatch_spread(Wsp, gauss, cz, g, sorted, bins, bin_start, bin_end, 1, nsplit, nG, dec, spread_out,
+                              s));
   }
-  SPSIM_HIP_TRY(hipGetLastError());
-  if (local_bins)
-    hipLaunchKernelGGL(bin_scatter_local, dim3(runs), dim3(kThreads), 0, s, rec, natoms, bins, start, cursor, sorted);
-  else
-    hipLaunchKernelGGL(bin_scatter, dim3(ablocks), dim3(kThreads), 0, s, rec, natoms, bins, start, cursor, sorted);
-  SPSIM_HIP_TRY(hipGetLastError());
-  SPSIM_TRY(dispatch_spread(Wsp, gauss, gauss && g.d[2].cs != 0.0, g, sorted, bins, start, counts, nsplit, nG, dec,
-                            nsplit > 1 ? gpart : buf1, s));
 
   // 4. Pruned FFT along x, y, z: G (buf1, or the sum of the nsplit partial
   // grids, added up as the x pass loads them) -> A (buf2) -> B (buf1) -> H (buf2).

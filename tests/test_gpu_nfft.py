@@ -489,6 +489,45 @@ def test_nfft_per_atom_b_rotation_volume_and_high_b(gpu):
     assert cplx_err(Fh[:, 0] + 1j * Fh[:, 1], F_ref) < 1e-5, info
 
 
+def test_nfft_results_repeat_bit_for_bit(gpu):
+    """The NUFFT's spread binning is a stable sort and its candidate lists keep
+    lane order, so repeated calls on the same input give the same bits: a large
+    grid (more than 8192 spread blocks), a small one (each block's atoms split
+    over several workgroups whose partial grids the first FFT pass adds), per-atom
+    B factors (Gaussian spreading) and the many-molecule call."""
+    import torch
+
+    ff = dv.form_factor_table(device=gpu)
+    rng = np.random.default_rng(71)
+    Zl, posl, _ = wl.random_molecule(30000, 12e-9, rng)
+    ql = dv.hkl_detector(1024, 1024, 1024 * 75e-6, 1024 * 75e-6, 0.1, 0.15e-9, device=gpu)
+    Zs, poss = molecule(2000, 3e-9, seed=72, z0=0)
+    qs = torch.from_numpy(np.ascontiguousarray(detector_q(40), np.float32)).to(gpu)
+    Bs = rng.uniform(10, 40, len(Zs)).astype(np.float32)
+    cases = [("large grid", ql, dv.DeviceAtoms(dv.pack_atoms(Zl, posl, mode=_lib.PACK_NFFT), gpu)),
+             ("small grid", qs, dv.DeviceAtoms(dv.pack_atoms(Zs, poss, mode=_lib.PACK_NFFT), gpu)),
+             ("per-atom B", qs, dv.DeviceAtoms(dv.pack_atoms(Zs, poss, atom_b=Bs), gpu))]
+    for name, q, atoms in cases:
+        runs = []
+        for _ in range(3):
+            F, _, info = dv.structure_factor_nfft(q, atoms, ff)
+            torch.cuda.synchronize()
+            runs.append(F.cpu().numpy().copy())
+        if name == "large grid":
+            blocks = np.prod([(m + 7) // 8 for m in info["M"]])
+            assert blocks * 4 > 8192, info
+        assert all(np.array_equal(runs[0], r) for r in runs[1:]), name
+    mols = [molecule(n, r, seed, z0=0) for n, r, seed in ((500, 2e-9, 73), (1500, 4e-9, 74), (60, 1e-9, 75))]
+    dm = dv.DeviceMolecules([dv.pack_atoms(Z, pos) for Z, pos in mols], gpu)
+    rots = wl.uniform_rotations(len(mols), np.random.default_rng(76))
+    runs = []
+    for _ in range(3):
+        F, _ = dv.structure_factor_nfft_molecules(qs, dm, ff, rots)
+        torch.cuda.synchronize()
+        runs.append(F.cpu().numpy().copy())
+    assert all(np.array_equal(runs[0], r) for r in runs[1:]), "molecules"
+
+
 def test_nfft_many_molecules_in_one_call(gpu):
     """spsim_hip_sf_nfft_molecules: molecules of uneven sizes (one empty), each at
     its own rotation, with one synchronisation for all their bounds. There are
