@@ -392,7 +392,8 @@ int spsim_hip_detector_chain(const spsim_hip_detector *det, const float *d_I, in
    pixels on the reference's Gaussian branch (mean >= 1000, src/noise.c:44-46)
    keep their own draw. The binned real output therefore has the distribution
    of the per-pixel chain's, not its values; the noiseless output is bitwise
-   the per-pixel chain's. */
+   the per-pixel chain's. With d_real_output, uses the library workspace (one
+   bit per bin and pattern marks the bins its second pass recomputes). */
 int spsim_hip_detector_chain_binned(const spsim_hip_detector *det, const float *d_I, int npatterns, uint64_t seed,
                                     uint64_t pattern_base, float *d_real_output, float *d_noiseless_output,
                                     void *stream);

@@ -163,7 +163,10 @@ __global__ __launch_bounds__(kT) void chain_binned_runs_kernel(DetConst d, const
   const int nraw = d.bin_x * d.bin_y * d.bin_z;
   const long long nb = (long long)nbx * nby * nbz;
   const long long gid = (long long)blockIdx.x * kT + threadIdx.x;
-  if (gid >= nb) return;  // lanes use only their own LDS slots: no barrier
+  // lanes use only their own LDS slots: no barrier; a wave's first lane has a
+  // bin whenever any lane does (kT is a multiple of 64), so it writes the word
+  if (gid >= nb) return;
+  const long long nw = (nb + 63) / 64;
   const int bin = (int)gid;
   const int bx = bin % nbx, by = (bin / nbx) % nby, bz = bin / (nbx * nby);
   {
@@ -224,9 +227,10 @@ __global__ __launch_bounds__(kT) void chain_binned_runs_kernel(DetConst d, const
 //
 // Two passes, so that the common pass carries no register-hungry branch:
 // chain_bin_draws_kernel finishes every bin whose summed mean is below 10 and
-// that has no Gaussian pixel, and marks the others' real output with a NaN;
-// chain_bin_fixup_kernel recomputes the marked bins with PTRS and the
-// Gaussian branch. Each lane of the first pass keeps the Thomson factor x
+// that has no Gaussian pixel, and marks the others: one bit per bin in a word
+// per wave and pattern (marks[p][bin / 64], from a ballot), and a NaN in
+// their real output; chain_bin_fixup_kernel reads the words and recomputes the
+// marked bins with PTRS and the Gaussian branch. Each lane of the first pass keeps the Thomson factor x
 // solid angle of its bin's raw pixels in LDS for a run of patterns; VEC4
 // reads a bin row of 4 pixels as one 16-byte load (bin_x = 4, nx a multiple
 // of 4, I 16-byte aligned), B44 fixes 4 x 4 x 1 bins at compile time.
@@ -235,7 +239,8 @@ constexpr int kBinDrawMaxRaw = 64;  // raw pixels per bin kept in LDS (64 KiB pe
 template <bool VEC4, bool B44, bool GEO_LDS>
 __global__ __launch_bounds__(kT) void chain_bin_draws_kernel(DetConst d, const float *__restrict__ I, int npatterns,
                                                              int run, uint64_t seed, uint64_t pattern_base,
-                                                             float *__restrict__ real, float *__restrict__ noiseless) {
+                                                             float *__restrict__ real, float *__restrict__ noiseless,
+                                                             unsigned long long *__restrict__ marks) {
 #pragma clang fp contract(off)
   extern __shared__ float thom[];
   if constexpr (B44) {
@@ -247,7 +252,10 @@ __global__ __launch_bounds__(kT) void chain_bin_draws_kernel(DetConst d, const f
   const int nraw = d.bin_x * d.bin_y * d.bin_z;
   const long long nb = (long long)nbx * nby * nbz;
   const long long gid = (long long)blockIdx.x * kT + threadIdx.x;
-  if (gid >= nb) return;  // lanes use only their own LDS slots: no barrier
+  // lanes use only their own LDS slots: no barrier; a wave's first lane has a
+  // bin whenever any lane does (kT is a multiple of 64), so it writes the word
+  if (gid >= nb) return;
+  const long long nw = (nb + 63) / 64;
   const int bin = (int)gid;
   const int bx = bin % nbx, by = (bin / nbx) % nby, bz = bin / (nbx * nby);
   constexpr bool in_lds = GEO_LDS;  // nraw <= kBinDrawMaxRaw
@@ -302,8 +310,11 @@ __global__ __launch_bounds__(kT) void chain_bin_draws_kernel(DetConst d, const f
     const size_t ob = (size_t)p * nb + bin;
     if (noiseless) noiseless[ob] = accn;
     if (!real) continue;
-    if (peak >= 1000.0f || lam >= 10.0f) {
-      real[ob] = __builtin_nanf("");  // for chain_bin_fixup_kernel
+    const bool marked = peak >= 1000.0f || lam >= 10.0f;
+    const unsigned long long m = __ballot(marked);
+    if (threadIdx.x % 64 == 0) marks[(size_t)p * nw + gid / 64] = m;
+    if (marked) {
+      real[ob] = __builtin_nanf("");  // recomputed by chain_bin_fixup_kernel
       continue;
     }
     const U4 r = draw(seed, (uint32_t)bin, pat, kRngPoissonBin, 0u);
@@ -317,30 +328,25 @@ __global__ __launch_bounds__(kT) void chain_bin_draws_kernel(DetConst d, const f
 }
 
 // Second pass: the bins the first pass marked (summed mean >= 10 or a
-// Gaussian pixel), with every branch. Each lane scans 16 consecutive entries
-// of the [npatterns][nb] real output (16 loads in flight) and recomputes the
-// marked ones.
-constexpr int kFixupPerLane = 16;
+// Gaussian pixel), with every branch. One lane per mark word (nw words of 64
+// bins per pattern, npatterns x nw in all) recomputes the bins whose bits are
+// set.
 __device__ __noinline__ void fixup_bin(DetConst d, const float *__restrict__ I, long long nb, long long gid,
                                        uint64_t seed, uint64_t pattern_base, float *__restrict__ real);
 
-// nb binned pixels per pattern, total = nb x npatterns entries.
 __global__ __launch_bounds__(kT) void chain_bin_fixup_kernel(DetConst d, const float *__restrict__ I, long long nb,
-                                                             long long total, uint64_t seed, uint64_t pattern_base,
+                                                             long long nw, long long nwords, uint64_t seed,
+                                                             uint64_t pattern_base,
+                                                             const unsigned long long *__restrict__ marks,
                                                              float *__restrict__ real) {
-  const long long base = ((long long)blockIdx.x * kT + threadIdx.x) * kFixupPerLane;
-  if (base >= total) return;
-  float v[kFixupPerLane];
-#pragma unroll
-  for (int k = 0; k < kFixupPerLane; ++k) v[k] = real[min(base + k, total - 1)];  // all loads in flight at once
-  unsigned marked = 0;
-#pragma unroll
-  for (int k = 0; k < kFixupPerLane; ++k)
-    if (base + k < total && __builtin_isnan(v[k])) marked |= 1u << k;
-  while (marked) {
-    const int k = __builtin_ctz(marked);
-    marked &= marked - 1;
-    fixup_bin(d, I, nb, base + k, seed, pattern_base, real);
+  const long long i = (long long)blockIdx.x * kT + threadIdx.x;
+  if (i >= nwords) return;
+  unsigned long long m = marks[i];
+  const long long p = i / nw, first = (i - p * nw) * 64;
+  while (m) {
+    const int k = __builtin_ctzll(m);
+    m &= m - 1;
+    fixup_bin(d, I, nb, p * nb + first + k, seed, pattern_base, real);
   }
 }
 
@@ -596,9 +602,17 @@ int launch_chain_bin_draws(const DetConst &d, const float *I, int npatterns, uin
   const bool b44 = vec4 && d.bin_y == 4 && d.bin_z == 1;
   const bool geo_lds = nraw <= kBinDrawMaxRaw;
   const dim3 grid((unsigned)gx, (unsigned)groups);
+  // the mark words (real output only): from the library workspace
+  const long long nw = (nb + 63) / 64, nwords = nw * npatterns;
+  unsigned long long *marks = nullptr;
+  if (real) {
+    void *ws = nullptr;
+    SPSIM_TRY(workspace(sizeof(unsigned long long) * (size_t)nwords, &ws));
+    marks = (unsigned long long *)ws;
+  }
 #define SPSIM_BIN_DRAWS(V, B, G)                                                                                \
   hipLaunchKernelGGL((chain_bin_draws_kernel<V, B, G>), grid, dim3(kT), G ? lds : 0, s, d, I, npatterns, run, \
-                     seed, pattern_base, real, noiseless)
+                     seed, pattern_base, real, noiseless, marks)
   if (b44)
     SPSIM_BIN_DRAWS(true, true, true);
   else if (vec4 && geo_lds)
@@ -612,9 +626,8 @@ int launch_chain_bin_draws(const DetConst &d, const float *I, int npatterns, uin
 #undef SPSIM_BIN_DRAWS
   SPSIM_HIP_TRY(hipGetLastError());
   if (real) {
-    const long long lanes = (nb * npatterns + kFixupPerLane - 1) / kFixupPerLane;
-    hipLaunchKernelGGL(chain_bin_fixup_kernel, dim3(blocks_for(lanes)), dim3(kT), 0, s, d, I, nb, nb * npatterns,
-                       seed, pattern_base, real);
+    hipLaunchKernelGGL(chain_bin_fixup_kernel, dim3(blocks_for(nwords)), dim3(kT), 0, s, d, I, nb, nw, nwords, seed,
+                       pattern_base, marks, real);
     SPSIM_HIP_TRY(hipGetLastError());
   }
   return SPSIM_HIP_OK;
