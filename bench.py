@@ -61,8 +61,11 @@ W_TEXTBOOK_DW = 36.0
 # back edge; v_sin/v_cos/v_exp counted once each) and checked against the
 # SQ_INSTS_VALU counter of profiled launches. Keys: (family, per-atom B).
 #   octant: 8 pixels per lane (5 phases: 10 sin/cos), quad1/quad2: 1 or 2
-#   mirror quads per lane (3 phases per quad), pixelK: K pixels per lane.
+#   mirror quads per lane (3 phases per quad), pixelK: K pixels per lane,
+#   spectrum2: the wavelength-spectrum kernel at 2 pixels per lane when every
+#   lane takes the fitted sample factor (|s_c q.r| within its range).
 VALU_PER_WAVE_ATOM = {
+    ("spectrum2", False): 62, ("spectrum2", True): 66,
     ("octant", False): 46, ("octant", True): 48,
     ("quad1", False): 27, ("quad1", True): 29,
     ("quad2", False): 53, ("quad2", True): 57,
@@ -880,7 +883,8 @@ def spectrum(args, torch, dv, wl, _lib):
     ff = dv.form_factor_table(device=dev)
     sc, w = dv.spectrum_samples(det.wavelength, 0.02, 5)
     I = torch.empty(det.npix, dtype=torch.float32, device=dev)
-    run = lambda: dv.structure_factor_spectrum(hkl, atoms, ff, sc, w, I=I, want_F=False)
+    tile = dv.make_tile(2, 256, 0)  # the library's default for this list
+    run = lambda: dv.structure_factor_spectrum(hkl, atoms, ff, sc, w, I=I, want_F=False, tile=tile)
     run()
     torch.cuda.synchronize()
     ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(3)]
@@ -892,6 +896,20 @@ def spectrum(args, torch, dv, wl, _lib):
     ms = float(np.median([a.elapsed_time(b) for a, b in ev]))
     ts = float(det.npix) * atoms.natoms * len(sc)
     w_eq = 20.0 + 10.0 / len(sc)  # per term-sample: sincos + 2 FMA, plus the per-term phase and sum
+    # every lane takes the fitted sample factor when |s_c q.r| <= tlim =
+    # 2 / (2 pi max|s_k / s_c - 1|), s_c the midpoint of the scales
+    s_c = 0.5 * (min(sc) + max(sc))
+    tlim = 2.0 / (2 * math.pi * max(abs(x / s_c - 1) for x in sc))
+    t_max = s_c * torch.linalg.vector_norm(hkl.double(), dim=1).max().item() * float(np.max(np.linalg.norm(pos, axis=1)))
+    roof = None
+    if t_max <= tlim:
+        wave_atoms = -(-det.npix // (2 * LANES)) * atoms.natoms
+        roof = executed_roofline("spectrum2", False, wave_atoms, ms, ts / len(sc), "sf_spectrum_kernel<2, 256, false>")
+        roof["textbook"] = {
+            "w_eq_per_term_sample": w_eq, "term_samples_per_launch": ts,
+            "frac_of_peak": round(ts * w_eq / (ms * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, 4),
+            "note": "a sine and cosine per wavelength sample; the kernel evaluates the sample sum as one "
+                    "sine and cosine times a fitted FP32 polynomial, so this figure is not bounded by the peak"}
     err = None
     try:
         import oracle
@@ -923,8 +941,9 @@ def spectrum(args, torch, dv, wl, _lib):
                     f"{len(sc)} wavelength samples over a 2 % bandwidth (amplitude average, reference GPU semantics)",
         "kernel": "sf_spectrum_kernel", "ms": round(ms, 3),
         "gterm_samples_per_s": round(ts / (ms * 1e-3) / 1e9, 1),
-        "roofline_frac": round(ts * w_eq / (ms * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, 4),
-        "w_eq_flops_per_term_sample": w_eq,
+        "roofline": roof,
+        "roofline_frac": roof["frac"] if roof else None,
+        "max_phase_turns": round(t_max, 2), "fit_range_turns": round(tlim, 2),
         "rel_err_vs_fp64": err,
         "nufft": {"entry": "spsim_hip_sf_spectrum_nfft", "eps": 1e-6, "ms": round(ms_n, 3),
                   "speedup_vs_direct": round(ms / ms_n, 2), "rel_err_vs_direct_whole_pattern": err_n_direct},

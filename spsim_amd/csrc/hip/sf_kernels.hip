@@ -24,6 +24,7 @@
 #include "form_factor.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <mutex>
 
 namespace spsim {
@@ -161,9 +162,17 @@ __device__ __forceinline__ void tile_sum(const double (&qx)[K], const double (&q
 // amplitude over wavelength samples: each atom contributes
 // f(|q|) w_j sum_k W_k e^{-2 pi i s_k q.r_j} with phase scales s_k = lambda/lambda_k
 // and normalised weights W_k (the form factor stays at the unscaled |q|). The
-// FP64 phase q.r is formed once per atom and pixel and scaled by each s_k in
-// FP64 before the reduction to turns; the weighted sample sum is then folded
-// into the atom's amplitude like a per-atom weight.
+// FP64 phase q.r is formed once per atom and pixel. With the centre scale sc,
+// t = sc q.r and e_k = s_k / sc - 1, the sample sum factors as
+//   sum_k W_k e^{-2 pi i s_k q.r} = e^{-2 pi i t} G(t),
+//   G(t) = sum_k W_k e^{-2 pi i e_k t},
+// where only the first factor needs the FP64 reduction of t to turns. G is a
+// smooth function of t alone (|e_k| is about half the bandwidth), evaluated as
+// two FP32 polynomials in (t / tlim)^2 fitted on the host (spectrum_fit), so a
+// term costs one sine and cosine whatever the number of samples. Lanes with
+// |t| > tlim, and spectra of fewer than 4 samples, take the per-sample sum:
+// each s_k q.r formed and reduced to turns in FP64. The weighted sample sum is
+// then folded into the atom's amplitude like a per-atom weight.
 template <int K, int CHUNK, bool HAS_B>
 __device__ __forceinline__ void spectrum_sum(const double (&qx)[K], const double (&qy)[K], const double (&qz)[K],
                                              const float (&s2)[K], const spsim_hip_atom *__restrict__ atoms,
@@ -206,11 +215,30 @@ __device__ __forceinline__ void spectrum_sum(const double (&qx)[K], const double
 #pragma unroll
         for (int k = 0; k < K; ++k) {
           const double t = __builtin_fma(qz[k], a.z, __builtin_fma(qy[k], a.y, qx[k] * a.x));
+          const double tc = sp.sc * t;
+          const float tf = (float)tc;
           float ur = 0.0f, ui = 0.0f;
-          for (int m = 0; m < ns; ++m) {
-            const float ph = (float)__builtin_amdgcn_fract(sp.s[m] * t);
-            ur = __builtin_fmaf(sp.w[m], __builtin_amdgcn_cosf(ph), ur);
-            ui = __builtin_fmaf(sp.w[m], __builtin_amdgcn_sinf(ph), ui);
+          if (__builtin_fabsf(tf) <= sp.tlim) {
+            const float ph = (float)__builtin_amdgcn_fract(tc);
+            const float cs = __builtin_amdgcn_cosf(ph);
+            const float sn = __builtin_amdgcn_sinf(ph);
+            const float v = tf * sp.inv_tlim;
+            const float u = v * v;
+            float gc = sp.p[kSpectrumDegree], gq = sp.q[kSpectrumDegree];
+#pragma unroll
+            for (int d = kSpectrumDegree - 1; d >= 0; --d) {
+              gc = __builtin_fmaf(gc, u, sp.p[d]);
+              gq = __builtin_fmaf(gq, u, sp.q[d]);
+            }
+            const float gs = v * gq;
+            ur = __builtin_fmaf(cs, gc, -sn * gs);
+            ui = __builtin_fmaf(sn, gc, cs * gs);
+          } else {
+            for (int m = 0; m < ns; ++m) {
+              const float ph = (float)__builtin_amdgcn_fract(sp.s[m] * t);
+              ur = __builtin_fmaf(sp.w[m], __builtin_amdgcn_cosf(ph), ur);
+              ui = __builtin_fmaf(sp.w[m], __builtin_amdgcn_sinf(ph), ui);
+            }
           }
           float wt;
           if constexpr (HAS_B) {
@@ -1008,6 +1036,77 @@ int launch_spectrum(const PatternLaunch &a, const Spectrum &sp, const spsim_hip_
   return launch_single(Kind::kSpectrum, a, 0, 0, t, splits, s, &sp);
 }
 
+// Interpolation in u = v^2 on [0, 1] at the kSpectrumDegree + 1 Chebyshev-
+// Lobatto points (u = 0 among them, so P(0) = sum_k W_k and the small phases
+// of most terms carry no bias) of
+//   P(u) = sum_k W_k cos(a_k v),  Q(u) = sum_k W_k sin(a_k v) / v  (a_k at v = 0),
+// a_k = 2 pi e_k tlim, both entire in u; the monomial coefficients come from
+// the Vandermonde system (Gaussian elimination with partial pivoting). With
+// tlim = kSpectrumReach / (2 pi max|e_k|), the FP32 polynomials stay within
+// 2e-7 of the sample sums over |t| <= tlim, and within 6e-8 over the first
+// third of that range.
+void spectrum_fit(Spectrum &sp) {
+  constexpr int N = kSpectrumDegree + 1;
+  constexpr double kTwoPi = 6.283185307179586476925;
+  for (int d = 0; d < N; ++d) sp.p[d] = sp.q[d] = 0.0f;
+  sp.sc = 1.0;
+  sp.tlim = -1.0f;
+  sp.inv_tlim = 0.0f;
+  if (sp.n < 4) return;
+  double smin = sp.s[0], smax = sp.s[0];
+  for (int k = 1; k < sp.n; ++k) {
+    smin = std::min(smin, sp.s[k]);
+    smax = std::max(smax, sp.s[k]);
+  }
+  const double sc = 0.5 * (smin + smax);
+  if (!(sc > 0.0) || !std::isfinite(sc)) return;
+  double emax = 0.0;
+  for (int k = 0; k < sp.n; ++k) emax = std::max(emax, std::fabs(sp.s[k] / sc - 1.0));
+  const double tlim = emax > 0.0 ? std::min(1e30, kSpectrumReach / (kTwoPi * emax)) : 1e30;
+  double A[N][N + 2];
+  for (int j = 0; j < N; ++j) {
+    const double u = j == N - 1 ? 0.0 : 0.5 * (1.0 + std::cos(M_PI * j / (N - 1)));
+    const double v = std::sqrt(u);
+    double pc = 0.0, qc = 0.0;
+    for (int k = 0; k < sp.n; ++k) {
+      const double ak = kTwoPi * (sp.s[k] / sc - 1.0) * tlim;
+      pc += (double)sp.w[k] * std::cos(ak * v);
+      qc += (double)sp.w[k] * (v > 0.0 ? std::sin(ak * v) / v : ak);
+    }
+    double ud = 1.0;
+    for (int d = 0; d < N; ++d, ud *= u) A[j][d] = ud;
+    A[j][N] = pc;
+    A[j][N + 1] = qc;
+  }
+  for (int c = 0; c < N; ++c) {
+    int piv = c;
+    for (int r = c + 1; r < N; ++r)
+      if (std::fabs(A[r][c]) > std::fabs(A[piv][c])) piv = r;
+    for (int x = 0; x < N + 2; ++x) std::swap(A[c][x], A[piv][x]);
+    for (int r = c + 1; r < N; ++r) {
+      const double f = A[r][c] / A[c][c];
+      for (int x = c; x < N + 2; ++x) A[r][x] -= f * A[c][x];
+    }
+  }
+  double cp[N], cq[N];
+  for (int r = N - 1; r >= 0; --r) {
+    double vp = A[r][N], vq = A[r][N + 1];
+    for (int x = r + 1; x < N; ++x) {
+      vp -= A[r][x] * cp[x];
+      vq -= A[r][x] * cq[x];
+    }
+    cp[r] = vp / A[r][r];
+    cq[r] = vq / A[r][r];
+  }
+  for (int d = 0; d < N; ++d) {
+    sp.p[d] = (float)cp[d];
+    sp.q[d] = (float)cq[d];
+  }
+  sp.sc = sc;
+  sp.tlim = (float)tlim;
+  sp.inv_tlim = (float)(1.0 / tlim);
+}
+
 int batch_blocks_per_cu(const spsim_hip_tile &t, bool has_b, bool grid, int *blocks) {
   const void *k = kernel_for(grid ? Kind::kBatchGrid : Kind::kBatch, t, has_b);
   if (!k) return fail(SPSIM_HIP_ERR_INVALID, "unsupported tile (per lane %d, chunk %d)", t.pixels_per_lane, t.chunk_atoms);
@@ -1178,6 +1277,9 @@ extern "C" int spsim_hip_sf_spectrum(const float *d_hkl, int npix, const spsim_h
   SPSIM_TRY(ensure_device(&cus));
   if (npix == 0) return SPSIM_HIP_OK;
   spsim_hip_tile t = tile ? *tile : default_tile(natoms, npix, true);
+  // two pixels per lane share the atom loads and loop overhead (3 % on a
+  // 512 x 512 detector); smaller lists keep one for more workgroups
+  if (!tile && npix >= 256 * 256) t.pixels_per_lane = 2;
   if (kernel_for(Kind::kSpectrum, t, false) == nullptr || t.atom_splits < 0)
     return fail(SPSIM_HIP_ERR_INVALID, "unsupported spectrum tile (pixels_per_lane=%d waves=%d chunk_atoms=%d)",
                 t.pixels_per_lane, t.waves, t.chunk_atoms);
@@ -1187,6 +1289,7 @@ extern "C" int spsim_hip_sf_spectrum(const float *d_hkl, int npix, const spsim_h
     sp.s[k] = k < nsamples ? scales[k] : 0.0;
     sp.w[k] = k < nsamples ? (float)weights[k] : 0.0f;
   }
+  spectrum_fit(sp);
   const int tiles = (npix + tile_units(t) - 1) / tile_units(t);
   const int splits = t.atom_splits > 0
                          ? t.atom_splits

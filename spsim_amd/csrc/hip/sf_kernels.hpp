@@ -18,12 +18,28 @@ struct Rot {
   double r[9];
 };
 
-// Wavelength samples of a spectrum pattern: phase scales and normalised weights.
+// Degree of the spectrum factor's polynomials (in v^2) and the reach of the
+// fit: |2 pi e_k t| <= kSpectrumReach radians for every sample k.
+constexpr int kSpectrumDegree = 4;
+constexpr double kSpectrumReach = 2.0;
+
+// Wavelength samples of a spectrum pattern: phase scales and normalised
+// weights, and the polynomial form of their sum about the centre scale sc:
+// with e_k = s_k / sc - 1 and v = t / tlim (t = sc q.r in turns),
+//   sum_k W_k cos(2 pi e_k t) ~ P(v^2),  sum_k W_k sin(2 pi e_k t) ~ v Q(v^2)
+// for |t| <= tlim. tlim < 0 turns the polynomial off.
 struct Spectrum {
   int n;
   float w[SPSIM_HIP_MAX_SPECTRUM];
   double s[SPSIM_HIP_MAX_SPECTRUM];
+  double sc;
+  float tlim, inv_tlim;
+  float p[kSpectrumDegree + 1], q[kSpectrumDegree + 1];
 };
+
+// Fills sp.sc, sp.tlim, sp.inv_tlim, sp.p and sp.q from sp.n, sp.s and sp.w
+// (host code). The polynomial is used from 4 samples on.
+void spectrum_fit(Spectrum &sp);
 
 struct PatternLaunch {
   const float *hkl;
