@@ -240,8 +240,15 @@ extern "C" int spsim_hip_sf_adjoint(const float *d_hkl, int npix, const spsim_hi
     covered += g.count;
   }
   if (covered != natoms) return fail(SPSIM_HIP_ERR_INVALID, "segments cover %d atoms, natoms=%d", covered, natoms);
-  // Enough pixel ranges for ~4 workgroups per CU, each range at least a few chunks.
-  const long long want = 4LL * cus;
+  const bool hb = d_atom_b != nullptr, wb = d_grad_b != nullptr;
+  const void *k = hb ? (wb ? adjoint_ptr<KA, true, true>() : adjoint_ptr<KA, true, false>())
+                     : (wb ? adjoint_ptr<KA, false, true>() : adjoint_ptr<KA, false, false>());
+  // Enough pixel ranges to fill every CU to the kernel's occupancy (5 or 6
+  // workgroups at 2 atoms per lane; 4 left a third of the wave slots empty),
+  // each range at least a few chunks.
+  int per_cu = 4;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, kT, 0) != hipSuccess || per_cu < 1) per_cu = 4;
+  const long long want = (long long)per_cu * cus;
   int nsplit = (int)std::max<long long>(1, (want + (long long)tiles.size() - 1) / (long long)tiles.size());
   nsplit = std::max(1, std::min(nsplit, std::max(1, npix / (4 * kCP))));
   const size_t tiles_bytes = ((tiles.size() * sizeof(AtomTile) + 255) / 256) * 256;
@@ -256,9 +263,6 @@ extern "C" int spsim_hip_sf_adjoint(const float *d_hkl, int npix, const spsim_hi
     return SPSIM_HIP_OK;
   }
   AdjArgs a{d_hkl, npix, d_atoms, d_atom_b, d_tiles, d_ff, b_global, (const float2 *)d_F, d_gI, partial, natoms};
-  const bool hb = d_atom_b != nullptr, wb = d_grad_b != nullptr;
-  const void *k = hb ? (wb ? adjoint_ptr<KA, true, true>() : adjoint_ptr<KA, true, false>())
-                     : (wb ? adjoint_ptr<KA, false, true>() : adjoint_ptr<KA, false, false>());
   void *params[] = {(void *)&a};
   SPSIM_HIP_TRY(hipLaunchKernel(k, dim3((unsigned)tiles.size(), nsplit), dim3(kT), params, 0, s));
   hipLaunchKernelGGL(adjoint_reduce_kernel, dim3((natoms + kT - 1) / kT), dim3(kT), 0, s, (const double4 *)partial,
