@@ -981,11 +981,16 @@ int auto_splits(int tiles, int natoms, int chunk_atoms, int slots, int max_split
   int best = 1;
   double best_cost = 1e300;
   for (int s = 1; s <= hi; ++s) {
-    const long long items = (long long)tiles * s;
-    const long long rounds = (items + std::max(slots, 1) - 1) / std::max(slots, 1);
-    // rounds of resident workgroups x (atoms per range + a per-item cost of
-    // about 64 atoms), plus the partial sums' write and read (~0.02 % per range)
-    const double cost = (double)rounds * ((double)natoms / s + 64.0) * (1.0 + 2e-4 * s);
+    // Rounds of resident workgroups x (atoms per range + a per-item cost of
+    // about 64 atoms). Up to 4 rounds the items of a round start together and
+    // the count is whole rounds; beyond that they finish at scattered times,
+    // and the launch lasts the items' share per slot plus about half an item
+    // (on 100,000 atoms x 1024^2 octants, 16 to 48 ranges measured
+    // 18.25 + 14.4/s + 0.0073 s ms). The partial sums' write and read add
+    // ~0.04 % per range.
+    const double r = (double)tiles * s / std::max(slots, 1);
+    const double rounds = r <= 4.0 ? std::ceil(r) : r + 0.5;
+    const double cost = rounds * ((double)natoms / s + 64.0) * (1.0 + 4e-4 * s);
     if (cost < best_cost) {
       best_cost = cost;
       best = s;

@@ -138,8 +138,8 @@ int auto_splits(int tiles, int natoms, int chunk_atoms, int slots, int max_split
 // Workgroups of `kernel` resident on the device at once (cus CUs).
 int resident_slots(const void *kernel, int cus);
 // Largest automatic partial-sum buffer of a single pattern (splits x npix double2
-// entries in the library workspace): 256 MiB.
-constexpr size_t kWorkspacePartials = (size_t)1 << 24;
+// entries in the library workspace): 512 MiB, 32 atom ranges of a 1024 x 1024 pattern.
+constexpr size_t kWorkspacePartials = (size_t)1 << 25;
 // True when the [ny][nx][3] host list is exactly mirror-symmetric in x and y.
 bool host_mirror_symmetric(const float *hkl, int nx, int ny);
 // True when the n x n host list is also transpose-symmetric: q(y, x) = (qy, qx, qz)(x, y).
