@@ -99,6 +99,8 @@ def parse():
                    help="only the non-uniform FFT section on the large-single-pattern geometry (for profiling)")
     p.add_argument("--spectrum-only", action="store_true",
                    help="only the wavelength-spectrum section, direct and through the NUFFT (for profiling)")
+    p.add_argument("--adjoint-only", action="store_true",
+                   help="only the differentiation section: forward and adjoint (for profiling)")
     p.add_argument("--batched-nfft-only", action="store_true",
                    help="only the batched (many molecules) workload through the NUFFT (for profiling)")
     p.add_argument("--pipeline-nfft-only", action="store_true",
@@ -1122,6 +1124,10 @@ def main():
     if args.spectrum_only:
         log("spectrum only: direct and through the NUFFT")
         print(json.dumps({"spectrum": spectrum(args, torch, dv, wl, _lib)}))
+        return
+    if args.adjoint_only:
+        log("differentiation only: forward and adjoint")
+        print(json.dumps({"differentiation": differentiation(args, torch, dv, wl, _lib)}))
         return
     if args.batched_nfft_only:
         log("batched_nfft only: the batched workload through the NUFFT path")
