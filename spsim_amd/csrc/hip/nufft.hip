@@ -1602,11 +1602,13 @@ __global__ __launch_bounds__(kThreads) void nufft_interp(const float *__restrict
 // region and then gather from LDS, which serves twice the bytes per clock of
 // the vector L1. A wave whose box is larger gathers from global memory.
 // At least 4 waves per SIMD (at most 128 VGPRs): measured faster than the 3
-// that the unconstrained allocation (136 VGPRs at W = 7) leaves.
+// that the unconstrained allocation (136 VGPRs at W = 7) leaves. At W = 6, 5
+// waves (76 VGPRs, no scratch) measured 7 % faster on the orientation batch;
+// at W = 7 (84 VGPRs) 5 % slower, and W = 5 would spill.
 constexpr int kBoxCap = 1024;  // grid points per wave (8 KiB of LDS per species)
 
 template <int W>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void nufft_interp_lds(
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(W == 6 ? 5 : 4))) void nufft_interp_lds(
     const float *__restrict__ hkl, int npix, NufftGeom g, const float2 *__restrict__ H, int nch,
     const spsim_hip_segment *__restrict__ segs, const float *__restrict__ ff, float b_global,
     const double *__restrict__ rots, int p0, float2 *__restrict__ F, float *__restrict__ I) {
