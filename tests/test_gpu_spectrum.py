@@ -87,6 +87,23 @@ def test_spectrum_large_phases(gpu, nsamples, bw):
         assert rel_l2(np.c_[F.real, F.imag], np.c_[F_ref.real, F_ref.imag]) < TOL, tile
 
 
+@pytest.mark.parametrize("nsamples,lo,hi,seed", [(4, 0.97, 1.05, 1), (6, 0.8, 1.25, 2), (17, 1.6, 1.7, 3)])
+def test_spectrum_arbitrary_samples(gpu, nsamples, lo, hi, seed):
+    """Caller-given scales and weights (spsim_hip_sf_spectrum takes any): unevenly
+    spaced, asymmetric about 1 or far from it, with unnormalised random weights, so
+    the fitted sample factor is centred on the scales' midpoint and spans an
+    uneven set. Against the FP64 spectrum oracle given the same samples."""
+    rng = np.random.default_rng(30 + seed)
+    Z, pos, B, q = setup_case(seed=seed)
+    sc = np.sort(rng.uniform(lo, hi, nsamples))
+    w = rng.uniform(0.1, 1.0, nsamples)
+    F_ref, I_ref = oracle.sf_spectrum_f64(q, Z, pos, sc, w, atom_b=B, threads=oracle_threads())
+    for tile in (None, dv.make_tile(1, 128, 2)):
+        F, I = run(gpu, q, Z, pos, sc, w, B=B, tile=tile)
+        assert rel_l2(I, I_ref) < TOL, tile
+        assert rel_l2(np.c_[F.real, F.imag], np.c_[F_ref.real, F_ref.imag]) < TOL, tile
+
+
 def test_single_sample_matches_plain_kernel(gpu):
     import torch
 
