@@ -59,10 +59,13 @@ constexpr int kThreads = 256;
 constexpr int kMinW = 4, kMaxW = 10;
 constexpr int kMaxQuad = 2 * kMaxW + 6;
 // LDS per FFT workgroup: one buffer of T lines (T nf <= kFftLds complex values,
-// 15 KiB) and the twiddle table (up to 15 KiB), or one line of up to kMaxNf
-// points (30 KiB, twiddles read from global memory); plus two pointers per line.
+// 15 KiB; kFixedLds, 16 KiB, for the fixed-length kernels, so that 4 lines of
+// 500 points fit) and the twiddle table (up to 15 KiB), or one line of up to
+// kMaxNf points (30 KiB, twiddles read from global memory); plus two pointers
+// per line.
 constexpr int kMaxNf = 3840;      // longest FFT line
 constexpr int kFftLds = 1920;     // complex values per LDS buffer for several lines
+constexpr int kFixedLds = 2048;   // the same for the fixed-length kernels (16 KiB)
 constexpr int kMaxStages = 16;
 constexpr int kMaxT = 256;          // lines per FFT workgroup
 constexpr double kSigma = 2.0;     // oversampling of the spread window (and of the default second window)
@@ -1404,7 +1407,7 @@ template <int NF, int MODE>
 __global__ __launch_bounds__(kThreads) void fft_fixed(FftPass p, const float2 *__restrict__ in,
                                                      float2 *__restrict__ out, const float2 *__restrict__ tw_g) {
   using Plan = FixedPlanOf<NF>;
-  constexpr int T = kFftLds / NF, PITCH = NF + 1;
+  constexpr int T = kFixedLds / NF < kMaxT ? kFixedLds / NF : kMaxT, PITCH = NF + 1;
   static_assert(Plan::plan.ns > 0 && T >= 1 && T <= kMaxT, "fixed FFT length");
   extern __shared__ float2 lds[];
   float2 *x = lds;
@@ -1971,14 +1974,20 @@ int launch_fft(const FftPass &pass, const float2 *in, float2 *out, const float2 
       {fft_lines<true, kFftGeneral>, fft_lines<true, kFftContig>, fft_lines<true, kFftStrided>}};
   Kernel k = kernels[twl ? 1 : 0][mode];
   size_t lds_k = lds;
-  if (mode != kFftGeneral && p.T == kFftLds / p.nf) {
+  long long nblk = blocks;
+  if (mode != kFftGeneral && p.T == std::min(kFftLds / p.nf, kMaxT)) {
     const int f = fixed_index(p.nf);
     if (f >= 0) {
+      // the fixed kernels take kFixedLds / nf lines per workgroup
       k = mode == kFftContig ? kFixedKernels[f].contig : kFixedKernels[f].strided;
-      lds_k += (size_t)p.T * sizeof(float2);  // line pitch nf + 1
+      p.T = std::min(kFixedLds / p.nf, kMaxT);
+      p.dT = fast_div((unsigned)p.T);
+      nblk = (p.nlines + p.T - 1) / p.T;
+      lds_k = ((size_t)p.T + 1) * p.nf * sizeof(float2) + 2 * (size_t)p.T * sizeof(long long) +
+              (size_t)p.T * sizeof(float2);  // line pitch nf + 1
     }
   }
-  hipLaunchKernelGGL(k, dim3((unsigned)blocks), dim3(kThreads), lds_k, stream, p, in, out, tw);
+  hipLaunchKernelGGL(k, dim3((unsigned)nblk), dim3(kThreads), lds_k, stream, p, in, out, tw);
   SPSIM_HIP_TRY(hipGetLastError());
   return SPSIM_HIP_OK;
 }
