@@ -275,55 +275,75 @@ __global__ __launch_bounds__(kT) void chain_bin_draws_kernel(DetConst d, const f
   }
   const size_t npix = (size_t)d.nx * d.ny * d.nz;
   const int p0 = (int)blockIdx.y * run, p1 = min(npatterns, p0 + run);
-  for (int p = p0; p < p1; ++p) {
-    const uint32_t pat = (uint32_t)(pattern_base + p);
-    const float *Ip = I + (size_t)p * npix;
-    float accn = 0.0f, lam = 0.0f, peak = 0.0f;
+  // Two patterns of the bin per iteration: their loads, sums and Philox blocks
+  // are independent, so the two chains interleave.
+  for (int p = p0; p < p1; p += 2) {
+    const bool two = p + 1 < p1;  // uniform
+    const float *IpA = I + (size_t)p * npix;
+    const float *IpB = two ? IpA + npix : IpA;
+    float accn[2] = {0.0f, 0.0f}, lam[2] = {0.0f, 0.0f}, peak[2] = {0.0f, 0.0f};
     int t = 0;
     for (int zi = 0; zi < d.bin_z; ++zi) {
       for (int yi = 0; yi < d.bin_y; ++yi) {
         const int y = by * d.bin_y + yi, z = bz * d.bin_z + zi;
         const size_t row = ((size_t)z * d.ny + y) * d.nx + (size_t)bx * d.bin_x;
-        float v4[4];
+        float v4[2][4];
         if constexpr (VEC4) {
-          const float4 q = *reinterpret_cast<const float4 *>(Ip + row);
-          v4[0] = q.x;
-          v4[1] = q.y;
-          v4[2] = q.z;
-          v4[3] = q.w;
+          const float4 qa = *reinterpret_cast<const float4 *>(IpA + row);
+          const float4 qb = *reinterpret_cast<const float4 *>(IpB + row);
+          v4[0][0] = qa.x;
+          v4[0][1] = qa.y;
+          v4[0][2] = qa.z;
+          v4[0][3] = qa.w;
+          v4[1][0] = qb.x;
+          v4[1][1] = qb.y;
+          v4[1][2] = qb.z;
+          v4[1][3] = qb.w;
         }
         for (int xi = 0; xi < d.bin_x; ++xi, ++t) {
-          const float Ii = VEC4 ? v4[xi & 3] : Ip[row + xi];
           const int x = bx * d.bin_x + xi;
           float g;
           if constexpr (in_lds)
             g = thom[t * kT + threadIdx.x];
           else
             g = thomson_at(d, x, y) * solid_angle_at(d, x, y);
-          const float mean = g * Ii * d.beam_intensity * d.qe;  // (((th om) I) beam) QE
-          accn += mean * d.e_per_photon_noiseless * d.adc;
-          lam += mean;
-          peak = fmaxf(peak, mean);
+#pragma unroll
+          for (int k = 0; k < 2; ++k) {
+            const float Ii = VEC4 ? v4[k][xi & 3] : (k ? IpB : IpA)[row + xi];
+            const float mean = g * Ii * d.beam_intensity * d.qe;  // (((th om) I) beam) QE
+            accn[k] += mean * d.e_per_photon_noiseless * d.adc;
+            lam[k] += mean;
+            peak[k] = fmaxf(peak[k], mean);
+          }
         }
       }
     }
-    const size_t ob = (size_t)p * nb + bin;
-    if (noiseless) noiseless[ob] = accn;
-    if (!real) continue;
-    const bool marked = peak >= 1000.0f || lam >= 10.0f;
-    const unsigned long long m = __ballot(marked);
-    if (threadIdx.x % 64 == 0) marks[(size_t)p * nw + gid / 64] = m;
-    if (marked) {
-      real[ob] = __builtin_nanf("");  // recomputed by chain_bin_fixup_kernel
-      continue;
+    U4 r[2];
+    if (real) {
+      // a marked bin's block goes unused (its bin is redrawn by the second pass)
+      r[0] = draw(seed, (uint32_t)bin, (uint32_t)(pattern_base + p), kRngPoissonBin, 0u);
+      r[1] = draw(seed, (uint32_t)bin, (uint32_t)(pattern_base + p + 1), kRngPoissonBin, 0u);
     }
-    const U4 r = draw(seed, (uint32_t)bin, pat, kRngPoissonBin, 0u);
-    float acc = dark_sum;
-    acc += (float)bin_count_small(lam, r) * d.e_per_photon;
-    acc += d.readout_noise * box_muller_f32(uniform32(r.v[2]), uniform32(r.v[3]));
-    if (acc < 0) acc = 0;
-    acc *= d.adc;
-    real[ob] = acc;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      if (k == 1 && !two) break;
+      const size_t ob = (size_t)(p + k) * nb + bin;
+      if (noiseless) noiseless[ob] = accn[k];
+      if (!real) continue;
+      const bool marked = peak[k] >= 1000.0f || lam[k] >= 10.0f;
+      const unsigned long long m = __ballot(marked);
+      if (threadIdx.x % 64 == 0) marks[(size_t)(p + k) * nw + gid / 64] = m;
+      if (marked) {
+        real[ob] = __builtin_nanf("");  // recomputed by chain_bin_fixup_kernel
+        continue;
+      }
+      float acc = dark_sum;
+      acc += (float)bin_count_small(lam[k], r[k]) * d.e_per_photon;
+      acc += d.readout_noise * box_muller_f32(uniform32(r[k].v[2]), uniform32(r[k].v[3]));
+      if (acc < 0) acc = 0;
+      acc *= d.adc;
+      real[ob] = acc;
+    }
   }
 }
 
