@@ -339,8 +339,10 @@ int spsim_hip_batch_set_item_atoms(spsim_hip_batch *b, int item_atoms);
 int spsim_hip_batch_describe(const spsim_hip_batch *b, int *nitems, int *item_atoms, int *split_tiles);
 /* Autotuner: times every supported tile (pixels per lane x LDS chunk) on a
    slice of the plan's own work (patterns spread over its atom-count range),
-   keeps the fastest in pixel-atom terms/s and returns it (best_rate may be
-   NULL). Synchronises the stream. */
+   keeps the kernel family (pixels or quads per lane, octants) with the
+   highest pixel-atom terms/s and, within it, the LDS chunk whose sample runs
+   fastest, and returns it (best_rate: its terms/s; may be NULL). Synchronises
+   the stream. */
 int spsim_hip_batch_autotune(spsim_hip_batch *b, void *stream, spsim_hip_tile *best, double *best_rate);
 /* Total pixel-atom terms of one run (sum over patterns of natoms * npix). */
 int spsim_hip_batch_terms(const spsim_hip_batch *b, double *terms);
