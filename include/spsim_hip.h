@@ -269,7 +269,7 @@ int spsim_hip_sf_spectrum_nfft(const float *d_hkl, int npix, const spsim_hip_ato
    molecule's first atom (host offset arrays of nmol + 1 entries, from 0). The
    bounds of every molecule are read with one synchronisation; the molecules'
    stages are then enqueued back to back, so the host does not wait between
-   molecules. They run on `stream` and on four library-owned streams forked
+   molecules. They run on `stream` and on 31 library-owned streams forked
    from it and joined back into it (work enqueued on `stream` afterwards runs
    after every pattern), each stream with its own slice of the library
    workspace. Calls
