@@ -899,9 +899,9 @@ def spectrum(args, torch, dv, wl, _lib):
     ts = float(det.npix) * atoms.natoms * len(sc)
     w_eq = 20.0 + 10.0 / len(sc)  # per term-sample: sincos + 2 FMA, plus the per-term phase and sum
     # every lane takes the fitted sample factor when |s_c q.r| <= tlim =
-    # 2 / (2 pi max|s_k / s_c - 1|), s_c the midpoint of the scales
+    # 1.5 / (2 pi max|s_k / s_c - 1|), s_c the midpoint of the scales
     s_c = 0.5 * (min(sc) + max(sc))
-    tlim = 2.0 / (2 * math.pi * max(abs(x / s_c - 1) for x in sc))
+    tlim = 1.5 / (2 * math.pi * max(abs(x / s_c - 1) for x in sc))
     t_max = s_c * torch.linalg.vector_norm(hkl.double(), dim=1).max().item() * float(np.max(np.linalg.norm(pos, axis=1)))
     roof = None
     if t_max <= tlim:

@@ -1048,8 +1048,10 @@ int launch_spectrum(const PatternLaunch &a, const Spectrum &sp, const spsim_hip_
 // a_k = 2 pi e_k tlim, both entire in u; the monomial coefficients come from
 // the Vandermonde system (Gaussian elimination with partial pivoting). With
 // tlim = kSpectrumReach / (2 pi max|e_k|), the FP32 polynomials stay within
-// 2e-7 of the sample sums over |t| <= tlim, and within 6e-8 over the first
-// third of that range.
+// 3e-7 sum_k |W_k| of the sample sums over |t| <= tlim, for any placement of
+// the samples (1.2e-7 for Gaussian spectra, whose extreme samples weigh
+// little). A reach of 2 radians left up to 1.1e-6 for samples weighted evenly
+// out to the extremes.
 void spectrum_fit(Spectrum &sp) {
   constexpr int N = kSpectrumDegree + 1;
   constexpr double kTwoPi = 6.283185307179586476925;

@@ -21,7 +21,7 @@ struct Rot {
 // Degree of the spectrum factor's polynomials (in v^2) and the reach of the
 // fit: |2 pi e_k t| <= kSpectrumReach radians for every sample k.
 constexpr int kSpectrumDegree = 4;
-constexpr double kSpectrumReach = 2.0;
+constexpr double kSpectrumReach = 1.5;
 
 // Wavelength samples of a spectrum pattern: phase scales and normalised
 // weights, and the polynomial form of their sum about the centre scale sc:
